@@ -1,0 +1,318 @@
+// Explicit (FTCS + forward-Euler HH) step, FP64 plain form, one fused kernel.
+//
+// Semantics follow the reference CPU solver, the general-topology one
+// (solverEx-cpu.cpp:45-123 with hhRate_Const coupling.cpp:25-48); the reference
+// GPU solver splits the same step into four launches (solverEx-gpu.cpp:87-194)
+// and assumes chain numbering, which this kernel does not.
+//
+// Layout in HBM (structure of arrays, 8-byte values, 4-byte codes):
+//   V[2][N]        double-buffered membrane potential
+//   coefA[N]       dT/(c_m*dx)           (solverEx-cpu.cpp:96)
+//   rdx[N]         r_a*dx                (divisor of the axial terms)
+//   wdx[N], kdx[N] W*dx, K*dx of passive elements (constant in time)
+//   code[N]        element class + index (plain chain / HH / record / skip)
+//   recs[R]        exception records (branch points, daughters, stimulus
+//                  sites, sources of sealed-end copies)
+//   HH SoA [H]     m, h, n (state) and E_Na, E_K, E_L, g_Na, g_K, G_L,
+//                  ls_Na, ls_K, rest, dx (parameters)
+// Algorithmic bytes per step: 48 per passive element (V r/w, coefA, rdx, wdx,
+// kdx), 152 per HH element (V r/w, coefA, rdx, m/h/n r/w, 9 parameters).
+//
+// Every operation keeps the reference's expression order and divisions, and the
+// library is built with -ffp-contract=off, so each element update is the same
+// IEEE double sequence as the reference CPU build; the only differing operation
+// is exp() (device libm vs host libm, both within an ulp or so).
+#include <algorithm>
+#include "nrt_internal.hpp"
+
+namespace nrt {
+
+struct HHDev {
+  double* m;
+  double* h;
+  double* n;
+  const double* E_Na;
+  const double* E_K;
+  const double* E_L;
+  const double* g_Na;
+  const double* g_K;
+  const double* G_L;
+  const double* ls_Na;
+  const double* ls_K;
+  const double* rest;
+  const double* dx;
+};
+
+struct Explicit {
+  int N = 0, H = 0, R = 0;
+  double dT = 0, dt_ms = 0;
+  DBuf<double> V[2];
+  DBuf<double> coefA, rdx, wdx, kdx;
+  DBuf<uint32_t> code;
+  DBuf<ExRec> recs;
+  DBuf<double> m, h, n, E_Na, E_K, E_L, g_Na, g_K, G_L, ls_Na, ls_K, rest, hdx;
+  DBuf<double> trace;
+  int cur = 0;  // buffer holding the latest potential (the reference's prevT)
+  int64_t trace_cap = 0;
+  HHDev hd() {
+    return HHDev{m.p, h.p, n.p, E_Na.p, E_K.p, E_L.p, g_Na.p, g_K.p, G_L.p, ls_Na.p, ls_K.p,
+                 rest.p, hdx.p};
+  }
+};
+
+// Forward-Euler HH gating of HH element i at potential v (solverEx-cpu.cpp:62-82,
+// coupling.cpp:25-48, discretization.cpp:81-86); returns W*dx and K*dx.
+__device__ __forceinline__ void hh_gate(const HHDev& hd, int i, double v, double dt_ms,
+                                        double& wdx, double& kdx) {
+  const double rest = hd.rest[i];
+  double vna = v + hd.ls_Na[i];
+  double vk = v + hd.ls_K[i];
+  vna = (vna - rest) / 0.001;
+  vk = (vk - rest) / 0.001;
+  const double am = (2.5 - 0.1 * vna) / (exp(2.5 - 0.1 * vna) - 1);
+  const double bm = 4 * exp(-vna / 18);
+  const double ah = 0.07 * exp(-vna / 20);
+  const double bh = 1 / (exp(3 - 0.1 * vna) + 1);
+  const double an = (0.1 - 0.01 * vk) / (exp(1 - 0.1 * vk) - 1);
+  const double bn = 0.125 * exp(-vk / 80);
+  const double m0 = hd.m[i], h0 = hd.h[i], n0 = hd.n[i];
+  const double m = m0 + dt_ms * (am * (1 - m0) - bm * m0);
+  const double h = h0 + dt_ms * (ah * (1 - h0) - bh * h0);
+  const double n = n0 + dt_ms * (an * (1 - n0) - bn * n0);
+  hd.m[i] = m;
+  hd.h[i] = h;
+  hd.n[i] = n;
+  const double gna = hd.g_Na[i] * m * m * m * h;
+  const double gk = hd.g_K[i] * n * n * n * n;
+  const double gl = hd.G_L[i];
+  const double W = -(gna + gk + gl);
+  const double K = gna * hd.E_Na[i] + gk * hd.E_K[i] + gl * hd.E_L[i];
+  const double dx = hd.dx[i];
+  wdx = W * dx;
+  kdx = K * dx;
+}
+
+// One explicit step for every element (thread per element).
+__global__ __launch_bounds__(256) void k_explicit_fp64(
+    int N, const double* __restrict__ Vo, double* __restrict__ Vn,
+    const double* __restrict__ coefA, const double* __restrict__ rdx,
+    const double* __restrict__ wdx, const double* __restrict__ kdx,
+    const uint32_t* __restrict__ code, const ExRec* __restrict__ recs, HHDev hd, double dt_ms,
+    int active) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const uint32_t c = code[j];
+  const double v = Vo[j];
+  if (c & kCodeSkip) {
+    // Terminal / ghost: its potential is a sealed-end copy written by its source,
+    // but an HH terminal still advances its gates (solverEx-cpu.cpp:62).
+    if (c & kCodeHH) {
+      double wd, kd;
+      hh_gate(hd, (int)(c & kCodeIdx), v, dt_ms, wd, kd);
+    }
+    return;
+  }
+  int mo = j - 1, dr = j + 1, dl = -1, hh = -1;
+  const ExRec* rec = nullptr;
+  if (c & kCodeRec) {
+    rec = &recs[c & kCodeIdx];
+    mo = rec->mother;
+    dr = rec->dr;
+    dl = (rec->flags & 1) ? rec->dl : -1;
+    hh = rec->hh;
+  } else if (c & kCodeHH) {
+    hh = (int)(c & kCodeIdx);
+  }
+  double wd, kd;
+  if (hh >= 0) {
+    hh_gate(hd, hh, v, dt_ms, wd, kd);
+  } else {
+    wd = wdx[j];
+    kd = kdx[j];
+  }
+  // branching*(DL term) + DR term + mother term + W*dX*V + K*dX + input
+  const double tdr = (Vo[dr] - v) / rdx[dr];
+  double s = (dl >= 0) ? ((Vo[dl] - v) / rdx[dl] + tdr) : tdr;
+  s = s + (Vo[mo] - v) / rdx[j];
+  s = s + wd * v;
+  s = s + kd;
+  if (rec && (rec->flags & 2) && active) s = s + rec->input;
+  const double vn = v + coefA[j] * s;
+  Vn[j] = vn;
+  if (rec) {
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+      if (rec->tgt[t] >= 0) Vn[rec->tgt[t]] = vn;
+  }
+}
+
+Explicit* explicit_create(nrt_handle* hnd, const HostProblem& p) {
+  hipStream_t s = hnd->stream;
+  const int N = p.N, H = p.H;
+  auto* e = new Explicit;
+  try {
+    e->N = N;
+    e->H = H;
+    e->dT = p.dT;
+    e->dt_ms = p.dT * 1000;  // solverEx-cpu.cpp:51
+    std::vector<int32_t> hh_of(N, -1);
+    for (int i = 0; i < H; i++) {
+      NRT_REQUIRE(hh_of[p.hh_elem[i]] < 0, NRT_ERR_INVALID, "duplicate HH element index");
+      hh_of[p.hh_elem[i]] = i;
+    }
+    // Sealed-end copies (solverEx-cpu.cpp:112-116): terminals[0] copies element
+    // terminals[0]+1, every other terminal copies its mother.
+    std::vector<uint8_t> is_term(N, 0);
+    for (int k = 0; k < p.T; k++) is_term[p.terminals[k]] = 1;
+    std::vector<std::vector<int32_t>> targets(N);
+    for (int k = 0; k < p.T; k++) {
+      const int t = p.terminals[k];
+      const int src = (k == 0) ? t + 1 : p.mother[t];
+      NRT_REQUIRE(src >= 1 && src <= N - 2 && !is_term[src], NRT_ERR_TOPOLOGY,
+                  "terminal " + std::to_string(t) + " copies element " + std::to_string(src) +
+                      ", which is not an interior non-terminal element");
+      auto& tg = targets[src];
+      if (std::find(tg.begin(), tg.end(), t) == tg.end()) tg.push_back(t);
+      NRT_REQUIRE(tg.size() <= 3, NRT_ERR_TOPOLOGY,
+                  "more than three sealed ends copy element " + std::to_string(src));
+    }
+    std::vector<uint32_t> code(N, 0);
+    std::vector<ExRec> recs;
+    std::vector<double> coefA(N, 0.0), rdx(N), wdx(N, 0.0), kdx(N, 0.0);
+    for (int j = 0; j < N; j++) rdx[j] = p.r_a[j] * p.dx[j];
+    for (int j = 0; j < N; j++) {
+      const int hh = hh_of[j];
+      const bool computed = j >= 1 && j <= N - 2 && !is_term[j];
+      if (!computed) {
+        code[j] = kCodeSkip | (hh >= 0 ? (kCodeHH | (uint32_t)hh) : 0u);
+        continue;
+      }
+      const int mo = p.mother[j], dr = p.dr[j], dl = p.dl[j];
+      NRT_REQUIRE(mo >= 0 && mo < N && dr >= 0 && dr < N, NRT_ERR_TOPOLOGY,
+                  "element " + std::to_string(j) + " has an out-of-range neighbour");
+      NRT_REQUIRE(!p.branching[j] || (dl >= 0 && dl < N), NRT_ERR_TOPOLOGY,
+                  "branching element " + std::to_string(j) + " has an out-of-range DL");
+      coefA[j] = p.dT / (p.c_m[j] * p.dx[j]);
+      if (hh < 0) {
+        wdx[j] = p.W[j] * p.dx[j];
+        kdx[j] = p.K[j] * p.dx[j];
+      }
+      const bool plain = mo == j - 1 && dr == j + 1 && !p.branching[j] && p.input[j] == 0.0 &&
+                         targets[j].empty();
+      if (plain) {
+        code[j] = hh >= 0 ? (kCodeHH | (uint32_t)hh) : 0u;
+        continue;
+      }
+      ExRec r{};
+      r.mother = mo;
+      r.dr = dr;
+      r.dl = dl;
+      r.hh = hh;
+      for (int t = 0; t < 3; t++) r.tgt[t] = t < (int)targets[j].size() ? targets[j][t] : -1;
+      r.flags = (p.branching[j] ? 1 : 0) | (p.input[j] != 0.0 ? 2 : 0);
+      r.input = p.input[j];
+      NRT_REQUIRE(recs.size() < kCodeIdx, NRT_ERR_INVALID, "too many exception records");
+      code[j] = kCodeRec | (uint32_t)recs.size();
+      recs.push_back(r);
+    }
+    e->R = (int)recs.size();
+    e->V[0].upload(p.V0, s);  // prevT (solverEx-cpu.cpp:26-27)
+    e->V[1].upload(p.V1, s);  // currT
+    e->cur = 0;
+    e->coefA.upload(coefA, s);
+    e->rdx.upload(rdx, s);
+    e->wdx.upload(wdx, s);
+    e->kdx.upload(kdx, s);
+    e->code.upload(code, s);
+    e->recs.upload(recs, s);
+    std::vector<double> hdx(H);
+    for (int i = 0; i < H; i++) hdx[i] = p.dx[p.hh_elem[i]];
+    e->m.upload(p.m, s);
+    e->h.upload(p.h, s);
+    e->n.upload(p.n, s);
+    e->E_Na.upload(p.E_Na, s);
+    e->E_K.upload(p.E_K, s);
+    e->E_L.upload(p.E_L, s);
+    e->g_Na.upload(p.g_Na, s);
+    e->g_K.upload(p.g_K, s);
+    e->G_L.upload(p.G_L, s);
+    e->ls_Na.upload(p.ls_Na, s);
+    e->ls_K.upload(p.ls_K, s);
+    e->rest.upload(p.rest, s);
+    e->hdx.upload(hdx, s);
+    NRT_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    delete e;
+    throw;
+  }
+  return e;
+}
+
+void explicit_destroy(Explicit* e) { delete e; }
+
+static void launch_step(nrt_handle* hnd, Explicit* e, bool active) {
+  const int block = 256;
+  const double* Vo = e->V[e->cur].p;
+  double* Vn = e->V[1 - e->cur].p;
+  hipLaunchKernelGGL(k_explicit_fp64, dim3(grid_for(e->N, block)), dim3(block), 0, hnd->stream,
+                     e->N, Vo, Vn, e->coefA.p, e->rdx.p, e->wdx.p, e->kdx.p, e->code.p, e->recs.p,
+                     e->hd(), e->dt_ms, active ? 1 : 0);
+  NRT_HIP(hipGetLastError());
+  e->cur = 1 - e->cur;  // flip (solverEx-cpu.cpp:119-121)
+}
+
+void explicit_step(nrt_handle* hnd, bool active, double* trace_dst, int32_t trace_elem) {
+  Explicit* e = hnd->ex;
+  launch_step(hnd, e, active);
+  if (trace_dst)
+    NRT_HIP(hipMemcpyAsync(trace_dst, e->V[e->cur].p + trace_elem, sizeof(double),
+                           hipMemcpyDeviceToDevice, hnd->stream));
+}
+
+void explicit_snapshot(nrt_handle* hnd, double* host_out) {
+  Explicit* e = hnd->ex;
+  NRT_HIP(hipMemcpyAsync(host_out, e->V[e->cur].p, sizeof(double) * e->N,
+                         hipMemcpyDeviceToHost, hnd->stream));
+  NRT_HIP(hipStreamSynchronize(hnd->stream));
+}
+
+void explicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
+                  int32_t trace_elem, int64_t interval, double* trace_out, int64_t* ns) {
+  Explicit* e = hnd->ex;
+  const bool tracing = interval >= 0;
+  const int64_t max_samples = tracing ? num / (interval + 1) : 0;
+  if (tracing && trace_out && max_samples > e->trace_cap) {
+    e->trace.alloc((size_t)max_samples);
+    e->trace_cap = max_samples;
+  }
+  hipEvent_t t0, t1;
+  NRT_HIP(hipEventCreate(&t0));
+  NRT_HIP(hipEventCreate(&t1));
+  NRT_HIP(hipEventRecord(t0, hnd->stream));
+  int64_t gl = 0, k = 0;
+  for (int64_t i = first; i < first + num; i++) {
+    const bool active = i >= a0 && i <= a1;
+    double* dst = nullptr;
+    if (tracing && ++gl > interval) {
+      gl = 0;
+      if (trace_out) dst = e->trace.p + k;
+      k++;
+    }
+    explicit_step(hnd, active, dst, trace_elem);
+  }
+  NRT_HIP(hipEventRecord(t1, hnd->stream));
+  if (trace_out && k > 0)
+    NRT_HIP(hipMemcpyAsync(trace_out, e->trace.p, sizeof(double) * k, hipMemcpyDeviceToHost,
+                           hnd->stream));
+  NRT_HIP(hipStreamSynchronize(hnd->stream));
+  float ms = 0;
+  NRT_HIP(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  hnd->stats.kernel_ms = ms;
+  hnd->stats.kernel_launches = num;
+  hnd->stats.steps += num;
+  if (ns) *ns = k;
+}
+
+}  // namespace nrt

@@ -1,0 +1,768 @@
+// Implicit (backward-Euler) step, FP64 plain form.
+//
+// Matrix: the CSR of the reference GPU solver (solverIm-gpu.cpp:199-369, row types
+// of :31-37), assembled on the device: a row-length kernel, an exclusive scan for
+// the row pointers, then a fill kernel. HH rows are refreshed every step with the
+// Update_Mat formulas (solverIm-gpu.cpp:103-156); the right-hand side follows
+// Update_B (:162-171).
+//
+// Solve: BiCGSTAB with the reference's recurrences (solverIm-gpu.cpp:725-843), run
+// as ONE persistent cooperative kernel per call: the HH gating, HH-row refresh and
+// right-hand side of each time step, then every Krylov iteration, with device
+// resident scalars and no host round trip. Grid-wide dependencies use a bounded
+// grid barrier; each iteration needs three (one per global reduction). Vectors that
+// a neighbour block reads while the owner rewrites them are double buffered, and the
+// off-block entries of p and s that an SpMV needs are recomputed from their
+// defining recurrences (p = r + beta*(p - omega*v), s = r - alpha*v) rather than
+// exchanged, which removes two barriers per iteration.
+//
+// Dot products: per-wave DPP reduction, per-block LDS reduction, then every block
+// sums the per-block partials in the same fixed order, so all blocks hold
+// bit-identical scalars and take identical control-flow decisions.
+//
+// Layout in HBM (FP64 unless noted): x(=pot), b, r0, r, p[2], v[2], s, t (N each);
+// rowptr/col (int32), val; per element cmdx = c_m*dx, kdx = K*dx, rdx = r_a*dx,
+// input, code (uint32); HH SoA as in the explicit step plus row type/position/DL.
+#include <algorithm>
+#include <numeric>
+#include "nrt_internal.hpp"
+#include "nrt_device.hpp"
+
+namespace nrt {
+
+enum : int32_t {
+  RT_NORMAL = 1, RT_BRANCH = 2, RT_TERM = 4, RT_NEXT_TERM = 8, RT_FIRST_NORMAL = 16,
+  RT_FIRST_BRANCH = 32, RT_OTHER = 64
+};
+
+struct ImpHH {
+  double* m;
+  double* h;
+  double* n;
+  const double* E_Na;
+  const double* E_K;
+  const double* E_L;
+  const double* g_Na;
+  const double* g_K;
+  const double* G_L;
+  const double* ls_Na;
+  const double* ls_K;
+  const double* rest;
+  const int32_t* type;
+  const int32_t* mat;
+  const int32_t* dl;
+};
+
+struct ImpStats {
+  unsigned long long iters_total;
+  unsigned long long steps;
+  int last_iters;
+  int status;
+  double last_rr;
+  double last_bb;
+};
+
+struct ImpDev {
+  int N;
+  int nblocks;
+  int rpb;  // rows per block
+  double dT;
+  double dt_ms;
+  const int32_t* rowptr;
+  const int32_t* col;
+  double* val;
+  const double* cmdx;
+  const double* kdx;
+  const double* input;
+  const double* rdx;
+  const double* dxv;
+  const uint32_t* code;
+  ImpHH hh;
+  double* x;
+  double* b;
+  double* r0;
+  double* r;
+  double* p0;
+  double* p1;
+  double* v0;
+  double* v1;
+  double* s;
+  double* t;
+  double* part;
+  unsigned* bar;
+  unsigned* err;
+  ImpStats* st;
+  double tol;
+  int tol_mode;
+  int x0_mode;
+  int check_every;
+  int max_iter;
+  unsigned long long spin_limit;
+  long long first, num, a0, a1, interval;
+  int trace_elem;
+  double* trace;
+};
+
+struct Implicit {
+  int N = 0, H = 0, nnz = 0, nblocks = 0;
+  double dT = 0;
+  DBuf<int32_t> rowptr, col, len, scan_tmp;
+  DBuf<double> val;
+  DBuf<double> cmdx, kdx, input, rdx, dxv, Wc;
+  DBuf<int32_t> mother, dr, dl;
+  DBuf<uint8_t> branching;
+  DBuf<uint32_t> code;
+  DBuf<double> m, h, n, E_Na, E_K, E_L, g_Na, g_K, G_L, ls_Na, ls_K, rest;
+  DBuf<int32_t> hh_type, hh_mat, hh_dl;
+  DBuf<double> x, b, r0, r, p[2], v[2], s, t;
+  DBuf<double> part;
+  DBuf<unsigned> bar;
+  DBuf<ImpStats> st;
+  DBuf<double> trace;
+  int64_t trace_cap = 0;
+  int64_t step = 1;
+};
+
+// ------------------------------------------------------------------ assembly
+// Row length per the reference GPU assembly (solverIm-gpu.cpp:205-368).
+__global__ void k_row_len(int N, const int32_t* __restrict__ dr, const int32_t* __restrict__ dl,
+                          int32_t* __restrict__ len) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int l;
+  if (i == 0 || i == N - 1) l = 1;
+  else if (i == 1) l = 2 + (dl[1] != 0 ? 1 : 0);
+  else if (i == N - 2) l = 2;
+  else if (dr[i] == i) l = 1;
+  else if (dr[i + 1] == i + 1) l = 2;
+  else l = 3 + (dl[i] != 0 ? 1 : 0);
+  len[i] = l;
+}
+
+// Exclusive scan of int32 in three passes (per-block sums, scan of the block sums
+// in one block, per-block local scans plus offsets). out has n+1 entries.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(int n, const int32_t* in, int32_t* sums) {
+  __shared__ int32_t sh[kScanBlock / kWave];
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  int v = i < n ? in[i] : 0;
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < kScanBlock / kWave; w++) s += sh[w];
+    sums[blockIdx.x] = s;
+  }
+}
+__device__ __forceinline__ int block_excl_scan(int v, int* sh, int& total) {
+  // inclusive wave scan by shuffles, then wave offsets through LDS
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  int x = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) sh[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); w++) {
+      int t = sh[w];
+      sh[w] = acc;
+      acc += t;
+    }
+    sh[kScanBlock / kWave] = acc;
+  }
+  __syncthreads();
+  int res = x - v + sh[wid];
+  total = sh[kScanBlock / kWave];
+  __syncthreads();
+  return res;
+}
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(int nb, int32_t* sums) {
+  __shared__ int sh[kScanBlock / kWave + 1];
+  int carry = 0;
+  for (int base = 0; base < nb; base += kScanBlock) {
+    const int i = base + threadIdx.x;
+    int v = i < nb ? sums[i] : 0, total;
+    int e = block_excl_scan(v, sh, total);
+    if (i < nb) sums[i] = e + carry;
+    carry += total;
+  }
+}
+__global__ __launch_bounds__(kScanBlock) void k_scan_final(int n, const int32_t* in,
+                                                            const int32_t* sums, int32_t* out) {
+  __shared__ int sh[kScanBlock / kWave + 1];
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  int v = i < n ? in[i] : 0, total;
+  int e = block_excl_scan(v, sh, total);
+  if (i < n) out[i] = e + sums[blockIdx.x];
+  if (i == n - 1) out[n] = e + sums[blockIdx.x] + v;
+}
+
+// Fill cols/values (solverIm-gpu.cpp:199-369) and the HH row metadata.
+__global__ void k_fill(int N, double dT, const int32_t* __restrict__ rowptr,
+                       int32_t* __restrict__ col, double* __restrict__ val,
+                       const int32_t* __restrict__ mother, const int32_t* __restrict__ dr,
+                       const int32_t* __restrict__ dl, const uint8_t* __restrict__ branching,
+                       const double* __restrict__ rdx, const double* __restrict__ cmdx,
+                       const double* __restrict__ dxv, const double* __restrict__ W,
+                       const uint32_t* __restrict__ code, int32_t* hh_type, int32_t* hh_mat,
+                       int32_t* hh_dl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int k = rowptr[i];
+  const uint32_t c = code[i];
+  const int hh = (c & kCodeHH) ? (int)(c & kCodeIdx) : -1;
+  int type;
+  int dlv = -1;
+  if (i == 0 || i == N - 1) {
+    col[k] = i;
+    val[k] = 1;
+    type = RT_OTHER;
+  } else if (i != N - 2 && i != 1 && dr[i] == i) {
+    col[k] = i;
+    val[k] = 1;
+    type = RT_TERM;
+  } else {
+    const int DR = dr[i], DL = dl[i];
+    const double alpha = -1 / rdx[i];
+    const double betaR = 1 / rdx[DR];
+    const double betaL = (i == N - 2) ? 0.0 : (double)branching[i] / rdx[DL];
+    const double betaMe = -W[i] * dxv[i];
+    const double beta = (i == N - 2) ? cmdx[i] / dT + betaMe - alpha + betaR
+                                     : cmdx[i] / dT + betaMe - alpha + betaL + betaR;
+    const double delta = -betaR;
+    if (i == 1) {
+      col[k] = 1;
+      val[k] = alpha + beta;
+      col[k + 1] = 2;
+      val[k + 1] = delta;
+      if (DL != 0) {
+        col[k + 2] = DL;
+        val[k + 2] = -betaL;
+        type = RT_FIRST_BRANCH;
+        dlv = DL;
+      } else {
+        type = RT_FIRST_NORMAL;
+      }
+    } else if (i == N - 2 || dr[i + 1] == i + 1) {
+      col[k] = mother[i];
+      val[k] = alpha;
+      col[k + 1] = i;
+      val[k + 1] = beta + delta;
+      type = RT_NEXT_TERM;
+    } else {
+      col[k] = mother[i];
+      val[k] = alpha;
+      col[k + 1] = i;
+      val[k + 1] = beta;
+      col[k + 2] = DR;
+      val[k + 2] = delta;
+      if (DL != 0) {
+        col[k + 3] = DL;
+        val[k + 3] = -betaL;
+        type = RT_BRANCH;
+        dlv = DL;
+      } else {
+        type = RT_NORMAL;
+      }
+    }
+  }
+  if (hh >= 0) {
+    hh_type[hh] = type;
+    hh_mat[hh] = k;
+    hh_dl[hh] = dlv;
+  }
+}
+
+// --------------------------------------------------------- persistent kernel
+constexpr int kImpBlock = 256;
+
+// HH gating (solverIm-gpu.cpp:61-98) at potential v; returns W and K.
+__device__ __forceinline__ void imp_gate(const ImpHH& hd, int i, double v, double dt_ms,
+                                         double& W, double& K) {
+  const double rest = hd.rest[i];
+  double vna = v + hd.ls_Na[i];
+  double vk = v + hd.ls_K[i];
+  vna = (vna - rest) / 0.001;
+  vk = (vk - rest) / 0.001;
+  const double am = (2.5 - 0.1 * vna) / (exp(2.5 - 0.1 * vna) - 1);
+  const double bm = 4 * exp(-vna / 18);
+  const double ah = 0.07 * exp(-vna / 20);
+  const double bh = 1 / (exp(3 - 0.1 * vna) + 1);
+  const double an = (0.1 - 0.01 * vk) / (exp(1 - 0.1 * vk) - 1);
+  const double bn = 0.125 * exp(-vk / 80);
+  const double m0 = hd.m[i], h0 = hd.h[i], n0 = hd.n[i];
+  const double m = m0 + dt_ms * (am * (1 - m0) - bm * m0);
+  const double h = h0 + dt_ms * (ah * (1 - h0) - bh * h0);
+  const double n = n0 + dt_ms * (an * (1 - n0) - bn * n0);
+  hd.m[i] = m;
+  hd.h[i] = h;
+  hd.n[i] = n;
+  const double gna = hd.g_Na[i] * m * m * m * h;
+  const double gk = hd.g_K[i] * n * n * n * n;
+  const double gl = hd.G_L[i];
+  W = -(gna + gk + gl);
+  K = gna * hd.E_Na[i] + gk * hd.E_K[i] + gl * hd.E_L[i];
+}
+
+// Update_Mat (solverIm-gpu.cpp:103-156) for the HH row of element iv.
+__device__ __forceinline__ void imp_update_row(const ImpDev& d, int hh, int iv, double W) {
+  const int im = d.hh.mat[hh];
+  double* v = d.val;
+  const double alpha = -1 / d.rdx[iv];
+  const double betaR = 1 / d.rdx[iv + 1];
+  const double betaMe = -(W * d.dxv[iv]) + 1 / d.rdx[iv];
+  const double beta = d.cmdx[iv] / d.dT + betaMe + betaR;
+  const double delta = -betaR;
+  double betaL;
+  switch (d.hh.type[hh]) {
+    case RT_NEXT_TERM:
+      v[im] = alpha;
+      v[im + 1] = beta + delta;
+      break;
+    case RT_BRANCH:
+      betaL = 1 / d.rdx[d.hh.dl[hh]];
+      v[im] = alpha;
+      v[im + 1] = beta + betaL;
+      v[im + 2] = delta;
+      v[im + 3] = -betaL;
+      break;
+    case RT_NORMAL:
+      v[im] = alpha;
+      v[im + 1] = beta;
+      v[im + 2] = delta;
+      break;
+    case RT_FIRST_NORMAL:
+      v[im] = alpha + beta;
+      v[im + 1] = delta;
+      break;
+    case RT_FIRST_BRANCH:
+      betaL = 1 / d.rdx[d.hh.dl[hh]];
+      v[im] = alpha + beta + betaL;
+      v[im + 1] = delta;
+      v[im + 2] = -betaL;
+      break;
+    default:
+      break;
+  }
+}
+
+// Grid-wide sum of NV values: block partials go to one of two alternating slots,
+// one barrier, then every block sums all partials in the same order. A block can
+// only refill a slot after passing the next reduction's barrier, by which time
+// every block has read that slot, so one barrier per reduction suffices.
+template <int NV>
+__device__ __forceinline__ bool reduce_all(const ImpDev& d, double (&v)[NV], double* lds,
+                                           int& slot) {
+  block_sum<NV>(v, lds);
+  double* part = d.part + (size_t)slot * d.nblocks * 4;
+  slot ^= 1;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) part[blockIdx.x * 4 + k] = v[k];
+  }
+  grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+  grid_partials_sum<NV>(part, 4, d.nblocks, v, lds);
+  return __hip_atomic_load(d.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+}
+
+__global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) {
+  __shared__ double lds[4 * 16];
+  const int r_begin = blockIdx.x * d.rpb;
+  const int r_end = min(d.N, r_begin + d.rpb);
+  const double* __restrict__ val = d.val;
+  const int32_t* __restrict__ rowptr = d.rowptr;
+  const int32_t* __restrict__ col = d.col;
+  long long gl = 0, ksamp = 0;
+  int pb = 0, slot = 0;
+  for (long long step = d.first; step < d.first + d.num; step++) {
+    const bool active = step >= d.a0 && step <= d.a1;
+    // ---- update phase: gating, HH rows, right-hand side, initial guess
+    for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
+      const double v = d.x[i];
+      const uint32_t c = d.code[i];
+      double kd;
+      if (c & kCodeHH) {
+        const int hh = (int)(c & kCodeIdx);
+        double W, K;
+        imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+        imp_update_row(d, hh, i, W);
+        kd = K * d.dxv[i];
+      } else {
+        kd = d.kdx[i];
+      }
+      double bi = d.cmdx[i] * v / d.dT + kd;
+      if (active) bi = bi + d.input[i];
+      d.b[i] = bi;
+      if (d.x0_mode == NRT_X0_RHS) d.x[i] = bi;
+    }
+    grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+    // ---- r0 = b - A x0 ; r = r0 ; rho = r0.r0 ; bb = b.b
+    double red2[2] = {0.0, 0.0};
+    for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
+      double ax = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * d.x[col[k]];
+      const double bi = d.b[i];
+      const double ri = bi + (-1.0) * ax;
+      d.r0[i] = ri;
+      d.r[i] = ri;
+      red2[0] = red2[0] + ri * ri;
+      red2[1] = red2[1] + bi * bi;
+    }
+    bool ok = reduce_all<2>(d, red2, lds, slot);
+    double rho = red2[0], rr = red2[0];
+    const double bb = red2[1];
+    const double thr = d.tol_mode == NRT_TOL_RELATIVE ? d.tol * sqrt(bb) : d.tol;
+    const double limit = thr * thr;
+    int status = (rr <= limit) ? NRT_OK : NRT_ERR_NOT_CONVERGED;
+    int it = 0;
+    double alpha = 0, omega = 0, beta = 0;
+    bool first_it = true;
+    while (ok && status == NRT_ERR_NOT_CONVERGED && it < d.max_iter) {
+      double* __restrict__ pold = pb ? d.p1 : d.p0;
+      double* __restrict__ pnew = pb ? d.p0 : d.p1;
+      double* __restrict__ vold = pb ? d.v1 : d.v0;
+      double* __restrict__ vnew = pb ? d.v0 : d.v1;
+      const double nomega = -omega;
+      // ---- phase A: p = r + beta*(p - omega*v) ; v = A p ; apr = v.r0
+      double redA[1] = {0.0};
+      for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
+        const double pi = first_it ? d.r[i] : d.r[i] + beta * (pold[i] + nomega * vold[i]);
+        pnew[i] = pi;
+        double acc = 0;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+          const int c = col[k];
+          const double pc = first_it ? d.r[c] : d.r[c] + beta * (pold[c] + nomega * vold[c]);
+          acc = acc + val[k] * pc;
+        }
+        vnew[i] = acc;
+        redA[0] = redA[0] + acc * d.r0[i];
+      }
+      ok = reduce_all<1>(d, redA, lds, slot);
+      if (!ok) break;
+      alpha = rho / redA[0];
+      const double nalpha = -(rho / redA[0]);
+      // ---- phase B: s = r - alpha*v ; t = A s ; w1 = t.s ; w2 = t.t
+      double redB[2] = {0.0, 0.0};
+      for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
+        const double si = d.r[i] + nalpha * vnew[i];
+        d.s[i] = si;
+        double acc = 0;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+          const int c = col[k];
+          acc = acc + val[k] * (d.r[c] + nalpha * vnew[c]);
+        }
+        d.t[i] = acc;
+        redB[0] = redB[0] + acc * si;
+        redB[1] = redB[1] + acc * acc;
+      }
+      ok = reduce_all<2>(d, redB, lds, slot);
+      if (!ok) break;
+      omega = redB[0] / redB[1];
+      const double nw = -(redB[0] / redB[1]);
+      // ---- phase C: x += alpha*p + omega*s ; r = s - omega*t ; p1 = r.r0 ; rr = r.r
+      double redC[2] = {0.0, 0.0};
+      for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
+        const double si = d.s[i];
+        double xi = d.x[i] + alpha * pnew[i];
+        xi = xi + omega * si;
+        d.x[i] = xi;
+        const double ri = si + nw * d.t[i];
+        d.r[i] = ri;
+        redC[0] = redC[0] + ri * d.r0[i];
+        redC[1] = redC[1] + ri * ri;
+      }
+      ok = reduce_all<2>(d, redC, lds, slot);
+      if (!ok) break;
+      beta = (alpha / omega) * (redC[0] / rho);
+      rho = redC[0];
+      rr = redC[1];
+      pb ^= 1;
+      first_it = false;
+      it++;
+      if (!isfinite(rr) || !isfinite(beta)) {
+        status = NRT_ERR_NUMERIC;
+        break;
+      }
+      if ((it % d.check_every) == 0 && rr <= limit) status = NRT_OK;
+    }
+    if (!ok) status = NRT_ERR_HIP;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      d.st->iters_total += (unsigned long long)it;
+      d.st->steps += 1;
+      d.st->last_iters = it;
+      d.st->last_rr = rr;
+      d.st->last_bb = bb;
+      if (d.st->status == NRT_OK && status != NRT_OK) d.st->status = status;
+    }
+    if (!ok) return;
+    // ---- trace sample (neurite.cpp:241-249 cadence)
+    if (d.interval >= 0 && ++gl > d.interval) {
+      gl = 0;
+      if (d.trace && d.trace_elem >= r_begin && d.trace_elem < r_end && threadIdx.x == 0)
+        d.trace[ksamp] = d.x[d.trace_elem];
+      ksamp++;
+    }
+    // the next step's update phase reads only rows this block owns
+  }
+}
+
+// ---------------------------------------------------------------- host side
+static void device_exclusive_scan(const int32_t* in, int32_t* out, int n, DBuf<int32_t>& tmp,
+                                  hipStream_t s) {
+  const int nb = grid_for(n, kScanBlock);
+  tmp.alloc((size_t)nb);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, n, in, tmp.p);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, nb, tmp.p);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, n, in, tmp.p, out);
+  NRT_HIP(hipGetLastError());
+}
+
+static int choose_blocks(nrt_handle* hnd, int N) {
+  int dev = hnd->device, cus = 0, per_cu = 0;
+  NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_implicit_bicgstab_fp64,
+                                                       kImpBlock, 0));
+  int max_coresident = std::max(1, cus * std::max(1, per_cu));
+  int want = hnd->opt.num_blocks > 0 ? hnd->opt.num_blocks : grid_for(N, kImpBlock * 8);
+  want = std::min(want, cus);  // one block per CU at most unless asked explicitly
+  if (hnd->opt.num_blocks > 0) want = hnd->opt.num_blocks;
+  NRT_REQUIRE(want <= max_coresident, NRT_ERR_INVALID,
+              "num_blocks exceeds the co-resident limit " + std::to_string(max_coresident));
+  return std::max(1, want);
+}
+
+Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
+  hipStream_t s = hnd->stream;
+  const int N = p.N, H = p.H;
+  NRT_REQUIRE(N >= 4, NRT_ERR_INVALID, "the implicit solver needs at least 4 elements");
+  auto* e = new Implicit;
+  try {
+    e->N = N;
+    e->H = H;
+    e->dT = p.dT;
+    std::vector<int32_t> hh_of(N, -1);
+    for (int i = 0; i < H; i++) {
+      NRT_REQUIRE(hh_of[p.hh_elem[i]] < 0, NRT_ERR_INVALID, "duplicate HH element index");
+      hh_of[p.hh_elem[i]] = i;
+    }
+    for (int i = 1; i < N - 1; i++) {
+      NRT_REQUIRE(p.mother[i] >= 0 && p.mother[i] < N && p.dr[i] >= 0 && p.dr[i] < N &&
+                      p.dl[i] >= 0 && p.dl[i] < N,
+                  NRT_ERR_TOPOLOGY, "element " + std::to_string(i) + " has an out-of-range neighbour");
+    }
+    for (int i = 0; i < H; i++) {
+      const int iv = p.hh_elem[i];
+      NRT_REQUIRE(iv >= 1 && iv <= N - 2, NRT_ERR_TOPOLOGY,
+                  "HH element " + std::to_string(iv) + " lies on an identity row");
+    }
+    std::vector<double> cmdx(N), kdx(N, 0.0), rdx(N), Wc(N);
+    std::vector<uint32_t> code(N, 0);
+    for (int i = 0; i < N; i++) {
+      cmdx[i] = p.c_m[i] * p.dx[i];
+      rdx[i] = p.r_a[i] * p.dx[i];
+      Wc[i] = p.W[i];
+      if (hh_of[i] >= 0) code[i] = kCodeHH | (uint32_t)hh_of[i];
+      else kdx[i] = p.K[i] * p.dx[i];
+    }
+    e->cmdx.upload(cmdx, s);
+    e->kdx.upload(kdx, s);
+    e->rdx.upload(rdx, s);
+    e->dxv.upload(p.dx, s);
+    e->Wc.upload(Wc, s);
+    e->input.upload(p.input, s);
+    e->mother.upload(p.mother, s);
+    e->dr.upload(p.dr, s);
+    e->dl.upload(p.dl, s);
+    e->branching.upload(p.branching, s);
+    e->code.upload(code, s);
+    e->m.upload(p.m, s);
+    e->h.upload(p.h, s);
+    e->n.upload(p.n, s);
+    e->E_Na.upload(p.E_Na, s);
+    e->E_K.upload(p.E_K, s);
+    e->E_L.upload(p.E_L, s);
+    e->g_Na.upload(p.g_Na, s);
+    e->g_K.upload(p.g_K, s);
+    e->G_L.upload(p.G_L, s);
+    e->ls_Na.upload(p.ls_Na, s);
+    e->ls_K.upload(p.ls_K, s);
+    e->rest.upload(p.rest, s);
+    e->hh_type.alloc(H);
+    e->hh_mat.alloc(H);
+    e->hh_dl.alloc(H);
+    // device assembly
+    e->len.alloc(N);
+    e->rowptr.alloc(N + 1);
+    hipLaunchKernelGGL(k_row_len, dim3(grid_for(N, 256)), dim3(256), 0, s, N, e->dr.p, e->dl.p,
+                       e->len.p);
+    NRT_HIP(hipGetLastError());
+    device_exclusive_scan(e->len.p, e->rowptr.p, N, e->scan_tmp, s);
+    int32_t nnz = 0;
+    NRT_HIP(hipMemcpyAsync(&nnz, e->rowptr.p + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NRT_HIP(hipStreamSynchronize(s));
+    NRT_REQUIRE(nnz > 0 && nnz <= 4 * N, NRT_ERR_HIP, "device assembly produced a bad nnz");
+    e->nnz = nnz;
+    e->col.alloc(nnz);
+    e->val.alloc(nnz);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256)), dim3(256), 0, s, N, p.dT, e->rowptr.p,
+                       e->col.p, e->val.p, e->mother.p, e->dr.p, e->dl.p, e->branching.p,
+                       e->rdx.p, e->cmdx.p, e->dxv.p, e->Wc.p, e->code.p, e->hh_type.p,
+                       e->hh_mat.p, e->hh_dl.p);
+    NRT_HIP(hipGetLastError());
+    // Krylov vectors; x starts as potential[0] (solverIm-gpu.cpp:523-524)
+    e->x.upload(p.V0, s);
+    for (DBuf<double>* vbuf : {&e->b, &e->r0, &e->r, &e->p[0], &e->p[1], &e->v[0], &e->v[1],
+                               &e->s, &e->t}) {
+      vbuf->alloc(N);
+      NRT_HIP(hipMemsetAsync(vbuf->p, 0, sizeof(double) * N, s));
+    }
+    e->nblocks = choose_blocks(hnd, N);
+    e->part.alloc((size_t)e->nblocks * 4 * 2);
+    e->bar.alloc(4);
+    NRT_HIP(hipMemsetAsync(e->bar.p, 0, sizeof(unsigned) * 4, s));
+    e->st.alloc(1);
+    NRT_HIP(hipMemsetAsync(e->st.p, 0, sizeof(ImpStats), s));
+    NRT_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    delete e;
+    throw;
+  }
+  return e;
+}
+
+void implicit_destroy(Implicit* e) { delete e; }
+
+static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
+                             int32_t trace_elem, int64_t interval, double* trace_dev) {
+  Implicit* e = hnd->im;
+  const nrt_options& o = hnd->opt;
+  ImpDev d{};
+  d.N = e->N;
+  d.nblocks = e->nblocks;
+  d.rpb = grid_for(e->N, e->nblocks);
+  d.dT = e->dT;
+  d.dt_ms = e->dT * 1000;
+  d.rowptr = e->rowptr.p;
+  d.col = e->col.p;
+  d.val = e->val.p;
+  d.cmdx = e->cmdx.p;
+  d.kdx = e->kdx.p;
+  d.input = e->input.p;
+  d.rdx = e->rdx.p;
+  d.dxv = e->dxv.p;
+  d.code = e->code.p;
+  d.hh = ImpHH{e->m.p, e->h.p, e->n.p, e->E_Na.p, e->E_K.p, e->E_L.p, e->g_Na.p, e->g_K.p,
+               e->G_L.p, e->ls_Na.p, e->ls_K.p, e->rest.p, e->hh_type.p, e->hh_mat.p, e->hh_dl.p};
+  d.x = e->x.p;
+  d.b = e->b.p;
+  d.r0 = e->r0.p;
+  d.r = e->r.p;
+  d.p0 = e->p[0].p;
+  d.p1 = e->p[1].p;
+  d.v0 = e->v[0].p;
+  d.v1 = e->v[1].p;
+  d.s = e->s.p;
+  d.t = e->t.p;
+  d.part = e->part.p;
+  d.bar = e->bar.p;
+  d.err = e->bar.p + 2;
+  d.st = e->st.p;
+  d.tol = o.tol;
+  d.tol_mode = o.tol_mode;
+  d.x0_mode = o.x0_mode;
+  d.check_every = std::max(1, o.check_every);
+  d.max_iter = o.max_iter > 0 ? o.max_iter : e->N;
+  d.spin_limit = 1ull << 28;
+  if (const char* sl = getenv("NRT_SPIN_LIMIT")) d.spin_limit = strtoull(sl, nullptr, 10);
+  d.first = first;
+  d.num = num;
+  d.a0 = a0;
+  d.a1 = a1;
+  d.interval = interval;
+  d.trace_elem = trace_elem;
+  d.trace = trace_dev;
+  void* args[] = {&d};
+  NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_bicgstab_fp64, dim3(e->nblocks),
+                                     dim3(kImpBlock), args, 0, hnd->stream));
+  return NRT_OK;
+}
+
+static int collect_stats(nrt_handle* hnd) {
+  Implicit* e = hnd->im;
+  ImpStats st{};
+  unsigned err = 0;
+  NRT_HIP(hipMemcpyAsync(&st, e->st.p, sizeof(ImpStats), hipMemcpyDeviceToHost, hnd->stream));
+  NRT_HIP(hipMemcpyAsync(&err, e->bar.p + 2, sizeof(unsigned), hipMemcpyDeviceToHost, hnd->stream));
+  NRT_HIP(hipStreamSynchronize(hnd->stream));
+  hnd->stats.steps = (int64_t)st.steps;
+  hnd->stats.krylov_iters = (int64_t)st.iters_total;
+  hnd->stats.last_iters = st.last_iters;
+  hnd->stats.last_residual = sqrt(st.last_rr);
+  hnd->stats.last_rhs_norm = sqrt(st.last_bb);
+  hnd->stats.last_status = st.status;
+  if (err) throw Error(NRT_ERR_HIP, "grid barrier timed out inside the persistent Krylov kernel");
+  // a reported failure is returned once, then cleared
+  if (st.status != NRT_OK) {
+    const int zero = 0;
+    NRT_HIP(hipMemcpyAsync(&e->st.p->status, &zero, sizeof(int), hipMemcpyHostToDevice,
+                           hnd->stream));
+    NRT_HIP(hipStreamSynchronize(hnd->stream));
+  }
+  return st.status;
+}
+
+void implicit_update(nrt_handle* hnd, bool active) { hnd->input_active = active; }
+
+int implicit_calculate(nrt_handle* hnd) {
+  Implicit* e = hnd->im;
+  const int64_t st = e->step;
+  launch_persistent(hnd, st, 1, hnd->input_active ? st : st + 1, st, 0, -1, nullptr);
+  e->step++;
+  return collect_stats(hnd);
+}
+
+void implicit_snapshot(nrt_handle* hnd, double* host_out) {
+  Implicit* e = hnd->im;
+  NRT_HIP(hipMemcpyAsync(host_out, e->x.p, sizeof(double) * e->N, hipMemcpyDeviceToHost,
+                         hnd->stream));
+  NRT_HIP(hipStreamSynchronize(hnd->stream));
+}
+
+int implicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
+                 int32_t trace_elem, int64_t interval, double* trace_out, int64_t* ns) {
+  Implicit* e = hnd->im;
+  const bool tracing = interval >= 0;
+  const int64_t max_samples = tracing ? num / (interval + 1) : 0;
+  if (tracing && trace_out && max_samples > e->trace_cap) {
+    e->trace.alloc((size_t)max_samples);
+    e->trace_cap = max_samples;
+  }
+  hipEvent_t t0, t1;
+  NRT_HIP(hipEventCreate(&t0));
+  NRT_HIP(hipEventCreate(&t1));
+  NRT_HIP(hipEventRecord(t0, hnd->stream));
+  launch_persistent(hnd, first, num, a0, a1, trace_elem, interval,
+                    (tracing && trace_out) ? e->trace.p : nullptr);
+  NRT_HIP(hipEventRecord(t1, hnd->stream));
+  const int64_t k = max_samples;
+  if (trace_out && k > 0)
+    NRT_HIP(hipMemcpyAsync(trace_out, e->trace.p, sizeof(double) * k, hipMemcpyDeviceToHost,
+                           hnd->stream));
+  NRT_HIP(hipStreamSynchronize(hnd->stream));
+  float ms = 0;
+  NRT_HIP(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  e->step = first + num;
+  const int status = collect_stats(hnd);
+  hnd->stats.kernel_ms = ms;
+  hnd->stats.kernel_launches = 1;
+  if (ns) *ns = k;
+  return status;
+}
+
+}  // namespace nrt

@@ -1,0 +1,223 @@
+/*
+ * neurite_hip.h -- C ABI of the MI355X-native Neurite cable-equation solvers.
+ *
+ * Plain C types only (pointers, sizes, status codes); no HIP or torch types cross
+ * this boundary. Every entry point returns an int status (NRT_OK == 0) and, on
+ * failure, leaves a message retrievable with nrt_last_error(handle) (or
+ * nrt_last_error(NULL) for errors raised before a handle exists).
+ *
+ * Reference interfaces replaced (Neurite, ModelDBRepository/168861):
+ *   inc/solver.h:12-24          Solver::update / calculate / snapshot / ~Solver
+ *   inc/solverEx-gpu.h:79-85    SolverExGPU constructor + methods
+ *   inc/solverIm-gpu.h:111-118  SolverImGPU constructor + methods
+ *   inc/cudaException.h:331-357 cudaException (status + message here)
+ *   src/solverIm-gpu.cpp:440-473,766-822  cuBLAS/cuSPARSE calls (csrmv, dot, axpy)
+ *   src/init.cpp:138-298, inc/discretization.h:50-238  setup chain (strain, HH init)
+ *
+ * Units are the reference's SI units: volts, seconds, metres, ohms, farads,
+ * siemens, amperes.
+ */
+#ifndef NEURITE_HIP_H
+#define NEURITE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define NRT_OK                 0
+#define NRT_ERR_INVALID        1  /* bad argument / inconsistent arrays          */
+#define NRT_ERR_HIP            2  /* HIP runtime error (cudaException analogue)  */
+#define NRT_ERR_NOMEM          3  /* device allocation failed                    */
+#define NRT_ERR_TOPOLOGY       4  /* topology the solver cannot represent        */
+#define NRT_ERR_NOT_CONVERGED  5  /* Krylov hit max_iter (reference: exit(0))     */
+#define NRT_ERR_NUMERIC        6  /* NaN / Inf in a Krylov scalar or residual    */
+#define NRT_ERR_UNSUPPORTED    7  /* option combination not built                */
+#define NRT_ERR_ADMISSION      8  /* not enough free HBM to admit a request      */
+#define NRT_ERR_NO_DEVICE      9  /* no usable GPU                               */
+
+/* ------------------------------------------------------------- options */
+#define NRT_EXPLICIT 0   /* FTCS + forward-Euler HH gates (solverEx-*.cpp) */
+#define NRT_IMPLICIT 1   /* backward Euler + Krylov (solverIm-*.cpp)       */
+
+#define NRT_FP64          0  /* plain form: every step in FP64                      */
+#define NRT_FP32_REFINE   1  /* FP32 gates/SpMV/Krylov + FP64 residual & update     */
+
+#define NRT_BICGSTAB 0   /* solverIm-gpu.cpp:725-843 */
+#define NRT_GMRES    1   /* solverIm-cpu.cpp:192-272 (restarted GMRES)       */
+
+#define NRT_TOL_ABSOLUTE 0  /* ||b - A x||_2 <= tol      (reference: 1e-14)  */
+#define NRT_TOL_RELATIVE 1  /* ||b - A x||_2 <= tol*||b||_2                  */
+
+#define NRT_X0_PREVIOUS 0   /* Krylov starts from V of the previous step      */
+#define NRT_X0_RHS      1   /* Krylov starts from b (solverIm-gpu.cpp:169,769) */
+
+/* Per-element material description (discretization.cpp:19-97 defaults). */
+typedef struct nrt_material {
+  int32_t kind;          /* 0 = passive cable (Cable), 1 = Hodgkin-Huxley      */
+  int32_t layers_my;     /* myelin layers (Cable, 45 by default)                */
+  double rho_a;          /* axial resistivity  [ohm m]      (1.87)             */
+  double rho_m;          /* membrane resistivity [ohm m]   (2.5e9)             */
+  double per_m;          /* membrane permittivity [F/m]    (4e-11)             */
+  double thickness;      /* membrane thickness [m]         (4e-9)              */
+  double rest_pot;       /* resting potential [V]          (-0.0655)           */
+  double thickness_my;   /* myelin layer thickness [m]     (18e-9)             */
+  double per_my;         /* myelin permittivity [F/m]      (1.08e-10)          */
+  double ro_my;          /* myelin resistivity [ohm m]     (4.44e6)            */
+  double sigma_Na;       /* HH sodium conductivity         (4.8e-6)            */
+  double sigma_K;        /* HH potassium conductivity      (1.44e-6)           */
+  double sigma_L;        /* HH leak conductivity           (1.2e-8)            */
+  double dE_Na;          /* E_Na0 = rest_pot + dE_Na       (0.115)             */
+  double dE_K;           /* E_K0  = rest_pot + dE_K        (-0.012)            */
+  double n_damage;       /* damage exponent                (2)                 */
+} nrt_material;
+
+/*
+ * Raw geometry of one neurite: what a space.cpp builder produces before the
+ * setup chain runs (unstrained sizes, per-element strain, topology).
+ * Element 0 is the fictitious first element; terminals[0] is conventionally 0.
+ */
+typedef struct nrt_geometry {
+  int32_t num_elem;
+  const int32_t* mother;     /* [N] (-1 for element 0)                          */
+  const int32_t* dr;         /* [N] right daughter (== self for terminals)      */
+  const int32_t* dl;         /* [N] left daughter (0 when none)                 */
+  const uint8_t* branching;  /* [N] 1 at branching points                       */
+  const double* diameter;    /* [N] unstrained diameter [m]                     */
+  const double* dx0;         /* [N] unstrained element length [m]               */
+  const double* epsilon;     /* [N] axial strain                                */
+  const double* input;       /* [N] stimulus current when active [A]            */
+  const int32_t* material;   /* [N] index into materials                        */
+  int32_t num_materials;
+  const nrt_material* materials;
+  int32_t myelinate;         /* cable_init_constants flag (params.HH_model)     */
+  int32_t num_term;
+  const int32_t* terminals;  /* [num_term]                                      */
+  double E_MAX;              /* damage threshold for reversal potentials        */
+} nrt_geometry;
+
+/*
+ * Derived per-element state: exactly the arrays the reference GPU solver
+ * constructors copy out of the discretization objects
+ * (solverEx-gpu.cpp:327-399, solverIm-gpu.cpp:527-595) plus the topology the
+ * CPU solver reads through element[] (solverEx-cpu.cpp:92-116).
+ * Caller-owned; read only during nrt_create.
+ */
+typedef struct nrt_problem {
+  int32_t num_elem;
+  double dT;                 /* time step [s]                                   */
+  const double* potential0;  /* [N] potential[0] : V at the current step       */
+  const double* potential1;  /* [N] potential[1] : the second buffer            */
+  const double* dx;          /* [N] strained element length                     */
+  const double* c_m;         /* [N] membrane capacitance per length             */
+  const double* r_a;         /* [N] axial resistance per length                 */
+  const double* W;           /* [N] W (CT: -1/r_m; HH: overwritten each step)   */
+  const double* K;           /* [N] K (CT: Vr/r_m; HH: overwritten each step)   */
+  const double* input;       /* [N] input_on values                             */
+  const int32_t* mother;     /* [N]                                             */
+  const int32_t* dr;         /* [N]                                             */
+  const int32_t* dl;         /* [N]                                             */
+  const uint8_t* branching;  /* [N]                                             */
+  int32_t num_term;
+  const int32_t* terminals;  /* [num_term]                                      */
+  int32_t num_hh;
+  const int32_t* hh_elem;    /* [num_hh] element index of each HH element       */
+  const double* hh_m;        /* [num_hh] gates                                  */
+  const double* hh_h;
+  const double* hh_n;
+  const double* E_Na;        /* [num_hh] reversal potentials                    */
+  const double* E_K;
+  const double* E_L;
+  const double* g_Na;        /* [num_hh] maximal conductances (per length)      */
+  const double* g_K;
+  const double* G_L;
+  const double* ls_Na;       /* [num_hh] left shifts                            */
+  const double* ls_K;
+  const double* rest_pot;    /* [num_hh] resting potential                      */
+} nrt_problem;
+
+/* Device-computed setup (nrt_setup) output buffers, caller-owned host arrays. */
+typedef struct nrt_setup_out {
+  double* dx; double* c_m; double* r_a; double* r_m; double* W; double* K;
+  double* crit_dT;           /* [N] critical time step per element              */
+  double* rest;              /* [N] resting potential per element               */
+  int32_t* hh_elem;          /* [num_hh] (num_hh = number of HH-kind elements)  */
+  double* hh_m; double* hh_h; double* hh_n;
+  double* E_Na; double* E_K; double* E_L;
+  double* g_Na; double* g_K; double* G_L;
+  double* ls_Na; double* ls_K; double* hh_rest;
+  double dT_crit;            /* min crit_dT over elements 1..N-2                */
+  int32_t num_hh;
+} nrt_setup_out;
+
+typedef struct nrt_options {
+  int32_t kind;         /* NRT_EXPLICIT / NRT_IMPLICIT                          */
+  int32_t precision;    /* NRT_FP64 / NRT_FP32_REFINE                           */
+  int32_t krylov;       /* NRT_BICGSTAB / NRT_GMRES                             */
+  int32_t restart;      /* GMRES restart length (reference CPU: 150)           */
+  int32_t tol_mode;     /* NRT_TOL_ABSOLUTE / NRT_TOL_RELATIVE                  */
+  int32_t x0_mode;      /* NRT_X0_PREVIOUS / NRT_X0_RHS                         */
+  int32_t check_every;  /* BiCGSTAB convergence test period (reference: 30)    */
+  int32_t max_iter;     /* 0 -> N (reference MAX_ITER)                          */
+  int32_t device;       /* -1 -> $idGPU or 0                                    */
+  int32_t num_blocks;   /* 0 -> autotuned grid size for persistent kernels     */
+  double tol;           /* Krylov tolerance (see tol_mode)                      */
+  double refine_tol;    /* FP32_REFINE: outer FP64 residual tolerance (relative)*/
+  int32_t refine_max;   /* FP32_REFINE: maximum refinement sweeps               */
+  int32_t reserved[7];
+} nrt_options;
+
+typedef struct nrt_stats {
+  int64_t steps;            /* time steps taken                                 */
+  int64_t krylov_iters;     /* Krylov iterations summed over steps              */
+  int64_t refine_sweeps;    /* FP64 refinement sweeps summed over steps         */
+  int32_t last_iters;       /* Krylov iterations of the last step               */
+  int32_t last_status;      /* status of the last step                          */
+  double last_residual;     /* ||b - A x||_2 at exit of the last step           */
+  double last_rhs_norm;     /* ||b||_2 of the last step                         */
+  double kernel_ms;         /* device time of the dominant kernel, last run     */
+  int64_t kernel_launches;  /* launches of the dominant kernel, last run        */
+} nrt_stats;
+
+typedef struct nrt_handle nrt_handle;
+
+/* Library / device */
+int nrt_version(void);
+int nrt_device_count(int* count);
+const char* nrt_last_error(const nrt_handle* h);
+const char* nrt_status_string(int status);
+void nrt_default_options(nrt_options* opt);
+
+/* Setup chain on the device: geometry -> derived arrays (init.cpp:138-298). */
+int nrt_setup(const nrt_geometry* g, int device, nrt_setup_out* out);
+
+/* Solver lifecycle: the Solver interface of inc/solver.h. */
+int nrt_create(const nrt_problem* p, const nrt_options* opt, nrt_handle** out);
+int nrt_update(nrt_handle* h, int input_active);      /* Solver::update    */
+int nrt_calculate(nrt_handle* h);                     /* Solver::calculate */
+int nrt_snapshot(nrt_handle* h, double* host_out);    /* Solver::snapshot  */
+int nrt_destroy(nrt_handle* h);                       /* Solver::~Solver   */
+
+/*
+ * Device-resident time loop (neurite.cpp:224-255 without host round trips):
+ * runs steps i = first_step .. first_step+num_steps-1; the stimulus is on for
+ * active_first <= i <= active_last; every trace_every steps (counting as the
+ * reference's GL_cont) V[trace_elem] is recorded into trace_out.
+ * trace_out may be NULL when trace_every <= 0. Returns the number of samples in
+ * *num_samples.
+ */
+int nrt_run(nrt_handle* h, int64_t first_step, int64_t num_steps,
+            int64_t active_first, int64_t active_last,
+            int32_t trace_elem, int64_t trace_every,
+            double* trace_out, int64_t* num_samples);
+int nrt_get_stats(nrt_handle* h, nrt_stats* st);
+/* Synchronise the handle's stream (for timing by the caller). */
+int nrt_sync(nrt_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEURITE_HIP_H */

@@ -1,0 +1,310 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY.
+//
+// Driver linked against the reference's own CPU sources (compiled where they lie
+// under the reference tree by oracle/Makefile; nothing of them is copied here).
+// It replaces neurite.cpp's GLUT-bound main: it runs the same setup sequence
+// (neurite.cpp:119-184), the reference SolverExCPU and the same time loop
+// (neurite.cpp:224-255), and dumps everything the build is checked against as
+// JSON with round-trip (%.17g) doubles:
+//   * the raw geometry right after the builder (before strain is applied),
+//   * the derived per-element arrays the GPU constructors would copy
+//     (solverEx-gpu.cpp:327-399),
+//   * dT, numStepsT, the measurement point, the .dat trace and full V snapshots.
+//
+// Usage: simulation=<case> working_directory=<dir> neurite_ref <reference CLI>
+//   env REF_DUMP=<json path>, REF_MAX_STEPS=<cap on time steps>,
+//       REF_CHECKPOINTS=<comma list of steps at which V is dumped>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "configuration.h"
+#include "neurite.h"
+#include "discretization.h"
+#include "init.h"
+#include "input.h"
+#include "coupling.h"
+#include "space.h"
+#include "mechanical_model.h"
+#include "solver.h"
+#include "solverEx-cpu.h"
+
+// Globals the reference sources expect from neurite.cpp.
+int numStepsX = 0, numStepsT = 0;
+TYPE xspan = 0., tspan = 0., dist_mes = 0;
+extern const TYPE pi;
+const TYPE pi = 3.14159265;
+int* terminals;
+int num_terminals;
+TYPE **potential, *potentialR;
+TYPE *open_GL_graphE, **open_GL_graphR, ***open_GL_graph;
+char *log_file, *output;
+FILE *standard, *study;
+int open_GL_steps = 1000;
+char* simulation;
+
+static FILE* J;
+static bool first_field;
+
+static void key(const char* k) {
+  fprintf(J, "%s\n\"%s\": ", first_field ? "" : ",", k);
+  first_field = false;
+}
+static void dnum(const char* k, double v) {
+  key(k);
+  fprintf(J, "%.17g", v);
+}
+static void inum(const char* k, long v) {
+  key(k);
+  fprintf(J, "%ld", v);
+}
+static void darr(const char* k, const std::vector<double>& v) {
+  key(k);
+  fputc('[', J);
+  for (size_t i = 0; i < v.size(); i++) fprintf(J, "%s%.17g", i ? "," : "", v[i]);
+  fputc(']', J);
+}
+static void iarr(const char* k, const std::vector<long>& v) {
+  key(k);
+  fputc('[', J);
+  for (size_t i = 0; i < v.size(); i++) fprintf(J, "%s%ld", i ? "," : "", v[i]);
+  fputc(']', J);
+}
+// Element-field accessors (gnu++98: no lambdas).
+typedef double (*DGet)(discretization*);
+typedef long (*IGet)(discretization*);
+typedef double (*HGet)(Hodgkin_Huxley*);
+static std::vector<double> ecol(discretization** e, int n, DGet f) {
+  std::vector<double> v(n);
+  for (int i = 0; i < n; i++) v[i] = f(e[i]);
+  return v;
+}
+static std::vector<long> icol(discretization** e, int n, IGet f) {
+  std::vector<long> v(n);
+  for (int i = 0; i < n; i++) v[i] = f(e[i]);
+  return v;
+}
+static std::vector<double> hcol(Hodgkin_Huxley** e, int n, HGet f) {
+  std::vector<double> v(n);
+  for (int i = 0; i < n; i++) v[i] = f(e[i]);
+  return v;
+}
+static long g_mother(discretization* e) { return e->get_mother(); }
+static long g_dr(discretization* e) { return e->get_DR(); }
+static long g_dl(discretization* e) { return e->get_DL(); }
+static long g_br(discretization* e) { return e->get_branching() ? 1 : 0; }
+static long g_hh(discretization* e) { return e->get_kind_HH() ? 1 : 0; }
+static double g_diam(discretization* e) { return e->get_diameter(); }
+static double g_dx(discretization* e) { return e->get_dX(); }
+static double g_eps(discretization* e) { return e->get_epsilon(); }
+static double g_in(discretization* e) { return e->get_input_current(); }
+static double g_cm(discretization* e) { return e->get_c_m(); }
+static double g_ra(discretization* e) { return e->get_r_a(); }
+static double g_rm(discretization* e) { return e->get_r_m(); }
+static double g_W(discretization* e) { return e->get_W(); }
+static double g_K(discretization* e) { return e->get_K(); }
+static double g_rest(discretization* e) { return e->get_rest_pot(); }
+static double g_crit(discretization* e) { return e->get_critical_dT(); }
+static double h_m(Hodgkin_Huxley* e) { return e->get_m(); }
+static double h_h(Hodgkin_Huxley* e) { return e->get_h(); }
+static double h_n(Hodgkin_Huxley* e) { return e->get_n(); }
+static double h_ENa(Hodgkin_Huxley* e) { return e->get_E_Na(); }
+static double h_EK(Hodgkin_Huxley* e) { return e->get_E_K(); }
+static double h_EL(Hodgkin_Huxley* e) { return e->get_E_L(); }
+static double h_gNa(Hodgkin_Huxley* e) { return e->get_g_Na(); }
+static double h_gK(Hodgkin_Huxley* e) { return e->get_g_K(); }
+static double h_GL(Hodgkin_Huxley* e) { return e->get_G_L(); }
+static double h_lsNa(Hodgkin_Huxley* e) { return e->get_Left_Shift_Na(); }
+static double h_lsK(Hodgkin_Huxley* e) { return e->get_Left_Shift_K(); }
+static double h_rest(Hodgkin_Huxley* e) { return e->get_rest_pot(); }
+
+int main(int argc, char** argv) {
+  TYPE dT = 0., time_run = 0., temp = 0.;
+  int i = 0;
+  char* args;
+  input in;
+  params_t params = {false};
+  int GL_cont = 0, interval = 0;
+  discretization** element;
+  Hodgkin_Huxley** HH_elements = NULL;
+  Cable** CT_elements = NULL;
+  int num_HH_elements = 0, num_CT_elements = 0;
+  TYPE E_MAX = 0;
+  int dist_mes_points = 0;
+
+  const char* dump = getenv("REF_DUMP");
+  long max_steps = getenv("REF_MAX_STEPS") ? atol(getenv("REF_MAX_STEPS")) : -1;
+  std::vector<long> checkpoints;
+  if (getenv("REF_CHECKPOINTS")) {
+    std::string s = getenv("REF_CHECKPOINTS");
+    size_t p = 0;
+    while (p < s.size()) {
+      size_t q = s.find(',', p);
+      if (q == std::string::npos) q = s.size();
+      checkpoints.push_back(atol(s.substr(p, q - p).c_str()));
+      p = q + 1;
+    }
+  }
+  simulation = getenv("simulation");
+  if (!simulation || !dump) {
+    fprintf(stderr, "need env simulation and REF_DUMP\n");
+    return 2;
+  }
+  J = fopen(dump, "w");
+  first_field = true;
+  fputc('{', J);
+
+  // neurite.cpp:126-146 defaults and CLI parsing.
+  in.IRE = 0;
+  in.NRE = 0;
+  in.epsilon = 0.;
+  in.nu = 0.00261813;
+  in.dt_factor = 1.;
+  in.init_time = 0.;
+  in.E_MAX = 1;
+  args = new char[argc]();
+  inputArgs(argc, argv, args, in);
+  E_MAX = in.E_MAX;
+  study = fopen("/dev/null", "w");
+  standard = fopen("/dev/null", "w");
+  args[argc - 1] = '\0';
+  flags_init(args, &params);
+  test_logical_options(&params, in);
+
+  inputInit(in, &params, element, HH_elements, num_HH_elements, CT_elements, num_CT_elements);
+  int N = numStepsX;
+  // Raw geometry, straight from the builder.
+  inum("N", N);
+  iarr("mother", icol(element, N, g_mother));
+  iarr("dr", icol(element, N, g_dr));
+  iarr("dl", icol(element, N, g_dl));
+  iarr("branching", icol(element, N, g_br));
+  iarr("is_hh", icol(element, N, g_hh));
+  darr("diameter0", ecol(element, N, g_diam));
+  darr("dx0", ecol(element, N, g_dx));
+  darr("epsilon", ecol(element, N, g_eps));
+  darr("input", ecol(element, N, g_in));
+  iarr("terminals", std::vector<long>(terminals, terminals + num_terminals));
+  inum("myelinate", params.HH_model ? 1 : 0);
+  dnum("E_MAX", E_MAX);
+  dnum("init_time", in.init_time);
+  dnum("end_input", in.end_input);
+  dnum("in_epsilon", in.epsilon);
+
+  mechanical_model_initialization(in, element, dist_mes);
+  dist_mes_points = measurement_point(element, dist_mes);
+  neuriteInit(&params);
+  electrical_properties(HH_elements, num_HH_elements, E_MAX);
+  setting_rest_pot(potential, element);
+  hhIconds(potential[0], HH_elements, num_HH_elements, E_MAX);
+  cable_init_constants(CT_elements, num_CT_elements, params.HH_model);
+  critical_time_step_class(HH_elements, num_HH_elements, CT_elements, num_CT_elements);
+  dT = critical_time_step_selection(element);
+  dnum("dT_crit", dT);
+  dT = dT * in.dt_factor;
+  temp = tspan / dT;
+  numStepsT = int(temp);
+  interval = int(numStepsT / open_GL_steps);
+  dnum("dT", dT);
+  dnum("dt_factor", in.dt_factor);
+  dnum("tspan", tspan);
+  inum("numStepsT", numStepsT);
+  inum("interval", interval);
+  inum("dist_mes_points", dist_mes_points);
+  dnum("dist_mes", dist_mes);
+
+  // Derived arrays (what SolverExGPU copies: solverEx-gpu.cpp:334-399).
+  darr("dx", ecol(element, N, g_dx));
+  darr("c_m", ecol(element, N, g_cm));
+  darr("r_a", ecol(element, N, g_ra));
+  darr("r_m", ecol(element, N, g_rm));
+  darr("W", ecol(element, N, g_W));
+  darr("K", ecol(element, N, g_K));
+  darr("rest", ecol(element, N, g_rest));
+  darr("crit_dT", ecol(element, N, g_crit));
+  darr("diameter", ecol(element, N, g_diam));
+  darr("potential0", std::vector<double>(potential[0], potential[0] + N));
+  darr("potential1", std::vector<double>(potential[1], potential[1] + N));
+  int H = num_HH_elements;
+  std::vector<long> hh_idx(H);
+  for (int k = 0; k < H; k++) hh_idx[k] = HH_elements[k]->get_element();
+  iarr("hh_elem", hh_idx);
+  darr("hh_m", hcol(HH_elements, H, h_m));
+  darr("hh_h", hcol(HH_elements, H, h_h));
+  darr("hh_n", hcol(HH_elements, H, h_n));
+  darr("E_Na", hcol(HH_elements, H, h_ENa));
+  darr("E_K", hcol(HH_elements, H, h_EK));
+  darr("E_L", hcol(HH_elements, H, h_EL));
+  darr("g_Na", hcol(HH_elements, H, h_gNa));
+  darr("g_K", hcol(HH_elements, H, h_gK));
+  darr("G_L", hcol(HH_elements, H, h_GL));
+  darr("ls_Na", hcol(HH_elements, H, h_lsNa));
+  darr("ls_K", hcol(HH_elements, H, h_lsK));
+  darr("hh_rest", hcol(HH_elements, H, h_rest));
+
+  Solver* solver = NULL;
+  if (params.explicit_mode) {
+    solver = new SolverExCPU(dT, potential, num_terminals, terminals, in.input_current,
+                             num_HH_elements, HH_elements, numStepsX, element);
+  } else {
+    fprintf(stderr, "only the explicit CPU solver is buildable (MKL headers absent)\n");
+    return 3;
+  }
+
+  long last = numStepsT - 1;  // loop runs i = 1 .. numStepsT-2
+  if (max_steps >= 0 && 1 + max_steps < last) last = 1 + max_steps;
+  std::vector<double> trace_t, trace_v, snap(N);
+  std::vector<long> trace_i;
+  std::vector<std::vector<double> > cps;
+  std::vector<long> cp_steps;
+  std::vector<int> act;
+  for (i = 1; i < last; i++) {
+    time_run = i * dT;
+    bool active = (time_run >= in.init_time) && (time_run <= in.end_input);
+    act.push_back(active ? 1 : 0);
+    solver->update(active);
+    solver->calculate();
+    if (++GL_cont > interval) {
+      GL_cont = 0;
+      solver->snapshot(potential[1]);
+      trace_i.push_back(i);
+      trace_t.push_back(time_run);
+      trace_v.push_back(potential[1][dist_mes_points]);
+    }
+    for (size_t c = 0; c < checkpoints.size(); c++) {
+      if (checkpoints[c] == i) {
+        solver->snapshot(snap.data());
+        cps.push_back(snap);
+        cp_steps.push_back(i);
+      }
+    }
+  }
+  solver->snapshot(snap.data());
+  inum("steps_run", last - 1);
+  long a0 = -1, a1 = -2;
+  for (size_t k = 0; k < act.size(); k++)
+    if (act[k]) {
+      if (a0 < 0) a0 = (long)k + 1;
+      a1 = (long)k + 1;
+    }
+  inum("active_first", a0);
+  inum("active_last", a1);
+  iarr("trace_step", trace_i);
+  darr("trace_t", trace_t);
+  darr("trace_v", trace_v);
+  iarr("checkpoint_steps", cp_steps);
+  key("checkpoints");
+  fputc('[', J);
+  for (size_t c = 0; c < cps.size(); c++) {
+    fputs(c ? ",[" : "[", J);
+    for (int k = 0; k < N; k++) fprintf(J, "%s%.17g", k ? "," : "", cps[c][k]);
+    fputc(']', J);
+  }
+  fputc(']', J);
+  darr("final_v", snap);
+  fputs("\n}\n", J);
+  fclose(J);
+  delete solver;
+  return 0;
+}

@@ -1,0 +1,60 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import neurite_hip as nh
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(REPO, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(nrt_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ["nrt_create", "nrt_update", "nrt_calculate", "nrt_snapshot", "nrt_destroy",
+              "nrt_run", "nrt_setup", "nrt_last_error"]:
+        assert n in names
+
+
+def test_library_exports_all_declared_symbols():
+    L = nh.lib()
+    missing = [n for n in sorted(declared_functions()) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_status_strings_and_options():
+    L = nh.lib()
+    L.nrt_status_string.restype = ctypes.c_char_p
+    assert L.nrt_status_string(0) == b"ok"
+    assert b"converge" in L.nrt_status_string(5)
+    o = nh.default_options()
+    assert o.kind == nh.EXPLICIT and o.precision == nh.FP64 and o.check_every == 1
+    assert L.nrt_version() >= 1
+
+
+def test_invalid_problem_is_rejected_before_any_device_work():
+    import numpy as np
+    p = nh.Problem(1e-6, [-1, 0, 1], [1, 2, 2], [0, 0, 0], [0, 0, 0], [0, 2], [],
+                   potential0=np.zeros(3), dx=np.ones(3), c_m=np.ones(3), r_a=np.ones(3),
+                   hh_m=[], hh_h=[], hh_n=[], E_Na=[], E_K=[], E_L=[], g_Na=[], g_K=[],
+                   G_L=[], ls_Na=[], ls_K=[], rest_pot=[])
+    p.dT = -1.0
+    try:
+        nh.Solver(p)
+        raise AssertionError("negative dT accepted")
+    except nh.NrtError as e:
+        assert e.status == 1
+        assert "dT" in str(e)

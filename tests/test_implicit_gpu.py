@@ -1,0 +1,91 @@
+"""Implicit FP64 plain form on the GPU (through the C ABI).
+
+The FP64 truth for an implicit step is the exact solve of the assembled
+backward-Euler system (oracle tree elimination, itself pinned against dense
+numpy solves). The reference's own Krylov tolerance (absolute 1e-14 A) is too
+loose for 1e-6 mV on these matrices, so parity runs the GPU BiCGSTAB at a tight
+relative tolerance. Bar: max |dV| <= 1e-9 V over non-ghost elements.
+"""
+import numpy as np
+import pytest
+
+import neurite_hip as nh
+import workloads as wl
+from common import TOL_V, ghost_mask, load_golden, max_abs_diff, problem_from_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _oracle_direct(orc, prob, steps, a0, a1):
+    ref = orc.Implicit(prob, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    ref.run(1, steps, a0, a1)
+    return ref.snapshot()
+
+
+@pytest.mark.parametrize("name,dtf", [("ex3_hh_y", 100), ("ex4_ct_cable", 100),
+                                      ("ex5_ct_y", 1), ("ex6_ct_tree", 10),
+                                      ("ex7_hh_axon_strain", 100), ("ex8_neuron", 100)])
+def test_golden_problems_vs_exact(name, dtf, nrt, orc):
+    g = load_golden(name)
+    dT = g["dT_crit"] * dtf
+    p = problem_from_golden(g, dT=dT)
+    steps = 200
+    a0, a1 = nrt.active_range(dT, 10 ** 9, g["init_time"], g["end_input"])
+    _say(f"implicit golden {name} N={p.N} dT={dT:.3e}")
+    with nrt.Solver(p, kind=nh.IMPLICIT, tol=1e-14, tol_mode=nh.TOL_RELATIVE) as s:
+        s.run(1, steps, a0, a1)
+        v = s.snapshot()
+        st = s.stats()
+    ref = _oracle_direct(orc, p, steps, a0, a1)
+    m = ghost_mask(p)
+    assert max_abs_diff(v, ref, m) <= TOL_V, st
+    assert st["krylov_iters"] > 0
+
+
+def test_solver_interface_steps(nrt, orc):
+    """update/calculate/snapshot per step == oracle BiCGSTAB restatement."""
+    g = load_golden("ex6_ct_tree")
+    dT = g["dT_crit"] * 10
+    p = problem_from_golden(g, dT=dT)
+    _say("implicit step-by-step ex6")
+    ref = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_BICGSTAB, tol=1e-14, tol_mode=1,
+                       x0_mode=0, check_every=1)
+    m = ghost_mask(p)
+    with nrt.Solver(p, kind=nh.IMPLICIT, tol=1e-14, tol_mode=nh.TOL_RELATIVE) as s:
+        for i in range(1, 51):
+            act = i <= 30
+            s.update(act)
+            s.calculate()
+            ref.update(act)
+            assert ref.calculate() == 0
+            assert max_abs_diff(s.snapshot(), ref.snapshot(), m) <= TOL_V, f"step {i}"
+
+
+def test_config2_short(nrt, orc):
+    """Config 2 (myelinated axon, N = 20,202, dt = 10 us): 20 steps vs exact."""
+    geom, run = wl.myelinated_axon()
+    prob, _ = nrt.setup(geom, dT=run.dT)
+    a0, a1 = nrt.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
+    steps = 20
+    _say(f"implicit config2 N={prob.N} steps={steps}")
+    with nrt.Solver(prob, kind=nh.IMPLICIT, tol=1e-13, tol_mode=nh.TOL_RELATIVE) as s:
+        tr = s.run(1, steps, a0, a1, trace_elem=1, interval=0)
+        v = s.snapshot()
+        st = s.stats()
+    _say(f"config2 iterations {st['krylov_iters']} over {steps} steps")
+    ref = orc.Implicit(prob, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    rtr, _ = ref.run(1, steps, a0, a1, trace_elem=1, interval=0)
+    m = ghost_mask(prob)
+    assert max_abs_diff(v, ref.snapshot(), m) <= TOL_V
+    assert max_abs_diff(tr, rtr) <= TOL_V
+
+
+def test_small_system_rejected(nrt):
+    g = load_golden("ex1_only_hh")
+    p = problem_from_golden(g)
+    with pytest.raises(nh.NrtError):
+        nh.Solver(p, kind=nh.IMPLICIT)
