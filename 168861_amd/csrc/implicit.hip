@@ -60,6 +60,11 @@ struct ImpStats {
   int status;
   double last_rr;
   double last_bb;
+  // first failure: step, iteration and the scalars of that iteration
+  long long fail_step;
+  int fail_iter;
+  int restarts;
+  double fail_rho, fail_apr, fail_w1, fail_w2, fail_rr;
 };
 
 struct ImpDev {
@@ -279,6 +284,8 @@ __global__ void k_fill(int N, double dT, const int32_t* __restrict__ rowptr,
 
 // --------------------------------------------------------- persistent kernel
 constexpr int kImpBlock = 256;
+// breakdown restarts allowed per time step before the step reports NOT_CONVERGED
+constexpr int kMaxRestarts = 64;
 
 // HH gating (solverIm-gpu.cpp:61-98) at potential v; returns W and K.
 __device__ __forceinline__ void imp_gate(const ImpHH& hd, int i, double v, double dt_ms,
@@ -310,6 +317,9 @@ __device__ __forceinline__ void imp_gate(const ImpHH& hd, int i, double v, doubl
 
 // Update_Mat (solverIm-gpu.cpp:103-156) for the HH row of element iv.
 __device__ __forceinline__ void imp_update_row(const ImpDev& d, int hh, int iv, double W) {
+  const int type = d.hh.type[hh];
+  // identity rows (ghosts, sealed ends) keep their 1 and read no neighbour
+  if (type == RT_OTHER || type == RT_TERM) return;
   const int im = d.hh.mat[hh];
   double* v = d.val;
   const double alpha = -1 / d.rdx[iv];
@@ -318,7 +328,7 @@ __device__ __forceinline__ void imp_update_row(const ImpDev& d, int hh, int iv, 
   const double beta = d.cmdx[iv] / d.dT + betaMe + betaR;
   const double delta = -betaR;
   double betaL;
-  switch (d.hh.type[hh]) {
+  switch (type) {
     case RT_NEXT_TERM:
       v[im] = alpha;
       v[im + 1] = beta + delta;
@@ -373,7 +383,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
   __shared__ double lds[4 * 16];
   const int r_begin = blockIdx.x * d.rpb;
   const int r_end = min(d.N, r_begin + d.rpb);
-  const double* __restrict__ val = d.val;
+  const double* val = d.val;  // HH rows are rewritten through d.val in the update phase
   const int32_t* __restrict__ rowptr = d.rowptr;
   const int32_t* __restrict__ col = d.col;
   long long gl = 0, ksamp = 0;
@@ -418,9 +428,10 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
     const double thr = d.tol_mode == NRT_TOL_RELATIVE ? d.tol * sqrt(bb) : d.tol;
     const double limit = thr * thr;
     int status = (rr <= limit) ? NRT_OK : NRT_ERR_NOT_CONVERGED;
-    int it = 0;
+    int it = 0, restarts = 0;
     double alpha = 0, omega = 0, beta = 0;
-    bool first_it = true;
+    double f_apr = 0, f_w1 = 0, f_w2 = 0;
+    bool first_it = true, restart = false;
     while (ok && status == NRT_ERR_NOT_CONVERGED && it < d.max_iter) {
       double* __restrict__ pold = pb ? d.p1 : d.p0;
       double* __restrict__ pnew = pb ? d.p0 : d.p1;
@@ -428,8 +439,10 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
       double* __restrict__ vnew = pb ? d.v0 : d.v1;
       const double nomega = -omega;
       // ---- phase A: p = r + beta*(p - omega*v) ; v = A p ; apr = v.r0
+      // (after a breakdown of rho the shadow residual restarts as r0 = r)
       double redA[1] = {0.0};
       for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
+        if (restart) d.r0[i] = d.r[i];
         const double pi = first_it ? d.r[i] : d.r[i] + beta * (pold[i] + nomega * vold[i]);
         pnew[i] = pi;
         double acc = 0;
@@ -441,12 +454,27 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
         vnew[i] = acc;
         redA[0] = redA[0] + acc * d.r0[i];
       }
+      restart = false;
       ok = reduce_all<1>(d, redA, lds, slot);
       if (!ok) break;
+      f_apr = redA[0];
+      if (!isfinite(redA[0])) {
+        status = NRT_ERR_NUMERIC;
+        break;
+      }
+      if (redA[0] == 0.0 || !isfinite(rho / redA[0])) {
+        // Lanczos breakdown (v orthogonal to the shadow residual): restart
+        it++;
+        if (++restarts > kMaxRestarts) break;
+        restart = true;
+        first_it = true;
+        rho = rr;
+        continue;
+      }
       alpha = rho / redA[0];
       const double nalpha = -(rho / redA[0]);
-      // ---- phase B: s = r - alpha*v ; t = A s ; w1 = t.s ; w2 = t.t
-      double redB[2] = {0.0, 0.0};
+      // ---- phase B: s = r - alpha*v ; t = A s ; w1 = t.s ; w2 = t.t ; ss = s.s
+      double redB[3] = {0.0, 0.0, 0.0};
       for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
         const double si = d.r[i] + nalpha * vnew[i];
         d.s[i] = si;
@@ -458,9 +486,27 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
         d.t[i] = acc;
         redB[0] = redB[0] + acc * si;
         redB[1] = redB[1] + acc * acc;
+        redB[2] = redB[2] + si * si;
       }
-      ok = reduce_all<2>(d, redB, lds, slot);
+      ok = reduce_all<3>(d, redB, lds, slot);
       if (!ok) break;
+      f_w1 = redB[0];
+      f_w2 = redB[1];
+      it++;
+      if (!isfinite(redB[2]) || !isfinite(redB[1])) {
+        status = NRT_ERR_NUMERIC;
+        break;
+      }
+      if (redB[2] <= limit || redB[1] == 0.0) {
+        // converged at the half step: x += alpha*p, r = s
+        for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
+          d.x[i] = d.x[i] + alpha * pnew[i];
+          d.r[i] = d.s[i];
+        }
+        rr = redB[2];
+        status = NRT_OK;
+        break;
+      }
       omega = redB[0] / redB[1];
       const double nw = -(redB[0] / redB[1]);
       // ---- phase C: x += alpha*p + omega*s ; r = s - omega*t ; p1 = r.r0 ; rr = r.r
@@ -482,12 +528,21 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
       rr = redC[1];
       pb ^= 1;
       first_it = false;
-      it++;
-      if (!isfinite(rr) || !isfinite(beta)) {
+      if (!isfinite(rr)) {
         status = NRT_ERR_NUMERIC;
         break;
       }
-      if ((it % d.check_every) == 0 && rr <= limit) status = NRT_OK;
+      if ((it % d.check_every) == 0 && rr <= limit) {
+        status = NRT_OK;
+        break;
+      }
+      if (!isfinite(beta) || rho == 0.0 || omega == 0.0) {
+        // breakdown of the bi-orthogonality: restart with r0 = r, p = r
+        if (++restarts > kMaxRestarts) break;
+        restart = true;
+        first_it = true;
+        rho = rr;
+      }
     }
     if (!ok) status = NRT_ERR_HIP;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -496,7 +551,17 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
       d.st->last_iters = it;
       d.st->last_rr = rr;
       d.st->last_bb = bb;
-      if (d.st->status == NRT_OK && status != NRT_OK) d.st->status = status;
+      d.st->restarts += restarts;
+      if (d.st->status == NRT_OK && status != NRT_OK) {
+        d.st->status = status;
+        d.st->fail_step = step;
+        d.st->fail_iter = it;
+        d.st->fail_rho = rho;
+        d.st->fail_apr = f_apr;
+        d.st->fail_w1 = f_w1;
+        d.st->fail_w2 = f_w2;
+        d.st->fail_rr = rr;
+      }
     }
     if (!ok) return;
     // ---- trace sample (neurite.cpp:241-249 cadence)
@@ -553,11 +618,6 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
       NRT_REQUIRE(p.mother[i] >= 0 && p.mother[i] < N && p.dr[i] >= 0 && p.dr[i] < N &&
                       p.dl[i] >= 0 && p.dl[i] < N,
                   NRT_ERR_TOPOLOGY, "element " + std::to_string(i) + " has an out-of-range neighbour");
-    }
-    for (int i = 0; i < H; i++) {
-      const int iv = p.hh_elem[i];
-      NRT_REQUIRE(iv >= 1 && iv <= N - 2, NRT_ERR_TOPOLOGY,
-                  "HH element " + std::to_string(iv) + " lies on an identity row");
     }
     std::vector<double> cmdx(N), kdx(N, 0.0), rdx(N), Wc(N);
     std::vector<uint32_t> code(N, 0);
@@ -704,9 +764,18 @@ static int collect_stats(nrt_handle* hnd) {
   hnd->stats.last_residual = sqrt(st.last_rr);
   hnd->stats.last_rhs_norm = sqrt(st.last_bb);
   hnd->stats.last_status = st.status;
+  hnd->stats.matrix_nnz = e->nnz;
+  hnd->stats.num_blocks = e->nblocks;
+  hnd->stats.restarts = st.restarts;
   if (err) throw Error(NRT_ERR_HIP, "grid barrier timed out inside the persistent Krylov kernel");
   // a reported failure is returned once, then cleared
   if (st.status != NRT_OK) {
+    char msg[256];
+    snprintf(msg, sizeof(msg),
+             "step %lld iteration %d: rho=%.3e apr=%.3e w1=%.3e w2=%.3e rr=%.3e (restarts %d)",
+             st.fail_step, st.fail_iter, st.fail_rho, st.fail_apr, st.fail_w1, st.fail_w2,
+             st.fail_rr, st.restarts);
+    hnd->last_error = msg;
     const int zero = 0;
     NRT_HIP(hipMemcpyAsync(&e->st.p->status, &zero, sizeof(int), hipMemcpyHostToDevice,
                            hnd->stream));

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libneurite_hip.so")
 NRT_OK = 0
 STATUS = {0: "ok", 1: "invalid", 2: "hip", 3: "nomem", 4: "topology", 5: "not_converged",
           6: "numeric", 7: "unsupported", 8: "admission", 9: "no_device"}
+STATUS_CODES = {v: k for k, v in STATUS.items()}
 EXPLICIT, IMPLICIT = 0, 1
 FP64, FP32_REFINE = 0, 1
 BICGSTAB, GMRES = 0, 1
@@ -75,7 +76,8 @@ class Stats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("krylov_iters", C.c_int64), ("refine_sweeps", C.c_int64),
                 ("last_iters", C.c_int32), ("last_status", C.c_int32),
                 ("last_residual", C.c_double), ("last_rhs_norm", C.c_double),
-                ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64)]
+                ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64),
+                ("matrix_nnz", C.c_int64), ("num_blocks", C.c_int32), ("restarts", C.c_int32)]
 
 
 class NrtError(RuntimeError):

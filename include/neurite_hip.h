@@ -180,6 +180,9 @@ typedef struct nrt_stats {
   double last_rhs_norm;     /* ||b||_2 of the last step                         */
   double kernel_ms;         /* device time of the dominant kernel, last run     */
   int64_t kernel_launches;  /* launches of the dominant kernel, last run        */
+  int64_t matrix_nnz;       /* implicit: non-zeros of the assembled CSR         */
+  int32_t num_blocks;       /* persistent kernels: workgroups in the grid       */
+  int32_t restarts;         /* Krylov breakdown restarts summed over steps      */
 } nrt_stats;
 
 typedef struct nrt_handle nrt_handle;

@@ -221,7 +221,10 @@ void gpu_assemble(Implicit& s) {
     push(delta, 2);
     if (q.dl[1] != 0) {
       push(-betaL, q.dl[1]);
-      tag(1, T_FIRST_BRANCH, 1, 1);
+      // solverIm-gpu.cpp:240 stores indVec_DL = 1 here, so Update_Mat would take
+      // betaL from element 1 instead of its left daughter; the CPU solver
+      // (solverIm-cpu.cpp:231-262) uses the daughter, which this follows.
+      tag(1, T_FIRST_BRANCH, 1, q.dl[1]);
     } else {
       tag(1, T_FIRST_NORMAL, 1, -1);
     }
