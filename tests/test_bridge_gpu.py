@@ -1,0 +1,68 @@
+"""Drop-in at the C++ class level: the reference's setup chain, space.cpp builders
+and time loop (oracle/_ref/neurite_ref_hip, built from the reference sources by
+oracle/Makefile `bridge`) construct SolverExHIP / SolverImHIP through the bridge
+classes with the reference's constructor signature, and the run must reproduce
+the reference's own trajectories (tests/golden) within 1e-9 V."""
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from common import TOL_V, ghost_mask, load_golden, max_abs_diff, problem_from_golden
+
+pytestmark = pytest.mark.gpu
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(REPO, "oracle", "_ref", "neurite_ref_hip")
+
+# scenarios whose builders need no input files (examples 1-6)
+CASES = {
+    "ex1_only_hh": ("Only_HH_Model", "e out o.dat log o.log", -1),
+    "ex3_hh_y": ("HH_Y_branching", "e Tt 0.1 dtf 0.8 out o.dat log o.log", 30000),
+    "ex4_ct_cable": ("CT_cable", "e Tt 0.1 dtf 0.8 out o.dat log o.log", 6000),
+    "ex5_ct_y": ("CT_Y_branching", "e Tt 0.1 dtf 0.8 out o.dat log o.log", -1),
+    "ex6_ct_tree": ("CT_symmetric_tree", "e Tt 0.1 dtf 0.8 out o.dat log o.log", 6000),
+}
+
+
+def _run(sim, cli, max_steps, checkpoints=(1, 2, 10, 100, 1000, 2999)):
+    with tempfile.TemporaryDirectory() as td:
+        dump = os.path.join(td, "d.json")
+        env = dict(os.environ, simulation=sim, REF_DUMP=dump, working_directory=td,
+                   REF_MAX_STEPS=str(max_steps),
+                   REF_CHECKPOINTS=",".join(str(c) for c in checkpoints))
+        r = subprocess.run([EXE] + cli.split(), env=env, cwd=td, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        with open(dump) as f:
+            return json.load(f)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_exe():
+    if not os.path.exists(EXE):
+        pytest.skip("oracle/_ref/neurite_ref_hip not built (reference tree absent at build time)")
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_bridge_explicit_matches_reference(name):
+    sim, cli, steps = CASES[name]
+    print(f"[gpu-case] bridge explicit {name}", flush=True)
+    d = _run(sim, cli, steps)
+    g = load_golden(name)
+    assert d["steps_run"] == g["steps_run"]
+    assert max_abs_diff(d["trace_v"], g["trace_v"]) <= TOL_V
+    for got, ref in zip(d["checkpoints"], g["checkpoints"]):
+        assert max_abs_diff(got, ref) <= TOL_V
+    assert max_abs_diff(d["final_v"], g["final_v"]) <= TOL_V
+
+
+def test_bridge_implicit_matches_exact(orc):
+    print("[gpu-case] bridge implicit ex6 dtf 10", flush=True)
+    d = _run("CT_symmetric_tree", "i Tt 0.1 dtf 10 out o.dat log o.log", 300)
+    p = problem_from_golden(d)
+    ref = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    ref.run(1, d["steps_run"], d["active_first"], d["active_last"])
+    assert max_abs_diff(d["final_v"], ref.snapshot(), ghost_mask(p)) <= TOL_V
