@@ -180,10 +180,15 @@ int nrt_create(const nrt_problem* p, const nrt_options* opt, nrt_handle** out) {
     if (opt) o = *opt;
     else nrt_default_options(&o);
     NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.kind == NRT_IMPLICIT, NRT_ERR_INVALID, "bad kind");
-    NRT_REQUIRE(o.precision == NRT_FP64, NRT_ERR_UNSUPPORTED,
-                "only the FP64 plain form is available through nrt_create");
-    NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB, NRT_ERR_UNSUPPORTED,
-                "only BiCGSTAB is available for the FP64 implicit solver");
+    NRT_REQUIRE(o.precision == NRT_FP64 || o.precision == NRT_FP32_REFINE, NRT_ERR_INVALID,
+                "bad precision");
+    NRT_REQUIRE(o.precision == NRT_FP64 || (o.kind == NRT_IMPLICIT && o.krylov == NRT_DIRECT),
+                NRT_ERR_UNSUPPORTED,
+                "FP32 with FP64 refinement is available for the implicit tree solve");
+    NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB || o.krylov == NRT_DIRECT,
+                NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB or NRT_DIRECT");
+    NRT_REQUIRE(o.precond == NRT_PRECOND_NONE, NRT_ERR_UNSUPPORTED,
+                "preconditioned Krylov is not built; use NRT_DIRECT for the tree solve");
     HostProblem q = load_problem(p);
     auto* h = new nrt_handle;
     try {

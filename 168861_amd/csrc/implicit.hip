@@ -27,6 +27,7 @@
 #include <numeric>
 #include "nrt_internal.hpp"
 #include "nrt_device.hpp"
+#include "tree.hpp"
 
 namespace nrt {
 
@@ -126,6 +127,15 @@ struct Implicit {
   DBuf<double> trace;
   int64_t trace_cap = 0;
   int64_t step = 1;
+  // tree partition solve (method != 0)
+  int method = 0;        // 0 BiCGSTAB, 1 exact FP64 tree solve, 2 FP64 refinement over FP32
+  bool grid_mode = false;
+  int threads = 256;
+  TreePlanHost plan_h;
+  DBuf<int32_t> plan_buf;
+  TreePlanDev plan{};
+  DBuf<double> tw64;
+  DBuf<float> tw32;
 };
 
 // ------------------------------------------------------------------ assembly
@@ -575,6 +585,192 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
   }
 }
 
+// --------------------------------------------- tree partition solve kernels
+// One persistent kernel per call, as the BiCGSTAB one, with the Krylov loop
+// replaced by the exact tree partition solve (tree.hpp): either one FP64 solve
+// per step (direct), or FP64 iterative refinement around FP32 solves whose
+// residual b - A x and update x += d are the only FP64 vector work. Block mode:
+// one workgroup owns the whole tree and synchronises with workgroup barriers;
+// grid mode (large trees): every workgroup takes a share of the rows and chunks,
+// grid barriers separate the levels, and workgroup 0 solves the compressed forest.
+constexpr int kTreeBlock = 1024;
+
+template <class T>
+struct TreeArgs {
+  TreePlanDev P;
+  TreeWork<T> w;
+  int direct;
+  double refine_tol;
+  int refine_max;
+};
+
+template <bool GRID>
+__device__ __forceinline__ void tsync(const ImpDev& d) {
+  if (GRID) grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+  else __syncthreads();
+}
+
+template <class T, bool GRID>
+__device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w) {
+  const int wid = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)threadIdx.x;
+  const int nw = GRID ? (int)(gridDim.x * blockDim.x) : (int)blockDim.x;
+  for (int lv = 0; lv < P.levels; lv++) {
+    const int c1 = P.chunk_off[lv + 1];
+    for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw) tps_reduce_chunk(P, w, lv, ch);
+    tsync<GRID>(d);
+  }
+  if (!GRID || blockIdx.x == 0) {
+    for (int h = 0; h < P.heights; h++) {
+      for (int k = P.fin_hoff[h] + (int)threadIdx.x; k < P.fin_hoff[h + 1]; k += blockDim.x)
+        tps_final_up(P, w, P.fin_order[k]);
+      __syncthreads();
+    }
+    for (int h = P.heights - 1; h >= 0; h--) {
+      for (int k = P.fin_hoff[h] + (int)threadIdx.x; k < P.fin_hoff[h + 1]; k += blockDim.x)
+        tps_final_down(P, w, P.fin_order[k]);
+      __syncthreads();
+    }
+  }
+  if (GRID) tsync<GRID>(d);
+  for (int lv = P.levels - 1; lv >= 0; lv--) {
+    const int n = P.nslots[lv];
+    for (int v = wid; v < n; v += nw) tps_back(P, w, lv, v);
+    tsync<GRID>(d);
+  }
+}
+
+// Update phase of a step for rows [r0, r1) with stride `stride` (as in the
+// BiCGSTAB kernel): gating, HH row refresh, right-hand side.
+__device__ __forceinline__ void imp_update_rows(const ImpDev& d, int r0, int r1, int stride,
+                                                bool active) {
+  for (int i = r0 + (int)threadIdx.x; i < r1; i += stride) {
+    const double v = d.x[i];
+    const uint32_t c = d.code[i];
+    double kd;
+    if (c & kCodeHH) {
+      const int hh = (int)(c & kCodeIdx);
+      double W, K;
+      imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+      imp_update_row(d, hh, i, W);
+      kd = K * d.dxv[i];
+    } else {
+      kd = d.kdx[i];
+    }
+    double bi = d.cmdx[i] * v / d.dT + kd;
+    if (active) bi = bi + d.input[i];
+    d.b[i] = bi;
+    if (d.x0_mode == NRT_X0_RHS) d.x[i] = bi;
+  }
+}
+
+template <class T, bool GRID>
+__global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree(ImpDev d,
+                                                                                 TreeArgs<T> t) {
+  __shared__ double lds[4 * 16];
+  const int r_begin = GRID ? (int)blockIdx.x * d.rpb : 0;
+  const int r_end = GRID ? min(d.N, r_begin + d.rpb) : d.N;
+  const int bs = blockDim.x;
+  const double* val = d.val;
+  const int32_t* __restrict__ rowptr = d.rowptr;
+  const int32_t* __restrict__ col = d.col;
+  long long gl = 0, ksamp = 0;
+  int slot = 0;
+  const double lim2 = t.refine_tol * t.refine_tol;
+  for (long long step = d.first; step < d.first + d.num; step++) {
+    const bool active = step >= d.a0 && step <= d.a1;
+    imp_update_rows(d, r_begin, r_end, bs, active);
+    tsync<GRID>(d);
+    int sweeps = 0, status = NRT_OK;
+    double rr = 0, bb = 0;
+    bool ok = true;
+    const int32_t* __restrict__ slot0 = t.P.slot0;
+    if (t.direct) {
+      for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+        const int si = slot0[i];
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+          const int c = col[k];
+          const T v = (T)val[k];
+          if (c == i) t.w.d[si] = v;
+          else if (c < i) t.w.up[si] = v;
+          else t.w.dn[slot0[c]] = v;
+        }
+        t.w.r[si] = (T)d.b[i];
+      }
+      tsync<GRID>(d);
+      tps_run<T, GRID>(d, t.P, t.w);
+      for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs)
+        d.x[i] = (double)t.w.x[slot0[i]];
+      sweeps = 1;
+    } else {
+      while (true) {
+        // FP64 residual r = b - A x; the FP32 tree system is refreshed from the CSR
+        double red[2] = {0.0, 0.0};
+        for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+          const int si = slot0[i];
+          double ax = 0;
+          for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+            const int c = col[k];
+            const double v = val[k];
+            ax = ax + v * d.x[c];
+            if (c == i) t.w.d[si] = (T)v;
+            else if (c < i) t.w.up[si] = (T)v;
+            else t.w.dn[slot0[c]] = (T)v;
+          }
+          const double bi = d.b[i];
+          const double ri = bi - ax;
+          t.w.r[si] = (T)ri;
+          red[0] = red[0] + ri * ri;
+          red[1] = red[1] + bi * bi;
+        }
+        if (GRID) ok = reduce_all<2>(d, red, lds, slot);
+        else block_sum<2>(red, lds);
+        if (!ok) break;
+        rr = red[0];
+        bb = red[1];
+        if (!isfinite(rr)) {
+          status = NRT_ERR_NUMERIC;
+          break;
+        }
+        if (rr <= lim2 * bb) break;
+        if (sweeps >= t.refine_max) {
+          status = NRT_ERR_NOT_CONVERGED;
+          break;
+        }
+        tps_run<T, GRID>(d, t.P, t.w);
+        for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs)
+          d.x[i] = d.x[i] + (double)t.w.x[slot0[i]];
+        tsync<GRID>(d);
+        sweeps++;
+      }
+    }
+    if (GRID && __hip_atomic_load(d.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+      ok = false;
+    if (!ok) status = NRT_ERR_HIP;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      d.st->iters_total += (unsigned long long)sweeps;
+      d.st->steps += 1;
+      d.st->last_iters = sweeps;
+      d.st->last_rr = rr;
+      d.st->last_bb = bb;
+      if (d.st->status == NRT_OK && status != NRT_OK) {
+        d.st->status = status;
+        d.st->fail_step = step;
+        d.st->fail_iter = sweeps;
+        d.st->fail_rr = rr;
+      }
+    }
+    if (!ok) return;
+    if (d.interval >= 0 && ++gl > d.interval) {
+      gl = 0;
+      if (d.trace && d.trace_elem >= r_begin && d.trace_elem < r_end && threadIdx.x == 0)
+        d.trace[ksamp] = d.x[d.trace_elem];
+      ksamp++;
+    }
+    // x of a row is written by the thread that owns it and read back only by it
+    // in the next update phase; other rows' x are read after the next barrier
+  }
+}
+
 // ---------------------------------------------------------------- host side
 static void device_exclusive_scan(const int32_t* in, int32_t* out, int n, DBuf<int32_t>& tmp,
                                   hipStream_t s) {
@@ -586,11 +782,14 @@ static void device_exclusive_scan(const int32_t* in, int32_t* out, int n, DBuf<i
   NRT_HIP(hipGetLastError());
 }
 
-static int choose_blocks(nrt_handle* hnd, int N) {
+static int choose_blocks(nrt_handle* hnd, int N, const void* kernel = nullptr) {
   int dev = hnd->device, cus = 0, per_cu = 0;
   NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_implicit_bicgstab_fp64,
-                                                       kImpBlock, 0));
+  if (kernel)
+    NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kImpBlock, 0));
+  else
+    NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_implicit_bicgstab_fp64,
+                                                         kImpBlock, 0));
   int max_coresident = std::max(1, cus * std::max(1, per_cu));
   int want = hnd->opt.num_blocks > 0 ? hnd->opt.num_blocks : grid_for(N, kImpBlock * 8);
   want = std::min(want, cus);  // one block per CU at most unless asked explicitly
@@ -680,7 +879,57 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
       vbuf->alloc(N);
       NRT_HIP(hipMemsetAsync(vbuf->p, 0, sizeof(double) * N, s));
     }
-    e->nblocks = choose_blocks(hnd, N);
+    const nrt_options& o = hnd->opt;
+    e->method = o.krylov == NRT_DIRECT ? (o.precision == NRT_FP64 ? 1 : 2) : 0;
+    if (e->method != 0) {
+      // plan of the tree partition solve from the assembled pattern
+      std::vector<int32_t> hrp(N + 1), hcol(nnz);
+      NRT_HIP(hipMemcpyAsync(hrp.data(), e->rowptr.p, sizeof(int32_t) * (N + 1),
+                             hipMemcpyDeviceToHost, s));
+      NRT_HIP(hipMemcpyAsync(hcol.data(), e->col.p, sizeof(int32_t) * nnz,
+                             hipMemcpyDeviceToHost, s));
+      NRT_HIP(hipStreamSynchronize(s));
+      std::vector<int32_t> par;
+      try {
+        par = csr_parents(N, hrp.data(), hcol.data());
+      } catch (const std::exception& ex) {
+        throw Error(NRT_ERR_TOPOLOGY, std::string("tree solve: ") + ex.what());
+      }
+      for (int i = 0; i < N; i++)
+        for (int k = hrp[i]; k < hrp[i + 1]; k++)
+          NRT_REQUIRE(hcol[k] <= i || par[hcol[k]] == i, NRT_ERR_TOPOLOGY,
+                      "tree solve: row " + std::to_string(i) + " couples to row " +
+                          std::to_string(hcol[k]) + ", which does not couple back");
+      const int chunk = o.chunk > 0 ? o.chunk : 32;
+      e->plan_h = build_tree_plan(par, chunk);
+      const TreePlanHost& P = e->plan_h;
+      // one device buffer holding every plan array
+      std::vector<int32_t> flat;
+      plan_view(P, nullptr, &flat);
+      e->plan_buf.upload(flat, s);
+      e->plan = plan_view(P, e->plan_buf.p, nullptr);
+      const size_t T = (size_t)P.total_nodes();
+      if (e->method == 1) {
+        e->tw64.alloc(8 * T);
+        NRT_HIP(hipMemsetAsync(e->tw64.p, 0, sizeof(double) * 8 * T, s));
+      } else {
+        e->tw32.alloc(8 * T);
+        NRT_HIP(hipMemsetAsync(e->tw32.p, 0, sizeof(float) * 8 * T, s));
+      }
+      e->grid_mode = o.num_blocks > 1 || (o.num_blocks == 0 && N > (1 << 17));
+    }
+    if (e->method != 0 && !e->grid_mode) {
+      e->nblocks = 1;
+      e->threads = kTreeBlock;
+    } else if (e->method != 0) {
+      const void* k = e->method == 1 ? (const void*)k_implicit_tree<double, true>
+                                     : (const void*)k_implicit_tree<float, true>;
+      e->nblocks = choose_blocks(hnd, N, k);
+      e->threads = kImpBlock;
+    } else {
+      e->nblocks = choose_blocks(hnd, N);
+      e->threads = kImpBlock;
+    }
     e->part.alloc((size_t)e->nblocks * 4 * 2);
     e->bar.alloc(4);
     NRT_HIP(hipMemsetAsync(e->bar.p, 0, sizeof(unsigned) * 4, s));
@@ -745,9 +994,36 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.interval = interval;
   d.trace_elem = trace_elem;
   d.trace = trace_dev;
-  void* args[] = {&d};
-  NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_bicgstab_fp64, dim3(e->nblocks),
-                                     dim3(kImpBlock), args, 0, hnd->stream));
+  if (e->method == 0) {
+    void* args[] = {&d};
+    NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_bicgstab_fp64, dim3(e->nblocks),
+                                       dim3(kImpBlock), args, 0, hnd->stream));
+    return NRT_OK;
+  }
+  const size_t T = (size_t)e->plan_h.total_nodes();
+  auto launch = [&](auto targs, const void* kgrid, const void* kblock) {
+    void* args[] = {&d, &targs};
+    if (e->grid_mode)
+      NRT_HIP(hipLaunchCooperativeKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, 0,
+                                         hnd->stream));
+    else
+      NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));
+  };
+  if (e->method == 1) {
+    double* b = e->tw64.p;
+    TreeArgs<double> t{e->plan, TreeWork<double>{b, b + T, b + 2 * T, b + 3 * T, b + 4 * T,
+                                                 b + 5 * T, b + 6 * T, b + 7 * T},
+                       1, o.refine_tol, o.refine_max};
+    launch(t, (const void*)k_implicit_tree<double, true>,
+           (const void*)k_implicit_tree<double, false>);
+  } else {
+    float* b = e->tw32.p;
+    TreeArgs<float> t{e->plan, TreeWork<float>{b, b + T, b + 2 * T, b + 3 * T, b + 4 * T,
+                                               b + 5 * T, b + 6 * T, b + 7 * T},
+                      0, o.refine_tol, o.refine_max};
+    launch(t, (const void*)k_implicit_tree<float, true>,
+           (const void*)k_implicit_tree<float, false>);
+  }
   return NRT_OK;
 }
 

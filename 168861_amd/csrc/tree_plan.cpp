@@ -1,0 +1,237 @@
+// Host-side plan of the tree partition solve (see tree.hpp).
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include "tree.hpp"
+#include "../../include/neurite_hip.h"
+
+namespace nrt {
+
+std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* col) {
+  std::vector<int32_t> par(N, -1);
+  for (int i = 0; i < N; i++) {
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      const int c = col[k];
+      if (c < i) {
+        if (par[i] >= 0 && par[i] != c)
+          throw std::runtime_error("row " + std::to_string(i) + " couples to two earlier rows");
+        par[i] = c;
+      }
+    }
+  }
+  return par;
+}
+
+namespace {
+struct LocalLevel {
+  std::vector<int32_t> par;                 // level-local parents
+  std::vector<int32_t> cs, cl;              // chunks (local start, length)
+  std::vector<int32_t> next, es, ee;        // local next-level index / chunk ends
+  std::vector<int32_t> slot;                // local node -> slot
+  int nchunk = 0, cmax = 0, nslots = 0;
+};
+}  // namespace
+
+TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk) {
+  TreePlanHost P;
+  P.C = std::max(3, chunk);
+  const int N0 = (int)par0.size();
+  for (int i = 0; i < N0; i++)
+    if (par0[i] >= i) throw std::runtime_error("tree plan needs parent < child");
+  std::vector<LocalLevel> lv;
+  std::vector<int32_t> par = par0;
+  while (true) {
+    LocalLevel L;
+    L.par = par;
+    const int n = (int)par.size();
+    std::vector<int32_t> nchild(n, 0);
+    for (int v = 0; v < n; v++)
+      if (par[v] >= 0) nchild[par[v]]++;
+    bool reduces = false;
+    int v = 0;
+    while (v < n) {
+      int b = v;
+      while (b + 1 < n && par[b + 1] == b && nchild[b] == 1) b++;
+      for (int a = v; a <= b; a += P.C) {
+        const int m = std::min(P.C, b - a + 1);
+        L.cs.push_back(a);
+        L.cl.push_back(m);
+        if (m >= 3) reduces = true;
+      }
+      v = b + 1;
+    }
+    if (!reduces) {  // final forest
+      L.cs.clear();
+      L.cl.clear();
+      lv.push_back(std::move(L));
+      break;
+    }
+    L.next.assign(n, -1);
+    L.es.assign(n, -1);
+    L.ee.assign(n, -1);
+    int cnt = 0;
+    for (size_t k = 0; k < L.cs.size(); k++) {
+      const int a = L.cs[k], m = L.cl[k];
+      L.next[a] = cnt++;
+      if (m >= 2) L.next[a + m - 1] = cnt++;
+      for (int j = a + 1; j < a + m - 1; j++) {
+        L.es[j] = a;
+        L.ee[j] = a + m - 1;
+      }
+    }
+    std::vector<int32_t> npar(cnt, -1);
+    for (size_t k = 0; k < L.cs.size(); k++) {
+      const int a = L.cs[k], m = L.cl[k];
+      if (par[a] >= 0) {
+        if (L.next[par[a]] < 0) throw std::runtime_error("tree plan: chunk parent eliminated");
+        npar[L.next[a]] = L.next[par[a]];
+      }
+      if (m >= 2) npar[L.next[a + m - 1]] = L.next[a];
+    }
+    lv.push_back(std::move(L));
+    par.swap(npar);
+  }
+  // slot maps
+  const int nl = (int)lv.size();
+  P.levels = nl - 1;
+  for (int l = 0; l < nl; l++) {
+    LocalLevel& L = lv[l];
+    const int n = (int)L.par.size();
+    L.slot.assign(n, -1);
+    if (l == nl - 1) {
+      for (int u = 0; u < n; u++) L.slot[u] = u;
+      L.nslots = n;
+    } else {
+      L.nchunk = (int)L.cs.size();
+      for (int m : L.cl) L.cmax = std::max(L.cmax, m);
+      for (int c = 0; c < L.nchunk; c++)
+        for (int k = 0; k < L.cl[c]; k++) L.slot[L.cs[c] + k] = k * L.nchunk + c;
+      L.nslots = L.cmax * L.nchunk;
+    }
+  }
+  P.node_off.push_back(0);
+  for (int l = 0; l < nl; l++) {
+    P.nslots.push_back(lv[l].nslots);
+    P.nchunk.push_back(lv[l].nchunk);
+    P.node_off.push_back(P.node_off.back() + lv[l].nslots);
+  }
+  // flat slot-indexed arrays
+  P.next.assign(P.total_nodes(), -1);
+  P.es.assign(P.total_nodes(), -1);
+  P.ee.assign(P.total_nodes(), -1);
+  P.chunk_off.push_back(0);
+  for (int l = 0; l + 1 < nl; l++) {
+    const LocalLevel& L = lv[l];
+    const LocalLevel& Nx = lv[l + 1];
+    const int base = P.node_off[l];
+    for (size_t u = 0; u < L.par.size(); u++) {
+      const int g = base + L.slot[u];
+      if (L.next[u] >= 0) P.next[g] = Nx.slot[L.next[u]];
+      if (L.es[u] >= 0) {
+        P.es[g] = Nx.slot[L.next[L.es[u]]];
+        P.ee[g] = Nx.slot[L.next[L.ee[u]]];
+      }
+    }
+    P.chunk_len.insert(P.chunk_len.end(), L.cl.begin(), L.cl.end());
+    P.chunk_off.push_back((int32_t)P.chunk_len.size());
+  }
+  P.slot0 = lv[0].slot;
+  // final forest (local index == slot): heights, children, order
+  const std::vector<int32_t>& fp = lv[nl - 1].par;
+  const int n = (int)fp.size();
+  P.fin_par = fp;
+  std::vector<int32_t> h(n, 0);
+  for (int u = n - 1; u >= 0; u--)
+    if (fp[u] >= 0) h[fp[u]] = std::max(h[fp[u]], h[u] + 1);
+  int H = 0;
+  for (int u = 0; u < n; u++) H = std::max(H, h[u] + 1);
+  P.heights = H;
+  P.fin_hoff.assign(H + 1, 0);
+  for (int u = 0; u < n; u++) P.fin_hoff[h[u] + 1]++;
+  for (int k = 0; k < H; k++) P.fin_hoff[k + 1] += P.fin_hoff[k];
+  P.fin_order.assign(n, 0);
+  std::vector<int32_t> fill(P.fin_hoff.begin(), P.fin_hoff.end() - 1);
+  for (int u = 0; u < n; u++) P.fin_order[fill[h[u]]++] = u;
+  P.fin_cptr.assign(n + 1, 0);
+  for (int u = 0; u < n; u++)
+    if (fp[u] >= 0) P.fin_cptr[fp[u] + 1]++;
+  for (int u = 0; u < n; u++) P.fin_cptr[u + 1] += P.fin_cptr[u];
+  P.fin_child.assign(P.fin_cptr[n], 0);
+  std::vector<int32_t> pos(P.fin_cptr.begin(), P.fin_cptr.end() - 1);
+  for (int u = 0; u < n; u++)
+    if (fp[u] >= 0) P.fin_child[pos[fp[u]]++] = u;
+  P.nchunk.push_back(0);  // keeps nchunk[levels] addressable
+  return P;
+}
+
+// Device view over one flat int32 buffer holding every plan array; `flat`
+// receives the concatenation, the returned view has offsets relative to base.
+TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
+  const std::vector<int32_t>* parts[] = {&P.nslots, &P.nchunk, &P.node_off, &P.next,
+                                         &P.es, &P.ee, &P.chunk_off, &P.chunk_len,
+                                         &P.slot0, &P.fin_par, &P.fin_order, &P.fin_hoff,
+                                         &P.fin_cptr, &P.fin_child};
+  size_t off[14];
+  size_t pos = 0;
+  for (int k = 0; k < 14; k++) {
+    off[k] = pos;
+    if (flat) {
+      flat->insert(flat->end(), parts[k]->begin(), parts[k]->end());
+      flat->push_back(0);
+    }
+    pos += parts[k]->size() + 1;
+  }
+  return TreePlanDev{P.levels,        P.heights,     base + off[0],  base + off[1],
+                     base + off[2],   base + off[3], base + off[4],  base + off[5],
+                     base + off[6],   base + off[7], base + off[8],  base + off[9],
+                     base + off[10],  base + off[11], base + off[12], base + off[13]};
+}
+
+}  // namespace nrt
+
+using namespace nrt;
+
+// Test hook (not part of the solver ABI): the tree partition solve of an assembled
+// CSR system run on the host with the same per-chunk / per-node routines the
+// device kernels execute, so CPU tests can check the plan logic against the
+// oracle's sequential elimination. No solver handle ever calls it.
+extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int32_t* col,
+                                        const double* val, const double* rhs, int chunk,
+                                        double* x, int* levels, int* heights) {
+  try {
+    std::vector<int32_t> par = csr_parents(N, rowptr, col);
+    TreePlanHost H = build_tree_plan(par, chunk);
+    std::vector<int32_t> flat;
+    TreePlanDev tmp = plan_view(H, nullptr, &flat);
+    (void)tmp;
+    TreePlanDev P = plan_view(H, flat.data(), nullptr);
+    const size_t T = (size_t)H.total_nodes();
+    std::vector<double> buf(8 * T, 0.0);
+    double* b = buf.data();
+    TreeWork<double> w{b, b + T, b + 2 * T, b + 3 * T, b + 4 * T, b + 5 * T, b + 6 * T, b + 7 * T};
+    const int32_t* s0 = H.slot0.data();
+    for (int i = 0; i < N; i++) {
+      w.r[s0[i]] = rhs[i];
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const int c = col[k];
+        if (c == i) w.d[s0[i]] += val[k];
+        else if (c < i) w.up[s0[i]] = val[k];
+        else w.dn[s0[c]] = val[k];
+      }
+    }
+    for (int lv = 0; lv < H.levels; lv++)
+      for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++) tps_reduce_chunk(P, w, lv, ch);
+    for (int h = 0; h < H.heights; h++)
+      for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_up(P, w, H.fin_order[k]);
+    for (int h = H.heights - 1; h >= 0; h--)
+      for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_down(P, w, H.fin_order[k]);
+    for (int lv = H.levels - 1; lv >= 0; lv--)
+      for (int v = 0; v < H.nslots[lv]; v++) tps_back(P, w, lv, v);
+    for (int i = 0; i < N; i++) x[i] = w.x[s0[i]];
+    if (levels) *levels = H.levels;
+    if (heights) *heights = H.heights;
+    return NRT_OK;
+  } catch (const std::exception&) {
+    return NRT_ERR_TOPOLOGY;
+  }
+}

@@ -20,7 +20,8 @@ STATUS = {0: "ok", 1: "invalid", 2: "hip", 3: "nomem", 4: "topology", 5: "not_co
 STATUS_CODES = {v: k for k, v in STATUS.items()}
 EXPLICIT, IMPLICIT = 0, 1
 FP64, FP32_REFINE = 0, 1
-BICGSTAB, GMRES = 0, 1
+BICGSTAB, GMRES, DIRECT = 0, 1, 2
+PRECOND_NONE, PRECOND_TREE = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 X0_PREVIOUS, X0_RHS = 0, 1
 
@@ -69,7 +70,8 @@ class Options(C.Structure):
                 ("restart", C.c_int32), ("tol_mode", C.c_int32), ("x0_mode", C.c_int32),
                 ("check_every", C.c_int32), ("max_iter", C.c_int32), ("device", C.c_int32),
                 ("num_blocks", C.c_int32), ("tol", C.c_double), ("refine_tol", C.c_double),
-                ("refine_max", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("refine_max", C.c_int32), ("precond", C.c_int32), ("chunk", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class Stats(C.Structure):

@@ -4,16 +4,19 @@
 Workload (the configuration the headline metric is quoted on): a myelinated
 axon of 200 HH nodes of Ranvier (1 um) alternating with 1 mm passive internodes
 (100 x 10 um, Cm = 0.005 uF/cm^2, gL = 0.001 mS/cm^2), d = 10 um, N = 20,202
-elements, backward Euler at dt = 10 us, Krylov relative residual 1e-10,
-stimulus 10 nA for 0.5 ms at the first node. A "step" is one time step: HH
-gating, matrix refresh, right-hand side and the full Krylov solve, all inside
-the library's persistent kernel; inputs are resident on the device.
+elements, backward Euler at dt = 10 us, stimulus 10 nA for 0.5 ms at the first
+node. A "step" is one time step: HH gating, matrix refresh, right-hand side and
+the solve, all inside one persistent kernel launch per run; inputs are resident
+on the device.
 
-Prints ONE JSON line. `roofline` uses algorithmic bytes (per element and per
-Krylov iteration, formulas below) over the kernel's device time measured with
-HIP events on the library's stream. `cpu_baseline` times the reference CPU
-method (restarted GMRES(150), x0 = 0, ||r|| <= 1e-14 A) on this host's cores
-for a bounded sample of the same workload.
+Forms measured on the same workload:
+  fp64_tree_direct     exact tree partition solve in FP64
+  mixed_tree_ir        FP32 tree solves inside FP64 iterative refinement
+                       (residual b - A x and x += d in FP64, relative 1e-10)
+  fp64_plain_bicgstab  the reference method: unpreconditioned BiCGSTAB,
+                       relative residual 1e-10 (few steps: it is slow)
+`value` is the fastest form whose max |dV| against the FP64 exact solve after
+the same steps is <= 1e-4 mV. Prints ONE JSON line.
 """
 import argparse
 import json
@@ -21,20 +24,51 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(REPO, "168861_amd")]
 
 METRIC = "compartment-steps/s at max|ΔV|≤1e-4 mV vs FP64; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
+TOL_MV = 1e-4
 
 
-def implicit_bytes_per_run(N, H, nnz, steps, iters):
-    """Algorithmic HBM bytes of `steps` implicit FP64 steps with `iters` BiCGSTAB
-    iterations in total: per step 144 B per HH element (V, gates r/w,
-    parameters, diagonal) + 40 B per element (right-hand side); per iteration
-    2 SpMV (12 nnz + 4 (N+1) + 16 N bytes each) + 21 vector passes of 8 N."""
+def bicgstab_bytes(N, H, nnz, steps, iters):
+    """Per step 144 B per HH element + 40 B per element (update phase); per
+    BiCGSTAB iteration 2 SpMV (12 nnz + 4 (N+1) + 16 N) + 21 vector passes of 8 N."""
     spmv = 12 * nnz + 4 * (N + 1) + 16 * N
     return steps * (144 * H + 40 * N) + iters * (2 * spmv + 21 * 8 * N)
+
+
+def tree_bytes(N, H, nnz, steps, solves, vbytes):
+    """Per step the update phase (144 B per HH element + 40 B per element) and the
+    write of V (8 N); per tree solve: the CSR read (12 nnz + 4 (N+1)), the level-0
+    system write + read (4 values), the elimination coefficients written twice and
+    read once (3 values each), and the solution write (1 value), vbytes per value;
+    refinement adds the FP64 residual pass (8 N read of x, b) per solve."""
+    per_solve = 12 * nnz + 4 * (N + 1) + N * vbytes * (4 + 4 + 9 + 1)
+    if vbytes == 4:
+        per_solve += 24 * N  # FP64 residual: x, b read; x += d write
+    return steps * (144 * H + 40 * N + 8 * N) + solves * per_solve
+
+
+def run_form(nh, prob, opts, first, steps, a0, a1):
+    s = nh.Solver(prob, kind=nh.IMPLICIT, device=0, **opts)
+    if first > 1:
+        s.run(1, first - 1, a0, a1)
+    s.sync()
+    st0 = s.stats()
+    t0 = time.perf_counter()
+    s.run(first, steps, a0, a1)
+    s.sync()
+    t1 = time.perf_counter()
+    st1 = s.stats()
+    v = s.snapshot()
+    s.close()
+    return {"wall": t1 - t0, "kernel_ms": st1["kernel_ms"],
+            "inner": st1["krylov_iters"] - st0["krylov_iters"], "nnz": st1["matrix_nnz"],
+            "blocks": st1["num_blocks"], "v": v}
 
 
 def cpu_baseline(prob, budget_s=15.0, max_steps=3):
@@ -49,19 +83,28 @@ def cpu_baseline(prob, budget_s=15.0, max_steps=3):
         im.update(True)
         im.calculate()
     dt = time.perf_counter() - t0
-    return {"value": prob.N * n / dt, "unit": "compartment-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"first {n} steps of the same workload (N={prob.N}) with the reference "
-                      "CPU method: restarted GMRES(150), x0=0, ||r||<=1e-14 A "
-                      "(solverIm-cpu.cpp semantics), oracle C++ build, 1 thread"}
+    out = {"value": prob.N * n / dt, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
+           "sample": f"first {n} steps of the same workload (N={prob.N}) with the reference "
+                     "CPU method: restarted GMRES(150), x0=0, ||r||<=1e-14 A "
+                     "(solverIm-cpu.cpp semantics), oracle C++ build, 1 thread"}
+    ex = oracle.Implicit(prob, variant=oracle.VAR_GPU, method=oracle.M_DIRECT)
+    t0 = time.perf_counter()
+    n = 0
+    while n < 2000 and time.perf_counter() - t0 < 5.0:
+        n += 1
+        ex.update(n <= 50)
+        ex.calculate()
+    out["exact_tree_elimination_1core"] = prob.N * n / (time.perf_counter() - t0)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -74,52 +117,72 @@ def main():
 
     geom, run = wl.myelinated_axon()
     prob, _ = nh.setup(geom, dT=run.dT)
+    N, H = prob.N, prob.H
     a0, a1 = nh.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
-    s = nh.Solver(prob, kind=nh.IMPLICIT, precision=nh.FP64, krylov=nh.BICGSTAB, tol=a.tol,
-                  tol_mode=nh.TOL_RELATIVE, device=0)
-    if a.warmup > 0:
-        s.run(1, a.warmup, a0, a1)
-    s.sync()
-    st0 = s.stats()
-    t0 = time.perf_counter()
-    s.run(a.warmup + 1, a.steps, a0, a1)
-    s.sync()
-    t1 = time.perf_counter()
-    st1 = s.stats()
-    wall = t1 - t0
-    iters = st1["krylov_iters"] - st0["krylov_iters"]
-    N, H, nnz = prob.N, prob.H, st1["matrix_nnz"]
-    kernel_s = st1["kernel_ms"] / 1e3
-    nbytes = implicit_bytes_per_run(N, H, nnz, a.steps, iters)
-    achieved = nbytes / kernel_s / 1e9
+    first = a.warmup + 1
+    forms = {}
+    ref = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP64), first, a.steps, a0, a1)
+    forms["fp64_tree_direct"] = ref
+    forms["mixed_tree_ir"] = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP32_REFINE,
+                                                     refine_tol=1e-10), first, a.steps, a0, a1)
+    if not a.no_plain:
+        plain = run_form(nh, prob, dict(krylov=nh.BICGSTAB, precision=nh.FP64, tol=1e-10,
+                                        tol_mode=nh.TOL_RELATIVE), first, a.plain_steps, a0, a1)
+        pref = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP64), first,
+                        a.plain_steps, a0, a1)
+        plain["ref_v"] = pref["v"]
+        forms["fp64_plain_bicgstab"] = plain
+    mask = np.ones(N, bool)
+    mask[[0, N - 1]] = False
+    summary = {}
+    for name, f in forms.items():
+        steps = a.plain_steps if name == "fp64_plain_bicgstab" else a.steps
+        rv = f.get("ref_v", ref["v"])
+        err_mv = float(np.max(np.abs(f["v"] - rv)[mask])) * 1e3
+        if name == "fp64_plain_bicgstab":
+            nbytes = bicgstab_bytes(N, H, f["nnz"], steps, f["inner"])
+        else:
+            nbytes = tree_bytes(N, H, f["nnz"], steps, f["inner"], 8 if name.startswith("fp64")
+                                else 4)
+        summary[name] = {"compartment_steps_per_s": N * steps / f["wall"],
+                         "ms_per_step": f["wall"] / steps * 1e3, "steps": steps,
+                         "max_abs_dV_mV_vs_fp64": err_mv, "inner_per_step": f["inner"] / steps,
+                         "kernel_ms": f["kernel_ms"], "algorithmic_bytes": nbytes,
+                         "workgroups": f["blocks"]}
+    ok = {k: v for k, v in summary.items() if v["max_abs_dV_mV_vs_fp64"] <= TOL_MV}
+    best = max(ok, key=lambda k: ok[k]["compartment_steps_per_s"])
+    b = summary[best]
+    kernel = {"fp64_tree_direct": "k_implicit_tree<double>",
+              "mixed_tree_ir": "k_implicit_tree<float>",
+              "fp64_plain_bicgstab": "k_implicit_bicgstab_fp64"}[best]
+    achieved = b["algorithmic_bytes"] / (b["kernel_ms"] / 1e3) / 1e9
     out = {
         "metric": METRIC,
-        "value": N * a.steps / wall,
+        "value": b["compartment_steps_per_s"],
         "unit": "compartment-steps/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": b["steps"],
         "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if best.startswith("fp64") else "f32+f64-refine",
         "data": "synthetic",
         "config": {"workload": "myelinated axon, 200 HH nodes (1 um) + 200 x 1 mm passive "
                                "internodes (100 x 10 um), d=10 um, N=20202, backward Euler "
                                "dt=10 us, stimulus 10 nA x 0.5 ms",
-                   "N": N, "hh_elements": H, "nnz": nnz, "solver": "BiCGSTAB",
-                   "tolerance": f"relative {a.tol:g}", "form": "fp64-plain",
-                   "num_blocks": st1["num_blocks"]},
-        "krylov_iters_per_step": iters / a.steps,
+                   "N": N, "hh_elements": H, "nnz": forms[best]["nnz"], "form": best,
+                   "tolerance": "relative 1e-10 (refinement and BiCGSTAB)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_implicit_bicgstab_fp64", "launches": st1["kernel_launches"],
-                     "kernel_ms": st1["kernel_ms"], "algorithmic_bytes": nbytes},
-        "accuracy": {"form": "fp64-plain", "max_abs_dV_mV_vs_fp64": 0.0,
-                     "note": "plain FP64 form; parity vs the FP64 oracle is in tests/"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                     "launches": 1, "kernel_ms": b["kernel_ms"],
+                     "algorithmic_bytes": b["algorithmic_bytes"]},
+        "forms": summary,
     }
-    s.close()
+    if "fp64_plain_bicgstab" in summary:
+        out["speedup_vs_fp64_plain"] = (b["compartment_steps_per_s"] /
+                                        summary["fp64_plain_bicgstab"]["compartment_steps_per_s"])
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob)
     print(json.dumps(out), flush=True)

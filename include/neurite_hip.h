@@ -48,6 +48,10 @@ extern "C" {
 
 #define NRT_BICGSTAB 0   /* solverIm-gpu.cpp:725-843 */
 #define NRT_GMRES    1   /* solverIm-cpu.cpp:192-272 (restarted GMRES)       */
+#define NRT_DIRECT   2   /* exact tree partition solve (no Krylov)           */
+
+#define NRT_PRECOND_NONE 0  /* reference: unpreconditioned                    */
+#define NRT_PRECOND_TREE 1  /* exact tree solve as the preconditioner         */
 
 #define NRT_TOL_ABSOLUTE 0  /* ||b - A x||_2 <= tol      (reference: 1e-14)  */
 #define NRT_TOL_RELATIVE 1  /* ||b - A x||_2 <= tol*||b||_2                  */
@@ -167,7 +171,9 @@ typedef struct nrt_options {
   double tol;           /* Krylov tolerance (see tol_mode)                      */
   double refine_tol;    /* FP32_REFINE: outer FP64 residual tolerance (relative)*/
   int32_t refine_max;   /* FP32_REFINE: maximum refinement sweeps               */
-  int32_t reserved[7];
+  int32_t precond;      /* NRT_PRECOND_NONE / NRT_PRECOND_TREE                  */
+  int32_t chunk;        /* tree solve: chain chunk length (0 -> autotuned)      */
+  int32_t reserved[5];
 } nrt_options;
 
 typedef struct nrt_stats {

@@ -1,0 +1,92 @@
+"""The tree partition solve (TPS) plan and routines, run on the host with the same
+per-chunk / per-node code the device kernels execute, against the oracle's
+sequential exact elimination and dense numpy solves (CPU only)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import neurite_hip as nh
+import workloads as wl
+from common import load_golden, problem_from_golden
+from test_oracle_math import _problem, _setup
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+def tps_host(rp, col, val, rhs, chunk):
+    L = nh.lib()
+    f = L.nrt_test_tree_solve_host
+    f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, _dp, C.POINTER(C.c_int),
+                  C.POINTER(C.c_int)]
+    rp = np.ascontiguousarray(rp, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    x = np.empty(len(rhs))
+    lv, h = C.c_int(0), C.c_int(0)
+    st = f(len(rhs), rp.ctypes.data_as(_ip), col.ctypes.data_as(_ip), val.ctypes.data_as(_dp),
+           rhs.ctypes.data_as(_dp), chunk, x.ctypes.data_as(_dp), C.byref(lv), C.byref(h))
+    assert st == 0
+    return x, lv.value, h.value
+
+
+def _system(orc, p):
+    im = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    im.update(True)
+    return im.matrix()
+
+
+@pytest.mark.parametrize("name", ["ex3_hh_y", "ex4_ct_cable", "ex5_ct_y", "ex6_ct_tree",
+                                  "ex8_neuron"])
+@pytest.mark.parametrize("chunk", [3, 4, 7, 32, 64])
+def test_tps_golden_systems(name, chunk, orc):
+    g = load_golden(name)
+    p = problem_from_golden(g, dT=g["dT_crit"] * 10)
+    rp, col, val, rhs = _system(orc, p)
+    x, lv, h = tps_host(rp, col, val, rhs, chunk)
+    ref = orc.tree_solve(rp, col, val, rhs)
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("chunk", [4, 16, 64])
+def test_tps_long_chain_and_tree(chunk, orc):
+    for geom, dT in [(wl.myelinated_axon()[0], 1e-5),
+                     (wl.random_tree(wl.seed_for(3), depth=8)[0], 2.5e-5)]:
+        s, bufs, H = _setup(orc, geom)
+        p = _problem(geom, s, bufs, H, dT)
+        rp, col, val, rhs = _system(orc, p)
+        x, lv, h = tps_host(rp, col, val, rhs, chunk)
+        ref = orc.tree_solve(rp, col, val, rhs)
+        assert np.max(np.abs(x - ref)) <= 1e-11 * np.max(np.abs(ref))
+        # the compressed forest must be shallow: at most 2 nodes per chain remain
+        assert lv >= 1
+
+
+def test_tps_random_forest_dense(orc):
+    """Random diagonally dominant forests (parent < child), against numpy."""
+    rng = np.random.default_rng(7)
+    for trial in range(20):
+        N = int(rng.integers(2, 400))
+        par = [-1] + [int(rng.integers(-1, i)) if rng.random() < 0.3 else i - 1
+                      for i in range(1, N)]
+        A = np.zeros((N, N))
+        for i in range(N):
+            A[i, i] = 4.0 + rng.random()
+            if par[i] >= 0:
+                A[i, par[i]] = -rng.random()
+                A[par[i], i] = -rng.random()
+        # make it diagonally dominant
+        for i in range(N):
+            A[i, i] = 1.0 + np.sum(np.abs(A[i])) - abs(A[i, i]) + A[i, i]
+        rp, col, val = [0], [], []
+        for i in range(N):
+            for j in np.nonzero(A[i])[0]:
+                col.append(j)
+                val.append(A[i, j])
+            rp.append(len(col))
+        b = rng.standard_normal(N)
+        for chunk in (3, 5, 32):
+            x, _, _ = tps_host(rp, col, val, b, chunk)
+            np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-11, atol=1e-12)
