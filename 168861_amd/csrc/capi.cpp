@@ -189,6 +189,9 @@ int nrt_create(const nrt_problem* p, const nrt_options* opt, nrt_handle** out) {
                 NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB or NRT_DIRECT");
     NRT_REQUIRE(o.precond == NRT_PRECOND_NONE, NRT_ERR_UNSUPPORTED,
                 "preconditioned Krylov is not built; use NRT_DIRECT for the tree solve");
+    NRT_REQUIRE(o.chunk >= 0 && o.chunk <= kMaxChunk && o.chunk_hi >= 0 &&
+                    o.chunk_hi <= kMaxChunk,
+                NRT_ERR_INVALID, "tree chunk lengths must be in [0, 4096]");
     HostProblem q = load_problem(p);
     auto* h = new nrt_handle;
     try {

@@ -52,6 +52,7 @@ struct ImpHH {
   const int32_t* type;
   const int32_t* mat;
   const int32_t* dl;
+  const int32_t* row;  // element (row) of each HH entry
 };
 
 struct ImpStats {
@@ -70,6 +71,7 @@ struct ImpStats {
 
 struct ImpDev {
   int N;
+  int H;
   int nblocks;
   int rpb;  // rows per block
   double dT;
@@ -119,7 +121,7 @@ struct Implicit {
   DBuf<uint8_t> branching;
   DBuf<uint32_t> code;
   DBuf<double> m, h, n, E_Na, E_K, E_L, g_Na, g_K, G_L, ls_Na, ls_K, rest;
-  DBuf<int32_t> hh_type, hh_mat, hh_dl;
+  DBuf<int32_t> hh_type, hh_mat, hh_dl, hh_row;
   DBuf<double> x, b, r0, r, p[2], v[2], s, t;
   DBuf<double> part;
   DBuf<unsigned> bar;
@@ -325,11 +327,13 @@ __device__ __forceinline__ void imp_gate(const ImpHH& hd, int i, double v, doubl
   K = gna * hd.E_Na[i] + gk * hd.E_K[i] + gl * hd.E_L[i];
 }
 
-// Update_Mat (solverIm-gpu.cpp:103-156) for the HH row of element iv.
-__device__ __forceinline__ void imp_update_row(const ImpDev& d, int hh, int iv, double W) {
+// Update_Mat (solverIm-gpu.cpp:103-156) for the HH row of element iv. Returns
+// false for identity rows (left as they are), else true with the new diagonal.
+__device__ __forceinline__ bool imp_update_row(const ImpDev& d, int hh, int iv, double W,
+                                               double& diag) {
   const int type = d.hh.type[hh];
   // identity rows (ghosts, sealed ends) keep their 1 and read no neighbour
-  if (type == RT_OTHER || type == RT_TERM) return;
+  if (type == RT_OTHER || type == RT_TERM) return false;
   const int im = d.hh.mat[hh];
   double* v = d.val;
   const double alpha = -1 / d.rdx[iv];
@@ -340,33 +344,38 @@ __device__ __forceinline__ void imp_update_row(const ImpDev& d, int hh, int iv, 
   double betaL;
   switch (type) {
     case RT_NEXT_TERM:
+      diag = beta + delta;
       v[im] = alpha;
-      v[im + 1] = beta + delta;
-      break;
+      v[im + 1] = diag;
+      return true;
     case RT_BRANCH:
       betaL = 1 / d.rdx[d.hh.dl[hh]];
+      diag = beta + betaL;
       v[im] = alpha;
-      v[im + 1] = beta + betaL;
+      v[im + 1] = diag;
       v[im + 2] = delta;
       v[im + 3] = -betaL;
-      break;
+      return true;
     case RT_NORMAL:
+      diag = beta;
       v[im] = alpha;
-      v[im + 1] = beta;
+      v[im + 1] = diag;
       v[im + 2] = delta;
-      break;
+      return true;
     case RT_FIRST_NORMAL:
-      v[im] = alpha + beta;
+      diag = alpha + beta;
+      v[im] = diag;
       v[im + 1] = delta;
-      break;
+      return true;
     case RT_FIRST_BRANCH:
       betaL = 1 / d.rdx[d.hh.dl[hh]];
-      v[im] = alpha + beta + betaL;
+      diag = alpha + beta + betaL;
+      v[im] = diag;
       v[im + 1] = delta;
       v[im + 2] = -betaL;
-      break;
+      return true;
     default:
-      break;
+      return false;
   }
 }
 
@@ -409,7 +418,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
         const int hh = (int)(c & kCodeIdx);
         double W, K;
         imp_gate(d.hh, hh, v, d.dt_ms, W, K);
-        imp_update_row(d, hh, i, W);
+        double diag;
+        (void)imp_update_row(d, hh, i, W, diag);
         kd = K * d.dxv[i];
       } else {
         kd = d.kdx[i];
@@ -611,7 +621,7 @@ __device__ __forceinline__ void tsync(const ImpDev& d) {
 }
 
 template <class T, bool GRID>
-__device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w) {
+__device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w, bool acc) {
   const int wid = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)threadIdx.x;
   const int nw = GRID ? (int)(gridDim.x * blockDim.x) : (int)blockDim.x;
   for (int lv = 0; lv < P.levels; lv++) {
@@ -632,34 +642,88 @@ __device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>
     }
   }
   if (GRID) tsync<GRID>(d);
-  for (int lv = P.levels - 1; lv >= 0; lv--) {
-    const int n = P.nslots[lv];
-    for (int v = wid; v < n; v += nw) tps_back(P, w, lv, v);
+  for (int lv = P.levels - 1; lv >= 1; lv--) {
+    const int c1 = P.chunk_off[lv + 1];
+    for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw)
+      tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr);
     tsync<GRID>(d);
   }
+  // level 0 lands in the FP64 potential directly (x = solution, or x += correction)
+  if (P.levels == 0) {
+    const int n0 = P.nslots[0];
+    for (int v = wid; v < n0; v += nw) {
+      if (acc) tps_final_out<OUT_ADD>(P, w, v, d.x);
+      else tps_final_out<OUT_ASSIGN>(P, w, v, d.x);
+    }
+  } else {
+    const int c1 = P.chunk_off[1];
+    for (int ch = wid; ch < c1; ch += nw) {
+      if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, ch, d.x);
+      else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, d.x);
+    }
+  }
+  tsync<GRID>(d);
 }
 
-// Update phase of a step for rows [r0, r1) with stride `stride` (as in the
-// BiCGSTAB kernel): gating, HH row refresh, right-hand side.
-__device__ __forceinline__ void imp_update_rows(const ImpDev& d, int r0, int r1, int stride,
-                                                bool active) {
-  for (int i = r0 + (int)threadIdx.x; i < r1; i += stride) {
-    const double v = d.x[i];
-    const uint32_t c = d.code[i];
-    double kd;
-    if (c & kCodeHH) {
-      const int hh = (int)(c & kCodeIdx);
-      double W, K;
-      imp_gate(d.hh, hh, v, d.dt_ms, W, K);
-      imp_update_row(d, hh, i, W);
-      kd = K * d.dxv[i];
-    } else {
-      kd = d.kdx[i];
+// Update phase of a tree-solved step. HH elements are spread one per thread over
+// the whole launch (their gating and row refresh are long dependent chains, so
+// they must not queue up inside one wave); passive rows follow in the row
+// distribution of the launch. Each writes its right-hand side straight into the
+// level-0 tree system (direct) or into b (refinement, whose FP64 residual reads
+// it); an HH row also refreshes its CSR values and the level-0 diagonal.
+template <class T, bool GRID>
+__device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& t, int r0, int r1,
+                                            bool active) {
+  const int tid = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)threadIdx.x;
+  const int nt = GRID ? (int)(gridDim.x * blockDim.x) : (int)blockDim.x;
+  const double* __restrict__ xv = d.x;
+  const uint32_t* __restrict__ code = d.code;
+  const double* __restrict__ cmdx = d.cmdx;
+  const double* __restrict__ kdx = d.kdx;
+  const double* __restrict__ input = d.input;
+  const int32_t* __restrict__ slot0 = t.P.slot0;
+  T* __restrict__ rt = t.w.r;
+  double* __restrict__ b = d.b;
+  const double dT = d.dT;
+  for (int hh = tid; hh < d.H; hh += nt) {
+    const int i = d.hh.row[hh];
+    const double v = xv[i];
+    double W, K, diag;
+    imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+    if (imp_update_row(d, hh, i, W, diag)) t.w.d[slot0[i]] = (T)diag;
+    double bi = cmdx[i] * v / dT + K * d.dxv[i];
+    if (active) bi = bi + input[i];
+    if (t.direct) rt[slot0[i]] = (T)bi;
+    else b[i] = bi;
+  }
+  // passive rows, loads of kTpsBatch rows per thread issued together
+  const int st = (int)blockDim.x;
+  for (int i0 = r0 + (int)threadIdx.x; i0 < r1; i0 += kTpsBatch * st) {
+    uint32_t bc[kTpsBatch];
+    int bs[kTpsBatch];
+    double bv[kTpsBatch], bm[kTpsBatch], bk[kTpsBatch], bin[kTpsBatch];
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int i = i0 + j * st;
+      if (i < r1) {
+        bc[j] = code[i];
+        bv[j] = xv[i];
+        bm[j] = cmdx[i];
+        bk[j] = kdx[i];
+        bin[j] = active ? input[i] : 0.0;
+        bs[j] = slot0[i];
+      }
     }
-    double bi = d.cmdx[i] * v / d.dT + kd;
-    if (active) bi = bi + d.input[i];
-    d.b[i] = bi;
-    if (d.x0_mode == NRT_X0_RHS) d.x[i] = bi;
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int i = i0 + j * st;
+      if (i < r1 && !(bc[j] & kCodeHH)) {
+        double bi = bm[j] * bv[j] / dT + bk[j];
+        if (active) bi = bi + bin[j];
+        if (t.direct) rt[bs[j]] = (T)bi;
+        else b[i] = bi;
+      }
+    }
   }
 }
 
@@ -673,52 +737,44 @@ __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree
   const double* val = d.val;
   const int32_t* __restrict__ rowptr = d.rowptr;
   const int32_t* __restrict__ col = d.col;
+  const int32_t* __restrict__ slot0 = t.P.slot0;
   long long gl = 0, ksamp = 0;
   int slot = 0;
   const double lim2 = t.refine_tol * t.refine_tol;
+  // Level-0 tree system from the CSR. Off-diagonals never change; the diagonal of
+  // HH rows is refreshed by the update phase of every step, by the thread that
+  // owns the HH entry (not the row), hence the barrier before the first update.
+  for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+    const int si = slot0[i];
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      const int c = col[k];
+      const T v = (T)val[k];
+      if (c == i) t.w.d[si] = v;
+      else if (c < i) t.w.up[si] = v;
+      else t.w.dn[slot0[c]] = v;
+    }
+  }
+  tsync<GRID>(d);
   for (long long step = d.first; step < d.first + d.num; step++) {
     const bool active = step >= d.a0 && step <= d.a1;
-    imp_update_rows(d, r_begin, r_end, bs, active);
+    tree_update<T, GRID>(d, t, r_begin, r_end, active);
     tsync<GRID>(d);
     int sweeps = 0, status = NRT_OK;
     double rr = 0, bb = 0;
     bool ok = true;
-    const int32_t* __restrict__ slot0 = t.P.slot0;
     if (t.direct) {
-      for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
-        const int si = slot0[i];
-        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-          const int c = col[k];
-          const T v = (T)val[k];
-          if (c == i) t.w.d[si] = v;
-          else if (c < i) t.w.up[si] = v;
-          else t.w.dn[slot0[c]] = v;
-        }
-        t.w.r[si] = (T)d.b[i];
-      }
-      tsync<GRID>(d);
-      tps_run<T, GRID>(d, t.P, t.w);
-      for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs)
-        d.x[i] = (double)t.w.x[slot0[i]];
+      tps_run<T, GRID>(d, t.P, t.w, false);
       sweeps = 1;
     } else {
       while (true) {
-        // FP64 residual r = b - A x; the FP32 tree system is refreshed from the CSR
+        // FP64 residual r = b - A x into the level-0 right-hand side
         double red[2] = {0.0, 0.0};
         for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
-          const int si = slot0[i];
           double ax = 0;
-          for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-            const int c = col[k];
-            const double v = val[k];
-            ax = ax + v * d.x[c];
-            if (c == i) t.w.d[si] = (T)v;
-            else if (c < i) t.w.up[si] = (T)v;
-            else t.w.dn[slot0[c]] = (T)v;
-          }
+          for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * d.x[col[k]];
           const double bi = d.b[i];
           const double ri = bi - ax;
-          t.w.r[si] = (T)ri;
+          t.w.r[slot0[i]] = (T)ri;
           red[0] = red[0] + ri * ri;
           red[1] = red[1] + bi * bi;
         }
@@ -736,10 +792,7 @@ __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree
           status = NRT_ERR_NOT_CONVERGED;
           break;
         }
-        tps_run<T, GRID>(d, t.P, t.w);
-        for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs)
-          d.x[i] = d.x[i] + (double)t.w.x[slot0[i]];
-        tsync<GRID>(d);
+        tps_run<T, GRID>(d, t.P, t.w, true);
         sweeps++;
       }
     }
@@ -766,8 +819,6 @@ __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree
         d.trace[ksamp] = d.x[d.trace_elem];
       ksamp++;
     }
-    // x of a row is written by the thread that owns it and read back only by it
-    // in the next update phase; other rows' x are read after the next barrier
   }
 }
 
@@ -853,6 +904,7 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
     e->hh_type.alloc(H);
     e->hh_mat.alloc(H);
     e->hh_dl.alloc(H);
+    e->hh_row.upload(p.hh_elem, s);
     // device assembly
     e->len.alloc(N);
     e->rowptr.alloc(N + 1);
@@ -900,8 +952,29 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
           NRT_REQUIRE(hcol[k] <= i || par[hcol[k]] == i, NRT_ERR_TOPOLOGY,
                       "tree solve: row " + std::to_string(i) + " couples to row " +
                           std::to_string(hcol[k]) + ", which does not couple back");
-      const int chunk = o.chunk > 0 ? o.chunk : 32;
-      e->plan_h = build_tree_plan(par, chunk);
+      // Workgroups: every reduction level costs a grid barrier, so a mid-size tree
+      // runs best on a few workgroups (measured on a 20,202-row cable: 1 -> 104 us,
+      // 4 -> 85 us, 32 -> 114 us per step) and only large trees spread wide.
+      int nb = o.num_blocks;
+      if (nb == 0 && N <= (1 << 17)) nb = N >= (1 << 14) ? 4 : 1;
+      e->grid_mode = nb != 1;
+      if (e->grid_mode) {
+        const void* k = e->method == 1 ? (const void*)k_implicit_tree<double, true>
+                                       : (const void*)k_implicit_tree<float, true>;
+        e->nblocks = o.num_blocks > 0 || nb <= 0 ? choose_blocks(hnd, N, k) : nb;
+        e->threads = kImpBlock;
+      } else {
+        e->nblocks = 1;
+        e->threads = kTreeBlock;
+      }
+      // Level 0: one chunk per worker covers an unbranched cable in one round;
+      // reduced levels: short chunks (the serial sweep is the latency, and each
+      // level shrinks a chain by C/2).
+      const int workers = e->nblocks * e->threads;
+      const int auto0 = std::min(64, std::max(8, (N + workers - 1) / workers));
+      const int chunk = o.chunk > 0 ? o.chunk : auto0;
+      const int chunk_hi = o.chunk_hi > 0 ? o.chunk_hi : (o.chunk > 0 ? o.chunk : 8);
+      e->plan_h = build_tree_plan(par, chunk, chunk_hi);
       const TreePlanHost& P = e->plan_h;
       // one device buffer holding every plan array
       std::vector<int32_t> flat;
@@ -916,16 +989,6 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
         e->tw32.alloc(8 * T);
         NRT_HIP(hipMemsetAsync(e->tw32.p, 0, sizeof(float) * 8 * T, s));
       }
-      e->grid_mode = o.num_blocks > 1 || (o.num_blocks == 0 && N > (1 << 17));
-    }
-    if (e->method != 0 && !e->grid_mode) {
-      e->nblocks = 1;
-      e->threads = kTreeBlock;
-    } else if (e->method != 0) {
-      const void* k = e->method == 1 ? (const void*)k_implicit_tree<double, true>
-                                     : (const void*)k_implicit_tree<float, true>;
-      e->nblocks = choose_blocks(hnd, N, k);
-      e->threads = kImpBlock;
     } else {
       e->nblocks = choose_blocks(hnd, N);
       e->threads = kImpBlock;
@@ -951,6 +1014,7 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   const nrt_options& o = hnd->opt;
   ImpDev d{};
   d.N = e->N;
+  d.H = e->H;
   d.nblocks = e->nblocks;
   d.rpb = grid_for(e->N, e->nblocks);
   d.dT = e->dT;
@@ -965,7 +1029,8 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.dxv = e->dxv.p;
   d.code = e->code.p;
   d.hh = ImpHH{e->m.p, e->h.p, e->n.p, e->E_Na.p, e->E_K.p, e->E_L.p, e->g_Na.p, e->g_K.p,
-               e->G_L.p, e->ls_Na.p, e->ls_K.p, e->rest.p, e->hh_type.p, e->hh_mat.p, e->hh_dl.p};
+               e->G_L.p, e->ls_Na.p, e->ls_K.p, e->rest.p, e->hh_type.p, e->hh_mat.p, e->hh_dl.p,
+               e->hh_row.p};
   d.x = e->x.p;
   d.b = e->b.p;
   d.r0 = e->r0.p;

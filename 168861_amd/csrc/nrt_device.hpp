@@ -111,9 +111,11 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
 // every block drains; callers test *err after each barrier.
 __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned nblocks, unsigned* err,
                                              unsigned long long spin_limit) {
-  __threadfence();
+  // The workgroup barrier orders every wave's stores before thread 0's agent-scope
+  // release (fences are cumulative), so one fence per workgroup publishes them.
   __syncthreads();
   if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned g = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned prev =
         __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);

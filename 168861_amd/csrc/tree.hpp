@@ -15,7 +15,8 @@
 //   * the final compressed forest is eliminated by height (a node pulls the
 //     contributions of its children, so siblings never write the same parent) and
 //     back-substituted by height;
-//   * eliminated rows are recovered in parallel as x = alpha + beta*x_s + gamma*x_e.
+//   * eliminated rows are recovered in parallel as x = alpha + beta*x_s + gamma*x_e
+//     (one worker per chunk again, from the solution of the chunk's two ends).
 // The plan (levels, chunks, heights) depends only on the topology and is built
 // once on the host; only values flow per solve.
 //
@@ -35,16 +36,17 @@ namespace nrt {
 
 // Host-side plan.
 struct TreePlanHost {
-  int C = 32;                       // chunk length
+  int C = 32;                       // chunk length of level 0
+  int C_hi = 32;                    // chunk length of the levels above
   int levels = 0;                   // reduction levels (L)
   std::vector<int32_t> nslots;      // slots per level, L+1 entries (last = final forest)
   std::vector<int32_t> nchunk;      // chunks per reduction level (L entries, +1 padding)
   std::vector<int32_t> node_off;    // offset of each level in the flat arrays, L+2
   std::vector<int32_t> next;        // flat: slot in level+1 of a surviving node (-1 otherwise)
-  std::vector<int32_t> es, ee;      // flat: level+1 slots of the chunk ends of an eliminated node
   std::vector<int32_t> chunk_off;   // per level, L+1 entries, into chunk_len
   std::vector<int32_t> chunk_len;
   std::vector<int32_t> slot0;       // level-0 slot of each row (N entries)
+  std::vector<int32_t> row0;        // row of each level-0 slot (-1 for padding)
   std::vector<int32_t> fin_par;     // final forest: parent (final-local) or -1
   std::vector<int32_t> fin_order;   // final-forest nodes ordered by height
   std::vector<int32_t> fin_hoff;    // height boundaries into fin_order, H+1 entries
@@ -53,8 +55,9 @@ struct TreePlanHost {
   int64_t total_nodes() const { return node_off.empty() ? 0 : node_off.back(); }
 };
 
-// Builds the plan for a forest given by parent indices (par[i] < i or -1).
-TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk);
+// Builds the plan for a forest given by parent indices (par[i] < i or -1), with
+// chains cut into chunks of `chunk` rows at level 0 and `chunk_hi` rows above.
+TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chunk_hi);
 
 // Parent of each row in an assembled CSR (the single column < row, or -1).
 std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* col);
@@ -71,11 +74,10 @@ struct TreePlanDev {
   const int32_t* nchunk;
   const int32_t* node_off;
   const int32_t* next;
-  const int32_t* es;
-  const int32_t* ee;
   const int32_t* chunk_off;
   const int32_t* chunk_len;
   const int32_t* slot0;
+  const int32_t* row0;
   const int32_t* fin_par;
   const int32_t* fin_order;
   const int32_t* fin_hoff;
@@ -105,6 +107,11 @@ struct TreeWork {
 #define NRT_HD inline
 #endif
 
+// Rows of a chunk whose loads are issued together: the sweeps are recurrences, but
+// their operands do not depend on them, so a batch of rows is fetched at once and
+// the latency is paid once per batch rather than once per row.
+constexpr int kTpsBatch = 8;
+
 // Eliminate the interior of chunk `ch` of level `lv` and write its ends to lv+1.
 template <class T>
 NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
@@ -129,74 +136,106 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
     }
     return;
   }
-  // forward sweep over interior rows k = 1..m-2: d'_k x_k + dn_{k+1} x_{k+1} + g_k x_0 = r'_k
-  int g = base + NC;
-  T dp = d[g], gk = up[g], rp = r[g];
-  ca[g] = dp;
-  cb[g] = gk;
-  cg[g] = rp;
-  // values of the next row are loaded one step ahead
-  T nu = up[g + NC], nd = d[g + NC], ndn = dn[g + NC], nr = r[g + NC];
-  for (int k = 2; k <= m - 2; k++) {
-    g += NC;
-    const T u0 = nu, d0 = nd, dn0 = ndn, r0 = nr;
-    nu = up[g + NC];
-    nd = d[g + NC];
-    ndn = dn[g + NC];
-    nr = r[g + NC];
-    const T f = u0 / dp;
-    dp = d0 - f * dn0;
-    gk = -f * gk;
-    rp = r0 - f * rp;
-    ca[g] = dp;
-    cb[g] = gk;
-    cg[g] = rp;
+  // start row s = 0 and first interior row 1
+  const T d0 = d[base], u0 = up[base], n0 = dn[base], r0 = r[base];
+  const int ts = nbase + P.next[base];
+  const int te = nbase + P.next[base + (m - 1) * NC];
+  const int g1 = base + NC;
+  const T dn1 = dn[g1];
+  // forward sweep over rows k = 1..m-1:
+  //   d'_k x_k + dn_{k+1} x_{k+1} + g_k x_s = r'_k,
+  // stored as (ca, cb, cg) = (d', g, r') for interior rows; row m-1 becomes the
+  // end row of the reduced system: G_e x_s + D_e x_e + (children) = R_e.
+  T dp = d[g1], gk = up[g1], rp = r[g1];
+  ca[g1] = dp;
+  cb[g1] = gk;
+  cg[g1] = rp;
+  T pdp = dp, pgk = gk, prp = rp;  // values of the row before the last one
+  T dnl = dn1;                       // dn of the last row
+  for (int k0 = 2; k0 <= m - 1; k0 += kTpsBatch) {
+    T bu[kTpsBatch], bd[kTpsBatch], bn[kTpsBatch], br[kTpsBatch];
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 + j;
+      if (k <= m - 1) {
+        const int g = base + k * NC;
+        bu[j] = up[g];
+        bd[j] = d[g];
+        bn[j] = dn[g];
+        br[j] = r[g];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 + j;
+      if (k <= m - 1) {
+        pdp = dp;
+        pgk = gk;
+        prp = rp;
+        const T f = bu[j] / dp;
+        dp = bd[j] - f * bn[j];
+        gk = -f * gk;
+        rp = br[j] - f * rp;
+        dnl = bn[j];
+        if (k <= m - 2) {
+          const int g = base + k * NC;
+          ca[g] = dp;
+          cb[g] = gk;
+          cg[g] = rp;
+        }
+      }
+    }
   }
-  // end row e = m-1 eliminates x_{m-2}: G_e x_s + D_e x_e + children = R_e
-  const int ge = base + (m - 1) * NC;
-  const int te = nbase + P.next[ge];
-  {
-    const T f = nu / dp;  // nu, nd, ndn, nr hold row m-1 now
-    d[te] = nd - f * ndn;
-    up[te] = -f * gk;
-    r[te] = nr - f * rp;
-  }
-  // backward sweep: x_k = alpha_k + beta_k x_s + gamma_k x_e
+  d[te] = dp;
+  up[te] = gk;
+  r[te] = rp;
+  // backward sweep: x_k = alpha_k + beta_k x_s + gamma_k x_e, into (ca, cb, cg)
   T al, be, ga;
   {
     const int g2 = base + (m - 2) * NC;
-    const T inv = T(1) / ca[g2];
-    al = cg[g2] * inv;
-    be = -cb[g2] * inv;
-    ga = -ndn * inv;
+    const T inv = T(1) / pdp;
+    al = prp * inv;
+    be = -pgk * inv;
+    ga = -dnl * inv;
     ca[g2] = al;
     cb[g2] = be;
     cg[g2] = ga;
   }
-  T dnx = dn[base + (m - 2) * NC];
-  for (int k = m - 3; k >= 1; k--) {
-    const int g2 = base + k * NC;
-    const T inv = T(1) / ca[g2];
-    const T r2 = cg[g2], gg = cb[g2];
-    const T dcur = dnx;
-    dnx = dn[g2];
-    al = (r2 - dcur * al) * inv;
-    be = (-gg - dcur * be) * inv;
-    ga = -dcur * ga * inv;
-    ca[g2] = al;
-    cb[g2] = be;
-    cg[g2] = ga;
+  for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatch) {
+    T bp[kTpsBatch], bg[kTpsBatch], brr[kTpsBatch], bn[kTpsBatch];
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 - j;
+      if (k >= 1) {
+        const int g = base + k * NC;
+        bp[j] = ca[g];
+        bg[j] = cb[g];
+        brr[j] = cg[g];
+        bn[j] = dn[g + NC];  // coupling k -> k+1
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 - j;
+      if (k >= 1) {
+        const T inv = T(1) / bp[j];
+        const T dcur = bn[j];
+        al = (brr[j] - dcur * al) * inv;
+        be = (-bg[j] - dcur * be) * inv;
+        ga = -dcur * ga * inv;
+        const int g = base + k * NC;
+        ca[g] = al;
+        cb[g] = be;
+        cg[g] = ga;
+      }
+    }
   }
   // start row s: up_s x_P + (d_s + dn_1 beta_1) x_s + dn_1 gamma_1 x_e = r_s - dn_1 alpha_1
-  {
-    const T dn1 = dn[base + NC];
-    const int ts = nbase + P.next[base];
-    d[ts] = d[base] + dn1 * be;
-    up[ts] = up[base];
-    dn[ts] = dn[base];
-    r[ts] = r[base] - dn1 * al;
-    dn[te] = dn1 * ga;  // s -> e coupling
-  }
+  d[ts] = d0 + dn1 * be;
+  up[ts] = u0;
+  dn[ts] = n0;
+  r[ts] = r0 - dn1 * al;
+  dn[te] = dn1 * ga;  // s -> e coupling
 }
 
 // Final forest, elimination of node v (all of its children are final already).
@@ -223,21 +262,61 @@ NRT_HD void tps_final_down(const TreePlanDev& P, const TreeWork<T>& w, int v) {
   w.x[base + v] = acc / w.d[base + v];
 }
 
-// Back-substitution of slot v of level lv from level lv+1.
-template <class T>
-NRT_HD void tps_back(const TreePlanDev& P, const TreeWork<T>& w, int lv, int v) {
-  const int base = P.node_off[lv], nbase = P.node_off[lv + 1];
-  const int g = base + v;
-  const int nx = P.next[g];
-  if (nx >= 0) {
-    w.x[g] = w.x[nbase + nx];
+// Stores of the back-substitution: into the level's x (OUT_LEVEL), or at level 0
+// into the caller's FP64 vector by row, assigned or accumulated (a refinement).
+enum TpsOut { OUT_LEVEL = 0, OUT_ASSIGN = 1, OUT_ADD = 2 };
+
+template <int MODE, class T>
+NRT_HD void tps_put(const TreePlanDev& P, const TreeWork<T>& w, int g, T v, double* out) {
+  if (MODE == OUT_LEVEL) {
+    w.x[g] = v;
   } else {
-    const int s = P.es[g];
-    if (s < 0) return;  // padding slot
-    const T xs = w.x[nbase + s];
-    const T xe = w.x[nbase + P.ee[g]];
-    w.x[g] = w.ca[g] + w.cb[g] * xs + w.cg[g] * xe;
+    const int row = P.row0[g];  // level 0 starts at offset 0
+    out[row] = MODE == OUT_ADD ? out[row] + (double)v : (double)v;
   }
+}
+
+// Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1.
+template <int MODE, class T>
+NRT_HD void tps_back_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch,
+                           double* out) {
+  const int NC = P.nchunk[lv];
+  const int c = ch - P.chunk_off[lv];
+  const int m = P.chunk_len[ch];
+  const int base = P.node_off[lv] + c, nbase = P.node_off[lv + 1];
+  const T* __restrict__ xn = w.x + nbase;
+  const int ge = base + (m - 1) * NC;
+  const T xs = xn[P.next[base]];
+  const T xe = m >= 2 ? xn[P.next[ge]] : xs;
+  tps_put<MODE>(P, w, base, xs, out);
+  if (m >= 2) tps_put<MODE>(P, w, ge, xe, out);
+  const T* __restrict__ ca = w.ca;
+  const T* __restrict__ cb = w.cb;
+  const T* __restrict__ cg = w.cg;
+  for (int k0 = 1; k0 <= m - 2; k0 += kTpsBatch) {
+    T a[kTpsBatch], b[kTpsBatch], g[kTpsBatch];
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 + j;
+      if (k <= m - 2) {
+        const int gg = base + k * NC;
+        a[j] = ca[gg];
+        b[j] = cb[gg];
+        g[j] = cg[gg];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 + j;
+      if (k <= m - 2) tps_put<MODE>(P, w, base + k * NC, a[j] + b[j] * xs + g[j] * xe, out);
+    }
+  }
+}
+
+// Level 0 as the final forest itself (a system with no chain longer than 2).
+template <int MODE, class T>
+NRT_HD void tps_final_out(const TreePlanDev& P, const TreeWork<T>& w, int v, double* out) {
+  tps_put<MODE>(P, w, v, w.x[v], out);
 }
 
 }  // namespace nrt

@@ -26,15 +26,16 @@ namespace {
 struct LocalLevel {
   std::vector<int32_t> par;                 // level-local parents
   std::vector<int32_t> cs, cl;              // chunks (local start, length)
-  std::vector<int32_t> next, es, ee;        // local next-level index / chunk ends
+  std::vector<int32_t> next;                // local next-level index
   std::vector<int32_t> slot;                // local node -> slot
   int nchunk = 0, cmax = 0, nslots = 0;
 };
 }  // namespace
 
-TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk) {
+TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int chunk_hi) {
   TreePlanHost P;
   P.C = std::max(3, chunk);
+  P.C_hi = std::max(3, chunk_hi);
   const int N0 = (int)par0.size();
   for (int i = 0; i < N0; i++)
     if (par0[i] >= i) throw std::runtime_error("tree plan needs parent < child");
@@ -47,13 +48,14 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk) {
     std::vector<int32_t> nchild(n, 0);
     for (int v = 0; v < n; v++)
       if (par[v] >= 0) nchild[par[v]]++;
+    const int C = lv.empty() ? P.C : P.C_hi;
     bool reduces = false;
     int v = 0;
     while (v < n) {
       int b = v;
       while (b + 1 < n && par[b + 1] == b && nchild[b] == 1) b++;
-      for (int a = v; a <= b; a += P.C) {
-        const int m = std::min(P.C, b - a + 1);
+      for (int a = v; a <= b; a += C) {
+        const int m = std::min(C, b - a + 1);
         L.cs.push_back(a);
         L.cl.push_back(m);
         if (m >= 3) reduces = true;
@@ -67,17 +69,11 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk) {
       break;
     }
     L.next.assign(n, -1);
-    L.es.assign(n, -1);
-    L.ee.assign(n, -1);
     int cnt = 0;
     for (size_t k = 0; k < L.cs.size(); k++) {
       const int a = L.cs[k], m = L.cl[k];
       L.next[a] = cnt++;
       if (m >= 2) L.next[a + m - 1] = cnt++;
-      for (int j = a + 1; j < a + m - 1; j++) {
-        L.es[j] = a;
-        L.ee[j] = a + m - 1;
-      }
     }
     std::vector<int32_t> npar(cnt, -1);
     for (size_t k = 0; k < L.cs.size(); k++) {
@@ -117,8 +113,6 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk) {
   }
   // flat slot-indexed arrays
   P.next.assign(P.total_nodes(), -1);
-  P.es.assign(P.total_nodes(), -1);
-  P.ee.assign(P.total_nodes(), -1);
   P.chunk_off.push_back(0);
   for (int l = 0; l + 1 < nl; l++) {
     const LocalLevel& L = lv[l];
@@ -127,15 +121,13 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk) {
     for (size_t u = 0; u < L.par.size(); u++) {
       const int g = base + L.slot[u];
       if (L.next[u] >= 0) P.next[g] = Nx.slot[L.next[u]];
-      if (L.es[u] >= 0) {
-        P.es[g] = Nx.slot[L.next[L.es[u]]];
-        P.ee[g] = Nx.slot[L.next[L.ee[u]]];
-      }
     }
     P.chunk_len.insert(P.chunk_len.end(), L.cl.begin(), L.cl.end());
     P.chunk_off.push_back((int32_t)P.chunk_len.size());
   }
   P.slot0 = lv[0].slot;
+  P.row0.assign(P.nslots[0], -1);
+  for (int i = 0; i < N0; i++) P.row0[P.slot0[i]] = i;
   // final forest (local index == slot): heights, children, order
   const std::vector<int32_t>& fp = lv[nl - 1].par;
   const int n = (int)fp.size();
@@ -167,13 +159,14 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk) {
 // Device view over one flat int32 buffer holding every plan array; `flat`
 // receives the concatenation, the returned view has offsets relative to base.
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
-  const std::vector<int32_t>* parts[] = {&P.nslots, &P.nchunk, &P.node_off, &P.next,
-                                         &P.es, &P.ee, &P.chunk_off, &P.chunk_len,
-                                         &P.slot0, &P.fin_par, &P.fin_order, &P.fin_hoff,
-                                         &P.fin_cptr, &P.fin_child};
-  size_t off[14];
+  const std::vector<int32_t>* parts[] = {&P.nslots,   &P.nchunk,   &P.node_off, &P.next,
+                                         &P.chunk_off, &P.chunk_len, &P.slot0,   &P.row0,
+                                         &P.fin_par,  &P.fin_order, &P.fin_hoff, &P.fin_cptr,
+                                         &P.fin_child};
+  constexpr int kParts = sizeof(parts) / sizeof(parts[0]);
+  size_t off[kParts];
   size_t pos = 0;
-  for (int k = 0; k < 14; k++) {
+  for (int k = 0; k < kParts; k++) {
     off[k] = pos;
     if (flat) {
       flat->insert(flat->end(), parts[k]->begin(), parts[k]->end());
@@ -181,10 +174,10 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
     }
     pos += parts[k]->size() + 1;
   }
-  return TreePlanDev{P.levels,        P.heights,     base + off[0],  base + off[1],
-                     base + off[2],   base + off[3], base + off[4],  base + off[5],
-                     base + off[6],   base + off[7], base + off[8],  base + off[9],
-                     base + off[10],  base + off[11], base + off[12], base + off[13]};
+  return TreePlanDev{P.levels,       P.heights,      base + off[0],  base + off[1],
+                     base + off[2],  base + off[3],  base + off[4],  base + off[5],
+                     base + off[6],  base + off[7],  base + off[8],  base + off[9],
+                     base + off[10], base + off[11], base + off[12]};
 }
 
 }  // namespace nrt
@@ -197,10 +190,10 @@ using namespace nrt;
 // oracle's sequential elimination. No solver handle ever calls it.
 extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int32_t* col,
                                         const double* val, const double* rhs, int chunk,
-                                        double* x, int* levels, int* heights) {
+                                        int chunk_hi, double* x, int* levels, int* heights) {
   try {
     std::vector<int32_t> par = csr_parents(N, rowptr, col);
-    TreePlanHost H = build_tree_plan(par, chunk);
+    TreePlanHost H = build_tree_plan(par, chunk, chunk_hi > 0 ? chunk_hi : chunk);
     std::vector<int32_t> flat;
     TreePlanDev tmp = plan_view(H, nullptr, &flat);
     (void)tmp;
@@ -225,9 +218,15 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
       for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_up(P, w, H.fin_order[k]);
     for (int h = H.heights - 1; h >= 0; h--)
       for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_down(P, w, H.fin_order[k]);
-    for (int lv = H.levels - 1; lv >= 0; lv--)
-      for (int v = 0; v < H.nslots[lv]; v++) tps_back(P, w, lv, v);
-    for (int i = 0; i < N; i++) x[i] = w.x[s0[i]];
+    for (int lv = H.levels - 1; lv >= 1; lv--)
+      for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
+        tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, x);
+    if (H.levels == 0) {
+      for (int v = 0; v < H.nslots[0]; v++) tps_final_out<OUT_ASSIGN>(P, w, v, x);
+    } else {
+      for (int ch = H.chunk_off[0]; ch < H.chunk_off[1]; ch++)
+        tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, x);
+    }
     if (levels) *levels = H.levels;
     if (heights) *heights = H.heights;
     return NRT_OK;

@@ -71,7 +71,7 @@ class Options(C.Structure):
                 ("check_every", C.c_int32), ("max_iter", C.c_int32), ("device", C.c_int32),
                 ("num_blocks", C.c_int32), ("tol", C.c_double), ("refine_tol", C.c_double),
                 ("refine_max", C.c_int32), ("precond", C.c_int32), ("chunk", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("chunk_hi", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):

@@ -163,7 +163,8 @@ typedef struct nrt_options {
   int32_t krylov;       /* NRT_BICGSTAB / NRT_GMRES                             */
   int32_t restart;      /* GMRES restart length (reference CPU: 150)           */
   int32_t tol_mode;     /* NRT_TOL_ABSOLUTE / NRT_TOL_RELATIVE                  */
-  int32_t x0_mode;      /* NRT_X0_PREVIOUS / NRT_X0_RHS                         */
+  int32_t x0_mode;      /* NRT_X0_PREVIOUS / NRT_X0_RHS (Krylov only; the tree
+                           solve's refinement always starts from V)             */
   int32_t check_every;  /* BiCGSTAB convergence test period (reference: 30)    */
   int32_t max_iter;     /* 0 -> N (reference MAX_ITER)                          */
   int32_t device;       /* -1 -> $idGPU or 0                                    */
@@ -172,8 +173,10 @@ typedef struct nrt_options {
   double refine_tol;    /* FP32_REFINE: outer FP64 residual tolerance (relative)*/
   int32_t refine_max;   /* FP32_REFINE: maximum refinement sweeps               */
   int32_t precond;      /* NRT_PRECOND_NONE / NRT_PRECOND_TREE                  */
-  int32_t chunk;        /* tree solve: chain chunk length (0 -> autotuned)      */
-  int32_t reserved[5];
+  int32_t chunk;        /* tree solve: chain chunk length at level 0 (0 -> auto) */
+  int32_t chunk_hi;     /* tree solve: chunk length of the reduced levels
+                           (0 -> chunk if set, else auto)                        */
+  int32_t reserved[4];
 } nrt_options;
 
 typedef struct nrt_stats {

@@ -15,10 +15,10 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
-def tps_host(rp, col, val, rhs, chunk):
+def tps_host(rp, col, val, rhs, chunk, chunk_hi=0):
     L = nh.lib()
     f = L.nrt_test_tree_solve_host
-    f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, _dp, C.POINTER(C.c_int),
+    f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int),
                   C.POINTER(C.c_int)]
     rp = np.ascontiguousarray(rp, np.int32)
     col = np.ascontiguousarray(col, np.int32)
@@ -27,7 +27,8 @@ def tps_host(rp, col, val, rhs, chunk):
     x = np.empty(len(rhs))
     lv, h = C.c_int(0), C.c_int(0)
     st = f(len(rhs), rp.ctypes.data_as(_ip), col.ctypes.data_as(_ip), val.ctypes.data_as(_dp),
-           rhs.ctypes.data_as(_dp), chunk, x.ctypes.data_as(_dp), C.byref(lv), C.byref(h))
+           rhs.ctypes.data_as(_dp), chunk, chunk_hi, x.ctypes.data_as(_dp), C.byref(lv),
+           C.byref(h))
     assert st == 0
     return x, lv.value, h.value
 
@@ -40,24 +41,25 @@ def _system(orc, p):
 
 @pytest.mark.parametrize("name", ["ex3_hh_y", "ex4_ct_cable", "ex5_ct_y", "ex6_ct_tree",
                                   "ex8_neuron"])
-@pytest.mark.parametrize("chunk", [3, 4, 7, 32, 64])
-def test_tps_golden_systems(name, chunk, orc):
+@pytest.mark.parametrize("chunk,chunk_hi", [(3, 0), (4, 0), (7, 0), (32, 0), (64, 0), (20, 6),
+                                            (5, 40)])
+def test_tps_golden_systems(name, chunk, chunk_hi, orc):
     g = load_golden(name)
     p = problem_from_golden(g, dT=g["dT_crit"] * 10)
     rp, col, val, rhs = _system(orc, p)
-    x, lv, h = tps_host(rp, col, val, rhs, chunk)
+    x, lv, h = tps_host(rp, col, val, rhs, chunk, chunk_hi)
     ref = orc.tree_solve(rp, col, val, rhs)
     assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("chunk", [4, 16, 64])
-def test_tps_long_chain_and_tree(chunk, orc):
+@pytest.mark.parametrize("chunk,chunk_hi", [(4, 0), (16, 0), (64, 0), (20, 8)])
+def test_tps_long_chain_and_tree(chunk, chunk_hi, orc):
     for geom, dT in [(wl.myelinated_axon()[0], 1e-5),
                      (wl.random_tree(wl.seed_for(3), depth=8)[0], 2.5e-5)]:
         s, bufs, H = _setup(orc, geom)
         p = _problem(geom, s, bufs, H, dT)
         rp, col, val, rhs = _system(orc, p)
-        x, lv, h = tps_host(rp, col, val, rhs, chunk)
+        x, lv, h = tps_host(rp, col, val, rhs, chunk, chunk_hi)
         ref = orc.tree_solve(rp, col, val, rhs)
         assert np.max(np.abs(x - ref)) <= 1e-11 * np.max(np.abs(ref))
         # the compressed forest must be shallow: at most 2 nodes per chain remain
