@@ -1,0 +1,404 @@
+// Batches of independent neurites (the nrt_batch_* entry points). Each slot is a
+// complete solver state (a handle without a stream of its own); the batch keeps a
+// device table of slot descriptors that is rebuilt only when membership changes,
+// and advances all slots with one launch per call (implicit, persistent over the
+// k steps) or one graph of k steps (explicit).
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include "nrt_internal.hpp"
+
+namespace nrt {
+// implicit.hip
+size_t implicit_slot_bytes(int precision);
+void implicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
+                        long long a1, int trace_elem, long long interval, double* trace,
+                        void* out);
+void implicit_batch_launch(int precision, int threads, const void* table, int nslots,
+                           long long g0, long long k, hipStream_t s);
+void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner);
+int64_t implicit_device_bytes(const Implicit* e);
+// explicit.hip
+size_t explicit_slot_bytes();
+void explicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
+                        long long a1, int trace_elem, long long interval, double* trace,
+                        void* out);
+void explicit_batch_step(const void* slots, const int2* blk, int nblk, int nslots,
+                         long long* gstep, hipStream_t s);
+const double* explicit_slot_v(nrt_handle* h, long long done);
+int64_t explicit_device_bytes(const Explicit* e);
+
+namespace {
+constexpr int kExBlock = 256;            // elements per block of the explicit batch
+constexpr int kTuneThreads[] = {256, 512, 1024};
+constexpr int kPlanThreads = 512;        // workers assumed by a slot's chunk choice
+// Footprint per element used for admission (bytes; the arrays a slot allocates,
+// rounded up): implicit tree solve ~ 7 FP64 + 3 int32 vectors, 3 CSR entries, the
+// plan and 8 tree work values over ~1.1 N level nodes; explicit ~ 6 FP64 + code.
+constexpr int64_t kImplicitBytesPerElem = 420;
+constexpr int64_t kExplicitBytesPerElem = 64;
+constexpr int64_t kAdmissionReserve = 512ll << 20;  // left free for the runtime
+
+struct Slot {
+  nrt_handle* h = nullptr;
+  int N = 0;
+  long long s0 = 0, num_steps = 0, a0 = 0, a1 = -1, interval = -1;
+  int trace_elem = 0;
+  DBuf<double> trace;
+  int64_t bytes = 0;
+  ~Slot() {
+    if (h) handle_destroy(h);
+  }
+  long long done(long long g) const { return std::min(std::max(0ll, g - s0), num_steps); }
+};
+}  // namespace
+}  // namespace nrt
+
+using namespace nrt;
+
+struct nrt_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  nrt_options opt{};
+  std::vector<std::unique_ptr<Slot>> slots;  // index = slot id; null once removed
+  long long g = 0;                           // batch steps done
+  bool dirty = true;
+  std::vector<int> order;  // slot ids in table order
+  DBuf<uint8_t> table;
+  DBuf<int2> blk;
+  int nblk = 0;
+  DBuf<long long> gdev;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long graph_k = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int threads = 0;                        // implicit workgroup size (0: tuning)
+  std::vector<std::pair<int, double>> tune;  // (threads, ms per compartment-step)
+  nrt_batch_stats stats{};
+  std::string last_error;
+
+  bool is_implicit() const { return opt.kind == NRT_IMPLICIT; }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    graph_k = 0;
+  }
+
+  Slot& slot(int id) {
+    NRT_REQUIRE(id >= 0 && id < (int)slots.size() && slots[id], NRT_ERR_INVALID,
+                "no slot " + std::to_string(id));
+    return *slots[id];
+  }
+
+  // Descriptor table of the slots that still have steps left.
+  void rebuild() {
+    order.clear();
+    for (int id = 0; id < (int)slots.size(); id++)
+      if (slots[id] && slots[id]->done(g) < slots[id]->num_steps) order.push_back(id);
+    drop_graph();
+    if (is_implicit()) {
+      // largest trees first: the long workgroups start early (LPT order)
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return slots[a]->N > slots[b]->N; });
+      const size_t sb = implicit_slot_bytes(opt.precision);
+      std::vector<uint8_t> host(sb * order.size());
+      for (size_t k = 0; k < order.size(); k++) {
+        Slot& s = *slots[order[k]];
+        implicit_slot_fill(s.h, s.s0, s.num_steps, s.a0, s.a1, s.trace_elem, s.interval,
+                           s.trace.p, host.data() + k * sb);
+      }
+      table.upload(host, stream);
+    } else {
+      const size_t sb = explicit_slot_bytes();
+      std::vector<uint8_t> host(sb * order.size());
+      std::vector<int2> bl;
+      for (size_t k = 0; k < order.size(); k++) {
+        Slot& s = *slots[order[k]];
+        explicit_slot_fill(s.h, s.s0, s.num_steps, s.a0, s.a1, s.trace_elem, s.interval,
+                           s.trace.p, host.data() + k * sb);
+        for (int e0 = 0; e0 < s.N; e0 += kExBlock) bl.push_back(make_int2((int)k, e0));
+      }
+      table.upload(host, stream);
+      blk.upload(bl, stream);
+      nblk = (int)bl.size();
+      if (!gdev.p) gdev.alloc(1);
+      NRT_HIP(hipMemcpyAsync(gdev.p, &g, sizeof(long long), hipMemcpyHostToDevice, stream));
+    }
+    NRT_HIP(hipStreamSynchronize(stream));
+    dirty = false;
+  }
+
+  int pick_threads() {
+    if (opt.batch_threads > 0) return opt.batch_threads;
+    if (threads > 0) return threads;
+    return kTuneThreads[tune.size()];
+  }
+
+  void step(long long k) {
+    NRT_REQUIRE(k >= 0, NRT_ERR_INVALID, "k < 0");
+    if (dirty) rebuild();
+    if (order.empty() || k == 0) {
+      g += k;
+      return;
+    }
+    int64_t cs = 0, ss = 0;
+    for (int id : order) {
+      const Slot& s = *slots[id];
+      const long long adv = std::min(k, s.num_steps - s.done(g));
+      ss += adv;
+      cs += adv * s.N;
+    }
+    float ms = 0;
+    if (is_implicit()) {
+      const int th = pick_threads();
+      NRT_HIP(hipEventRecord(ev0, stream));
+      implicit_batch_launch(opt.precision, th, table.p, (int)order.size(), g, k, stream);
+      NRT_HIP(hipEventRecord(ev1, stream));
+      NRT_HIP(hipEventSynchronize(ev1));
+      NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+      if (opt.batch_threads <= 0 && threads == 0 && cs > 0) {
+        tune.emplace_back(th, ms / (double)cs);
+        if (tune.size() == sizeof(kTuneThreads) / sizeof(int))
+          threads = std::min_element(tune.begin(), tune.end(),
+                                     [](auto& a, auto& b) { return a.second < b.second; })
+                        ->first;
+      }
+      stats.threads = th;
+      stats.launches += 1;
+    } else {
+      if (!gexec || graph_k != k) {
+        drop_graph();
+        NRT_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (long long i = 0; i < k; i++)
+          explicit_batch_step(table.p, blk.p, nblk, (int)order.size(), gdev.p, stream);
+        NRT_HIP(hipStreamEndCapture(stream, &graph));
+        NRT_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        graph_k = k;
+      }
+      NRT_HIP(hipEventRecord(ev0, stream));
+      NRT_HIP(hipGraphLaunch(gexec, stream));
+      NRT_HIP(hipEventRecord(ev1, stream));
+      NRT_HIP(hipEventSynchronize(ev1));
+      NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+      stats.launches += k;
+    }
+    g += k;
+    stats.kernel_ms += ms;
+    stats.slot_steps += ss;
+    stats.compartment_steps += cs;
+    for (int id : order)
+      if (slots[id]->done(g) >= slots[id]->num_steps) dirty = true;
+  }
+};
+
+namespace {
+template <class F>
+int guarded_b(nrt_batch* b, F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    if (b) b->last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    if (b) b->last_error = "host allocation failed";
+    return NRT_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    if (b) b->last_error = e.what();
+    return NRT_ERR_INVALID;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int nrt_batch_create(const nrt_options* opt, nrt_batch** out) {
+  if (!out) return NRT_ERR_INVALID;
+  *out = nullptr;
+  auto* b = new nrt_batch;
+  int st = guarded_b(b, [&] {
+    b->opt = resolve_options(opt);
+    NRT_REQUIRE(b->opt.kind == NRT_EXPLICIT || b->opt.krylov == NRT_DIRECT, NRT_ERR_UNSUPPORTED,
+                "implicit batches use the tree solve (krylov = NRT_DIRECT)");
+    NRT_REQUIRE(b->opt.batch_threads == 0 || b->opt.batch_threads == 256 ||
+                    b->opt.batch_threads == 512 || b->opt.batch_threads == 1024,
+                NRT_ERR_INVALID, "batch_threads must be 0, 256, 512 or 1024");
+    b->device = pick_device(b->opt.device);
+    b->opt.device = b->device;
+    NRT_HIP(hipSetDevice(b->device));
+    NRT_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    NRT_HIP(hipEventCreate(&b->ev0));
+    NRT_HIP(hipEventCreate(&b->ev1));
+    return NRT_OK;
+  });
+  if (st != NRT_OK) {
+    nrt_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return NRT_OK;
+}
+
+int nrt_batch_add(nrt_batch* b, const nrt_problem* p, int64_t num_steps, int64_t a0,
+                  int64_t a1, int32_t trace_elem, int64_t trace_every, int32_t* slot) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    NRT_REQUIRE(p && slot, NRT_ERR_INVALID, "NULL argument");
+    NRT_REQUIRE(num_steps >= 0, NRT_ERR_INVALID, "num_steps < 0");
+    NRT_REQUIRE(p->num_elem >= 3, NRT_ERR_INVALID, "num_elem must be >= 3");
+    NRT_REQUIRE(trace_every < 0 || (trace_elem >= 0 && trace_elem < p->num_elem),
+                NRT_ERR_INVALID, "trace_elem out of range");
+    NRT_HIP(hipSetDevice(b->device));
+    // admission by free device memory
+    size_t free_b = 0, total_b = 0;
+    NRT_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t per = b->is_implicit() ? kImplicitBytesPerElem : kExplicitBytesPerElem;
+    const int64_t samples = trace_every >= 0 ? num_steps / (trace_every + 1) : 0;
+    const int64_t need = per * (int64_t)p->num_elem + 8 * samples;
+    NRT_REQUIRE((int64_t)free_b >= need + kAdmissionReserve, NRT_ERR_ADMISSION,
+                "slot of " + std::to_string(p->num_elem) + " elements needs ~" +
+                    std::to_string(need >> 20) + " MiB; " + std::to_string(free_b >> 20) +
+                    " MiB free");
+    auto s = std::make_unique<Slot>();
+    s->h = handle_create(p, b->opt, b->stream,
+                         b->opt.batch_threads > 0 ? b->opt.batch_threads : kPlanThreads);
+    s->N = p->num_elem;
+    s->s0 = b->g;
+    s->num_steps = num_steps;
+    s->a0 = a0;
+    s->a1 = a1;
+    s->interval = trace_every;
+    s->trace_elem = trace_elem;
+    if (samples > 0) s->trace.alloc((size_t)samples);
+    s->bytes = (b->is_implicit() ? implicit_device_bytes(s->h->im)
+                                 : explicit_device_bytes(s->h->ex)) +
+               8 * samples;
+    *slot = (int32_t)b->slots.size();
+    b->slots.push_back(std::move(s));
+    b->dirty = true;
+    return NRT_OK;
+  });
+}
+
+int nrt_batch_step(nrt_batch* b, int64_t k) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    NRT_HIP(hipSetDevice(b->device));
+    b->step(k);
+    return NRT_OK;
+  });
+}
+
+int nrt_batch_snapshot(nrt_batch* b, int32_t slot, double* host_out) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    NRT_REQUIRE(host_out, NRT_ERR_INVALID, "host_out is NULL");
+    Slot& s = b->slot(slot);
+    NRT_HIP(hipSetDevice(b->device));
+    if (b->is_implicit()) {
+      implicit_snapshot(s.h, host_out);
+    } else {
+      const double* v = explicit_slot_v(s.h, s.done(b->g));
+      NRT_HIP(hipMemcpyAsync(host_out, v, sizeof(double) * s.N, hipMemcpyDeviceToHost,
+                             b->stream));
+      NRT_HIP(hipStreamSynchronize(b->stream));
+    }
+    return NRT_OK;
+  });
+}
+
+int nrt_batch_trace(nrt_batch* b, int32_t slot, double* host_out, int64_t capacity,
+                    int64_t* count) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    Slot& s = b->slot(slot);
+    const int64_t n =
+        s.interval >= 0 ? std::min<int64_t>(s.done(b->g) / (s.interval + 1), (int64_t)s.trace.n)
+                        : 0;
+    const int64_t c = std::min(n, std::max<int64_t>(capacity, 0));
+    if (c > 0) {
+      NRT_REQUIRE(host_out, NRT_ERR_INVALID, "host_out is NULL");
+      NRT_HIP(hipSetDevice(b->device));
+      NRT_HIP(hipMemcpyAsync(host_out, s.trace.p, sizeof(double) * c, hipMemcpyDeviceToHost,
+                             b->stream));
+      NRT_HIP(hipStreamSynchronize(b->stream));
+    }
+    if (count) *count = n;
+    return NRT_OK;
+  });
+}
+
+int nrt_batch_slot_info(nrt_batch* b, int32_t slot, nrt_slot_info* info) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    NRT_REQUIRE(info, NRT_ERR_INVALID, "info is NULL");
+    Slot& s = b->slot(slot);
+    memset(info, 0, sizeof(*info));
+    info->N = s.N;
+    info->steps_done = s.done(b->g);
+    info->num_steps = s.num_steps;
+    info->trace_samples = s.interval >= 0 ? s.done(b->g) / (s.interval + 1) : 0;
+    if (b->is_implicit()) {
+      NRT_HIP(hipSetDevice(b->device));
+      int st = 0;
+      int64_t inner = 0;
+      implicit_slot_stats(s.h, &st, &inner);
+      info->status = st;
+      info->inner_iters = inner;
+    }
+    return NRT_OK;
+  });
+}
+
+int nrt_batch_remove(nrt_batch* b, int32_t slot) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    b->slot(slot);
+    NRT_HIP(hipSetDevice(b->device));
+    NRT_HIP(hipStreamSynchronize(b->stream));
+    b->slots[slot].reset();
+    b->dirty = true;
+    return NRT_OK;
+  });
+}
+
+int nrt_batch_get_stats(nrt_batch* b, nrt_batch_stats* out) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    NRT_REQUIRE(out, NRT_ERR_INVALID, "out is NULL");
+    *out = b->stats;
+    out->active_slots = 0;
+    out->device_bytes = 0;
+    for (auto& s : b->slots) {
+      if (!s) continue;
+      if (s->done(b->g) < s->num_steps) out->active_slots++;
+      out->device_bytes += s->bytes;
+    }
+    if (b->is_implicit()) out->threads = b->pick_threads();
+    return NRT_OK;
+  });
+}
+
+const char* nrt_batch_last_error(const nrt_batch* b) { return b ? b->last_error.c_str() : ""; }
+
+int nrt_batch_destroy(nrt_batch* b) {
+  if (!b) return NRT_OK;
+  if (b->stream) {
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+  }
+  b->drop_graph();
+  b->slots.clear();
+  b->table.release();
+  b->blk.release();
+  b->gdev.release();
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return NRT_OK;
+}
+
+}  // extern "C"

@@ -72,7 +72,7 @@ HostProblem load_problem(const nrt_problem* p) {
   return q;
 }
 
-static int pick_device(int requested) {
+int pick_device(int requested) {
   int count = 0;
   hipError_t e = hipGetDeviceCount(&count);
   if (e != hipSuccess || count <= 0) {
@@ -87,6 +87,58 @@ static int pick_device(int requested) {
   NRT_REQUIRE(dev >= 0 && dev < count, NRT_ERR_INVALID,
               "device " + std::to_string(dev) + " out of range (" + std::to_string(count) + ")");
   return dev;
+}
+
+nrt_options resolve_options(const nrt_options* opt) {
+  nrt_options o;
+  if (opt) o = *opt;
+  else nrt_default_options(&o);
+  NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.kind == NRT_IMPLICIT, NRT_ERR_INVALID, "bad kind");
+  NRT_REQUIRE(o.precision == NRT_FP64 || o.precision == NRT_FP32_REFINE, NRT_ERR_INVALID,
+              "bad precision");
+  NRT_REQUIRE(o.precision == NRT_FP64 || (o.kind == NRT_IMPLICIT && o.krylov == NRT_DIRECT),
+              NRT_ERR_UNSUPPORTED,
+              "FP32 with FP64 refinement is available for the implicit tree solve");
+  NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB || o.krylov == NRT_DIRECT,
+              NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB or NRT_DIRECT");
+  NRT_REQUIRE(o.precond == NRT_PRECOND_NONE, NRT_ERR_UNSUPPORTED,
+              "preconditioned Krylov is not built; use NRT_DIRECT for the tree solve");
+  NRT_REQUIRE(o.chunk >= 0 && o.chunk <= kMaxChunk && o.chunk_hi >= 0 &&
+                  o.chunk_hi <= kMaxChunk,
+              NRT_ERR_INVALID, "tree chunk lengths must be in [0, 4096]");
+  return o;
+}
+
+nrt_handle* handle_create(const nrt_problem* p, const nrt_options& o, hipStream_t shared,
+                          int batch_threads) {
+  HostProblem q = load_problem(p);
+  auto* h = new nrt_handle;
+  try {
+    h->opt = o;
+    h->device = pick_device(o.device);
+    h->N = q.N;
+    NRT_HIP(hipSetDevice(h->device));
+    if (shared) {
+      h->stream = shared;
+      h->owns_stream = false;
+    } else {
+      NRT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    }
+    if (o.kind == NRT_EXPLICIT) h->ex = explicit_create(h, q);
+    else h->im = implicit_create(h, q, batch_threads);
+  } catch (...) {
+    if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    throw;
+  }
+  return h;
+}
+
+void handle_destroy(nrt_handle* h) {
+  if (h->ex) explicit_destroy(h->ex);
+  if (h->im) implicit_destroy(h->im);
+  if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
+  delete h;
 }
 
 }  // namespace nrt
@@ -176,38 +228,7 @@ int nrt_create(const nrt_problem* p, const nrt_options* opt, nrt_handle** out) {
   return guarded(nullptr, [&] {
     NRT_REQUIRE(out != nullptr, NRT_ERR_INVALID, "out is NULL");
     *out = nullptr;
-    nrt_options o;
-    if (opt) o = *opt;
-    else nrt_default_options(&o);
-    NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.kind == NRT_IMPLICIT, NRT_ERR_INVALID, "bad kind");
-    NRT_REQUIRE(o.precision == NRT_FP64 || o.precision == NRT_FP32_REFINE, NRT_ERR_INVALID,
-                "bad precision");
-    NRT_REQUIRE(o.precision == NRT_FP64 || (o.kind == NRT_IMPLICIT && o.krylov == NRT_DIRECT),
-                NRT_ERR_UNSUPPORTED,
-                "FP32 with FP64 refinement is available for the implicit tree solve");
-    NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB || o.krylov == NRT_DIRECT,
-                NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB or NRT_DIRECT");
-    NRT_REQUIRE(o.precond == NRT_PRECOND_NONE, NRT_ERR_UNSUPPORTED,
-                "preconditioned Krylov is not built; use NRT_DIRECT for the tree solve");
-    NRT_REQUIRE(o.chunk >= 0 && o.chunk <= kMaxChunk && o.chunk_hi >= 0 &&
-                    o.chunk_hi <= kMaxChunk,
-                NRT_ERR_INVALID, "tree chunk lengths must be in [0, 4096]");
-    HostProblem q = load_problem(p);
-    auto* h = new nrt_handle;
-    try {
-      h->opt = o;
-      h->device = pick_device(o.device);
-      h->N = q.N;
-      NRT_HIP(hipSetDevice(h->device));
-      NRT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-      if (o.kind == NRT_EXPLICIT) h->ex = explicit_create(h, q);
-      else h->im = implicit_create(h, q);
-    } catch (...) {
-      if (h->stream) (void)hipStreamDestroy(h->stream);
-      delete h;
-      throw;
-    }
-    *out = h;
+    *out = handle_create(p, resolve_options(opt), nullptr, 0);
     return NRT_OK;
   });
 }
@@ -286,10 +307,7 @@ int nrt_destroy(nrt_handle* h) {
     NRT_HIP(hipStreamSynchronize(h->stream));
     return NRT_OK;
   });
-  if (h->ex) explicit_destroy(h->ex);
-  if (h->im) implicit_destroy(h->im);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
-  delete h;
+  handle_destroy(h);
   return st;
 }
 

@@ -23,6 +23,7 @@
 // IEEE double sequence as the reference CPU build; the only differing operation
 // is exp() (device libm vs host libm, both within an ulp or so).
 #include <algorithm>
+#include <cstring>
 #include "nrt_internal.hpp"
 
 namespace nrt {
@@ -92,15 +93,13 @@ __device__ __forceinline__ void hh_gate(const HHDev& hd, int i, double v, double
   kdx = K * dx;
 }
 
-// One explicit step for every element (thread per element).
-__global__ __launch_bounds__(256) void k_explicit_fp64(
-    int N, const double* __restrict__ Vo, double* __restrict__ Vn,
+// Explicit step of element j: reads Vo, writes Vn[j] and its sealed-end copies.
+__device__ __forceinline__ void explicit_element(
+    int j, const double* __restrict__ Vo, double* __restrict__ Vn,
     const double* __restrict__ coefA, const double* __restrict__ rdx,
     const double* __restrict__ wdx, const double* __restrict__ kdx,
-    const uint32_t* __restrict__ code, const ExRec* __restrict__ recs, HHDev hd, double dt_ms,
-    int active) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
+    const uint32_t* __restrict__ code, const ExRec* __restrict__ recs, const HHDev& hd,
+    double dt_ms, bool active) {
   const uint32_t c = code[j];
   const double v = Vo[j];
   if (c & kCodeSkip) {
@@ -144,6 +143,18 @@ __global__ __launch_bounds__(256) void k_explicit_fp64(
     for (int t = 0; t < 3; t++)
       if (rec->tgt[t] >= 0) Vn[rec->tgt[t]] = vn;
   }
+}
+
+// One explicit step for every element (thread per element).
+__global__ __launch_bounds__(256) void k_explicit_fp64(
+    int N, const double* __restrict__ Vo, double* __restrict__ Vn,
+    const double* __restrict__ coefA, const double* __restrict__ rdx,
+    const double* __restrict__ wdx, const double* __restrict__ kdx,
+    const uint32_t* __restrict__ code, const ExRec* __restrict__ recs, HHDev hd, double dt_ms,
+    int active) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  explicit_element(j, Vo, Vn, coefA, rdx, wdx, kdx, code, recs, hd, dt_ms, active != 0);
 }
 
 Explicit* explicit_create(nrt_handle* hnd, const HostProblem& p) {
@@ -313,6 +324,108 @@ void explicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64
   hnd->stats.kernel_launches = num;
   hnd->stats.steps += num;
   if (ns) *ns = k;
+}
+
+// ------------------------------------------------------------ batch support
+// Slots of an explicit batch: one thread per element over all slots, one launch
+// per step; blocks map to (slot, first element). The batch step counter lives on
+// the device so a captured sequence of steps can be replayed.
+struct ExSlot {
+  int N;
+  int cur0;  // buffer holding V when the slot joined
+  double* V[2];
+  const double* coefA;
+  const double* rdx;
+  const double* wdx;
+  const double* kdx;
+  const uint32_t* code;
+  const ExRec* recs;
+  HHDev hd;
+  double dt_ms;
+  long long s0, num_steps, a0, a1, interval;
+  int trace_elem;
+  double* trace;
+};
+
+__global__ __launch_bounds__(256) void k_explicit_batch(const ExSlot* __restrict__ slots,
+                                                        const int2* __restrict__ blk,
+                                                        const long long* __restrict__ gstep) {
+  const int2 b = blk[blockIdx.x];
+  const ExSlot& S = slots[b.x];
+  const long long done = *gstep - S.s0;
+  if (done < 0 || done >= S.num_steps) return;
+  const int j = b.y + (int)threadIdx.x;
+  if (j >= S.N) return;
+  const int cur = (S.cur0 + (int)(done & 1)) & 1;
+  const long long step = done + 1;
+  explicit_element(j, S.V[cur], S.V[1 - cur], S.coefA, S.rdx, S.wdx, S.kdx, S.code, S.recs,
+                   S.hd, S.dt_ms, step >= S.a0 && step <= S.a1);
+}
+
+// After a step: the probe samples due at it, then the step counter advances.
+__global__ void k_explicit_batch_trace(const ExSlot* __restrict__ slots, int nslots,
+                                       const long long* __restrict__ gstep) {
+  const int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nslots) return;
+  const ExSlot& S = slots[si];
+  const long long done = *gstep - S.s0 + 1;  // steps done including this one
+  if (S.interval < 0 || !S.trace || done < 1 || done > S.num_steps) return;
+  if (done % (S.interval + 1) != 0) return;
+  const int cur = (S.cur0 + (int)(done & 1)) & 1;
+  S.trace[done / (S.interval + 1) - 1] = S.V[cur][S.trace_elem];
+}
+
+__global__ void k_batch_advance(long long* gstep) { *gstep = *gstep + 1; }
+
+size_t explicit_slot_bytes() { return sizeof(ExSlot); }
+
+void explicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
+                        long long a1, int trace_elem, long long interval, double* trace,
+                        void* out) {
+  Explicit* e = h->ex;
+  ExSlot S{};
+  S.N = e->N;
+  S.cur0 = e->cur;
+  S.V[0] = e->V[0].p;
+  S.V[1] = e->V[1].p;
+  S.coefA = e->coefA.p;
+  S.rdx = e->rdx.p;
+  S.wdx = e->wdx.p;
+  S.kdx = e->kdx.p;
+  S.code = e->code.p;
+  S.recs = e->recs.p;
+  S.hd = e->hd();
+  S.dt_ms = e->dt_ms;
+  S.s0 = s0;
+  S.num_steps = num_steps;
+  S.a0 = a0;
+  S.a1 = a1;
+  S.interval = interval;
+  S.trace_elem = trace_elem;
+  S.trace = trace;
+  std::memcpy(out, &S, sizeof(S));
+}
+
+// One batch step: the element kernel over `nblk` blocks, the probes, the counter.
+void explicit_batch_step(const void* slots, const int2* blk, int nblk, int nslots,
+                         long long* gstep, hipStream_t s) {
+  const auto* S = (const ExSlot*)slots;
+  if (nblk > 0) hipLaunchKernelGGL(k_explicit_batch, dim3(nblk), dim3(256), 0, s, S, blk, gstep);
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_explicit_batch_trace, dim3(grid_for(nslots, 256)), dim3(256), 0, s, S,
+                       nslots, gstep);
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, s, gstep);
+  NRT_HIP(hipGetLastError());
+}
+
+// Buffer holding V of a slot after `done` of its steps.
+const double* explicit_slot_v(nrt_handle* h, long long done) {
+  Explicit* e = h->ex;
+  return e->V[(e->cur + (int)(done & 1)) & 1].p;
+}
+
+int64_t explicit_device_bytes(const Explicit* e) {
+  return (int64_t)e->N * (8 * 6 + 4) + (int64_t)e->R * 40 + (int64_t)e->H * 8 * 13;
 }
 
 }  // namespace nrt

@@ -24,6 +24,7 @@
 // rowptr/col (int32), val; per element cmdx = c_m*dx, kdx = K*dx, rdx = r_a*dx,
 // input, code (uint32); HH SoA as in the explicit step plus row type/position/DL.
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 #include "nrt_internal.hpp"
 #include "nrt_device.hpp"
@@ -107,6 +108,7 @@ struct ImpDev {
   int max_iter;
   unsigned long long spin_limit;
   long long first, num, a0, a1, interval;
+  long long gl0;  // trace phase at step `first` (steps since the last sample)
   int trace_elem;
   double* trace;
 };
@@ -727,10 +729,10 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
   }
 }
 
+// Steps d.first .. d.first+d.num-1 of one tree (GRID: the whole launch works on
+// it; else one workgroup does).
 template <class T, bool GRID>
-__global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree(ImpDev d,
-                                                                                 TreeArgs<T> t) {
-  __shared__ double lds[4 * 16];
+__device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t, double* lds) {
   const int r_begin = GRID ? (int)blockIdx.x * d.rpb : 0;
   const int r_end = GRID ? min(d.N, r_begin + d.rpb) : d.N;
   const int bs = blockDim.x;
@@ -738,7 +740,7 @@ __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree
   const int32_t* __restrict__ rowptr = d.rowptr;
   const int32_t* __restrict__ col = d.col;
   const int32_t* __restrict__ slot0 = t.P.slot0;
-  long long gl = 0, ksamp = 0;
+  long long gl = d.gl0, ksamp = 0;
   int slot = 0;
   const double lim2 = t.refine_tol * t.refine_tol;
   // Level-0 tree system from the CSR. Off-diagonals never change; the diagonal of
@@ -822,6 +824,43 @@ __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree
   }
 }
 
+template <class T, bool GRID>
+__global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree(ImpDev d,
+                                                                                 TreeArgs<T> t) {
+  __shared__ double lds[4 * 16];
+  tree_steps<T, GRID>(d, t, lds);
+}
+
+// Batched form: independent trees ("slots"), one workgroup each, no grid-wide
+// dependency. A slot's descriptor is static while the batch's membership does not
+// change; the launch passes the batch's step counter g0 and the step count k, and
+// every slot derives its own step numbers from the step at which it joined (s0).
+template <class T>
+struct BatchSlot {
+  ImpDev d;
+  TreeArgs<T> t;
+  long long s0;         // batch step at which the slot joined
+  long long num_steps;  // the slot's length
+  double* trace_base;   // probe samples, num_steps / (interval + 1) of them
+};
+
+template <class T, int BS>
+__global__ __launch_bounds__(BS) void k_batch_tree(const BatchSlot<T>* __restrict__ slots,
+                                                  long long g0, long long k) {
+  __shared__ double lds[4 * 16];
+  const BatchSlot<T>& S = slots[blockIdx.x];
+  const long long done = g0 - S.s0;
+  if (done < 0 || done >= S.num_steps) return;
+  ImpDev d = S.d;
+  d.first = done + 1;
+  d.num = min(k, S.num_steps - done);
+  if (d.interval >= 0) {
+    d.gl0 = done % (d.interval + 1);
+    d.trace = S.trace_base ? S.trace_base + done / (d.interval + 1) : nullptr;
+  }
+  tree_steps<T, false>(d, S.t, lds);
+}
+
 // ---------------------------------------------------------------- host side
 static void device_exclusive_scan(const int32_t* in, int32_t* out, int n, DBuf<int32_t>& tmp,
                                   hipStream_t s) {
@@ -850,7 +889,7 @@ static int choose_blocks(nrt_handle* hnd, int N, const void* kernel = nullptr) {
   return std::max(1, want);
 }
 
-Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
+Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threads) {
   hipStream_t s = hnd->stream;
   const int N = p.N, H = p.H;
   NRT_REQUIRE(N >= 4, NRT_ERR_INVALID, "the implicit solver needs at least 4 elements");
@@ -924,14 +963,19 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
                        e->rdx.p, e->cmdx.p, e->dxv.p, e->Wc.p, e->code.p, e->hh_type.p,
                        e->hh_mat.p, e->hh_dl.p);
     NRT_HIP(hipGetLastError());
-    // Krylov vectors; x starts as potential[0] (solverIm-gpu.cpp:523-524)
-    e->x.upload(p.V0, s);
-    for (DBuf<double>* vbuf : {&e->b, &e->r0, &e->r, &e->p[0], &e->p[1], &e->v[0], &e->v[1],
-                               &e->s, &e->t}) {
-      vbuf->alloc(N);
-      NRT_HIP(hipMemsetAsync(vbuf->p, 0, sizeof(double) * N, s));
-    }
+    // Krylov vectors; x starts as potential[0] (solverIm-gpu.cpp:523-524). The tree
+    // solve needs only x and b.
     const nrt_options& o = hnd->opt;
+    e->x.upload(p.V0, s);
+    e->b.alloc(N);
+    NRT_HIP(hipMemsetAsync(e->b.p, 0, sizeof(double) * N, s));
+    if (o.krylov != NRT_DIRECT) {
+      for (DBuf<double>* vbuf : {&e->r0, &e->r, &e->p[0], &e->p[1], &e->v[0], &e->v[1], &e->s,
+                                 &e->t}) {
+        vbuf->alloc(N);
+        NRT_HIP(hipMemsetAsync(vbuf->p, 0, sizeof(double) * N, s));
+      }
+    }
     e->method = o.krylov == NRT_DIRECT ? (o.precision == NRT_FP64 ? 1 : 2) : 0;
     if (e->method != 0) {
       // plan of the tree partition solve from the assembled pattern
@@ -957,6 +1001,7 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
       // 4 -> 85 us, 32 -> 114 us per step) and only large trees spread wide.
       int nb = o.num_blocks;
       if (nb == 0 && N <= (1 << 17)) nb = N >= (1 << 14) ? 4 : 1;
+      if (batch_threads > 0) nb = 1;  // a batch slot: one workgroup of batch_threads
       e->grid_mode = nb != 1;
       if (e->grid_mode) {
         const void* k = e->method == 1 ? (const void*)k_implicit_tree<double, true>
@@ -965,7 +1010,7 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
         e->threads = kImpBlock;
       } else {
         e->nblocks = 1;
-        e->threads = kTreeBlock;
+        e->threads = batch_threads > 0 ? batch_threads : kTreeBlock;
       }
       // Level 0: one chunk per worker covers an unbranched cable in one round;
       // reduced levels: short chunks (the serial sweep is the latency, and each
@@ -999,6 +1044,10 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
     e->st.alloc(1);
     NRT_HIP(hipMemsetAsync(e->st.p, 0, sizeof(ImpStats), s));
     NRT_HIP(hipStreamSynchronize(s));
+    // assembly-only inputs
+    for (DBuf<int32_t>* ib : {&e->len, &e->scan_tmp, &e->mother, &e->dr, &e->dl}) ib->release();
+    e->Wc.release();
+    e->branching.release();
   } catch (...) {
     delete e;
     throw;
@@ -1008,8 +1057,8 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p) {
 
 void implicit_destroy(Implicit* e) { delete e; }
 
-static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
-                             int32_t trace_elem, int64_t interval, double* trace_dev) {
+// Device view of a handle's state (step window and trace left for the caller).
+static ImpDev imp_dev(nrt_handle* hnd) {
   Implicit* e = hnd->im;
   const nrt_options& o = hnd->opt;
   ImpDev d{};
@@ -1052,6 +1101,25 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.max_iter = o.max_iter > 0 ? o.max_iter : e->N;
   d.spin_limit = 1ull << 28;
   if (const char* sl = getenv("NRT_SPIN_LIMIT")) d.spin_limit = strtoull(sl, nullptr, 10);
+  d.interval = -1;
+  d.trace_elem = 0;
+  return d;
+}
+
+template <class T>
+static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b) {
+  const size_t T_ = (size_t)e->plan_h.total_nodes();
+  return TreeArgs<T>{e->plan,
+                     TreeWork<T>{b, b + T_, b + 2 * T_, b + 3 * T_, b + 4 * T_, b + 5 * T_,
+                                 b + 6 * T_, b + 7 * T_},
+                     e->method == 1 ? 1 : 0, o.refine_tol, o.refine_max};
+}
+
+static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
+                             int32_t trace_elem, int64_t interval, double* trace_dev) {
+  Implicit* e = hnd->im;
+  const nrt_options& o = hnd->opt;
+  ImpDev d = imp_dev(hnd);
   d.first = first;
   d.num = num;
   d.a0 = a0;
@@ -1065,7 +1133,6 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
                                        dim3(kImpBlock), args, 0, hnd->stream));
     return NRT_OK;
   }
-  const size_t T = (size_t)e->plan_h.total_nodes();
   auto launch = [&](auto targs, const void* kgrid, const void* kblock) {
     void* args[] = {&d, &targs};
     if (e->grid_mode)
@@ -1074,21 +1141,12 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
     else
       NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));
   };
-  if (e->method == 1) {
-    double* b = e->tw64.p;
-    TreeArgs<double> t{e->plan, TreeWork<double>{b, b + T, b + 2 * T, b + 3 * T, b + 4 * T,
-                                                 b + 5 * T, b + 6 * T, b + 7 * T},
-                       1, o.refine_tol, o.refine_max};
-    launch(t, (const void*)k_implicit_tree<double, true>,
+  if (e->method == 1)
+    launch(tree_args<double>(e, o, e->tw64.p), (const void*)k_implicit_tree<double, true>,
            (const void*)k_implicit_tree<double, false>);
-  } else {
-    float* b = e->tw32.p;
-    TreeArgs<float> t{e->plan, TreeWork<float>{b, b + T, b + 2 * T, b + 3 * T, b + 4 * T,
-                                               b + 5 * T, b + 6 * T, b + 7 * T},
-                      0, o.refine_tol, o.refine_max};
-    launch(t, (const void*)k_implicit_tree<float, true>,
+  else
+    launch(tree_args<float>(e, o, e->tw32.p), (const void*)k_implicit_tree<float, true>,
            (const void*)k_implicit_tree<float, false>);
-  }
   return NRT_OK;
 }
 
@@ -1173,6 +1231,66 @@ int implicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_
   hnd->stats.kernel_launches = 1;
   if (ns) *ns = k;
   return status;
+}
+
+// ------------------------------------------------------------ batch support
+size_t implicit_slot_bytes(int precision) {
+  return precision == NRT_FP64 ? sizeof(BatchSlot<double>) : sizeof(BatchSlot<float>);
+}
+
+void implicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
+                        long long a1, int trace_elem, long long interval, double* trace,
+                        void* out) {
+  Implicit* e = h->im;
+  NRT_REQUIRE(e->method != 0 && !e->grid_mode, NRT_ERR_UNSUPPORTED,
+              "batch slots use the block-mode tree solve");
+  ImpDev d = imp_dev(h);
+  d.a0 = a0;
+  d.a1 = a1;
+  d.interval = interval;
+  d.trace_elem = trace_elem;
+  if (e->method == 1) {
+    BatchSlot<double> S{d, tree_args<double>(e, h->opt, e->tw64.p), s0, num_steps, trace};
+    std::memcpy(out, &S, sizeof(S));
+  } else {
+    BatchSlot<float> S{d, tree_args<float>(e, h->opt, e->tw32.p), s0, num_steps, trace};
+    std::memcpy(out, &S, sizeof(S));
+  }
+}
+
+void implicit_batch_launch(int precision, int threads, const void* table, int nslots,
+                           long long g0, long long k, hipStream_t s) {
+  if (nslots <= 0) return;
+  const dim3 grid(nslots);
+  if (precision == NRT_FP64) {
+    auto* t = (const BatchSlot<double>*)table;
+    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<double, 256>), grid, dim3(256), 0, s, t, g0, k);
+    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<double, 512>), grid, dim3(512), 0, s, t, g0, k);
+    else hipLaunchKernelGGL((k_batch_tree<double, 1024>), grid, dim3(1024), 0, s, t, g0, k);
+  } else {
+    auto* t = (const BatchSlot<float>*)table;
+    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<float, 256>), grid, dim3(256), 0, s, t, g0, k);
+    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<float, 512>), grid, dim3(512), 0, s, t, g0, k);
+    else hipLaunchKernelGGL((k_batch_tree<float, 1024>), grid, dim3(1024), 0, s, t, g0, k);
+  }
+  NRT_HIP(hipGetLastError());
+}
+
+// Status and inner-iteration count of a slot (its failure is reported, not cleared).
+void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner) {
+  Implicit* e = h->im;
+  ImpStats st{};
+  NRT_HIP(hipMemcpyAsync(&st, e->st.p, sizeof(ImpStats), hipMemcpyDeviceToHost, h->stream));
+  NRT_HIP(hipStreamSynchronize(h->stream));
+  *status = st.status;
+  *inner = (int64_t)st.iters_total;
+}
+
+// Bytes of device memory a handle holds (approximate: its arrays).
+int64_t implicit_device_bytes(const Implicit* e) {
+  const int64_t N = e->N, T = e->plan_h.total_nodes();
+  return N * (8 * 7 + 4 * 3) + (int64_t)e->nnz * 12 + (int64_t)e->H * 8 * 12 +
+         (int64_t)e->plan_buf.n * 4 + T * 8 * (e->method == 1 ? 8 : 4);
 }
 
 }  // namespace nrt

@@ -109,6 +109,7 @@ struct nrt_handle {
   nrt::Explicit* ex = nullptr;
   nrt::Implicit* im = nullptr;
   int N = 0;
+  bool owns_stream = true;  // false for the slots of a batch (they use its stream)
 };
 
 namespace nrt {
@@ -120,7 +121,8 @@ void explicit_snapshot(nrt_handle* h, double* host_out);
 void explicit_run(nrt_handle* h, int64_t first, int64_t num, int64_t a0, int64_t a1,
                   int32_t trace_elem, int64_t interval, double* trace_out, int64_t* ns);
 // implicit.hip
-Implicit* implicit_create(nrt_handle* h, const HostProblem& p);
+// batch_threads > 0: a slot of a batch (one workgroup of that size per step).
+Implicit* implicit_create(nrt_handle* h, const HostProblem& p, int batch_threads = 0);
 void implicit_destroy(Implicit* e);
 void implicit_update(nrt_handle* h, bool active);
 int implicit_calculate(nrt_handle* h);
@@ -129,4 +131,12 @@ int implicit_run(nrt_handle* h, int64_t first, int64_t num, int64_t a0, int64_t 
                  int32_t trace_elem, int64_t interval, double* trace_out, int64_t* ns);
 
 inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+// capi.cpp
+int pick_device(int requested);
+nrt_options resolve_options(const nrt_options* opt);  // defaults + validation
+// A handle on its own stream (shared == nullptr) or on a batch's stream.
+nrt_handle* handle_create(const nrt_problem* p, const nrt_options& o, hipStream_t shared,
+                          int batch_threads);
+void handle_destroy(nrt_handle* h);
 }  // namespace nrt

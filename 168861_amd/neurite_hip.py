@@ -71,7 +71,8 @@ class Options(C.Structure):
                 ("check_every", C.c_int32), ("max_iter", C.c_int32), ("device", C.c_int32),
                 ("num_blocks", C.c_int32), ("tol", C.c_double), ("refine_tol", C.c_double),
                 ("refine_max", C.c_int32), ("precond", C.c_int32), ("chunk", C.c_int32),
-                ("chunk_hi", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("chunk_hi", C.c_int32), ("batch_threads", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):
@@ -80,6 +81,19 @@ class Stats(C.Structure):
                 ("last_residual", C.c_double), ("last_rhs_norm", C.c_double),
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64),
                 ("matrix_nnz", C.c_int64), ("num_blocks", C.c_int32), ("restarts", C.c_int32)]
+
+
+class SlotInfo(C.Structure):
+    _fields_ = [("N", C.c_int32), ("status", C.c_int32), ("steps_done", C.c_int64),
+                ("num_steps", C.c_int64), ("inner_iters", C.c_int64),
+                ("trace_samples", C.c_int64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("slot_steps", C.c_int64),
+                ("compartment_steps", C.c_int64), ("kernel_ms", C.c_double),
+                ("active_slots", C.c_int32), ("threads", C.c_int32),
+                ("device_bytes", C.c_int64)]
 
 
 class NrtError(RuntimeError):
@@ -114,6 +128,19 @@ def lib():
         L.nrt_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.nrt_sync.argtypes = [C.c_void_p]
         L.nrt_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.nrt_batch_create.argtypes = [C.POINTER(Options), C.POINTER(C.c_void_p)]
+        L.nrt_batch_add.argtypes = [C.c_void_p, C.POINTER(ProblemStruct), C.c_int64, C.c_int64,
+                                    C.c_int64, C.c_int32, C.c_int64, C.POINTER(C.c_int32)]
+        L.nrt_batch_step.argtypes = [C.c_void_p, C.c_int64]
+        L.nrt_batch_snapshot.argtypes = [C.c_void_p, C.c_int32, _dp]
+        L.nrt_batch_trace.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int64,
+                                      C.POINTER(C.c_int64)]
+        L.nrt_batch_slot_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SlotInfo)]
+        L.nrt_batch_remove.argtypes = [C.c_void_p, C.c_int32]
+        L.nrt_batch_get_stats.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
+        L.nrt_batch_last_error.argtypes = [C.c_void_p]
+        L.nrt_batch_last_error.restype = C.c_char_p
+        L.nrt_batch_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -259,6 +286,79 @@ class Solver:
         if getattr(self, "h", None):
             lib().nrt_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Batch:
+    """Many independent neurites advanced together (nrt_batch_*): each slot is the
+    problem a Solver would run alone; slots join and leave between steps."""
+
+    def __init__(self, options=None, **kw):
+        opts = options if options is not None else default_options(**kw)
+        b = C.c_void_p()
+        st = lib().nrt_batch_create(C.byref(opts), C.byref(b))
+        if st != 0:
+            raise NrtError(st, lib().nrt_last_error(None).decode() or "batch creation failed")
+        self.b = b
+        self._keep = {}
+
+    def _c(self, st):
+        if st != 0:
+            raise NrtError(st, lib().nrt_batch_last_error(self.b).decode())
+
+    def add(self, problem, num_steps, a0=1, a1=0, trace_elem=0, trace_every=-1):
+        s = problem.struct()
+        slot = C.c_int32(-1)
+        self._c(lib().nrt_batch_add(self.b, C.byref(s), num_steps, a0, a1, trace_elem,
+                                    trace_every, C.byref(slot)))
+        self._keep[slot.value] = problem.N
+        return slot.value
+
+    def step(self, k):
+        self._c(lib().nrt_batch_step(self.b, k))
+
+    def snapshot(self, slot):
+        out = np.empty(self._keep[slot])
+        self._c(lib().nrt_batch_snapshot(self.b, slot, out.ctypes.data_as(_dp)))
+        return out
+
+    def trace(self, slot):
+        n = C.c_int64(0)
+        self._c(lib().nrt_batch_trace(self.b, slot, None, 0, C.byref(n)))
+        out = np.empty(max(n.value, 1))
+        self._c(lib().nrt_batch_trace(self.b, slot, out.ctypes.data_as(_dp), n.value,
+                                      C.byref(n)))
+        return out[:n.value]
+
+    def info(self, slot):
+        i = SlotInfo()
+        self._c(lib().nrt_batch_slot_info(self.b, slot, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in SlotInfo._fields_}
+
+    def remove(self, slot):
+        self._c(lib().nrt_batch_remove(self.b, slot))
+        self._keep.pop(slot, None)
+
+    def stats(self):
+        s = BatchStats()
+        self._c(lib().nrt_batch_get_stats(self.b, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in BatchStats._fields_}
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib().nrt_batch_destroy(self.b)
+            self.b = None
 
     def __del__(self):
         try:

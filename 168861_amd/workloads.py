@@ -187,3 +187,45 @@ def random_tree(seed, depth=14, len_range=(20e-6, 200e-6), root_diam=4e-6, dx=2e
         work.append(("children", last, level, diam))
     geom = b.geometry([material(0), material(1)], myelinate=0)
     return geom, RunSpec(dT, tspan, 0.0, pulse, mes=soma)
+
+
+def _tree_of_size(seed, n_target, depth, dx, strain, amp, dT, tspan):
+    """A random_tree whose compartment count is about n_target: depth is capped so
+    that every segment keeps at least two compartments, and segment lengths are
+    uniform in [0.5, 1.5] x the mean that n_target implies."""
+    depth = int(max(2, min(depth, np.floor(np.log2(n_target / 2.0 + 1.0)))))
+    segs = 2 ** depth - 1
+    mean_len = max(2.0, n_target / segs) * dx
+    return random_tree(seed, depth=depth, len_range=(0.5 * mean_len, 1.5 * mean_len), dx=dx,
+                       strain=strain, amp=amp, dT=dT, tspan=tspan)
+
+
+def sweep_tree(index, n_range=(500, 50000), depth_range=(4, 12), strain=(0.0, 0.3),
+               amp=(1e-9, 20e-9), dx=2e-6, dT=25e-6, steps=200):
+    """Config 4 member `index` (of a batched parameter sweep): compartment count
+    log-uniform in n_range, depth uniform in depth_range (capped as in
+    _tree_of_size), one axial strain per tree uniform in `strain`, stimulus
+    amplitude uniform in `amp`; implicit dt = 25 us, 200 steps."""
+    rng = SplitMix64(seed_for(4, index))
+    n_target = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+    depth = rng.integers(*depth_range)
+    eps = rng.uniform(*strain)
+    a = rng.uniform(*amp)
+    return _tree_of_size(seed_for(4, index) ^ 0x5A5A5A5A, n_target, depth, dx, (eps, eps), a,
+                         dT, steps * dT)
+
+
+def served_request(index, median=20000.0, sigma=1.0, cap=2_000_000, duration=(5e-3, 50e-3),
+                   rate=200.0, dx=2e-6, dT=25e-6):
+    """Config 5 request `index`: compartment count log-normal (median, sigma),
+    capped; simulated duration uniform in `duration`; arrival gap exponential with
+    `rate` per second. Returns (geometry, run spec, inter-arrival gap [s])."""
+    rng = SplitMix64(seed_for(5, index))
+    n_target = min(float(cap), median * float(np.exp(sigma * rng.normal())))
+    tspan = rng.uniform(*duration)
+    gap = -np.log(max(rng.uniform(), 1e-300)) / rate
+    depth = int(np.clip(np.floor(np.log2(n_target / 50.0 + 1.0)), 2, 14))
+    eps = rng.uniform(0.0, 0.3)
+    geom, run = _tree_of_size(seed_for(5, index) ^ 0x5A5A5A5A, n_target, depth, dx, (eps, eps),
+                              10e-9, dT, tspan)
+    return geom, run, gap

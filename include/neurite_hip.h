@@ -176,7 +176,9 @@ typedef struct nrt_options {
   int32_t chunk;        /* tree solve: chain chunk length at level 0 (0 -> auto) */
   int32_t chunk_hi;     /* tree solve: chunk length of the reduced levels
                            (0 -> chunk if set, else auto)                        */
-  int32_t reserved[4];
+  int32_t batch_threads;/* batch (implicit): workgroup size per tree, 256, 512
+                           or 1024 (0 -> autotuned on the first launches)        */
+  int32_t reserved[3];
 } nrt_options;
 
 typedef struct nrt_stats {
@@ -228,6 +230,56 @@ int nrt_run(nrt_handle* h, int64_t first_step, int64_t num_steps,
 int nrt_get_stats(nrt_handle* h, nrt_stats* st);
 /* Synchronise the handle's stream (for timing by the caller). */
 int nrt_sync(nrt_handle* h);
+
+/* ------------------------------------------------------------------ batches
+ * No counterpart in the reference, which advances one neurite per process
+ * (neurite.cpp:185-255). A batch advances many independent neurites ("slots")
+ * together, each slot being exactly the problem a handle would solve alone:
+ *   implicit: one launch per nrt_batch_step moves every active slot by up to k
+ *     steps, one workgroup per tree, each with its own tree solve and its own
+ *     refinement scalars (NRT_DIRECT, FP64 or FP32_REFINE);
+ *   explicit: one thread per element over all slots, one launch per step, the k
+ *     steps of a call captured once as a HIP graph and replayed.
+ * Slots keep their whole state on the device and join or leave between calls
+ * (continuous batching). A slot counts its own steps from 1: its stimulus is on
+ * for steps a0..a1 and it completes after num_steps. Probe traces follow
+ * nrt_run: with trace_every >= 0, V[trace_elem] after every (trace_every+1)-th
+ * step. Calls on one batch are not re-entrant. */
+typedef struct nrt_batch nrt_batch;
+
+typedef struct nrt_slot_info {
+  int32_t N;               /* elements                                          */
+  int32_t status;          /* NRT_OK, or the slot's first failure               */
+  int64_t steps_done;
+  int64_t num_steps;
+  int64_t inner_iters;     /* tree solves / refinement sweeps (implicit)        */
+  int64_t trace_samples;   /* probe samples recorded so far                     */
+} nrt_slot_info;
+
+typedef struct nrt_batch_stats {
+  int64_t launches;          /* step launches (graph replays count each step) */
+  int64_t slot_steps;        /* sum over slots of the steps advanced          */
+  int64_t compartment_steps; /* sum over slots of N x steps advanced          */
+  double kernel_ms;          /* device time of the step launches              */
+  int32_t active_slots;      /* slots with steps left                         */
+  int32_t threads;           /* implicit: workgroup size in use               */
+  int64_t device_bytes;      /* device memory held by the slots               */
+} nrt_batch_stats;
+
+int nrt_batch_create(const nrt_options* opt, nrt_batch** out);
+/* Admits a problem as a new slot, starting at the next step; fails with
+   NRT_ERR_ADMISSION when free device memory does not cover its footprint. */
+int nrt_batch_add(nrt_batch* b, const nrt_problem* p, int64_t num_steps, int64_t a0,
+                  int64_t a1, int32_t trace_elem, int64_t trace_every, int32_t* slot);
+int nrt_batch_step(nrt_batch* b, int64_t k);
+int nrt_batch_snapshot(nrt_batch* b, int32_t slot, double* host_out);
+int nrt_batch_trace(nrt_batch* b, int32_t slot, double* host_out, int64_t capacity,
+                    int64_t* count);
+int nrt_batch_slot_info(nrt_batch* b, int32_t slot, nrt_slot_info* info);
+int nrt_batch_remove(nrt_batch* b, int32_t slot);
+int nrt_batch_get_stats(nrt_batch* b, nrt_batch_stats* out);
+const char* nrt_batch_last_error(const nrt_batch* b);
+int nrt_batch_destroy(nrt_batch* b);
 
 #ifdef __cplusplus
 }

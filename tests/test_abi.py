@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import neurite_hip as nh
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -58,3 +60,30 @@ def test_invalid_problem_is_rejected_before_any_device_work():
     except nh.NrtError as e:
         assert e.status == 1
         assert "dT" in str(e)
+
+
+def test_batch_entry_points_without_device(nrt):
+    """Without a GPU the batch constructor reports NO_DEVICE (no fallback), and the
+    batch calls reject NULL batches."""
+    import ctypes as C
+    L = nrt.lib()
+    opts = nrt.default_options(kind=nrt.IMPLICIT, krylov=nrt.DIRECT)
+    b = C.c_void_p()
+    st = L.nrt_batch_create(C.byref(opts), C.byref(b))
+    if nrt.device_count() > 0:
+        pytest.skip("a device is present")
+    assert st == nrt.STATUS_CODES["no_device"]
+    assert not b.value
+    assert L.nrt_batch_step(None, 1) == nrt.STATUS_CODES["invalid"]
+    assert L.nrt_batch_destroy(None) == 0
+
+
+def test_batch_rejects_krylov_and_bad_threads(nrt):
+    import ctypes as C
+    L = nrt.lib()
+    b = C.c_void_p()
+    for kw in (dict(kind=nrt.IMPLICIT, krylov=nrt.BICGSTAB),
+               dict(kind=nrt.IMPLICIT, krylov=nrt.DIRECT, batch_threads=300)):
+        st = L.nrt_batch_create(C.byref(nrt.default_options(**kw)), C.byref(b))
+        assert st in (nrt.STATUS_CODES["unsupported"], nrt.STATUS_CODES["invalid"])
+        assert not b.value

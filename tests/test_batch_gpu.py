@@ -1,0 +1,146 @@
+"""Batches of independent neurites (nrt_batch_*) on the GPU.
+
+Every slot must give what the same problem gives alone: FP64 implicit slots within
+1e-12 V of the single-problem tree solve and within 1e-9 V of the oracle's exact
+solve; FP32-with-refinement slots within 1e-7 V (1e-4 mV) of the oracle; explicit
+slots bit-identical to the single-problem explicit step (the same element code on
+the same inputs). Slots join at different batch steps (continuous batching) and
+leave when removed; traces continue across calls.
+"""
+import numpy as np
+import pytest
+
+import neurite_hip as nh
+import workloads as wl
+from common import TOL_V, ghost_mask, load_golden, max_abs_diff, problem_from_golden
+
+pytestmark = pytest.mark.gpu
+TOL_MIXED = 1e-7
+TOL_SAME = 1e-12
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _problems(nrt):
+    """Uneven sizes: golden problems, random trees of depth 5..9, a squid axon."""
+    out = []
+    for name in ("ex3_hh_y", "ex6_ct_tree", "ex8_neuron"):
+        g = load_golden(name)
+        out.append(problem_from_golden(g, dT=g["dT_crit"] * 50))
+    for i, depth in enumerate((5, 7, 9)):
+        geom, run = wl.random_tree(wl.seed_for(4, i), depth=depth, amp=(i + 1) * 3e-9)
+        out.append(nrt.setup(geom, dT=run.dT)[0])
+    geom, run = wl.squid_axon(n_comp=600)
+    out.append(nrt.setup(geom, dT=25e-6)[0])
+    return out
+
+
+def _alone(p, steps, a0, a1, **opt):
+    with nh.Solver(p, kind=nh.IMPLICIT, krylov=nh.DIRECT, **opt) as s:
+        s.run(1, steps, a0, a1)
+        return s.snapshot()
+
+
+def _oracle(orc, p, steps, a0, a1):
+    ref = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    ref.run(1, steps, a0, a1, 0, -1)
+    return ref.snapshot()
+
+
+@pytest.mark.parametrize("threads", [256, 1024, 0])
+def test_implicit_batch_matches_alone(threads, nrt, orc):
+    probs = _problems(nrt)
+    steps, a0, a1 = 60, 1, 25
+    _say(f"implicit batch fp64 threads={threads} sizes={[p.N for p in probs]}")
+    with nh.Batch(kind=nh.IMPLICIT, krylov=nh.DIRECT, batch_threads=threads) as b:
+        ids = [b.add(p, steps, a0, a1) for p in probs]
+        for _ in range(4):  # 4 calls of 15 steps (the autotuner takes the first 3)
+            b.step(15)
+        st = b.stats()
+        assert st["active_slots"] == 0
+        assert st["compartment_steps"] == steps * sum(p.N for p in probs)
+        if threads == 0:
+            assert st["threads"] in (256, 512, 1024)
+        for sid, p in zip(ids, probs):
+            v = b.snapshot(sid)
+            info = b.info(sid)
+            assert info["status"] == 0 and info["steps_done"] == steps
+            ref = _alone(p, steps, a0, a1)
+            assert max_abs_diff(v, ref, ghost_mask(p)) <= TOL_SAME
+            exact = _oracle(orc, p, steps, a0, a1)
+            assert max_abs_diff(v, exact, ghost_mask(p)) <= TOL_V
+
+
+def test_implicit_batch_mixed_precision(nrt, orc):
+    probs = _problems(nrt)
+    steps, a0, a1 = 50, 1, 20
+    _say("implicit batch fp32+refinement")
+    with nh.Batch(kind=nh.IMPLICIT, krylov=nh.DIRECT, precision=nh.FP32_REFINE,
+                  refine_tol=1e-10, batch_threads=512) as b:
+        ids = [b.add(p, steps, a0, a1) for p in probs]
+        b.step(steps)
+        # a step whose previous V already meets the residual test takes no sweep
+        assert sum(b.info(sid)["inner_iters"] for sid in ids) > 0
+        for sid, p in zip(ids, probs):
+            assert b.info(sid)["status"] == 0
+            exact = _oracle(orc, p, steps, a0, a1)
+            assert max_abs_diff(b.snapshot(sid), exact, ghost_mask(p)) <= TOL_MIXED
+
+
+def test_continuous_batching_join_leave(nrt, orc):
+    """Slots join at batch steps 0, 7 and 20, one leaves early; each slot's state
+    after its own steps equals the problem run alone for that many steps, and its
+    trace equals the single-problem trace."""
+    probs = _problems(nrt)[:5]
+    lengths = [40, 33, 25, 50, 12]
+    with nh.Batch(kind=nh.IMPLICIT, krylov=nh.DIRECT, batch_threads=256) as b:
+        ids = {}
+        ids[0] = b.add(probs[0], lengths[0], 1, 10, trace_elem=1, trace_every=2)
+        ids[1] = b.add(probs[1], lengths[1], 3, 12, trace_elem=2, trace_every=0)
+        b.step(7)
+        ids[2] = b.add(probs[2], lengths[2], 1, 5, trace_elem=1, trace_every=4)
+        ids[3] = b.add(probs[3], lengths[3], 2, 30)
+        b.step(13)
+        ids[4] = b.add(probs[4], lengths[4], 1, 12)
+        b.remove(ids[3])  # leaves after 13 of its steps
+        while b.stats()["active_slots"] > 0:
+            b.step(6)
+        for k in (0, 1, 2, 4):
+            p, n = probs[k], lengths[k]
+            a0, a1 = {0: (1, 10), 1: (3, 12), 2: (1, 5), 4: (1, 12)}[k]
+            te, ev = {0: (1, 2), 1: (2, 0), 2: (1, 4), 4: (0, -1)}[k]
+            with nh.Solver(p, kind=nh.IMPLICIT, krylov=nh.DIRECT) as s:
+                tr = s.run(1, n, a0, a1, te, ev)
+                v = s.snapshot()
+            assert b.info(ids[k])["steps_done"] == n
+            assert max_abs_diff(b.snapshot(ids[k]), v, ghost_mask(p)) <= TOL_SAME
+            if ev >= 0:
+                bt = b.trace(ids[k])
+                assert len(bt) == len(tr) == n // (ev + 1)
+                assert np.max(np.abs(bt - tr)) <= TOL_SAME
+
+
+def test_explicit_batch_bit_identical(nrt):
+    probs = []
+    for name in ("ex2_hh_axon", "ex3_hh_y", "ex6_ct_tree", "ex8_neuron"):
+        g = load_golden(name)
+        probs.append(problem_from_golden(g, dT=g["dT_crit"] * 0.8))
+    geom, _ = wl.squid_axon(n_comp=700)
+    p, crit = nrt.setup(geom, dT=None, dt_factor=0.8)
+    probs.append(p)
+    steps = 300
+    _say(f"explicit batch sizes={[q.N for q in probs]}")
+    with nh.Batch(kind=nh.EXPLICIT) as b:
+        ids = [b.add(q, steps, 1, 100, trace_elem=1, trace_every=9) for q in probs[:3]]
+        b.step(40)
+        ids += [b.add(q, steps, 1, 100, trace_elem=1, trace_every=9) for q in probs[3:]]
+        while b.stats()["active_slots"] > 0:
+            b.step(40)  # the same graph is replayed
+        for sid, q in zip(ids, probs):
+            with nh.Solver(q, kind=nh.EXPLICIT) as s:
+                tr = s.run(1, steps, 1, 100, 1, 9)
+                v = s.snapshot()
+            assert np.array_equal(b.snapshot(sid), v)
+            assert np.array_equal(b.trace(sid), tr)
