@@ -6,7 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
-#include "nrt_internal.hpp"
+#include "batch.hpp"
 
 namespace nrt {
 // implicit.hip
@@ -59,6 +59,7 @@ using namespace nrt;
 struct nrt_batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t adm = nullptr;  // slot construction, overlapping the step launches
   nrt_options opt{};
   std::vector<std::unique_ptr<Slot>> slots;  // index = slot id; null once removed
   long long g = 0;                           // batch steps done
@@ -72,6 +73,9 @@ struct nrt_batch {
   hipGraphExec_t gexec = nullptr;
   long long graph_k = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool pending = false;  // a launch is in flight (finish() accounts for it)
+  int64_t pend_cs = 0, pend_ss = 0, pend_launches = 0;
+  int pend_threads = 0;
   int threads = 0;                        // implicit workgroup size (0: tuning)
   std::vector<std::pair<int, double>> tune;  // (threads, ms per compartment-step)
   nrt_batch_stats stats{};
@@ -137,37 +141,29 @@ struct nrt_batch {
     return kTuneThreads[tune.size()];
   }
 
-  void step(long long k) {
+  // Enqueues k steps of every active slot and returns; slots added before
+  // finish() join at the batch step after these k.
+  void launch(long long k) {
     NRT_REQUIRE(k >= 0, NRT_ERR_INVALID, "k < 0");
+    NRT_REQUIRE(!pending, NRT_ERR_INVALID, "a launch is already in flight");
     if (dirty) rebuild();
-    if (order.empty() || k == 0) {
-      g += k;
-      return;
-    }
+    const long long g0 = g;
+    g += k;
+    if (order.empty() || k == 0) return;
     int64_t cs = 0, ss = 0;
     for (int id : order) {
       const Slot& s = *slots[id];
-      const long long adv = std::min(k, s.num_steps - s.done(g));
+      const long long adv = std::min(k, s.num_steps - s.done(g0));
       ss += adv;
       cs += adv * s.N;
     }
-    float ms = 0;
     if (is_implicit()) {
       const int th = pick_threads();
       NRT_HIP(hipEventRecord(ev0, stream));
-      implicit_batch_launch(opt.precision, th, table.p, (int)order.size(), g, k, stream);
+      implicit_batch_launch(opt.precision, th, table.p, (int)order.size(), g0, k, stream);
       NRT_HIP(hipEventRecord(ev1, stream));
-      NRT_HIP(hipEventSynchronize(ev1));
-      NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-      if (opt.batch_threads <= 0 && threads == 0 && cs > 0) {
-        tune.emplace_back(th, ms / (double)cs);
-        if (tune.size() == sizeof(kTuneThreads) / sizeof(int))
-          threads = std::min_element(tune.begin(), tune.end(),
-                                     [](auto& a, auto& b) { return a.second < b.second; })
-                        ->first;
-      }
-      stats.threads = th;
-      stats.launches += 1;
+      pend_threads = th;
+      pend_launches = 1;
     } else {
       if (!gexec || graph_k != k) {
         drop_graph();
@@ -181,16 +177,41 @@ struct nrt_batch {
       NRT_HIP(hipEventRecord(ev0, stream));
       NRT_HIP(hipGraphLaunch(gexec, stream));
       NRT_HIP(hipEventRecord(ev1, stream));
-      NRT_HIP(hipEventSynchronize(ev1));
-      NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-      stats.launches += k;
+      pend_launches = k;
     }
-    g += k;
+    pend_cs = cs;
+    pend_ss = ss;
+    pending = true;
+  }
+
+  // Waits for the launch in flight and accounts for it.
+  void finish() {
+    if (!pending) return;
+    pending = false;
+    float ms = 0;
+    NRT_HIP(hipEventSynchronize(ev1));
+    NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    if (is_implicit()) {
+      if (opt.batch_threads <= 0 && threads == 0 && pend_cs > 0) {
+        tune.emplace_back(pend_threads, ms / (double)pend_cs);
+        if (tune.size() == sizeof(kTuneThreads) / sizeof(int))
+          threads = std::min_element(tune.begin(), tune.end(),
+                                     [](auto& a, auto& b) { return a.second < b.second; })
+                        ->first;
+      }
+      stats.threads = pend_threads;
+    }
+    stats.launches += pend_launches;
     stats.kernel_ms += ms;
-    stats.slot_steps += ss;
-    stats.compartment_steps += cs;
+    stats.slot_steps += pend_ss;
+    stats.compartment_steps += pend_cs;
     for (int id : order)
-      if (slots[id]->done(g) >= slots[id]->num_steps) dirty = true;
+      if (slots[id] && slots[id]->done(g) >= slots[id]->num_steps) dirty = true;
+  }
+
+  void step(long long k) {
+    launch(k);
+    finish();
   }
 };
 
@@ -229,6 +250,7 @@ int nrt_batch_create(const nrt_options* opt, nrt_batch** out) {
     b->opt.device = b->device;
     NRT_HIP(hipSetDevice(b->device));
     NRT_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    NRT_HIP(hipStreamCreateWithFlags(&b->adm, hipStreamNonBlocking));
     NRT_HIP(hipEventCreate(&b->ev0));
     NRT_HIP(hipEventCreate(&b->ev1));
     return NRT_OK;
@@ -262,8 +284,11 @@ int nrt_batch_add(nrt_batch* b, const nrt_problem* p, int64_t num_steps, int64_t
                     std::to_string(need >> 20) + " MiB; " + std::to_string(free_b >> 20) +
                     " MiB free");
     auto s = std::make_unique<Slot>();
-    s->h = handle_create(p, b->opt, b->stream,
+    // built on the admission stream (overlapping a launch in flight), then
+    // moved to the batch stream, which every later operation on the slot uses
+    s->h = handle_create(p, b->opt, b->adm,
                          b->opt.batch_threads > 0 ? b->opt.batch_threads : kPlanThreads);
+    s->h->stream = b->stream;
     s->N = p->num_elem;
     s->s0 = b->g;
     s->num_steps = num_steps;
@@ -297,6 +322,7 @@ int nrt_batch_snapshot(nrt_batch* b, int32_t slot, double* host_out) {
     NRT_REQUIRE(host_out, NRT_ERR_INVALID, "host_out is NULL");
     Slot& s = b->slot(slot);
     NRT_HIP(hipSetDevice(b->device));
+    b->finish();
     if (b->is_implicit()) {
       implicit_snapshot(s.h, host_out);
     } else {
@@ -314,6 +340,8 @@ int nrt_batch_trace(nrt_batch* b, int32_t slot, double* host_out, int64_t capaci
   if (!b) return NRT_ERR_INVALID;
   return guarded_b(b, [&] {
     Slot& s = b->slot(slot);
+    NRT_HIP(hipSetDevice(b->device));
+    b->finish();
     const int64_t n =
         s.interval >= 0 ? std::min<int64_t>(s.done(b->g) / (s.interval + 1), (int64_t)s.trace.n)
                         : 0;
@@ -335,6 +363,8 @@ int nrt_batch_slot_info(nrt_batch* b, int32_t slot, nrt_slot_info* info) {
   return guarded_b(b, [&] {
     NRT_REQUIRE(info, NRT_ERR_INVALID, "info is NULL");
     Slot& s = b->slot(slot);
+    NRT_HIP(hipSetDevice(b->device));
+    b->finish();
     memset(info, 0, sizeof(*info));
     info->N = s.N;
     info->steps_done = s.done(b->g);
@@ -357,6 +387,7 @@ int nrt_batch_remove(nrt_batch* b, int32_t slot) {
   return guarded_b(b, [&] {
     b->slot(slot);
     NRT_HIP(hipSetDevice(b->device));
+    b->finish();
     NRT_HIP(hipStreamSynchronize(b->stream));
     b->slots[slot].reset();
     b->dirty = true;
@@ -389,6 +420,7 @@ int nrt_batch_destroy(nrt_batch* b) {
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
   }
+  if (b->adm) (void)hipStreamSynchronize(b->adm);
   b->drop_graph();
   b->slots.clear();
   b->table.release();
@@ -397,8 +429,19 @@ int nrt_batch_destroy(nrt_batch* b) {
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->adm) (void)hipStreamDestroy(b->adm);
   delete b;
   return NRT_OK;
 }
 
 }  // extern "C"
+
+namespace nrt {
+void batch_launch(nrt_batch* b, long long k) { b->launch(k); }
+void batch_finish(nrt_batch* b) { b->finish(); }
+hipStream_t batch_stream(nrt_batch* b) { return b->stream; }
+bool batch_slot_complete(nrt_batch* b, int slot) {
+  const Slot& s = b->slot(slot);
+  return s.done(b->g) >= s.num_steps;
+}
+}  // namespace nrt

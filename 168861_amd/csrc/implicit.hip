@@ -1116,7 +1116,8 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b) {
 }
 
 static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
-                             int32_t trace_elem, int64_t interval, double* trace_dev) {
+                             int32_t trace_elem, int64_t interval, double* trace_dev,
+                             int64_t gl0 = 0) {
   Implicit* e = hnd->im;
   const nrt_options& o = hnd->opt;
   ImpDev d = imp_dev(hnd);
@@ -1125,6 +1126,7 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.a0 = a0;
   d.a1 = a1;
   d.interval = interval;
+  d.gl0 = gl0;
   d.trace_elem = trace_elem;
   d.trace = trace_dev;
   if (e->method == 0) {
@@ -1274,6 +1276,14 @@ void implicit_batch_launch(int precision, int threads, const void* table, int ns
     else hipLaunchKernelGGL((k_batch_tree<float, 1024>), grid, dim3(1024), 0, s, t, g0, k);
   }
   NRT_HIP(hipGetLastError());
+}
+
+// Steps first..first+num-1 enqueued on the handle's stream without waiting; the
+// trace continues a run whose phase is gl0 (steps since its last sample).
+void implicit_steps_async(nrt_handle* h, int64_t first, int64_t num, int64_t a0, int64_t a1,
+                          int32_t trace_elem, int64_t interval, int64_t gl0, double* trace_dev) {
+  launch_persistent(h, first, num, a0, a1, trace_elem, interval, trace_dev, gl0);
+  h->im->step = first + num;
 }
 
 // Status and inner-iteration count of a slot (its failure is reported, not cleared).

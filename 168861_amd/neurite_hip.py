@@ -96,6 +96,26 @@ class BatchStats(C.Structure):
                 ("device_bytes", C.c_int64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("steps_per_launch", C.c_int32), ("big_threshold", C.c_int32),
+                ("max_big", C.c_int32), ("keep_results", C.c_int32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("N", C.c_int32),
+                ("num_steps", C.c_int64), ("steps_done", C.c_int64), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("finish_s", C.c_double),
+                ("trace_samples", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("queued", C.c_int64), ("running", C.c_int64), ("launches", C.c_int64),
+                ("compartment_steps", C.c_int64), ("wall_s", C.c_double),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("batch_kernel_ms", C.c_double)]
+
+
 class NrtError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS.get(status, status)}: {msg}")
@@ -141,6 +161,20 @@ def lib():
         L.nrt_batch_last_error.argtypes = [C.c_void_p]
         L.nrt_batch_last_error.restype = C.c_char_p
         L.nrt_batch_destroy.argtypes = [C.c_void_p]
+        L.nrt_default_server_config.argtypes = [C.POINTER(ServerConfig)]
+        L.nrt_server_create.argtypes = [C.POINTER(Options), C.POINTER(ServerConfig),
+                                        C.POINTER(C.c_void_p)]
+        L.nrt_server_submit.argtypes = [C.c_void_p, C.POINTER(ProblemStruct), C.c_int64,
+                                        C.c_int64, C.c_int64, C.c_int32, C.c_int64,
+                                        C.POINTER(C.c_int64)]
+        L.nrt_server_poll.argtypes = [C.c_void_p, C.POINTER(RequestResult), C.c_int32,
+                                      C.POINTER(C.c_int32)]
+        L.nrt_server_fetch.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64]
+        L.nrt_server_wait_idle.argtypes = [C.c_void_p, C.c_double]
+        L.nrt_server_get_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+        L.nrt_server_last_error.argtypes = [C.c_void_p]
+        L.nrt_server_last_error.restype = C.c_char_p
+        L.nrt_server_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -359,6 +393,83 @@ class Batch:
         if getattr(self, "b", None):
             lib().nrt_batch_destroy(self.b)
             self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """Request server (nrt_server_*): submitted simulations are admitted by free
+    device memory and advanced together (continuous batching) by a native loop."""
+
+    def __init__(self, options=None, steps_per_launch=16, big_threshold=1 << 17, max_big=4,
+                 keep_results=True, **kw):
+        opts = options if options is not None else default_options(**kw)
+        cfg = ServerConfig()
+        lib().nrt_default_server_config(C.byref(cfg))
+        cfg.steps_per_launch = steps_per_launch
+        cfg.big_threshold = big_threshold
+        cfg.max_big = max_big
+        cfg.keep_results = int(bool(keep_results))
+        s = C.c_void_p()
+        st = lib().nrt_server_create(C.byref(opts), C.byref(cfg), C.byref(s))
+        if st != 0:
+            raise NrtError(st, lib().nrt_last_error(None).decode() or "server creation failed")
+        self.s = s
+        self._req = {}
+
+    def _c(self, st):
+        if st != 0:
+            raise NrtError(st, lib().nrt_server_last_error(self.s).decode())
+
+    def submit(self, problem, num_steps, a0=1, a1=0, trace_elem=0, trace_every=-1):
+        ps = problem.struct()
+        rid = C.c_int64(0)
+        self._c(lib().nrt_server_submit(self.s, C.byref(ps), num_steps, a0, a1, trace_elem,
+                                        trace_every, C.byref(rid)))
+        self._req[rid.value] = (problem.N, num_steps // (trace_every + 1)
+                                if trace_every >= 0 else 0)
+        return rid.value
+
+    def poll(self, capacity=4096):
+        buf = (RequestResult * capacity)()
+        n = C.c_int32(0)
+        self._c(lib().nrt_server_poll(self.s, buf, capacity, C.byref(n)))
+        return [{f: getattr(buf[i], f) for f, _ in RequestResult._fields_}
+                for i in range(n.value)]
+
+    def fetch(self, rid):
+        """(status, final V, trace) of a completed request; releases them."""
+        N, ns = self._req.pop(rid)
+        v = np.empty(N)
+        tr = np.empty(max(ns, 1))
+        st = lib().nrt_server_fetch(self.s, rid, v.ctypes.data_as(_dp),
+                                    tr.ctypes.data_as(_dp), ns)
+        if st not in STATUS:
+            self._c(st)
+        return st, v, tr[:ns]
+
+    def wait_idle(self, timeout=-1.0):
+        self._c(lib().nrt_server_wait_idle(self.s, timeout))
+
+    def stats(self):
+        s = ServerStats()
+        self._c(lib().nrt_server_get_stats(self.s, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "s", None):
+            lib().nrt_server_destroy(self.s)
+            self.s = None
 
     def __del__(self):
         try:

@@ -281,6 +281,65 @@ int nrt_batch_get_stats(nrt_batch* b, nrt_batch_stats* out);
 const char* nrt_batch_last_error(const nrt_batch* b);
 int nrt_batch_destroy(nrt_batch* b);
 
+/* ------------------------------------------------------------------ serving
+ * No counterpart in the reference. A server runs a loop thread over one batch:
+ * submitted simulations queue in FIFO order, are admitted as device memory
+ * allows (free HBM from hipMemGetInfo against each request's footprint), keep
+ * their state on the device, and advance together steps_per_launch steps at a
+ * time; requests that arrive meanwhile join at the next launch (continuous
+ * batching) and completed ones leave, their final V and probe trace kept until
+ * fetched. A request above big_threshold elements runs on its own grid-wide
+ * handle, stepped in the same rhythm on the same stream. Submission, polling and
+ * fetching are thread-safe. Latency = completion - submission. */
+typedef struct nrt_server nrt_server;
+
+typedef struct nrt_server_config {
+  int32_t steps_per_launch;  /* k: steps per launch (default 16)                 */
+  int32_t big_threshold;     /* elements above which a request gets its own
+                                grid-wide handle (default 131072)                */
+  int32_t max_big;           /* such requests running at once (default 4)        */
+  int32_t keep_results;      /* keep V and trace until fetched (default 1)       */
+} nrt_server_config;
+
+typedef struct nrt_request_result {
+  int64_t id;
+  int32_t status;            /* NRT_OK, or why the request failed               */
+  int32_t N;
+  int64_t num_steps;
+  int64_t steps_done;
+  double submit_s;           /* seconds since the server was created            */
+  double start_s;            /* admission                                        */
+  double finish_s;           /* completion                                       */
+  int64_t trace_samples;
+} nrt_request_result;
+
+typedef struct nrt_server_stats {
+  int64_t submitted, completed, failed;
+  int64_t queued, running;    /* at the start of the last loop iteration        */
+  int64_t launches;           /* loop iterations that stepped work              */
+  int64_t compartment_steps;  /* of completed requests                          */
+  double wall_s;              /* since creation                                  */
+  double p50_ms, p99_ms, mean_ms;  /* request latency                            */
+  double batch_kernel_ms;     /* device time of the batched launches            */
+} nrt_server_stats;
+
+void nrt_default_server_config(nrt_server_config* c);
+int nrt_server_create(const nrt_options* opt, const nrt_server_config* cfg, nrt_server** out);
+int nrt_server_submit(nrt_server* s, const nrt_problem* p, int64_t num_steps, int64_t a0,
+                      int64_t a1, int32_t trace_elem, int64_t trace_every, int64_t* id);
+/* Completed requests not polled before, up to capacity. */
+int nrt_server_poll(nrt_server* s, nrt_request_result* out, int32_t capacity, int32_t* count);
+/* Copies a completed request's final V [N] and trace (up to trace_capacity) and
+   releases them; returns the request's status. */
+int nrt_server_fetch(nrt_server* s, int64_t id, double* v_out, double* trace_out,
+                     int64_t trace_capacity);
+/* Blocks until nothing is queued or running (timeout_s < 0: no limit). */
+int nrt_server_wait_idle(nrt_server* s, double timeout_s);
+int nrt_server_get_stats(nrt_server* s, nrt_server_stats* out);
+const char* nrt_server_last_error(const nrt_server* s);
+/* Finishes the queued and running requests, then stops the loop. */
+int nrt_server_destroy(nrt_server* s);
+
 #ifdef __cplusplus
 }
 #endif
