@@ -5,17 +5,20 @@
 // k steps) or one graph of k steps (explicit).
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 #include "batch.hpp"
 
 namespace nrt {
 // implicit.hip
 size_t implicit_slot_bytes(int precision);
-void implicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
-                        long long a1, int trace_elem, long long interval, double* trace,
-                        void* out);
-void implicit_batch_launch(int precision, int threads, const void* table, int nslots,
-                           long long g0, long long k, hipStream_t s);
+void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_elem,
+                        long long interval, double* trace, void* out);
+void implicit_batch_launch(int precision, int threads, const void* table, const void* work,
+                           const int2* wgmap, int nwg, hipStream_t s);
+size_t implicit_slot_work_bytes();
+void implicit_slot_work(void* out, long long done, long long steps, int vn);
+int implicit_batch_capacity(int precision, int threads);
 void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner);
 int64_t implicit_device_bytes(const Implicit* e);
 // explicit.hip
@@ -32,6 +35,7 @@ namespace {
 constexpr int kExBlock = 256;            // elements per block of the explicit batch
 constexpr int kTuneThreads[] = {256, 512, 1024};
 constexpr int kPlanThreads = 512;        // workers assumed by a slot's chunk choice
+constexpr int kRowsPerWorkgroup = 24576;  // a slot gets a workgroup per this many rows
 // Footprint per element used for admission (bytes; the arrays a slot allocates,
 // rounded up): implicit tree solve ~ 7 FP64 + 3 int32 vectors, 3 CSR entries, the
 // plan and 8 tree work values over ~1.1 N level nodes; explicit ~ 6 FP64 + code.
@@ -39,17 +43,28 @@ constexpr int64_t kImplicitBytesPerElem = 420;
 constexpr int64_t kExplicitBytesPerElem = 64;
 constexpr int64_t kAdmissionReserve = 512ll << 20;  // left free for the runtime
 
+struct Slot;
+}  // namespace
+int64_t batch_footprint(const nrt_batch* b, int N, int64_t num_steps, int64_t trace_every);
+void batch_admission_check(const nrt_batch* b, int N, int64_t num_steps, int64_t trace_every,
+                           int64_t reserved);
+int batch_plan_threads(const nrt_batch* b);
+int batch_add_handle(nrt_batch* b, nrt_handle* h, int64_t num_steps, int64_t a0, int64_t a1,
+                     int32_t trace_elem, int64_t trace_every);
+namespace {
 struct Slot {
   nrt_handle* h = nullptr;
   int N = 0;
-  long long s0 = 0, num_steps = 0, a0 = 0, a1 = -1, interval = -1;
+  long long s0 = 0;       // batch step at which it joined (explicit: its step = g - s0)
+  long long done = 0;     // steps taken, counting the launch in flight
+  long long num_steps = 0, a0 = 0, a1 = -1, interval = -1;
   int trace_elem = 0;
   DBuf<double> trace;
   int64_t bytes = 0;
   ~Slot() {
     if (h) handle_destroy(h);
   }
-  long long done(long long g) const { return std::min(std::max(0ll, g - s0), num_steps); }
+  long long left() const { return num_steps - done; }
 };
 }  // namespace
 }  // namespace nrt
@@ -68,6 +83,12 @@ struct nrt_batch {
   DBuf<uint8_t> table;
   DBuf<int2> blk;
   int nblk = 0;
+  DBuf<uint8_t> work;  // implicit: steps and workgroups per table entry
+  std::vector<uint8_t> work_h;
+  DBuf<int2> wgmap;    // implicit: (table entry, index in its virtual grid) per workgroup
+  std::vector<int2> wgmap_h;
+  std::vector<int> vn;  // workgroups per table entry in the launch being built
+  std::map<int, int> capacity;  // resident workgroups per workgroup size
   DBuf<long long> gdev;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -101,7 +122,7 @@ struct nrt_batch {
   void rebuild() {
     order.clear();
     for (int id = 0; id < (int)slots.size(); id++)
-      if (slots[id] && slots[id]->done(g) < slots[id]->num_steps) order.push_back(id);
+      if (slots[id] && slots[id]->left() > 0) order.push_back(id);
     drop_graph();
     if (is_implicit()) {
       // largest trees first: the long workgroups start early (LPT order)
@@ -111,10 +132,13 @@ struct nrt_batch {
       std::vector<uint8_t> host(sb * order.size());
       for (size_t k = 0; k < order.size(); k++) {
         Slot& s = *slots[order[k]];
-        implicit_slot_fill(s.h, s.s0, s.num_steps, s.a0, s.a1, s.trace_elem, s.interval,
-                           s.trace.p, host.data() + k * sb);
+        implicit_slot_fill(s.h, s.a0, s.a1, s.trace_elem, s.interval, s.trace.p,
+                           host.data() + k * sb);
       }
       table.upload(host, stream);
+      work.alloc(order.size() * implicit_slot_work_bytes());
+      work_h.resize(order.size() * implicit_slot_work_bytes());
+      vn.assign(order.size(), 1);
     } else {
       const size_t sb = explicit_slot_bytes();
       std::vector<uint8_t> host(sb * order.size());
@@ -141,26 +165,45 @@ struct nrt_batch {
     return kTuneThreads[tune.size()];
   }
 
-  // Enqueues k steps of every active slot and returns; slots added before
-  // finish() join at the batch step after these k.
-  void launch(long long k) {
+  // Enqueues steps of every active slot and returns: k each (lockstep), or, with
+  // elem_budget > 0 (implicit only), max(1, min(kmax, elem_budget / N)) each, so
+  // that one launch takes about the same time whatever the sizes and small
+  // slots do not wait on large ones. Slots added before finish() join after it.
+  void launch(long long k, long long elem_budget = 0, long long kmax = 0) {
     NRT_REQUIRE(k >= 0, NRT_ERR_INVALID, "k < 0");
     NRT_REQUIRE(!pending, NRT_ERR_INVALID, "a launch is already in flight");
+    NRT_REQUIRE(elem_budget <= 0 || is_implicit(), NRT_ERR_UNSUPPORTED,
+                "explicit batches step in lockstep");
     if (dirty) rebuild();
-    const long long g0 = g;
     g += k;
     if (order.empty() || k == 0) return;
+    int th = 0;
+    if (is_implicit()) {
+      th = pick_threads();
+      plan_workgroups(th);
+    }
     int64_t cs = 0, ss = 0;
-    for (int id : order) {
-      const Slot& s = *slots[id];
-      const long long adv = std::min(k, s.num_steps - s.done(g0));
+    const size_t wb = implicit_slot_work_bytes();
+    for (size_t i = 0; i < order.size(); i++) {
+      Slot& s = *slots[order[i]];
+      long long want = k;
+      if (elem_budget > 0) {
+        const long long v = is_implicit() ? vn[i] : 1;
+        want = std::max(1ll, std::min(kmax > 0 ? kmax : k, elem_budget * v / std::max(1, s.N)));
+      }
+      const long long adv = std::min(want, s.left());
+      if (is_implicit()) implicit_slot_work(work_h.data() + i * wb, s.done, adv, vn[i]);
+      s.done += adv;
       ss += adv;
       cs += adv * s.N;
     }
     if (is_implicit()) {
-      const int th = pick_threads();
+      NRT_HIP(hipMemcpyAsync(work.p, work_h.data(), work_h.size(), hipMemcpyHostToDevice,
+                             stream));
+      wgmap.upload(wgmap_h, stream);
       NRT_HIP(hipEventRecord(ev0, stream));
-      implicit_batch_launch(opt.precision, th, table.p, (int)order.size(), g0, k, stream);
+      implicit_batch_launch(opt.precision, th, table.p, work.p, wgmap.p, (int)wgmap_h.size(),
+                            stream);
       NRT_HIP(hipEventRecord(ev1, stream));
       pend_threads = th;
       pend_launches = 1;
@@ -182,6 +225,36 @@ struct nrt_batch {
     pend_cs = cs;
     pend_ss = ss;
     pending = true;
+  }
+
+  // Workgroups per slot: one per kRowsPerWorkgroup rows (at most kMaxSlotBlocks,
+  // at most opt.num_blocks when set), and only while every workgroup of the launch
+  // can be resident at once -- otherwise the largest grids are halved until they
+  // fit, down to one workgroup per slot, which needs no co-residency.
+  void plan_workgroups(int th) {
+    auto it = capacity.find(th);
+    if (it == capacity.end()) it = capacity.emplace(th, implicit_batch_capacity(opt.precision, th)).first;
+    const int cap = it->second;
+    const int vmax = opt.num_blocks > 0 ? std::min(opt.num_blocks, kMaxSlotBlocks) : kMaxSlotBlocks;
+    long long total = 0;
+    for (size_t i = 0; i < order.size(); i++) {
+      const int N = slots[order[i]]->N;
+      vn[i] = std::max(1, std::min(vmax, (N + kRowsPerWorkgroup - 1) / kRowsPerWorkgroup));
+      total += vn[i];
+    }
+    while (total > cap) {
+      size_t big = 0;
+      for (size_t i = 1; i < vn.size(); i++)
+        if (vn[i] > vn[big]) big = i;
+      if (vn[big] == 1) break;  // one workgroup per slot: no co-residency needed
+      const int half = vn[big] / 2;
+      total -= vn[big] - half;
+      vn[big] = half;
+    }
+    if (total > cap) std::fill(vn.begin(), vn.end(), 1);
+    wgmap_h.clear();
+    for (size_t i = 0; i < order.size(); i++)
+      for (int b = 0; b < vn[i]; b++) wgmap_h.push_back(make_int2((int)i, b));
   }
 
   // Waits for the launch in flight and accounts for it.
@@ -206,7 +279,7 @@ struct nrt_batch {
     stats.slot_steps += pend_ss;
     stats.compartment_steps += pend_cs;
     for (int id : order)
-      if (slots[id] && slots[id]->done(g) >= slots[id]->num_steps) dirty = true;
+      if (slots[id] && slots[id]->left() <= 0) dirty = true;
   }
 
   void step(long long k) {
@@ -273,36 +346,10 @@ int nrt_batch_add(nrt_batch* b, const nrt_problem* p, int64_t num_steps, int64_t
     NRT_REQUIRE(trace_every < 0 || (trace_elem >= 0 && trace_elem < p->num_elem),
                 NRT_ERR_INVALID, "trace_elem out of range");
     NRT_HIP(hipSetDevice(b->device));
-    // admission by free device memory
-    size_t free_b = 0, total_b = 0;
-    NRT_HIP(hipMemGetInfo(&free_b, &total_b));
-    const int64_t per = b->is_implicit() ? kImplicitBytesPerElem : kExplicitBytesPerElem;
-    const int64_t samples = trace_every >= 0 ? num_steps / (trace_every + 1) : 0;
-    const int64_t need = per * (int64_t)p->num_elem + 8 * samples;
-    NRT_REQUIRE((int64_t)free_b >= need + kAdmissionReserve, NRT_ERR_ADMISSION,
-                "slot of " + std::to_string(p->num_elem) + " elements needs ~" +
-                    std::to_string(need >> 20) + " MiB; " + std::to_string(free_b >> 20) +
-                    " MiB free");
-    auto s = std::make_unique<Slot>();
-    // built on the admission stream (overlapping a launch in flight), then
-    // moved to the batch stream, which every later operation on the slot uses
-    s->h = handle_create(p, b->opt, b->adm,
-                         b->opt.batch_threads > 0 ? b->opt.batch_threads : kPlanThreads);
-    s->h->stream = b->stream;
-    s->N = p->num_elem;
-    s->s0 = b->g;
-    s->num_steps = num_steps;
-    s->a0 = a0;
-    s->a1 = a1;
-    s->interval = trace_every;
-    s->trace_elem = trace_elem;
-    if (samples > 0) s->trace.alloc((size_t)samples);
-    s->bytes = (b->is_implicit() ? implicit_device_bytes(s->h->im)
-                                 : explicit_device_bytes(s->h->ex)) +
-               8 * samples;
-    *slot = (int32_t)b->slots.size();
-    b->slots.push_back(std::move(s));
-    b->dirty = true;
+    batch_admission_check(b, p->num_elem, num_steps, trace_every, 0);
+    // built on the admission stream (overlapping a launch in flight)
+    nrt_handle* h = handle_create(p, b->opt, b->adm, batch_plan_threads(b));
+    *slot = batch_add_handle(b, h, num_steps, a0, a1, trace_elem, trace_every);
     return NRT_OK;
   });
 }
@@ -326,7 +373,7 @@ int nrt_batch_snapshot(nrt_batch* b, int32_t slot, double* host_out) {
     if (b->is_implicit()) {
       implicit_snapshot(s.h, host_out);
     } else {
-      const double* v = explicit_slot_v(s.h, s.done(b->g));
+      const double* v = explicit_slot_v(s.h, s.done);
       NRT_HIP(hipMemcpyAsync(host_out, v, sizeof(double) * s.N, hipMemcpyDeviceToHost,
                              b->stream));
       NRT_HIP(hipStreamSynchronize(b->stream));
@@ -343,7 +390,7 @@ int nrt_batch_trace(nrt_batch* b, int32_t slot, double* host_out, int64_t capaci
     NRT_HIP(hipSetDevice(b->device));
     b->finish();
     const int64_t n =
-        s.interval >= 0 ? std::min<int64_t>(s.done(b->g) / (s.interval + 1), (int64_t)s.trace.n)
+        s.interval >= 0 ? std::min<int64_t>(s.done / (s.interval + 1), (int64_t)s.trace.n)
                         : 0;
     const int64_t c = std::min(n, std::max<int64_t>(capacity, 0));
     if (c > 0) {
@@ -367,9 +414,9 @@ int nrt_batch_slot_info(nrt_batch* b, int32_t slot, nrt_slot_info* info) {
     b->finish();
     memset(info, 0, sizeof(*info));
     info->N = s.N;
-    info->steps_done = s.done(b->g);
+    info->steps_done = s.done;
     info->num_steps = s.num_steps;
-    info->trace_samples = s.interval >= 0 ? s.done(b->g) / (s.interval + 1) : 0;
+    info->trace_samples = s.interval >= 0 ? s.done / (s.interval + 1) : 0;
     if (b->is_implicit()) {
       NRT_HIP(hipSetDevice(b->device));
       int st = 0;
@@ -404,7 +451,7 @@ int nrt_batch_get_stats(nrt_batch* b, nrt_batch_stats* out) {
     out->device_bytes = 0;
     for (auto& s : b->slots) {
       if (!s) continue;
-      if (s->done(b->g) < s->num_steps) out->active_slots++;
+      if (s->left() > 0) out->active_slots++;
       out->device_bytes += s->bytes;
     }
     if (b->is_implicit()) out->threads = b->pick_threads();
@@ -437,11 +484,59 @@ int nrt_batch_destroy(nrt_batch* b) {
 }  // extern "C"
 
 namespace nrt {
+int64_t batch_footprint(const nrt_batch* b, int N, int64_t num_steps, int64_t trace_every) {
+  const int64_t per = b->is_implicit() ? kImplicitBytesPerElem : kExplicitBytesPerElem;
+  const int64_t samples = trace_every >= 0 ? num_steps / (trace_every + 1) : 0;
+  return per * (int64_t)N + 8 * samples;
+}
+
+void batch_admission_check(const nrt_batch* b, int N, int64_t num_steps, int64_t trace_every,
+                           int64_t reserved) {
+  size_t free_b = 0, total_b = 0;
+  NRT_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int64_t need = batch_footprint(b, N, num_steps, trace_every);
+  NRT_REQUIRE((int64_t)free_b - reserved >= need + kAdmissionReserve, NRT_ERR_ADMISSION,
+              "slot of " + std::to_string(N) + " elements needs ~" + std::to_string(need >> 20) +
+                  " MiB; " + std::to_string(((int64_t)free_b - reserved) >> 20) +
+                  " MiB free");
+}
+
+int batch_plan_threads(const nrt_batch* b) {
+  return b->opt.batch_threads > 0 ? b->opt.batch_threads : kPlanThreads;
+}
+
+const nrt_options& batch_options(const nrt_batch* b) { return b->opt; }
+
+int batch_add_handle(nrt_batch* b, nrt_handle* h, int64_t num_steps, int64_t a0, int64_t a1,
+                     int32_t trace_elem, int64_t trace_every) {
+  auto s = std::make_unique<Slot>();
+  s->h = h;  // owned by the slot from here on, also if the rest throws
+  h->stream = b->stream;  // every later operation on the slot runs in stream order
+  s->N = h->N;
+  s->s0 = b->g;
+  s->num_steps = num_steps;
+  s->a0 = a0;
+  s->a1 = a1;
+  s->interval = trace_every;
+  s->trace_elem = trace_elem;
+  const int64_t samples = trace_every >= 0 ? num_steps / (trace_every + 1) : 0;
+  if (samples > 0) s->trace.alloc((size_t)samples);
+  s->bytes = (b->is_implicit() ? implicit_device_bytes(h->im) : explicit_device_bytes(h->ex)) +
+             8 * samples;
+  const int id = (int)b->slots.size();
+  b->slots.push_back(std::move(s));
+  b->dirty = true;
+  return id;
+}
+
 void batch_launch(nrt_batch* b, long long k) { b->launch(k); }
+void batch_launch_budget(nrt_batch* b, long long k, long long elem_budget, long long kmax) {
+  b->launch(k, elem_budget, kmax);
+}
 void batch_finish(nrt_batch* b) { b->finish(); }
 hipStream_t batch_stream(nrt_batch* b) { return b->stream; }
 bool batch_slot_complete(nrt_batch* b, int slot) {
   const Slot& s = b->slot(slot);
-  return s.done(b->g) >= s.num_steps;
+  return s.left() <= 0;
 }
 }  // namespace nrt

@@ -4,9 +4,22 @@
 
 namespace nrt {
 void batch_launch(nrt_batch* b, long long k);  // enqueue k steps, no wait
+// implicit: each slot takes max(1, min(kmax, elem_budget / N)) steps
+void batch_launch_budget(nrt_batch* b, long long k, long long elem_budget, long long kmax);
 void batch_finish(nrt_batch* b);               // wait for them
 hipStream_t batch_stream(nrt_batch* b);
 bool batch_slot_complete(nrt_batch* b, int slot);  // host-side, no device access
+// Admission: footprint estimate, and a check against free device memory less
+// `reserved` bytes (throws NRT_ERR_ADMISSION).
+int64_t batch_footprint(const nrt_batch* b, int N, int64_t num_steps, int64_t trace_every);
+void batch_admission_check(const nrt_batch* b, int N, int64_t num_steps, int64_t trace_every,
+                           int64_t reserved);
+int batch_plan_threads(const nrt_batch* b);  // batch_threads for handle_create
+const nrt_options& batch_options(const nrt_batch* b);
+// Registers a handle built elsewhere (handle_create with the batch's options and
+// batch_plan_threads) as a new slot; the batch owns it from then on.
+int batch_add_handle(nrt_batch* b, nrt_handle* h, int64_t num_steps, int64_t a0, int64_t a1,
+                     int32_t trace_elem, int64_t trace_every);
 // implicit.hip: steps of a single handle enqueued on its stream
 void implicit_steps_async(nrt_handle* h, int64_t first, int64_t num, int64_t a0, int64_t a1,
                           int32_t trace_elem, int64_t interval, int64_t gl0, double* trace_dev);

@@ -73,7 +73,8 @@ struct ImpStats {
 struct ImpDev {
   int N;
   int H;
-  int nblocks;
+  int nblocks;  // workgroups working on this tree (its "virtual grid")
+  int vb;       // this workgroup's index in that grid
   int rpb;  // rows per block
   double dT;
   double dt_ms;
@@ -393,7 +394,7 @@ __device__ __forceinline__ bool reduce_all(const ImpDev& d, double (&v)[NV], dou
   slot ^= 1;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) part[blockIdx.x * 4 + k] = v[k];
+    for (int k = 0; k < NV; k++) part[d.vb * 4 + k] = v[k];
   }
   grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
   grid_partials_sum<NV>(part, 4, d.nblocks, v, lds);
@@ -402,6 +403,7 @@ __device__ __forceinline__ bool reduce_all(const ImpDev& d, double (&v)[NV], dou
 
 __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) {
   __shared__ double lds[4 * 16];
+  d.vb = (int)blockIdx.x;
   const int r_begin = blockIdx.x * d.rpb;
   const int r_end = min(d.N, r_begin + d.rpb);
   const double* val = d.val;  // HH rows are rewritten through d.val in the update phase
@@ -624,14 +626,14 @@ __device__ __forceinline__ void tsync(const ImpDev& d) {
 
 template <class T, bool GRID>
 __device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w, bool acc) {
-  const int wid = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)threadIdx.x;
-  const int nw = GRID ? (int)(gridDim.x * blockDim.x) : (int)blockDim.x;
+  const int wid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
+  const int nw = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
   for (int lv = 0; lv < P.levels; lv++) {
     const int c1 = P.chunk_off[lv + 1];
     for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw) tps_reduce_chunk(P, w, lv, ch);
     tsync<GRID>(d);
   }
-  if (!GRID || blockIdx.x == 0) {
+  if (!GRID || d.vb == 0) {
     for (int h = 0; h < P.heights; h++) {
       for (int k = P.fin_hoff[h] + (int)threadIdx.x; k < P.fin_hoff[h + 1]; k += blockDim.x)
         tps_final_up(P, w, P.fin_order[k]);
@@ -676,8 +678,8 @@ __device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>
 template <class T, bool GRID>
 __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& t, int r0, int r1,
                                             bool active) {
-  const int tid = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)threadIdx.x;
-  const int nt = GRID ? (int)(gridDim.x * blockDim.x) : (int)blockDim.x;
+  const int tid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
+  const int nt = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
   const double* __restrict__ xv = d.x;
   const uint32_t* __restrict__ code = d.code;
   const double* __restrict__ cmdx = d.cmdx;
@@ -733,7 +735,7 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
 // it; else one workgroup does).
 template <class T, bool GRID>
 __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t, double* lds) {
-  const int r_begin = GRID ? (int)blockIdx.x * d.rpb : 0;
+  const int r_begin = GRID ? d.vb * d.rpb : 0;
   const int r_end = GRID ? min(d.N, r_begin + d.rpb) : d.N;
   const int bs = blockDim.x;
   const double* val = d.val;
@@ -801,7 +803,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     if (GRID && __hip_atomic_load(d.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
       ok = false;
     if (!ok) status = NRT_ERR_HIP;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (d.vb == 0 && threadIdx.x == 0) {
       d.st->iters_total += (unsigned long long)sweeps;
       d.st->steps += 1;
       d.st->last_iters = sweeps;
@@ -828,37 +830,53 @@ template <class T, bool GRID>
 __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree(ImpDev d,
                                                                                  TreeArgs<T> t) {
   __shared__ double lds[4 * 16];
+  d.vb = GRID ? (int)blockIdx.x : 0;
   tree_steps<T, GRID>(d, t, lds);
 }
 
 // Batched form: independent trees ("slots"), one workgroup each, no grid-wide
 // dependency. A slot's descriptor is static while the batch's membership does not
-// change; the launch passes the batch's step counter g0 and the step count k, and
-// every slot derives its own step numbers from the step at which it joined (s0).
+// change; each launch passes, per slot, the steps it has done and the steps it
+// takes now (slots may take different numbers of steps in one launch).
 template <class T>
 struct BatchSlot {
   ImpDev d;
   TreeArgs<T> t;
-  long long s0;         // batch step at which the slot joined
-  long long num_steps;  // the slot's length
-  double* trace_base;   // probe samples, num_steps / (interval + 1) of them
+  double* trace_base;  // probe samples, num_steps / (interval + 1) of them
+};
+
+// Steps of a slot in one launch: done before it, taken now, and the workgroups
+// that take them (vn > 1: a virtual grid of co-resident workgroups with the
+// slot's own grid barrier; the launch guarantees co-residency by its size).
+struct SlotWork {
+  long long done, steps;
+  int vn, pad;
 };
 
 template <class T, int BS>
 __global__ __launch_bounds__(BS) void k_batch_tree(const BatchSlot<T>* __restrict__ slots,
-                                                  long long g0, long long k) {
+                                                  const SlotWork* __restrict__ work,
+                                                  const int2* __restrict__ wgmap) {
   __shared__ double lds[4 * 16];
-  const BatchSlot<T>& S = slots[blockIdx.x];
-  const long long done = g0 - S.s0;
-  if (done < 0 || done >= S.num_steps) return;
+  const int2 m = wgmap[blockIdx.x];  // (slot, workgroup index in its grid)
+  const SlotWork wk = work[m.x];
+  if (wk.steps <= 0) return;
+  const BatchSlot<T>& S = slots[m.x];
   ImpDev d = S.d;
-  d.first = done + 1;
-  d.num = min(k, S.num_steps - done);
+  d.first = wk.done + 1;
+  d.num = wk.steps;
   if (d.interval >= 0) {
-    d.gl0 = done % (d.interval + 1);
-    d.trace = S.trace_base ? S.trace_base + done / (d.interval + 1) : nullptr;
+    d.gl0 = wk.done % (d.interval + 1);
+    d.trace = S.trace_base ? S.trace_base + wk.done / (d.interval + 1) : nullptr;
   }
-  tree_steps<T, false>(d, S.t, lds);
+  d.vb = m.y;
+  d.nblocks = wk.vn;
+  if (wk.vn > 1) {
+    d.rpb = (d.N + wk.vn - 1) / wk.vn;
+    tree_steps<T, true>(d, S.t, lds);
+  } else {
+    tree_steps<T, false>(d, S.t, lds);
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -1038,7 +1056,9 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       e->nblocks = choose_blocks(hnd, N);
       e->threads = kImpBlock;
     }
-    e->part.alloc((size_t)e->nblocks * 4 * 2);
+    // partial sums: two alternating sets of 4 per workgroup (a batch slot may
+    // run as a virtual grid of up to kMaxSlotBlocks workgroups)
+    e->part.alloc((size_t)std::max(e->nblocks, batch_threads > 0 ? kMaxSlotBlocks : 1) * 4 * 2);
     e->bar.alloc(4);
     NRT_HIP(hipMemsetAsync(e->bar.p, 0, sizeof(unsigned) * 4, s));
     e->st.alloc(1);
@@ -1240,9 +1260,8 @@ size_t implicit_slot_bytes(int precision) {
   return precision == NRT_FP64 ? sizeof(BatchSlot<double>) : sizeof(BatchSlot<float>);
 }
 
-void implicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
-                        long long a1, int trace_elem, long long interval, double* trace,
-                        void* out) {
+void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_elem,
+                        long long interval, double* trace, void* out) {
   Implicit* e = h->im;
   NRT_REQUIRE(e->method != 0 && !e->grid_mode, NRT_ERR_UNSUPPORTED,
               "batch slots use the block-mode tree solve");
@@ -1252,28 +1271,59 @@ void implicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long l
   d.interval = interval;
   d.trace_elem = trace_elem;
   if (e->method == 1) {
-    BatchSlot<double> S{d, tree_args<double>(e, h->opt, e->tw64.p), s0, num_steps, trace};
+    BatchSlot<double> S{d, tree_args<double>(e, h->opt, e->tw64.p), trace};
     std::memcpy(out, &S, sizeof(S));
   } else {
-    BatchSlot<float> S{d, tree_args<float>(e, h->opt, e->tw32.p), s0, num_steps, trace};
+    BatchSlot<float> S{d, tree_args<float>(e, h->opt, e->tw32.p), trace};
     std::memcpy(out, &S, sizeof(S));
   }
 }
 
-void implicit_batch_launch(int precision, int threads, const void* table, int nslots,
-                           long long g0, long long k, hipStream_t s) {
-  if (nslots <= 0) return;
-  const dim3 grid(nslots);
+size_t implicit_slot_work_bytes() { return sizeof(SlotWork); }
+
+void implicit_slot_work(void* out, long long done, long long steps, int vn) {
+  SlotWork w{done, steps, vn, 0};
+  std::memcpy(out, &w, sizeof(w));
+}
+
+template <class T, int BS>
+static int batch_capacity_of() {
+  int dev = 0, cus = 0, per = 0;
+  NRT_HIP(hipGetDevice(&dev));
+  NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_batch_tree<T, BS>, BS,
+                                                       0));
+  return cus * std::max(per, 0);
+}
+
+// Workgroups of the batched kernel that can be resident at once (an otherwise
+// idle device): virtual grids of several workgroups are only formed within it.
+int implicit_batch_capacity(int precision, int threads) {
+  if (precision == NRT_FP64) {
+    if (threads == 256) return batch_capacity_of<double, 256>();
+    if (threads == 512) return batch_capacity_of<double, 512>();
+    return batch_capacity_of<double, 1024>();
+  }
+  if (threads == 256) return batch_capacity_of<float, 256>();
+  if (threads == 512) return batch_capacity_of<float, 512>();
+  return batch_capacity_of<float, 1024>();
+}
+
+void implicit_batch_launch(int precision, int threads, const void* table, const void* work,
+                           const int2* wgmap, int nwg, hipStream_t s) {
+  if (nwg <= 0) return;
+  const dim3 grid(nwg);
+  const auto* w = (const SlotWork*)work;
   if (precision == NRT_FP64) {
     auto* t = (const BatchSlot<double>*)table;
-    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<double, 256>), grid, dim3(256), 0, s, t, g0, k);
-    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<double, 512>), grid, dim3(512), 0, s, t, g0, k);
-    else hipLaunchKernelGGL((k_batch_tree<double, 1024>), grid, dim3(1024), 0, s, t, g0, k);
+    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<double, 256>), grid, dim3(256), 0, s, t, w, wgmap);
+    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<double, 512>), grid, dim3(512), 0, s, t, w, wgmap);
+    else hipLaunchKernelGGL((k_batch_tree<double, 1024>), grid, dim3(1024), 0, s, t, w, wgmap);
   } else {
     auto* t = (const BatchSlot<float>*)table;
-    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<float, 256>), grid, dim3(256), 0, s, t, g0, k);
-    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<float, 512>), grid, dim3(512), 0, s, t, g0, k);
-    else hipLaunchKernelGGL((k_batch_tree<float, 1024>), grid, dim3(1024), 0, s, t, g0, k);
+    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<float, 256>), grid, dim3(256), 0, s, t, w, wgmap);
+    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<float, 512>), grid, dim3(512), 0, s, t, w, wgmap);
+    else hipLaunchKernelGGL((k_batch_tree<float, 1024>), grid, dim3(1024), 0, s, t, w, wgmap);
   }
   NRT_HIP(hipGetLastError());
 }

@@ -1,17 +1,22 @@
 // Request server (the nrt_server_* entry points): continuous batching of
 // independent simulations on one device.
 //
-// A loop thread owns one batch. Each iteration it
-//   1. takes newly submitted requests into its FIFO,
-//   2. enqueues k steps of every running request: the batched launch for the
-//      requests in batch slots, then one cooperative launch per large request
+// Builder threads take submitted requests in FIFO order once their footprint
+// fits in free device memory (hipMemGetInfo, less what other builders are about
+// to allocate) and build their device state (assembly, tree plan) on their own
+// streams, concurrently with the device work. A loop thread owns one batch; each
+// iteration it
+//   1. moves built requests into the batch (or the large-request list),
+//   2. enqueues steps of every running request: the batched launch for the
+//      requests in batch slots (about k * 16384 element-steps each, so a launch
+//      lasts about as long whatever the mix of sizes and small requests are not
+//      held back by large ones), then k steps of each large request, in one
+//      cooperative launch per request
 //      (more elements than big_threshold, which would hold a whole batch launch
 //      back if one workgroup had to carry it), all on the batch stream,
-//   3. admits waiting requests while that work runs: slot construction (device
-//      assembly, the tree plan) overlaps the launch; admission stops at the first
-//      request whose footprint does not fit in free device memory,
-//   4. waits for the launch, then retires completed requests: final V and the
-//      probe trace are copied out and the slot is freed.
+//   3. waits for the launch, then retires completed requests: final V and the
+//      probe trace are copied out and the slot is freed (which may let the FIFO
+//      head in).
 // Latency of a request = completion time - submission time (steady clock).
 #include <algorithm>
 #include <chrono>
@@ -26,6 +31,10 @@
 
 namespace nrt {
 namespace {
+
+// Implicit slots take steps_per_launch steps per launch at this size and
+// proportionally more when smaller (up to 8x), fewer when larger (at least 1).
+constexpr long long kBudgetElems = 16384;
 
 double now_s() {
   using namespace std::chrono;
@@ -100,15 +109,32 @@ struct BigRun {
 
 using namespace nrt;
 
+// A request whose device state has been built by a builder thread.
+struct Built {
+  Request req;
+  nrt_handle* h = nullptr;
+  bool big = false;
+  int status = NRT_OK;
+  std::string err;
+};
+
 struct nrt_server {
   nrt_options opt{};
   nrt_server_config cfg{};
   nrt_batch* batch = nullptr;
   std::thread loop;
+  std::vector<std::thread> builders;
   std::mutex mu;
-  std::condition_variable cv, idle_cv;
-  // shared with the callers (guarded by mu)
-  std::deque<Request> inbox;
+  std::condition_variable cv;        // loop thread: submissions, built requests, stop
+  std::condition_variable build_cv;  // builders: queue head admissible, stop
+  std::condition_variable idle_cv;   // wait_idle
+  // shared (guarded by mu)
+  std::deque<Request> to_build;  // FIFO of submitted requests
+  std::deque<Built> ready;       // built, waiting to join the batch
+  int building = 0;              // requests being built
+  int64_t reserved = 0;          // footprint of those, not yet visible as allocated
+  int big_count = 0;             // large requests built, building or running
+  int64_t active = 0;            // requests in the batch or running large
   std::map<int64_t, Result> results;
   std::deque<int64_t> unpolled;
   std::vector<double> latencies;
@@ -120,9 +146,16 @@ struct nrt_server {
   int loop_status = NRT_OK;
   double t_created = 0;
   // loop thread only
-  std::deque<Request> waiting;
   std::map<int, Request> running;  // batch slot -> request
   std::vector<std::unique_ptr<BigRun>> big;
+
+  bool is_big(const Request& r) const {
+    return opt.kind == NRT_IMPLICIT && r.N() > cfg.big_threshold;
+  }
+
+  int64_t footprint(const Request& r) const {
+    return is_big(r) ? (int64_t)r.N() * 600 : batch_footprint(batch, r.N(), r.num_steps, r.every);
+  }
 
   void finish_request(Request& r, int status, std::vector<double> V, std::vector<double> trace,
                       int64_t steps_done) {
@@ -152,49 +185,120 @@ struct nrt_server {
     results.emplace(r.id, std::move(res));
   }
 
-  // Admission in FIFO order; returns when the head does not fit.
-  void admit() {
-    while (!waiting.empty()) {
-      Request& r = waiting.front();
-      const bool large = opt.kind == NRT_IMPLICIT && r.N() > cfg.big_threshold;
-      if (large) {
-        if ((int)big.size() >= cfg.max_big) return;
-        auto br = std::make_unique<BigRun>();
-        try {
-          size_t free_b = 0, total_b = 0;
-          NRT_HIP(hipMemGetInfo(&free_b, &total_b));
-          NRT_REQUIRE((int64_t)free_b > (int64_t)r.N() * 600 + (512ll << 20), NRT_ERR_ADMISSION,
-                      "not enough free device memory");
+  // Builder threads: take the FIFO head when its footprint fits in free device
+  // memory (less what other builders are about to allocate), build its device
+  // state on their own stream, and hand it to the loop. When nothing at all is
+  // running or being built, the head is taken anyway, and fails if it does not
+  // fit: nothing will ever free memory for it.
+  void build_loop() {
+    hipStream_t stream = nullptr;
+    try {
+      NRT_HIP(hipSetDevice(opt.device));
+      NRT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    } catch (const Error& e) {
+      std::lock_guard<std::mutex> g(mu);
+      loop_status = e.status;
+      loop_error = e.what();
+      cv.notify_all();
+      return;
+    }
+    while (true) {
+      Request r;
+      int64_t need = 0;
+      bool big_req = false;
+      {
+        std::unique_lock<std::mutex> g(mu);
+        while (true) {
+          if (stop && to_build.empty()) {
+            if (stream) (void)hipStreamDestroy(stream);
+            return;
+          }
+          if (!to_build.empty()) {
+            Request& head = to_build.front();
+            big_req = is_big(head);
+            need = footprint(head);
+            size_t free_b = 0, total_b = 0;
+            (void)hipMemGetInfo(&free_b, &total_b);
+            const bool nothing_else = active == 0 && building == 0 && ready.empty();
+            const bool fits = (int64_t)free_b - reserved >= need + (512ll << 20);
+            const bool big_ok = !big_req || big_count < cfg.max_big;
+            if ((fits && big_ok) || nothing_else) break;
+          }
+          build_cv.wait(g);
+        }
+        r = std::move(to_build.front());
+        to_build.pop_front();
+        building++;
+        reserved += need;
+        if (big_req) big_count++;
+      }
+      Built b;
+      b.big = big_req;
+      try {
+        if (big_req) {
+          NRT_REQUIRE(cfg.max_big > 0, NRT_ERR_ADMISSION, "large requests are disabled");
           nrt_options o = opt;
           o.num_blocks = 0;  // grid mode for a tree this size
-          br->h = handle_create(&r.prob->view, o, batch_stream(batch), 0);
-          if (r.every >= 0 && r.num_steps / (r.every + 1) > 0)
-            br->trace.alloc((size_t)(r.num_steps / (r.every + 1)));
-        } catch (const Error& e) {
-          if (e.status == NRT_ERR_ADMISSION && (!running.empty() || !big.empty())) return;
-          finish_request(r, e.status, {}, {}, 0);
-          waiting.pop_front();
-          continue;
+          b.h = handle_create(&r.prob->view, o, stream, 0);
+        } else {
+          batch_admission_check(batch, r.N(), r.num_steps, r.every, 0);
+          b.h = handle_create(&r.prob->view, batch_options(batch), stream,
+                              batch_plan_threads(batch));
         }
-        r.t_start = now_s();
-        br->req = std::move(r);
-        waiting.pop_front();
-        big.push_back(std::move(br));
-        continue;
+      } catch (const Error& e) {
+        b.status = e.status;
+        b.err = e.what();
+      } catch (const std::exception& e) {
+        b.status = NRT_ERR_INVALID;
+        b.err = e.what();
       }
-      int32_t slot = -1;
-      const int st = nrt_batch_add(batch, &r.prob->view, r.num_steps, r.a0, r.a1, r.trace_elem,
-                                   r.every, &slot);
-      if (st == NRT_ERR_ADMISSION && (!running.empty() || !big.empty())) return;  // wait
-      if (st != NRT_OK) {
-        finish_request(r, st, {}, {}, 0);
-        waiting.pop_front();
-        continue;
-      }
-      r.t_start = now_s();
-      running.emplace(slot, std::move(r));
-      waiting.pop_front();
+      b.req = std::move(r);
+      std::lock_guard<std::mutex> g(mu);
+      building--;
+      reserved -= need;
+      ready.push_back(std::move(b));
+      cv.notify_all();
     }
+  }
+
+  // Loop thread: built requests join the batch (or the large-request list).
+  void join_ready(std::deque<Built>& got) {
+    for (Built& b : got) {
+      Request& r = b.req;
+      if (b.status == NRT_OK) {
+        r.t_start = now_s();
+        try {
+          if (b.big) {
+            auto br = std::make_unique<BigRun>();
+            br->h = b.h;  // owned by br from here on
+            b.h = nullptr;
+            br->h->stream = batch_stream(batch);  // stepped in the loop's rhythm
+            if (r.every >= 0 && r.num_steps / (r.every + 1) > 0)
+              br->trace.alloc((size_t)(r.num_steps / (r.every + 1)));
+            br->req = std::move(r);
+            big.push_back(std::move(br));
+          } else {
+            nrt_handle* h = b.h;
+            b.h = nullptr;  // owned by the batch from here on
+            const int slot =
+                batch_add_handle(batch, h, r.num_steps, r.a0, r.a1, r.trace_elem, r.every);
+            running.emplace(slot, std::move(r));
+          }
+          std::lock_guard<std::mutex> g(mu);
+          active++;
+          continue;
+        } catch (const Error& e) {
+          b.status = e.status;
+          b.err = e.what();
+        }
+      }
+      if (b.big) {
+        std::lock_guard<std::mutex> g(mu);
+        big_count--;
+      }
+      finish_request(b.req, b.status, {}, {}, 0);
+    }
+    got.clear();
   }
 
   void launch_big(long long k) {
@@ -215,6 +319,7 @@ struct nrt_server {
   }
 
   void retire() {
+    int freed = 0;
     for (auto it = running.begin(); it != running.end();) {
       if (!batch_slot_complete(batch, it->first)) {
         ++it;
@@ -237,6 +342,7 @@ struct nrt_server {
       finish_request(r, si.status, std::move(V), std::move(tr), si.steps_done);
       nrt_batch_remove(batch, it->first);
       it = running.erase(it);
+      freed++;
     }
     for (auto it = big.begin(); it != big.end();) {
       BigRun& b = **it;
@@ -259,64 +365,74 @@ struct nrt_server {
       implicit_slot_stats(b.h, &status, &inner);
       finish_request(b.req, status, std::move(V), std::move(tr), b.done);
       it = big.erase(it);
+      std::lock_guard<std::mutex> g(mu);
+      big_count--;
+      freed++;
+    }
+    if (freed) {
+      std::lock_guard<std::mutex> g(mu);
+      active -= freed;
+      build_cv.notify_all();  // memory freed: the FIFO head may fit now
     }
   }
 
   void run() {
     try {
       NRT_HIP(hipSetDevice(opt.device));
+      std::deque<Built> got;
       while (true) {
         {
           std::unique_lock<std::mutex> g(mu);
-          const bool idle = waiting.empty() && running.empty() && big.empty();
-          if (idle) {
-            busy = false;
-            idle_cv.notify_all();
-            cv.wait(g, [&] { return stop || !inbox.empty(); });
-            if (stop && inbox.empty()) return;
+          const auto has_work = [&] { return !running.empty() || !big.empty(); };
+          if (!has_work() && ready.empty()) {
+            if (to_build.empty() && building == 0) {
+              busy = false;
+              idle_cv.notify_all();
+              if (stop) return;
+            }
+            cv.wait(g, [&] { return stop || !ready.empty() || loop_status != NRT_OK; });
+            if (loop_status != NRT_OK) return;
+            if (stop && ready.empty() && to_build.empty() && building == 0) return;
+            if (ready.empty()) continue;
           }
           busy = true;
-          while (!inbox.empty()) {
-            waiting.push_back(std::move(inbox.front()));
-            inbox.pop_front();
-          }
-          stats.queued = (int64_t)waiting.size();
-          stats.running = (int64_t)(running.size() + big.size());
+          got.swap(ready);
+          stats.queued = (int64_t)to_build.size() + building;
+          stats.running = (int64_t)(running.size() + big.size() + got.size());
         }
+        const double t0 = now_s();
+        join_ready(got);
+        if (running.empty() && big.empty()) continue;  // every joiner failed
         const long long k = cfg.steps_per_launch;
-        const bool work = !running.empty() || !big.empty();
-        if (work) {
-          batch_launch(batch, k);
-          launch_big(k);
-        }
-        admit();  // overlaps the launch
-        if (work) batch_finish(batch);
+        if (opt.kind == NRT_IMPLICIT) batch_launch_budget(batch, k, k * kBudgetElems, 8 * k);
+        else batch_launch(batch, k);
+        launch_big(k);
+        const double t1 = now_s();
+        batch_finish(batch);
         NRT_HIP(hipStreamSynchronize(batch_stream(batch)));
-        if (work) {
-          std::lock_guard<std::mutex> g(mu);
-          stats.launches++;
-        }
+        const double t2 = now_s();
         retire();
-        if (!work && running.empty() && big.empty() && !waiting.empty()) {
-          // nothing runs and the head still does not fit: it never will
-          Request& r = waiting.front();
-          finish_request(r, NRT_ERR_ADMISSION, {}, {}, 0);
-          waiting.pop_front();
-        }
+        const double t3 = now_s();
+        std::lock_guard<std::mutex> g(mu);
+        stats.launches++;
+        stats.launch_ms += (t1 - t0) * 1e3;
+        stats.wait_ms += (t2 - t1) * 1e3;
+        stats.retire_ms += (t3 - t2) * 1e3;
       }
     } catch (const Error& e) {
-      std::lock_guard<std::mutex> g(mu);
-      loop_status = e.status;
-      loop_error = e.what();
-      busy = false;
-      idle_cv.notify_all();
+      fail(e.status, e.what());
     } catch (const std::exception& e) {
-      std::lock_guard<std::mutex> g(mu);
-      loop_status = NRT_ERR_HIP;
-      loop_error = e.what();
-      busy = false;
-      idle_cv.notify_all();
+      fail(NRT_ERR_HIP, e.what());
     }
+  }
+
+  void fail(int status, const std::string& what) {
+    std::lock_guard<std::mutex> g(mu);
+    loop_status = status;
+    loop_error = what;
+    busy = false;
+    idle_cv.notify_all();
+    build_cv.notify_all();
   }
 };
 
@@ -344,9 +460,10 @@ void nrt_default_server_config(nrt_server_config* c) {
   if (!c) return;
   memset(c, 0, sizeof(*c));
   c->steps_per_launch = 16;
-  c->big_threshold = 1 << 17;
+  c->big_threshold = 1 << 30;  // large slots run as virtual grids inside the batch
   c->max_big = 4;
   c->keep_results = 1;
+  c->builders = 4;
 }
 
 int nrt_server_create(const nrt_options* opt, const nrt_server_config* cfg, nrt_server** out) {
@@ -356,7 +473,8 @@ int nrt_server_create(const nrt_options* opt, const nrt_server_config* cfg, nrt_
   int st = guarded_s(s, [&] {
     if (cfg) s->cfg = *cfg;
     else nrt_default_server_config(&s->cfg);
-    NRT_REQUIRE(s->cfg.steps_per_launch >= 1 && s->cfg.max_big >= 0 && s->cfg.big_threshold >= 3,
+    NRT_REQUIRE(s->cfg.steps_per_launch >= 1 && s->cfg.max_big >= 0 &&
+                    s->cfg.big_threshold >= 3 && s->cfg.builders >= 1 && s->cfg.builders <= 16,
                 NRT_ERR_INVALID, "bad server configuration");
     s->opt = resolve_options(opt);
     s->opt.device = pick_device(s->opt.device);
@@ -364,6 +482,7 @@ int nrt_server_create(const nrt_options* opt, const nrt_server_config* cfg, nrt_
     if (bst != NRT_OK) throw Error(bst, "batch: " + std::string(nrt_last_error(nullptr)));
     s->t_created = now_s();
     s->loop = std::thread([s] { s->run(); });
+    for (int i = 0; i < s->cfg.builders; i++) s->builders.emplace_back([s] { s->build_loop(); });
     return NRT_OK;
   });
   if (st != NRT_OK) {
@@ -396,8 +515,9 @@ int nrt_server_submit(nrt_server* s, const nrt_problem* p, int64_t num_steps, in
     r.id = s->next_id++;
     if (id) *id = r.id;
     s->stats.submitted++;
-    s->inbox.push_back(std::move(r));
-    s->cv.notify_one();
+    s->busy = true;
+    s->to_build.push_back(std::move(r));
+    s->build_cv.notify_all();
     return NRT_OK;
   });
 }
@@ -440,7 +560,7 @@ int nrt_server_wait_idle(nrt_server* s, double timeout_s) {
   if (!s) return NRT_ERR_INVALID;
   return guarded_s(s, [&] {
     std::unique_lock<std::mutex> g(s->mu);
-    const auto pred = [&] { return (!s->busy && s->inbox.empty()) || s->loop_status != NRT_OK; };
+    const auto pred = [&] { return !s->busy || s->loop_status != NRT_OK; };
     if (timeout_s < 0) s->idle_cv.wait(g, pred);
     else if (!s->idle_cv.wait_for(g, std::chrono::duration<double>(timeout_s), pred))
       throw Error(NRT_ERR_NOT_CONVERGED, "server still busy after the timeout");
@@ -484,8 +604,18 @@ int nrt_server_destroy(nrt_server* s) {
     std::lock_guard<std::mutex> g(s->mu);
     s->stop = true;
     s->cv.notify_all();
+    s->build_cv.notify_all();
+  }
+  for (auto& t : s->builders)
+    if (t.joinable()) t.join();
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->cv.notify_all();
   }
   if (s->loop.joinable()) s->loop.join();
+  for (Built& b : s->ready)
+    if (b.h) handle_destroy(b.h);
+  s->ready.clear();
   s->running.clear();
   s->big.clear();
   if (s->batch) nrt_batch_destroy(s->batch);

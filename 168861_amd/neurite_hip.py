@@ -98,7 +98,7 @@ class BatchStats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("steps_per_launch", C.c_int32), ("big_threshold", C.c_int32),
-                ("max_big", C.c_int32), ("keep_results", C.c_int32)]
+                ("max_big", C.c_int32), ("keep_results", C.c_int32), ("builders", C.c_int32)]
 
 
 class RequestResult(C.Structure):
@@ -113,7 +113,8 @@ class ServerStats(C.Structure):
                 ("queued", C.c_int64), ("running", C.c_int64), ("launches", C.c_int64),
                 ("compartment_steps", C.c_int64), ("wall_s", C.c_double),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
-                ("batch_kernel_ms", C.c_double)]
+                ("batch_kernel_ms", C.c_double), ("launch_ms", C.c_double),
+                ("admit_ms", C.c_double), ("wait_ms", C.c_double), ("retire_ms", C.c_double)]
 
 
 class NrtError(RuntimeError):
@@ -411,8 +412,8 @@ class Server:
     """Request server (nrt_server_*): submitted simulations are admitted by free
     device memory and advanced together (continuous batching) by a native loop."""
 
-    def __init__(self, options=None, steps_per_launch=16, big_threshold=1 << 17, max_big=4,
-                 keep_results=True, **kw):
+    def __init__(self, options=None, steps_per_launch=16, big_threshold=1 << 30, max_big=4,
+                 keep_results=True, builders=4, **kw):
         opts = options if options is not None else default_options(**kw)
         cfg = ServerConfig()
         lib().nrt_default_server_config(C.byref(cfg))
@@ -420,6 +421,7 @@ class Server:
         cfg.big_threshold = big_threshold
         cfg.max_big = max_big
         cfg.keep_results = int(bool(keep_results))
+        cfg.builders = builders
         s = C.c_void_p()
         st = lib().nrt_server_create(C.byref(opts), C.byref(cfg), C.byref(s))
         if st != 0:

@@ -71,6 +71,110 @@ def run_form(nh, prob, opts, first, steps, a0, a1):
             "blocks": st1["num_blocks"], "v": v}
 
 
+def explicit_bytes(N, H):
+    """48 B per passive element (V r/w, a, r_a dx, W dx, K dx) + 4 B code; HH
+    elements 152 B (V r/w, a, r_a dx, m/h/n r/w, 9 parameters)."""
+    return 52 * (N - H) + 156 * H
+
+
+def batched_sweep(nh, wl, count, steps):
+    """Config 4: `count` seeded trees (sizes log-uniform 500..50,000), one batch per
+    form. Returns the per-form throughput and the accuracy of the mixed form."""
+    t0 = time.perf_counter()
+    geoms = [wl.sweep_tree(i) for i in range(count)]
+    imp = [nh.setup(g, dT=r.dT)[0] for g, r in geoms]
+    exp_ = [nh.setup(g, dT=None, dt_factor=0.8)[0] for g, _ in geoms]
+    gen_s = time.perf_counter() - t0
+    Ntot = sum(p.N for p in imp)
+    H = sum(p.H for p in imp)
+    out = {"workload": f"{count} seeded trees, compartments log-uniform 500..50000, depth "
+                       "U{4..12} (capped at >= 2 compartments per segment), per-tree strain "
+                       "U[0,0.3] and stimulus U[1,20] nA at the soma for 0.5 ms; implicit "
+                       "dt = 25 us, explicit dt = 0.8 x each tree's critical step",
+           "trees": count, "compartments": Ntot, "hh_elements": H, "setup_s": gen_s}
+    sample = list(range(0, count, max(1, count // 16)))
+    snaps = {}
+    for name, kw, probs in [
+            ("implicit_fp64", dict(kind=nh.IMPLICIT, krylov=nh.DIRECT), imp),
+            ("implicit_mixed", dict(kind=nh.IMPLICIT, krylov=nh.DIRECT,
+                                    precision=nh.FP32_REFINE, refine_tol=1e-10), imp),
+            ("explicit", dict(kind=nh.EXPLICIT), exp_)]:
+        b = nh.Batch(**kw)
+        t0 = time.perf_counter()
+        ids = [b.add(p, steps + 8, 1, 20) for p in probs]
+        add_s = time.perf_counter() - t0
+        b.step(4)  # warm-up (and the workgroup-size autotuner's first launches)
+        b.step(4)
+        st0 = b.stats()
+        t0 = time.perf_counter()
+        for _ in range(steps // 10):
+            b.step(10)
+        wall = time.perf_counter() - t0
+        st1 = b.stats()
+        cs = st1["compartment_steps"] - st0["compartment_steps"]
+        kms = st1["kernel_ms"] - st0["kernel_ms"]
+        rec = {"compartment_steps_per_s": cs / wall, "ms_per_step": wall / (steps // 10 * 10) * 1e3,
+               "kernel_ms": kms, "add_s": add_s, "device_bytes": st1["device_bytes"],
+               "workgroup_threads": st1["threads"] if name != "explicit" else 256}
+        if name == "explicit":
+            nb = explicit_bytes(Ntot, H) * (steps // 10 * 10)
+            ach = nb / (kms / 1e3) / 1e9
+            rec["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                               "kernel": "k_explicit_batch", "algorithmic_bytes": nb}
+        else:
+            snaps[name] = [b.snapshot(ids[i]) for i in sample]
+        out[name] = rec
+        b.close()
+    err = 0.0
+    for i, (vm, vd) in enumerate(zip(snaps["implicit_mixed"], snaps["implicit_fp64"])):
+        m = np.ones(len(vd), bool)
+        m[[0, len(vd) - 1]] = False
+        err = max(err, float(np.max(np.abs(vm - vd)[m])) * 1e3)
+    out["implicit_mixed"]["max_abs_dV_mV_vs_fp64"] = err
+    out["implicit_mixed"]["compared_trees"] = len(sample)
+    return out
+
+
+def served_requests(nh, wl, count):
+    reqs = []
+    for i in range(count):
+        g, r, gap = wl.served_request(i)
+        p = nh.setup(g, dT=r.dT)[0]
+        reqs.append((p, r.steps(r.dT), gap))
+    return reqs
+
+
+def served_stream(nh, reqs, precision):
+    """Config 5: requests with Poisson arrivals (200/s), sizes log-normal (median
+    20,000, sigma 1, cap 2M), 5..50 ms simulated at dt = 25 us, submitted in real
+    time to the request server."""
+    count = len(reqs)
+    kw = dict(kind=nh.IMPLICIT, krylov=nh.DIRECT, precision=precision, refine_tol=1e-10)
+    srv = nh.Server(steps_per_launch=16, **kw)
+    t0 = time.perf_counter()
+    due = 0.0
+    for p, n, gap in reqs:
+        due += gap
+        wait = t0 + due - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        srv.submit(p, n, 1, 20)
+    srv.wait_idle(600.0)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    res = srv.poll()
+    srv.close()
+    return {"requests": count, "completed": st["completed"], "failed": st["failed"],
+            "arrival_window_s": due, "wall_s": wall,
+            "compartment_steps_per_s": st["compartment_steps"] / wall,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "largest_request": max(p.N for p, _, _ in reqs),
+            "launches": st["launches"], "results": len(res),
+            "loop_ms": {k: st[k] for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms",
+                                           "batch_kernel_ms")}}
+
+
 def cpu_baseline(prob, budget_s=15.0, max_steps=3):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle  # checker library: timed here as the CPU baseline only
@@ -106,6 +210,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-trees", type=int, default=4096)
+    ap.add_argument("--sweep-steps", type=int, default=100)
+    ap.add_argument("--serve-requests", type=int, default=400)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -183,6 +290,14 @@ def main():
     if "fp64_plain_bicgstab" in summary:
         out["speedup_vs_fp64_plain"] = (b["compartment_steps_per_s"] /
                                         summary["fp64_plain_bicgstab"]["compartment_steps_per_s"])
+    if a.sweep_trees > 0:
+        out["batched_sweep"] = batched_sweep(nh, wl, a.sweep_trees, a.sweep_steps)
+    if a.serve_requests > 0:
+        reqs = served_requests(nh, wl, a.serve_requests)
+        out["served"] = {"workload": "Poisson arrivals 200/s, compartments log-normal "
+                                     "(median 20000, sigma 1, cap 2M), 5..50 ms at dt 25 us",
+                         "fp64": served_stream(nh, reqs, nh.FP64),
+                         "mixed": served_stream(nh, reqs, nh.FP32_REFINE)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob)
     print(json.dumps(out), flush=True)

@@ -294,11 +294,17 @@ int nrt_batch_destroy(nrt_batch* b);
 typedef struct nrt_server nrt_server;
 
 typedef struct nrt_server_config {
-  int32_t steps_per_launch;  /* k: steps per launch (default 16)                 */
+  int32_t steps_per_launch;  /* k: steps per launch; implicit requests of N
+                                elements take max(1, min(8k, k*16384/N)) (default
+                                16)                                              */
   int32_t big_threshold;     /* elements above which a request gets its own
-                                grid-wide handle (default 131072)                */
+                                grid-wide handle instead of a batch slot (default
+                                2^30: none; batch slots above 24,576 rows already
+                                run on several workgroups)                       */
   int32_t max_big;           /* such requests running at once (default 4)        */
   int32_t keep_results;      /* keep V and trace until fetched (default 1)       */
+  int32_t builders;          /* threads building admitted requests' device state
+                                (1..16, default 4)                               */
 } nrt_server_config;
 
 typedef struct nrt_request_result {
@@ -321,6 +327,10 @@ typedef struct nrt_server_stats {
   double wall_s;              /* since creation                                  */
   double p50_ms, p99_ms, mean_ms;  /* request latency                            */
   double batch_kernel_ms;     /* device time of the batched launches            */
+  /* loop thread wall time by phase (ms): enqueueing launches (with slot-table
+     rebuilds), admitting requests (slot construction), waiting for the device,
+     retiring completed requests (result copies, slot release)                  */
+  double launch_ms, admit_ms, wait_ms, retire_ms;  /* admit_ms: unused (builders) */
 } nrt_server_stats;
 
 void nrt_default_server_config(nrt_server_config* c);

@@ -67,6 +67,7 @@ def test_implicit_batch_matches_alone(threads, nrt, orc):
             v = b.snapshot(sid)
             info = b.info(sid)
             assert info["status"] == 0 and info["steps_done"] == steps
+            assert info["inner_iters"] == steps  # one exact solve per step, every slot
             ref = _alone(p, steps, a0, a1)
             assert max_abs_diff(v, ref, ghost_mask(p)) <= TOL_SAME
             exact = _oracle(orc, p, steps, a0, a1)
@@ -84,7 +85,9 @@ def test_implicit_batch_mixed_precision(nrt, orc):
         # a step whose previous V already meets the residual test takes no sweep
         assert sum(b.info(sid)["inner_iters"] for sid in ids) > 0
         for sid, p in zip(ids, probs):
-            assert b.info(sid)["status"] == 0
+            info = b.info(sid)
+            assert info["status"] == 0 and info["steps_done"] == steps
+            assert info["inner_iters"] <= 8 * steps  # refine_max sweeps per step at most
             exact = _oracle(orc, p, steps, a0, a1)
             assert max_abs_diff(b.snapshot(sid), exact, ghost_mask(p)) <= TOL_MIXED
 
@@ -144,3 +147,37 @@ def test_explicit_batch_bit_identical(nrt):
                 v = s.snapshot()
             assert np.array_equal(b.snapshot(sid), v)
             assert np.array_equal(b.trace(sid), tr)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_implicit_batch_virtual_grids(form, nrt, orc):
+    """Slots above 24,576 rows run on several co-resident workgroups with their own
+    grid barrier inside the batched launch; results must not change."""
+    probs = []
+    for depth in (12, 11):
+        geom, run = wl.random_tree(wl.seed_for(4, 40 + depth), depth=depth, amp=5e-9)
+        probs.append(nrt.setup(geom, dT=run.dT)[0])
+    geom, run = wl.myelinated_axon()
+    probs.append(nrt.setup(geom, dT=run.dT)[0])
+    geom, run = wl.squid_axon(n_comp=500)
+    probs.append(nrt.setup(geom, dT=25e-6)[0])
+    assert max(p.N for p in probs) > 8 * 24576
+    opts = dict(krylov=nh.DIRECT) if form == "fp64" else dict(
+        krylov=nh.DIRECT, precision=nh.FP32_REFINE, refine_tol=1e-10)
+    steps, a0, a1 = 30, 1, 20
+    _say(f"implicit batch virtual grids {form} sizes={[p.N for p in probs]}")
+    with nh.Batch(kind=nh.IMPLICIT, batch_threads=256, **opts) as b:
+        ids = [b.add(p, steps, a0, a1) for p in probs]
+        b.step(10)
+        b.step(20)
+        for sid, p in zip(ids, probs):
+            info = b.info(sid)
+            assert info["status"] == 0 and info["steps_done"] == steps
+            v = b.snapshot(sid)
+            exact = _oracle(orc, p, steps, a0, a1)
+            if form == "fp64":
+                assert info["inner_iters"] == steps
+                assert max_abs_diff(v, _alone(p, steps, a0, a1), ghost_mask(p)) <= TOL_SAME
+                assert max_abs_diff(v, exact, ghost_mask(p)) <= TOL_V
+            else:
+                assert max_abs_diff(v, exact, ghost_mask(p)) <= TOL_MIXED
