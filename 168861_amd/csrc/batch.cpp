@@ -21,6 +21,7 @@ void implicit_slot_work(void* out, long long done, long long steps, int vn);
 int implicit_batch_capacity(int precision, int threads);
 void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner);
 int64_t implicit_device_bytes(const Implicit* e);
+int64_t implicit_nnz(const Implicit* e);
 // explicit.hip
 size_t explicit_slot_bytes();
 void explicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
@@ -424,6 +425,7 @@ int nrt_batch_slot_info(nrt_batch* b, int32_t slot, nrt_slot_info* info) {
       implicit_slot_stats(s.h, &st, &inner);
       info->status = st;
       info->inner_iters = inner;
+      info->nnz = implicit_nnz(s.h->im);
     }
     return NRT_OK;
   });

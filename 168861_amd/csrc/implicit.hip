@@ -1346,6 +1346,8 @@ void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner) {
   *inner = (int64_t)st.iters_total;
 }
 
+int64_t implicit_nnz(const Implicit* e) { return e->nnz; }
+
 // Bytes of device memory a handle holds (approximate: its arrays).
 int64_t implicit_device_bytes(const Implicit* e) {
   const int64_t N = e->N, T = e->plan_h.total_nodes();

@@ -86,7 +86,7 @@ class Stats(C.Structure):
 class SlotInfo(C.Structure):
     _fields_ = [("N", C.c_int32), ("status", C.c_int32), ("steps_done", C.c_int64),
                 ("num_steps", C.c_int64), ("inner_iters", C.c_int64),
-                ("trace_samples", C.c_int64)]
+                ("trace_samples", C.c_int64), ("nnz", C.c_int64)]
 
 
 class BatchStats(C.Structure):

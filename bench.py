@@ -42,15 +42,26 @@ def bicgstab_bytes(N, H, nnz, steps, iters):
 
 
 def tree_bytes(N, H, nnz, steps, solves, vbytes):
-    """Per step the update phase (144 B per HH element + 40 B per element) and the
-    write of V (8 N); per tree solve: the CSR read (12 nnz + 4 (N+1)), the level-0
-    system write + read (4 values), the elimination coefficients written twice and
-    read once (3 values each), and the solution write (1 value), vbytes per value;
-    refinement adds the FP64 residual pass (8 N read of x, b) per solve."""
-    per_solve = 12 * nnz + 4 * (N + 1) + N * vbytes * (4 + 4 + 9 + 1)
-    if vbytes == 4:
-        per_solve += 24 * N  # FP64 residual: x, b read; x += d write
-    return steps * (144 * H + 40 * N + 8 * N) + solves * per_solve
+    """Algorithmic bytes of the tree-solved steps (k_implicit_tree):
+    update phase per step: 144 B per HH element (V, gates r/w, parameters, the
+      refreshed diagonal) and per element V 8 + code 4 + c_m dx 8 + K dx 8 +
+      input 8 + level-0 slot 4 + the right-hand side written (FP64 b for the
+      refinement: 8; into the tree system: vbytes);
+    per tree solve: the level-0 sweeps read d, up, dn, r and write the three
+      elimination coefficients, read them back with dn and write them again
+      (14 values), the back-substitution reads them (3 values), the row map (4)
+      and writes V (8; the refinement also reads V: +8);
+    refinement only, per residual evaluation (solves + steps of them): the CSR
+      (12 nnz + 4 (N+1)), the gathered V (8 nnz), b (8 N), the slot map and the
+      residual written into the tree system (4 + vbytes per element)."""
+    if vbytes == 8:
+        per_step = 144 * H + (8 + 4 + 8 + 8 + 8 + 4 + 8) * N
+        per_solve = (17 * 8 + 4 + 8) * N
+        return steps * per_step + solves * per_solve
+    per_step = 144 * H + (8 + 4 + 8 + 8 + 8 + 8) * N
+    per_solve = (17 * vbytes + 4 + 16) * N
+    per_resid = 12 * nnz + 4 * (N + 1) + 8 * nnz + 8 * N + (4 + vbytes) * N
+    return steps * per_step + solves * per_solve + (solves + steps) * per_resid
 
 
 def run_form(nh, prob, opts, first, steps, a0, a1):
@@ -116,14 +127,25 @@ def batched_sweep(nh, wl, count, steps):
         rec = {"compartment_steps_per_s": cs / wall, "ms_per_step": wall / (steps // 10 * 10) * 1e3,
                "kernel_ms": kms, "add_s": add_s, "device_bytes": st1["device_bytes"],
                "workgroup_threads": st1["threads"] if name != "explicit" else 256}
+        timed = steps // 10 * 10
         if name == "explicit":
-            nb = explicit_bytes(Ntot, H) * (steps // 10 * 10)
-            ach = nb / (kms / 1e3) / 1e9
-            rec["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                               "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                               "kernel": "k_explicit_batch", "algorithmic_bytes": nb}
+            nb = explicit_bytes(Ntot, H) * timed
+            kern = "k_explicit_batch"
         else:
             snaps[name] = [b.snapshot(ids[i]) for i in sample]
+            # solves in the timed steps: scaled from each slot's total (the warm-up
+            # steps take as many solves per step as the timed ones, within a sweep)
+            nb = 0
+            vb = 8 if name == "implicit_fp64" else 4
+            for sid, p in zip(ids, probs):
+                inf = b.info(sid)
+                solves = inf["inner_iters"] * timed / max(1, inf["steps_done"])
+                nb += tree_bytes(p.N, p.H, inf["nnz"], timed, solves, vb)
+            kern = f"k_batch_tree<{'double' if vb == 8 else 'float'},{st1['threads']}>"
+        ach = nb / (kms / 1e3) / 1e9
+        rec["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "kernel": kern,
+                           "algorithmic_bytes": nb}
         out[name] = rec
         b.close()
     err = 0.0

@@ -254,6 +254,7 @@ typedef struct nrt_slot_info {
   int64_t num_steps;
   int64_t inner_iters;     /* tree solves / refinement sweeps (implicit)        */
   int64_t trace_samples;   /* probe samples recorded so far                     */
+  int64_t nnz;             /* matrix entries (implicit)                         */
 } nrt_slot_info;
 
 typedef struct nrt_batch_stats {
