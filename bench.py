@@ -285,6 +285,15 @@ def main():
               "mixed_tree_ir": "k_implicit_tree<float>",
               "fp64_plain_bicgstab": "k_implicit_bicgstab_fp64"}[best]
     achieved = b["algorithmic_bytes"] / (b["kernel_ms"] / 1e3) / 1e9
+    # HBM bytes measured with rocprofv3 counters for this kernel and workload (per step,
+    # calibrated with a copy), scaled to this launch's steps
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", "traffic.json")
+    tkey = f"{kernel} myelinated axon N={N}"
+    if os.path.exists(tfile):
+        t = json.load(open(tfile)).get(tkey)
+        if t:
+            traffic = t["hbm_bytes_per_step"] * b["steps"]
     out = {
         "metric": METRIC,
         "value": b["compartment_steps_per_s"],
@@ -304,7 +313,7 @@ def main():
                    "N": N, "hh_elements": H, "nnz": forms[best]["nnz"], "form": best,
                    "tolerance": "relative 1e-10 (refinement and BiCGSTAB)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "launches": 1, "kernel_ms": b["kernel_ms"],
                      "algorithmic_bytes": b["algorithmic_bytes"]},
         "forms": summary,
