@@ -110,8 +110,18 @@ struct ImpDev {
   unsigned long long spin_limit;
   long long first, num, a0, a1, interval;
   long long gl0;  // trace phase at step `first` (steps since the last sample)
-  int trace_elem;
+  int trace_elem;  // element (BiCGSTAB) or its level-0 slot (tree solve)
   double* trace;
+  // tree solve: the state in level-0 slot order (ns0 slots, padding included)
+  int ns0;
+  const double* scmdx;
+  const double* skdx;
+  const double* sinput;
+  const uint32_t* scode;    // HH flag and index, or kCodeSkip for padding
+  const int32_t* srowptr;   // the FP64 matrix with rows in slot order
+  const int32_t* scol;      // column slots
+  const uint8_t* srole;     // 0 diagonal, 1 coupling to the parent, 2 to a child
+  const int32_t* hh_slot;   // slot of each HH entry's row
 };
 
 struct Implicit {
@@ -137,6 +147,11 @@ struct Implicit {
   bool grid_mode = false;
   int threads = 256;
   TreePlanHost plan_h;
+  int ns0 = 0;  // level-0 slots (the tree solve's vectors are in slot order)
+  DBuf<double> scmdx, skdx, sinput;
+  DBuf<uint32_t> scode;
+  DBuf<int32_t> srowptr, scol, hh_slot;
+  DBuf<uint8_t> srole;
   DBuf<int32_t> plan_buf;
   TreePlanDev plan{};
   DBuf<double> tw64;
@@ -669,42 +684,41 @@ __device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>
   tsync<GRID>(d);
 }
 
-// Update phase of a tree-solved step. HH elements are spread one per thread over
-// the whole launch (their gating and row refresh are long dependent chains, so
-// they must not queue up inside one wave); passive rows follow in the row
-// distribution of the launch. Each writes its right-hand side straight into the
-// level-0 tree system (direct) or into b (refinement, whose FP64 residual reads
-// it); an HH row also refreshes its CSR values and the level-0 diagonal.
+// Update phase of a tree-solved step, over slots. HH elements are spread one per
+// thread over the whole launch (their gating and row refresh are long dependent
+// chains, so they must not queue up inside one wave); the other slots of the
+// range follow. Each writes its right-hand side straight into the level-0 tree
+// system (direct) or into b (refinement, whose FP64 residual reads it); an HH row
+// also refreshes its CSR values and the level-0 diagonal.
 template <class T, bool GRID>
 __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& t, int r0, int r1,
                                             bool active) {
   const int tid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
   const int nt = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
   const double* __restrict__ xv = d.x;
-  const uint32_t* __restrict__ code = d.code;
-  const double* __restrict__ cmdx = d.cmdx;
-  const double* __restrict__ kdx = d.kdx;
-  const double* __restrict__ input = d.input;
-  const int32_t* __restrict__ slot0 = t.P.slot0;
+  const uint32_t* __restrict__ code = d.scode;
+  const double* __restrict__ cmdx = d.scmdx;
+  const double* __restrict__ kdx = d.skdx;
+  const double* __restrict__ input = d.sinput;
   T* __restrict__ rt = t.w.r;
   double* __restrict__ b = d.b;
   const double dT = d.dT;
   for (int hh = tid; hh < d.H; hh += nt) {
     const int i = d.hh.row[hh];
-    const double v = xv[i];
+    const int si = d.hh_slot[hh];
+    const double v = xv[si];
     double W, K, diag;
     imp_gate(d.hh, hh, v, d.dt_ms, W, K);
-    if (imp_update_row(d, hh, i, W, diag)) t.w.d[slot0[i]] = (T)diag;
-    double bi = cmdx[i] * v / dT + K * d.dxv[i];
-    if (active) bi = bi + input[i];
-    if (t.direct) rt[slot0[i]] = (T)bi;
-    else b[i] = bi;
+    if (imp_update_row(d, hh, i, W, diag)) t.w.d[si] = (T)diag;
+    double bi = d.cmdx[i] * v / dT + K * d.dxv[i];
+    if (active) bi = bi + d.input[i];
+    if (t.direct) rt[si] = (T)bi;
+    else b[si] = bi;
   }
-  // passive rows, loads of kTpsBatch rows per thread issued together
+  // the other slots, loads of kTpsBatch slots per thread issued together
   const int st = (int)blockDim.x;
   for (int i0 = r0 + (int)threadIdx.x; i0 < r1; i0 += kTpsBatch * st) {
     uint32_t bc[kTpsBatch];
-    int bs[kTpsBatch];
     double bv[kTpsBatch], bm[kTpsBatch], bk[kTpsBatch], bin[kTpsBatch];
 #pragma unroll
     for (int j = 0; j < kTpsBatch; j++) {
@@ -715,16 +729,15 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
         bm[j] = cmdx[i];
         bk[j] = kdx[i];
         bin[j] = active ? input[i] : 0.0;
-        bs[j] = slot0[i];
       }
     }
 #pragma unroll
     for (int j = 0; j < kTpsBatch; j++) {
       const int i = i0 + j * st;
-      if (i < r1 && !(bc[j] & kCodeHH)) {
+      if (i < r1 && !(bc[j] & (kCodeHH | kCodeSkip))) {
         double bi = bm[j] * bv[j] / dT + bk[j];
         if (active) bi = bi + bin[j];
-        if (t.direct) rt[bs[j]] = (T)bi;
+        if (t.direct) rt[i] = (T)bi;
         else b[i] = bi;
       }
     }
@@ -735,27 +748,26 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
 // it; else one workgroup does).
 template <class T, bool GRID>
 __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t, double* lds) {
+  // slot ranges (the tree solve's vectors are in level-0 slot order)
   const int r_begin = GRID ? d.vb * d.rpb : 0;
-  const int r_end = GRID ? min(d.N, r_begin + d.rpb) : d.N;
+  const int r_end = GRID ? min(d.ns0, r_begin + d.rpb) : d.ns0;
   const int bs = blockDim.x;
   const double* val = d.val;
-  const int32_t* __restrict__ rowptr = d.rowptr;
-  const int32_t* __restrict__ col = d.col;
-  const int32_t* __restrict__ slot0 = t.P.slot0;
+  const int32_t* __restrict__ rowptr = d.srowptr;
+  const int32_t* __restrict__ col = d.scol;
   long long gl = d.gl0, ksamp = 0;
   int slot = 0;
   const double lim2 = t.refine_tol * t.refine_tol;
-  // Level-0 tree system from the CSR. Off-diagonals never change; the diagonal of
-  // HH rows is refreshed by the update phase of every step, by the thread that
-  // owns the HH entry (not the row), hence the barrier before the first update.
+  // Level-0 tree system from the matrix. Off-diagonals never change; the diagonal
+  // of HH rows is refreshed by the update phase of every step, by the thread that
+  // owns the HH entry (not the slot), hence the barrier before the first update.
   for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
-    const int si = slot0[i];
     for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-      const int c = col[k];
       const T v = (T)val[k];
-      if (c == i) t.w.d[si] = v;
-      else if (c < i) t.w.up[si] = v;
-      else t.w.dn[slot0[c]] = v;
+      const int role = d.srole[k];
+      if (role == 0) t.w.d[i] = v;
+      else if (role == 1) t.w.up[i] = v;
+      else t.w.dn[col[k]] = v;
     }
   }
   tsync<GRID>(d);
@@ -778,7 +790,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
           for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * d.x[col[k]];
           const double bi = d.b[i];
           const double ri = bi - ax;
-          t.w.r[slot0[i]] = (T)ri;
+          t.w.r[i] = (T)ri;
           red[0] = red[0] + ri * ri;
           red[1] = red[1] + bi * bi;
         }
@@ -872,7 +884,7 @@ __global__ __launch_bounds__(BS) void k_batch_tree(const BatchSlot<T>* __restric
   d.vb = m.y;
   d.nblocks = wk.vn;
   if (wk.vn > 1) {
-    d.rpb = (d.N + wk.vn - 1) / wk.vn;
+    d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
     tree_steps<T, true>(d, S.t, lds);
   } else {
     tree_steps<T, false>(d, S.t, lds);
@@ -1044,6 +1056,66 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       plan_view(P, nullptr, &flat);
       e->plan_buf.upload(flat, s);
       e->plan = plan_view(P, e->plan_buf.p, nullptr);
+      // Slot order: V, b, the per-element coefficients and the FP64 matrix are laid
+      // out by level-0 slot, so the update, the residual and the back-substitution
+      // read and write consecutive addresses (chunk k-th rows side by side).
+      const int ns0 = P.nslots[0];
+      e->ns0 = ns0;
+      std::vector<double> hval(nnz);
+      std::vector<int32_t> hhm(H);
+      NRT_HIP(hipMemcpyAsync(hval.data(), e->val.p, sizeof(double) * nnz, hipMemcpyDeviceToHost,
+                             s));
+      if (H > 0)
+        NRT_HIP(hipMemcpyAsync(hhm.data(), e->hh_mat.p, sizeof(int32_t) * H,
+                               hipMemcpyDeviceToHost, s));
+      NRT_HIP(hipStreamSynchronize(s));
+      std::vector<double> sx(ns0, 0.0), scm(ns0, 0.0), skd(ns0, 0.0), sin_(ns0, 0.0);
+      std::vector<uint32_t> scode(ns0, kCodeSkip);
+      for (int i = 0; i < N; i++) {
+        const int si = P.slot0[i];
+        sx[si] = p.V0[i];
+        scm[si] = cmdx[i];
+        skd[si] = kdx[i];
+        sin_[si] = p.input[i];
+        scode[si] = code[i];
+      }
+      std::vector<int32_t> srp(ns0 + 1, 0), scol(nnz), hslot(H), hmat(H);
+      std::vector<double> sval(nnz);
+      std::vector<uint8_t> srole(nnz);
+      for (int si = 0; si < ns0; si++) {
+        const int row = P.row0[si];
+        srp[si + 1] = srp[si] + (row >= 0 ? hrp[row + 1] - hrp[row] : 0);
+      }
+      for (int si = 0; si < ns0; si++) {
+        const int row = P.row0[si];
+        if (row < 0) continue;
+        for (int k = hrp[row], t = srp[si]; k < hrp[row + 1]; k++, t++) {
+          const int c = hcol[k];
+          scol[t] = P.slot0[c];
+          sval[t] = hval[k];
+          srole[t] = c == row ? 0 : (c < row ? 1 : 2);
+        }
+      }
+      for (int hh = 0; hh < H; hh++) {
+        const int row = p.hh_elem[hh];
+        hslot[hh] = P.slot0[row];
+        hmat[hh] = srp[hslot[hh]] + (hhm[hh] - hrp[row]);  // same position within the row
+      }
+      e->x.upload(sx, s);
+      e->b.alloc(ns0);
+      NRT_HIP(hipMemsetAsync(e->b.p, 0, sizeof(double) * ns0, s));
+      e->scmdx.upload(scm, s);
+      e->skdx.upload(skd, s);
+      e->sinput.upload(sin_, s);
+      e->scode.upload(scode, s);
+      e->srowptr.upload(srp, s);
+      e->scol.upload(scol, s);
+      e->srole.upload(srole, s);
+      e->val.upload(sval, s);  // the refreshed HH rows are written here
+      e->hh_slot.upload(hslot, s);
+      e->hh_mat.upload(hmat, s);
+      e->rowptr.release();
+      e->col.release();
       const size_t T = (size_t)P.total_nodes();
       if (e->method == 1) {
         e->tw64.alloc(8 * T);
@@ -1085,7 +1157,7 @@ static ImpDev imp_dev(nrt_handle* hnd) {
   d.N = e->N;
   d.H = e->H;
   d.nblocks = e->nblocks;
-  d.rpb = grid_for(e->N, e->nblocks);
+  d.rpb = grid_for(e->method != 0 ? e->ns0 : e->N, e->nblocks);
   d.dT = e->dT;
   d.dt_ms = e->dT * 1000;
   d.rowptr = e->rowptr.p;
@@ -1123,7 +1195,22 @@ static ImpDev imp_dev(nrt_handle* hnd) {
   if (const char* sl = getenv("NRT_SPIN_LIMIT")) d.spin_limit = strtoull(sl, nullptr, 10);
   d.interval = -1;
   d.trace_elem = 0;
+  d.ns0 = e->ns0;
+  d.scmdx = e->scmdx.p;
+  d.skdx = e->skdx.p;
+  d.sinput = e->sinput.p;
+  d.scode = e->scode.p;
+  d.srowptr = e->srowptr.p;
+  d.scol = e->scol.p;
+  d.srole = e->srole.p;
+  d.hh_slot = e->hh_slot.p;
   return d;
+}
+
+// The probe's index in the vectors a launch works on: the element itself, or its
+// level-0 slot for the tree solve.
+static int32_t probe_index(const Implicit* e, int32_t elem) {
+  return e->method != 0 ? e->plan_h.slot0[elem] : elem;
 }
 
 template <class T>
@@ -1147,7 +1234,7 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.a1 = a1;
   d.interval = interval;
   d.gl0 = gl0;
-  d.trace_elem = trace_elem;
+  d.trace_elem = probe_index(e, trace_elem);
   d.trace = trace_dev;
   if (e->method == 0) {
     void* args[] = {&d};
@@ -1217,9 +1304,18 @@ int implicit_calculate(nrt_handle* hnd) {
 
 void implicit_snapshot(nrt_handle* hnd, double* host_out) {
   Implicit* e = hnd->im;
-  NRT_HIP(hipMemcpyAsync(host_out, e->x.p, sizeof(double) * e->N, hipMemcpyDeviceToHost,
+  if (e->method == 0) {
+    NRT_HIP(hipMemcpyAsync(host_out, e->x.p, sizeof(double) * e->N, hipMemcpyDeviceToHost,
+                           hnd->stream));
+    NRT_HIP(hipStreamSynchronize(hnd->stream));
+    return;
+  }
+  std::vector<double> xs(e->ns0);  // slot order -> element order
+  NRT_HIP(hipMemcpyAsync(xs.data(), e->x.p, sizeof(double) * e->ns0, hipMemcpyDeviceToHost,
                          hnd->stream));
   NRT_HIP(hipStreamSynchronize(hnd->stream));
+  const int32_t* s0 = e->plan_h.slot0.data();
+  for (int i = 0; i < e->N; i++) host_out[i] = xs[s0[i]];
 }
 
 int implicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
@@ -1269,7 +1365,7 @@ void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_ele
   d.a0 = a0;
   d.a1 = a1;
   d.interval = interval;
-  d.trace_elem = trace_elem;
+  d.trace_elem = probe_index(e, trace_elem);
   if (e->method == 1) {
     BatchSlot<double> S{d, tree_args<double>(e, h->opt, e->tw64.p), trace};
     std::memcpy(out, &S, sizeof(S));
@@ -1350,9 +1446,10 @@ int64_t implicit_nnz(const Implicit* e) { return e->nnz; }
 
 // Bytes of device memory a handle holds (approximate: its arrays).
 int64_t implicit_device_bytes(const Implicit* e) {
-  const int64_t N = e->N, T = e->plan_h.total_nodes();
-  return N * (8 * 7 + 4 * 3) + (int64_t)e->nnz * 12 + (int64_t)e->H * 8 * 12 +
-         (int64_t)e->plan_buf.n * 4 + T * 8 * (e->method == 1 ? 8 : 4);
+  const int64_t N = e->N, T = e->plan_h.total_nodes(), S = e->ns0;
+  return N * (8 * 5 + 4) + S * (8 * 5 + 4) + (int64_t)e->nnz * 13 + (S + 1) * 4 +
+         (int64_t)e->H * 8 * 13 + (int64_t)e->plan_buf.n * 4 +
+         T * 8 * (e->method == 1 ? 8 : 4);
 }
 
 }  // namespace nrt

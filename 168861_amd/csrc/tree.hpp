@@ -110,7 +110,7 @@ struct TreeWork {
 // Rows of a chunk whose loads are issued together: the sweeps are recurrences, but
 // their operands do not depend on them, so a batch of rows is fetched at once and
 // the latency is paid once per batch rather than once per row.
-constexpr int kTpsBatch = 8;
+constexpr int kTpsBatch = 4;
 
 // Eliminate the interior of chunk `ch` of level `lv` and write its ends to lv+1.
 template <class T>
@@ -263,17 +263,15 @@ NRT_HD void tps_final_down(const TreePlanDev& P, const TreeWork<T>& w, int v) {
 }
 
 // Stores of the back-substitution: into the level's x (OUT_LEVEL), or at level 0
-// into the caller's FP64 vector by row, assigned or accumulated (a refinement).
+// into the caller's FP64 vector, which is in level-0 slot order (level 0 starts at
+// offset 0), assigned or accumulated (a refinement).
 enum TpsOut { OUT_LEVEL = 0, OUT_ASSIGN = 1, OUT_ADD = 2 };
 
 template <int MODE, class T>
 NRT_HD void tps_put(const TreePlanDev& P, const TreeWork<T>& w, int g, T v, double* out) {
-  if (MODE == OUT_LEVEL) {
-    w.x[g] = v;
-  } else {
-    const int row = P.row0[g];  // level 0 starts at offset 0
-    out[row] = MODE == OUT_ADD ? out[row] + (double)v : (double)v;
-  }
+  (void)P;
+  if (MODE == OUT_LEVEL) w.x[g] = v;
+  else out[g] = MODE == OUT_ADD ? out[g] + (double)v : (double)v;
 }
 
 // Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1.

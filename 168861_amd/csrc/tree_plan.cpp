@@ -218,15 +218,17 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
       for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_up(P, w, H.fin_order[k]);
     for (int h = H.heights - 1; h >= 0; h--)
       for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_down(P, w, H.fin_order[k]);
+    std::vector<double> xs(H.nslots[0], 0.0);  // the solution in level-0 slot order
     for (int lv = H.levels - 1; lv >= 1; lv--)
       for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
-        tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, x);
+        tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data());
     if (H.levels == 0) {
-      for (int v = 0; v < H.nslots[0]; v++) tps_final_out<OUT_ASSIGN>(P, w, v, x);
+      for (int v = 0; v < H.nslots[0]; v++) tps_final_out<OUT_ASSIGN>(P, w, v, xs.data());
     } else {
       for (int ch = H.chunk_off[0]; ch < H.chunk_off[1]; ch++)
-        tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, x);
+        tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
     }
+    for (int i = 0; i < N; i++) x[i] = xs[s0[i]];
     if (levels) *levels = H.levels;
     if (heights) *heights = H.heights;
     return NRT_OK;
