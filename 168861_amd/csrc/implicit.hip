@@ -623,6 +623,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
 // grid mode (large trees): every workgroup takes a share of the rows and chunks,
 // grid barriers separate the levels, and workgroup 0 solves the compressed forest.
 constexpr int kTreeBlock = 1024;
+constexpr int kTreeArrays = 10;  // d, up, dn, r, x, ca, cb, cg, cf, ci
 
 template <class T>
 struct TreeArgs {
@@ -639,13 +640,34 @@ __device__ __forceinline__ void tsync(const ImpDev& d) {
   else __syncthreads();
 }
 
+// Full elimination of every chunk: the factors that the steps reuse for chunks
+// whose matrix values do not change (run once per launch, after the level-0 fill).
 template <class T, bool GRID>
-__device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w, bool acc) {
+__device__ void tps_factor(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w) {
   const int wid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
   const int nw = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
   for (int lv = 0; lv < P.levels; lv++) {
     const int c1 = P.chunk_off[lv + 1];
     for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw) tps_reduce_chunk(P, w, lv, ch);
+    tsync<GRID>(d);
+  }
+}
+
+// One solve. Dynamic chunks (holding a row whose values changed this step) are
+// eliminated in full, the others sweep the right-hand side with their kept
+// factors; all_static: the matrix did not change since the last solve (a later
+// refinement sweep of the same step), every chunk reuses its factors.
+template <class T, bool GRID>
+__device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w, bool acc,
+                        bool all_static) {
+  const int wid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
+  const int nw = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
+  for (int lv = 0; lv < P.levels; lv++) {
+    const int c1 = P.chunk_off[lv + 1];
+    for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw) {
+      if (!all_static && P.chunk_dyn[ch]) tps_reduce_chunk(P, w, lv, ch);
+      else tps_reduce_chunk_static(P, w, lv, ch);
+    }
     tsync<GRID>(d);
   }
   if (!GRID || d.vb == 0) {
@@ -771,6 +793,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     }
   }
   tsync<GRID>(d);
+  tps_factor<T, GRID>(d, t.P, t.w);
   for (long long step = d.first; step < d.first + d.num; step++) {
     const bool active = step >= d.a0 && step <= d.a1;
     tree_update<T, GRID>(d, t, r_begin, r_end, active);
@@ -779,7 +802,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     double rr = 0, bb = 0;
     bool ok = true;
     if (t.direct) {
-      tps_run<T, GRID>(d, t.P, t.w, false);
+      tps_run<T, GRID>(d, t.P, t.w, false, false);
       sweeps = 1;
     } else {
       while (true) {
@@ -808,7 +831,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
           status = NRT_ERR_NOT_CONVERGED;
           break;
         }
-        tps_run<T, GRID>(d, t.P, t.w, true);
+        tps_run<T, GRID>(d, t.P, t.w, true, sweeps > 0);
         sweeps++;
       }
     }
@@ -1026,11 +1049,12 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
           NRT_REQUIRE(hcol[k] <= i || par[hcol[k]] == i, NRT_ERR_TOPOLOGY,
                       "tree solve: row " + std::to_string(i) + " couples to row " +
                           std::to_string(hcol[k]) + ", which does not couple back");
-      // Workgroups: every reduction level costs a grid barrier, so a mid-size tree
-      // runs best on a few workgroups (measured on a 20,202-row cable: 1 -> 104 us,
-      // 4 -> 85 us, 32 -> 114 us per step) and only large trees spread wide.
+      // Workgroups: every reduction level costs a grid barrier (an L2 write-back), so
+      // a mid-size tree runs best in one workgroup (measured on a 20,202-row cable
+      // with reused factors: 1 -> 71 us, 4 -> 81 us per step) and only large trees
+      // spread over the chip.
       int nb = o.num_blocks;
-      if (nb == 0 && N <= (1 << 17)) nb = N >= (1 << 14) ? 4 : 1;
+      if (nb == 0 && N <= (1 << 17)) nb = 1;
       if (batch_threads > 0) nb = 1;  // a batch slot: one workgroup of batch_threads
       e->grid_mode = nb != 1;
       if (e->grid_mode) {
@@ -1049,7 +1073,10 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       const int auto0 = std::min(64, std::max(8, (N + workers - 1) / workers));
       const int chunk = o.chunk > 0 ? o.chunk : auto0;
       const int chunk_hi = o.chunk_hi > 0 ? o.chunk_hi : (o.chunk > 0 ? o.chunk : 8);
-      e->plan_h = build_tree_plan(par, chunk, chunk_hi);
+      // HH rows change their diagonal every step; every other chunk keeps its factors
+      std::vector<uint8_t> dyn(N, 0);
+      for (int hh = 0; hh < H; hh++) dyn[p.hh_elem[hh]] = 1;
+      e->plan_h = build_tree_plan(par, chunk, chunk_hi, &dyn);
       const TreePlanHost& P = e->plan_h;
       // one device buffer holding every plan array
       std::vector<int32_t> flat;
@@ -1118,11 +1145,11 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       e->col.release();
       const size_t T = (size_t)P.total_nodes();
       if (e->method == 1) {
-        e->tw64.alloc(8 * T);
-        NRT_HIP(hipMemsetAsync(e->tw64.p, 0, sizeof(double) * 8 * T, s));
+        e->tw64.alloc(kTreeArrays * T);
+        NRT_HIP(hipMemsetAsync(e->tw64.p, 0, sizeof(double) * kTreeArrays * T, s));
       } else {
-        e->tw32.alloc(8 * T);
-        NRT_HIP(hipMemsetAsync(e->tw32.p, 0, sizeof(float) * 8 * T, s));
+        e->tw32.alloc(kTreeArrays * T);
+        NRT_HIP(hipMemsetAsync(e->tw32.p, 0, sizeof(float) * kTreeArrays * T, s));
       }
     } else {
       e->nblocks = choose_blocks(hnd, N);
@@ -1218,7 +1245,7 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b) {
   const size_t T_ = (size_t)e->plan_h.total_nodes();
   return TreeArgs<T>{e->plan,
                      TreeWork<T>{b, b + T_, b + 2 * T_, b + 3 * T_, b + 4 * T_, b + 5 * T_,
-                                 b + 6 * T_, b + 7 * T_},
+                                 b + 6 * T_, b + 7 * T_, b + 8 * T_, b + 9 * T_},
                      e->method == 1 ? 1 : 0, o.refine_tol, o.refine_max};
 }
 
@@ -1449,7 +1476,7 @@ int64_t implicit_device_bytes(const Implicit* e) {
   const int64_t N = e->N, T = e->plan_h.total_nodes(), S = e->ns0;
   return N * (8 * 5 + 4) + S * (8 * 5 + 4) + (int64_t)e->nnz * 13 + (S + 1) * 4 +
          (int64_t)e->H * 8 * 13 + (int64_t)e->plan_buf.n * 4 +
-         T * 8 * (e->method == 1 ? 8 : 4);
+         T * kTreeArrays * (e->method == 1 ? 8 : 4);
 }
 
 }  // namespace nrt

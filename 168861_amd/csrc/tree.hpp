@@ -45,6 +45,7 @@ struct TreePlanHost {
   std::vector<int32_t> next;        // flat: slot in level+1 of a surviving node (-1 otherwise)
   std::vector<int32_t> chunk_off;   // per level, L+1 entries, into chunk_len
   std::vector<int32_t> chunk_len;
+  std::vector<int32_t> chunk_dyn;   // 1: the chunk's matrix values change between steps
   std::vector<int32_t> slot0;       // level-0 slot of each row (N entries)
   std::vector<int32_t> row0;        // row of each level-0 slot (-1 for padding)
   std::vector<int32_t> fin_par;     // final forest: parent (final-local) or -1
@@ -57,7 +58,13 @@ struct TreePlanHost {
 
 // Builds the plan for a forest given by parent indices (par[i] < i or -1), with
 // chains cut into chunks of `chunk` rows at level 0 and `chunk_hi` rows above.
-TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chunk_hi);
+// dyn_rows (optional, N entries): rows whose matrix values change between steps;
+// chunks holding one, and the reduced chunks that depend on them, are refactored
+// every step, the others reuse their factors. Within a level, chunks are placed
+// (slot columns) dynamic first, then by decreasing length, so a wave's lanes take
+// the same path and stay active together.
+TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chunk_hi,
+                             const std::vector<uint8_t>* dyn_rows = nullptr);
 
 // Parent of each row in an assembled CSR (the single column < row, or -1).
 std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* col);
@@ -76,6 +83,7 @@ struct TreePlanDev {
   const int32_t* next;
   const int32_t* chunk_off;
   const int32_t* chunk_len;
+  const int32_t* chunk_dyn;
   const int32_t* slot0;
   const int32_t* row0;
   const int32_t* fin_par;
@@ -86,9 +94,10 @@ struct TreePlanDev {
 };
 
 // Per-value work arrays of one solve, over all levels (total_nodes each): d, up
-// (row -> parent coupling), dn (parent -> row coupling), r, x; and the
-// elimination coefficients (three arrays a, b, g, used as (d', g', r') by the
-// forward sweep and as (alpha, beta, gamma) afterwards).
+// (row -> parent coupling), dn (parent -> row coupling), r, x; the elimination
+// coefficients (three arrays a, b, g, used as (d', g', r') by the forward sweep
+// and as (alpha, beta, gamma) afterwards); and the factors kept for the steps
+// that reuse them: the forward multipliers f and the pivots' reciprocals.
 template <class T>
 struct TreeWork {
   T* d;
@@ -99,6 +108,8 @@ struct TreeWork {
   T* ca;
   T* cb;
   T* cg;
+  T* cf;
+  T* ci;
 };
 
 #if defined(__HIPCC__)
@@ -122,6 +133,8 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
   T* __restrict__ ca = w.ca;
   T* __restrict__ cb = w.cb;
   T* __restrict__ cg = w.cg;
+  T* __restrict__ cf = w.cf;
+  T* __restrict__ ci = w.ci;
   const int NC = P.nchunk[lv];
   const int c = ch - P.chunk_off[lv];
   const int m = P.chunk_len[ch];
@@ -177,8 +190,9 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
         gk = -f * gk;
         rp = br[j] - f * rp;
         dnl = bn[j];
+        const int g = base + k * NC;
+        cf[g] = f;
         if (k <= m - 2) {
-          const int g = base + k * NC;
           ca[g] = dp;
           cb[g] = gk;
           cg[g] = rp;
@@ -200,6 +214,7 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
     ca[g2] = al;
     cb[g2] = be;
     cg[g2] = ga;
+    ci[g2] = inv;
   }
   for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatch) {
     T bp[kTpsBatch], bg[kTpsBatch], brr[kTpsBatch], bn[kTpsBatch];
@@ -227,6 +242,7 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
         ca[g] = al;
         cb[g] = be;
         cg[g] = ga;
+        ci[g] = inv;
       }
     }
   }
@@ -238,18 +254,98 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
   dn[te] = dn1 * ga;  // s -> e coupling
 }
 
+// Right-hand side only, for a chunk whose factors (f, 1/pivot, beta, gamma, the
+// reduced matrix values of its ends) are those of its last full elimination: the
+// same operations on r as tps_reduce_chunk, in the same order, so the results are
+// identical; the sweeps are multiply-adds with no division on the dependent path.
+template <class T>
+NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
+  T* __restrict__ r = w.r;
+  T* __restrict__ ca = w.ca;
+  const T* __restrict__ dn = w.dn;
+  const T* __restrict__ cf = w.cf;
+  const T* __restrict__ ci = w.ci;
+  const int NC = P.nchunk[lv];
+  const int c = ch - P.chunk_off[lv];
+  const int m = P.chunk_len[ch];
+  const int base = P.node_off[lv] + c, nbase = P.node_off[lv + 1];
+  if (m <= 2) {
+    for (int k = 0; k < m; k++) {
+      const int g = base + k * NC;
+      r[nbase + P.next[g]] = r[g];
+    }
+    return;
+  }
+  const T r0 = r[base];
+  const int ts = nbase + P.next[base];
+  const int te = nbase + P.next[base + (m - 1) * NC];
+  const T dn1 = dn[base + NC];
+  // forward: rp_k = r_k - f_k rp_{k-1}
+  T rp = r[base + NC];
+  ca[base + NC] = rp;
+  for (int k0 = 2; k0 <= m - 1; k0 += kTpsBatch) {
+    T br[kTpsBatch], bf[kTpsBatch];
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 + j;
+      if (k <= m - 1) {
+        const int g = base + k * NC;
+        br[j] = r[g];
+        bf[j] = cf[g];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 + j;
+      if (k <= m - 1) {
+        rp = br[j] - bf[j] * rp;
+        if (k <= m - 2) ca[base + k * NC] = rp;
+      }
+    }
+  }
+  r[te] = rp;
+  // backward: alpha_k = (rp_k - dn_{k+1} alpha_{k+1}) / pivot_k
+  const int g2 = base + (m - 2) * NC;
+  T al = ca[g2] * ci[g2];
+  ca[g2] = al;
+  for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatch) {
+    T bp[kTpsBatch], bi[kTpsBatch], bn[kTpsBatch];
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 - j;
+      if (k >= 1) {
+        const int g = base + k * NC;
+        bp[j] = ca[g];
+        bi[j] = ci[g];
+        bn[j] = dn[g + NC];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int k = k0 - j;
+      if (k >= 1) {
+        al = (bp[j] - bn[j] * al) * bi[j];
+        ca[base + k * NC] = al;
+      }
+    }
+  }
+  r[ts] = r0 - dn1 * al;
+}
+
 // Final forest, elimination of node v (all of its children are final already).
+// The eliminated pivot goes to ca (free at the final level): d must keep the
+// reduced matrix value, which a chunk that reuses its factors does not rewrite.
 template <class T>
 NRT_HD void tps_final_up(const TreePlanDev& P, const TreeWork<T>& w, int v) {
   const int base = P.node_off[P.levels];
   T dv = w.d[base + v], rv = w.r[base + v];
   for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) {
     const int c = base + P.fin_child[k];
-    const T f = w.dn[c] / w.d[c];
+    const T f = w.dn[c] / w.ca[c];
     dv = dv - f * w.up[c];
     rv = rv - f * w.r[c];
   }
-  w.d[base + v] = dv;
+  w.ca[base + v] = dv;
   w.r[base + v] = rv;
 }
 
@@ -259,7 +355,7 @@ NRT_HD void tps_final_down(const TreePlanDev& P, const TreeWork<T>& w, int v) {
   const int p = P.fin_par[v];
   T acc = w.r[base + v];
   if (p >= 0) acc = acc - w.up[base + v] * w.x[base + p];
-  w.x[base + v] = acc / w.d[base + v];
+  w.x[base + v] = acc / w.ca[base + v];
 }
 
 // Stores of the back-substitution: into the level's x (OUT_LEVEL), or at level 0

@@ -25,14 +25,18 @@ std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* co
 namespace {
 struct LocalLevel {
   std::vector<int32_t> par;                 // level-local parents
+  std::vector<uint8_t> dyn;                 // node values change between steps
   std::vector<int32_t> cs, cl;              // chunks (local start, length)
+  std::vector<uint8_t> cdyn;                // chunk holds a changing node
+  std::vector<int32_t> pos;                 // chunk -> slot column
   std::vector<int32_t> next;                // local next-level index
   std::vector<int32_t> slot;                // local node -> slot
   int nchunk = 0, cmax = 0, nslots = 0;
 };
 }  // namespace
 
-TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int chunk_hi) {
+TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int chunk_hi,
+                             const std::vector<uint8_t>* dyn_rows) {
   TreePlanHost P;
   P.C = std::max(3, chunk);
   P.C_hi = std::max(3, chunk_hi);
@@ -41,9 +45,15 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
     if (par0[i] >= i) throw std::runtime_error("tree plan needs parent < child");
   std::vector<LocalLevel> lv;
   std::vector<int32_t> par = par0;
+  std::vector<uint8_t> dyn(N0, 0);
+  if (dyn_rows) {
+    if ((int)dyn_rows->size() != N0) throw std::runtime_error("tree plan: dyn_rows size");
+    dyn = *dyn_rows;
+  }
   while (true) {
     LocalLevel L;
     L.par = par;
+    L.dyn = dyn;
     const int n = (int)par.size();
     std::vector<int32_t> nchild(n, 0);
     for (int v = 0; v < n; v++)
@@ -75,6 +85,18 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
       L.next[a] = cnt++;
       if (m >= 2) L.next[a + m - 1] = cnt++;
     }
+    // a chunk is dynamic when any of its nodes is; its surviving ends inherit that
+    L.cdyn.assign(L.cs.size(), 0);
+    std::vector<uint8_t> ndyn(cnt, 0);
+    for (size_t k = 0; k < L.cs.size(); k++) {
+      const int a = L.cs[k], m = L.cl[k];
+      uint8_t f = 0;
+      for (int j = a; j < a + m; j++) f |= L.dyn[j];
+      L.cdyn[k] = f;
+      ndyn[L.next[a]] = f;
+      if (m >= 2) ndyn[L.next[a + m - 1]] = f;
+    }
+    dyn.swap(ndyn);
     std::vector<int32_t> npar(cnt, -1);
     for (size_t k = 0; k < L.cs.size(); k++) {
       const int a = L.cs[k], m = L.cl[k];
@@ -100,8 +122,17 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
     } else {
       L.nchunk = (int)L.cs.size();
       for (int m : L.cl) L.cmax = std::max(L.cmax, m);
+      // slot columns: dynamic chunks first, then longest first (stable)
+      std::vector<int32_t> order(L.nchunk);
+      for (int c = 0; c < L.nchunk; c++) order[c] = c;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (L.cdyn[a] != L.cdyn[b]) return L.cdyn[a] > L.cdyn[b];
+        return L.cl[a] > L.cl[b];
+      });
+      L.pos.assign(L.nchunk, 0);
+      for (int p = 0; p < L.nchunk; p++) L.pos[order[p]] = p;
       for (int c = 0; c < L.nchunk; c++)
-        for (int k = 0; k < L.cl[c]; k++) L.slot[L.cs[c] + k] = k * L.nchunk + c;
+        for (int k = 0; k < L.cl[c]; k++) L.slot[L.cs[c] + k] = k * L.nchunk + L.pos[c];
       L.nslots = L.cmax * L.nchunk;
     }
   }
@@ -122,7 +153,14 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
       const int g = base + L.slot[u];
       if (L.next[u] >= 0) P.next[g] = Nx.slot[L.next[u]];
     }
-    P.chunk_len.insert(P.chunk_len.end(), L.cl.begin(), L.cl.end());
+    // chunk_len / chunk_dyn are indexed by slot column
+    const size_t off = P.chunk_len.size();
+    P.chunk_len.resize(off + L.nchunk);
+    P.chunk_dyn.resize(off + L.nchunk);
+    for (int c = 0; c < L.nchunk; c++) {
+      P.chunk_len[off + L.pos[c]] = L.cl[c];
+      P.chunk_dyn[off + L.pos[c]] = L.cdyn[c];
+    }
     P.chunk_off.push_back((int32_t)P.chunk_len.size());
   }
   P.slot0 = lv[0].slot;
@@ -159,10 +197,10 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
 // Device view over one flat int32 buffer holding every plan array; `flat`
 // receives the concatenation, the returned view has offsets relative to base.
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
-  const std::vector<int32_t>* parts[] = {&P.nslots,   &P.nchunk,   &P.node_off, &P.next,
-                                         &P.chunk_off, &P.chunk_len, &P.slot0,   &P.row0,
-                                         &P.fin_par,  &P.fin_order, &P.fin_hoff, &P.fin_cptr,
-                                         &P.fin_child};
+  const std::vector<int32_t>* parts[] = {&P.nslots,    &P.nchunk,   &P.node_off, &P.next,
+                                         &P.chunk_off, &P.chunk_len, &P.chunk_dyn, &P.slot0,
+                                         &P.row0,      &P.fin_par,  &P.fin_order, &P.fin_hoff,
+                                         &P.fin_cptr,  &P.fin_child};
   constexpr int kParts = sizeof(parts) / sizeof(parts[0]);
   size_t off[kParts];
   size_t pos = 0;
@@ -177,7 +215,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
   return TreePlanDev{P.levels,       P.heights,      base + off[0],  base + off[1],
                      base + off[2],  base + off[3],  base + off[4],  base + off[5],
                      base + off[6],  base + off[7],  base + off[8],  base + off[9],
-                     base + off[10], base + off[11], base + off[12]};
+                     base + off[10], base + off[11], base + off[12], base + off[13]};
 }
 
 }  // namespace nrt
@@ -190,21 +228,26 @@ using namespace nrt;
 // oracle's sequential elimination. No solver handle ever calls it.
 extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int32_t* col,
                                         const double* val, const double* rhs, int chunk,
-                                        int chunk_hi, double* x, int* levels, int* heights) {
+                                        int chunk_hi, const uint8_t* dyn_rows,
+                                        const double* rhs_factor, double* x, int* levels,
+                                        int* heights) {
   try {
     std::vector<int32_t> par = csr_parents(N, rowptr, col);
-    TreePlanHost H = build_tree_plan(par, chunk, chunk_hi > 0 ? chunk_hi : chunk);
+    std::vector<uint8_t> dyn;
+    if (dyn_rows) dyn.assign(dyn_rows, dyn_rows + N);
+    TreePlanHost H = build_tree_plan(par, chunk, chunk_hi > 0 ? chunk_hi : chunk,
+                                     dyn_rows ? &dyn : nullptr);
     std::vector<int32_t> flat;
     TreePlanDev tmp = plan_view(H, nullptr, &flat);
     (void)tmp;
     TreePlanDev P = plan_view(H, flat.data(), nullptr);
     const size_t T = (size_t)H.total_nodes();
-    std::vector<double> buf(8 * T, 0.0);
+    std::vector<double> buf(10 * T, 0.0);
     double* b = buf.data();
-    TreeWork<double> w{b, b + T, b + 2 * T, b + 3 * T, b + 4 * T, b + 5 * T, b + 6 * T, b + 7 * T};
+    TreeWork<double> w{b,         b + T,     b + 2 * T, b + 3 * T, b + 4 * T,
+                       b + 5 * T, b + 6 * T, b + 7 * T, b + 8 * T, b + 9 * T};
     const int32_t* s0 = H.slot0.data();
     for (int i = 0; i < N; i++) {
-      w.r[s0[i]] = rhs[i];
       for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
         const int c = col[k];
         if (c == i) w.d[s0[i]] += val[k];
@@ -212,21 +255,39 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
         else w.dn[s0[c]] = val[k];
       }
     }
-    for (int lv = 0; lv < H.levels; lv++)
-      for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++) tps_reduce_chunk(P, w, lv, ch);
-    for (int h = 0; h < H.heights; h++)
-      for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_up(P, w, H.fin_order[k]);
-    for (int h = H.heights - 1; h >= 0; h--)
-      for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) tps_final_down(P, w, H.fin_order[k]);
+    const bool reuse = dyn_rows != nullptr;
+    if (reuse) {
+      // full elimination with another right-hand side: factors for the static chunks
+      for (int i = 0; i < N; i++) w.r[s0[i]] = rhs_factor ? rhs_factor[i] : 0.0;
+      for (int lv = 0; lv < H.levels; lv++)
+        for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
+          tps_reduce_chunk(P, w, lv, ch);
+    }
+    // with reuse, two solves in a row (as consecutive steps do): the second must
+    // not depend on what the first left behind
     std::vector<double> xs(H.nslots[0], 0.0);  // the solution in level-0 slot order
-    for (int lv = H.levels - 1; lv >= 1; lv--)
-      for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
-        tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data());
-    if (H.levels == 0) {
-      for (int v = 0; v < H.nslots[0]; v++) tps_final_out<OUT_ASSIGN>(P, w, v, xs.data());
-    } else {
-      for (int ch = H.chunk_off[0]; ch < H.chunk_off[1]; ch++)
-        tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
+    for (int pass = 0; pass < (reuse ? 2 : 1); pass++) {
+      for (int i = 0; i < N; i++) w.r[s0[i]] = rhs[i];
+      for (int lv = 0; lv < H.levels; lv++)
+        for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++) {
+          if (reuse && !H.chunk_dyn[ch]) tps_reduce_chunk_static(P, w, lv, ch);
+          else tps_reduce_chunk(P, w, lv, ch);
+        }
+      for (int h = 0; h < H.heights; h++)
+        for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
+          tps_final_up(P, w, H.fin_order[k]);
+      for (int h = H.heights - 1; h >= 0; h--)
+        for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
+          tps_final_down(P, w, H.fin_order[k]);
+      for (int lv = H.levels - 1; lv >= 1; lv--)
+        for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
+          tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data());
+      if (H.levels == 0) {
+        for (int v = 0; v < H.nslots[0]; v++) tps_final_out<OUT_ASSIGN>(P, w, v, xs.data());
+      } else {
+        for (int ch = H.chunk_off[0]; ch < H.chunk_off[1]; ch++)
+          tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
+      }
     }
     for (int i = 0; i < N; i++) x[i] = xs[s0[i]];
     if (levels) *levels = H.levels;

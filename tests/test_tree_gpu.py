@@ -52,9 +52,9 @@ def test_tree_golden(form, name, dtf, nrt, orc):
 
 
 @pytest.mark.parametrize("form", sorted(FORMS))
-@pytest.mark.parametrize("nblocks", [0, 1, 3, 16])
+@pytest.mark.parametrize("nblocks", [0, 4, 3, 16])
 def test_tree_config2(form, nblocks, nrt, orc):
-    """Config 2 (N = 20,202): default (4 workgroups), one workgroup, forced grids."""
+    """Config 2 (N = 20,202): default (one workgroup) and forced grids."""
     geom, run = wl.myelinated_axon()
     prob, _ = nrt.setup(geom, dT=run.dT)
     a0, a1 = nrt.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)

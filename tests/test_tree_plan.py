@@ -15,20 +15,23 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
-def tps_host(rp, col, val, rhs, chunk, chunk_hi=0):
+def tps_host(rp, col, val, rhs, chunk, chunk_hi=0, dyn=None, rhs_factor=None):
     L = nh.lib()
     f = L.nrt_test_tree_solve_host
-    f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int),
-                  C.POINTER(C.c_int)]
+    f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _dp,
+                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
     rp = np.ascontiguousarray(rp, np.int32)
     col = np.ascontiguousarray(col, np.int32)
     val = np.ascontiguousarray(val, np.float64)
     rhs = np.ascontiguousarray(rhs, np.float64)
     x = np.empty(len(rhs))
     lv, h = C.c_int(0), C.c_int(0)
+    dyn = None if dyn is None else np.ascontiguousarray(dyn, np.uint8)
+    rf = None if rhs_factor is None else np.ascontiguousarray(rhs_factor, np.float64)
     st = f(len(rhs), rp.ctypes.data_as(_ip), col.ctypes.data_as(_ip), val.ctypes.data_as(_dp),
-           rhs.ctypes.data_as(_dp), chunk, chunk_hi, x.ctypes.data_as(_dp), C.byref(lv),
-           C.byref(h))
+           rhs.ctypes.data_as(_dp), chunk, chunk_hi,
+           None if dyn is None else dyn.ctypes.data, None if rf is None else rf.ctypes.data,
+           x.ctypes.data_as(_dp), C.byref(lv), C.byref(h))
     assert st == 0
     return x, lv.value, h.value
 
@@ -92,3 +95,22 @@ def test_tps_random_forest_dense(orc):
         for chunk in (3, 5, 32):
             x, _, _ = tps_host(rp, col, val, b, chunk)
             np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["ex3_hh_y", "ex6_ct_tree", "ex8_neuron"])
+@pytest.mark.parametrize("chunk,chunk_hi", [(4, 0), (20, 8), (7, 3)])
+def test_tps_reused_factors_identical(name, chunk, chunk_hi, orc):
+    """Chunks without a changing row reuse the factors of an elimination done with
+    another right-hand side: the solution must be bit-identical to a full solve."""
+    g = load_golden(name)
+    p = problem_from_golden(g, dT=g["dT_crit"] * 10)
+    rp, col, val, rhs = _system(orc, p)
+    rng = np.random.default_rng(3)
+    for frac in (0.0, 0.05, 0.5):
+        dyn = (rng.random(len(rhs)) < frac).astype(np.uint8)
+        full, _, _ = tps_host(rp, col, val, rhs, chunk, chunk_hi)
+        reuse, _, _ = tps_host(rp, col, val, rhs, chunk, chunk_hi, dyn=dyn,
+                               rhs_factor=rng.standard_normal(len(rhs)))
+        assert np.array_equal(full, reuse), frac
+        ref = orc.tree_solve(rp, col, val, rhs)
+        assert np.max(np.abs(reuse - ref)) <= 1e-12 * np.max(np.abs(ref))
