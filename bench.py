@@ -42,25 +42,27 @@ def bicgstab_bytes(N, H, nnz, steps, iters):
 
 
 def tree_bytes(N, H, nnz, steps, solves, vbytes):
-    """Algorithmic bytes of the tree-solved steps (k_implicit_tree):
+    """Algorithmic bytes of the tree-solved steps (k_implicit_tree, k_batch_tree),
+    counting every chunk as one that reuses its factors (chunks holding an HH row
+    are refactored each step and move more; they are a small share):
     update phase per step: 144 B per HH element (V, gates r/w, parameters, the
       refreshed diagonal) and per element V 8 + code 4 + c_m dx 8 + K dx 8 +
-      input 8 + level-0 slot 4 + the right-hand side written (FP64 b for the
-      refinement: 8; into the tree system: vbytes);
-    per tree solve: the level-0 sweeps read d, up, dn, r and write the three
-      elimination coefficients, read them back with dn and write them again
-      (14 values), the back-substitution reads them (3 values), the row map (4)
-      and writes V (8; the refinement also reads V: +8);
-    refinement only, per residual evaluation (solves + steps of them): the CSR
-      (12 nnz + 4 (N+1)), the gathered V (8 nnz), b (8 N), the slot map and the
-      residual written into the tree system (4 + vbytes per element)."""
+      input 8 + the right-hand side written (FP64 b for the refinement: 8; into the
+      tree system: vbytes);
+    per tree solve: the forward sweep reads r and the multiplier and writes the
+      partial right-hand side, the backward sweep reads it with the pivot
+      reciprocal and the coupling and writes alpha, the back-substitution reads
+      alpha, beta, gamma (10 values) and writes V (8; the refinement also reads V);
+    refinement only, per residual evaluation (solves + steps of them): the matrix
+      (12 nnz + 4 (N+1)), the gathered V (8 nnz), b (8 N), the residual written
+      into the tree system (vbytes N)."""
     if vbytes == 8:
-        per_step = 144 * H + (8 + 4 + 8 + 8 + 8 + 4 + 8) * N
-        per_solve = (17 * 8 + 4 + 8) * N
+        per_step = 144 * H + (8 + 4 + 8 + 8 + 8 + 8) * N
+        per_solve = (10 * 8 + 8) * N
         return steps * per_step + solves * per_solve
     per_step = 144 * H + (8 + 4 + 8 + 8 + 8 + 8) * N
-    per_solve = (17 * vbytes + 4 + 16) * N
-    per_resid = 12 * nnz + 4 * (N + 1) + 8 * nnz + 8 * N + (4 + vbytes) * N
+    per_solve = (10 * vbytes + 16) * N
+    per_resid = 12 * nnz + 4 * (N + 1) + 8 * nnz + 8 * N + vbytes * N
     return steps * per_step + solves * per_solve + (solves + steps) * per_resid
 
 
