@@ -20,6 +20,8 @@ size_t implicit_slot_work_bytes();
 void implicit_slot_work(void* out, long long done, long long steps, int vn);
 int implicit_batch_capacity(int precision, int threads);
 void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner);
+void implicit_slot_stats_on(nrt_handle* h, hipStream_t s, int* status, int64_t* inner);
+void implicit_snapshot_on(nrt_handle* hnd, double* host_out, hipStream_t s);
 int64_t implicit_device_bytes(const Implicit* e);
 int64_t implicit_nnz(const Implicit* e);
 // explicit.hip
@@ -496,11 +498,11 @@ void batch_admission_check(const nrt_batch* b, int N, int64_t num_steps, int64_t
                            int64_t reserved) {
   size_t free_b = 0, total_b = 0;
   NRT_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int64_t avail = (int64_t)free_b + pool_cached_bytes(b->device) - reserved;
   const int64_t need = batch_footprint(b, N, num_steps, trace_every);
-  NRT_REQUIRE((int64_t)free_b - reserved >= need + kAdmissionReserve, NRT_ERR_ADMISSION,
+  NRT_REQUIRE(avail >= need + kAdmissionReserve, NRT_ERR_ADMISSION,
               "slot of " + std::to_string(N) + " elements needs ~" + std::to_string(need >> 20) +
-                  " MiB; " + std::to_string(((int64_t)free_b - reserved) >> 20) +
-                  " MiB free");
+                  " MiB; " + std::to_string(avail >> 20) + " MiB free");
 }
 
 int batch_plan_threads(const nrt_batch* b) {
@@ -529,6 +531,50 @@ int batch_add_handle(nrt_batch* b, nrt_handle* h, int64_t num_steps, int64_t a0,
   b->slots.push_back(std::move(s));
   b->dirty = true;
   return id;
+}
+
+static bool in_flight(const nrt_batch* b, int slot) {
+  return b->pending && std::find(b->order.begin(), b->order.end(), slot) != b->order.end();
+}
+
+void batch_slot_collect(nrt_batch* b, int slot, hipStream_t side, std::vector<double>* V,
+                        std::vector<double>* trace, int* status, int64_t* steps_done) {
+  Slot& s = b->slot(slot);
+  NRT_REQUIRE(s.left() <= 0 && !in_flight(b, slot), NRT_ERR_INVALID,
+              "slot " + std::to_string(slot) + " is still running");
+  *status = NRT_OK;
+  *steps_done = s.done;
+  if (V) {
+    V->resize(s.N);
+    if (b->is_implicit()) {
+      implicit_snapshot_on(s.h, V->data(), side);
+    } else {
+      NRT_HIP(hipMemcpyAsync(V->data(), explicit_slot_v(s.h, s.done), sizeof(double) * s.N,
+                             hipMemcpyDeviceToHost, side));
+    }
+  }
+  if (trace) {
+    const int64_t n =
+        s.interval >= 0 ? std::min<int64_t>(s.done / (s.interval + 1), (int64_t)s.trace.n) : 0;
+    trace->resize((size_t)n);
+    if (n > 0)
+      NRT_HIP(hipMemcpyAsync(trace->data(), s.trace.p, sizeof(double) * n,
+                             hipMemcpyDeviceToHost, side));
+  }
+  if (b->is_implicit()) {
+    int64_t inner = 0;
+    implicit_slot_stats_on(s.h, side, status, &inner);  // synchronizes `side`
+  } else {
+    NRT_HIP(hipStreamSynchronize(side));
+  }
+}
+
+void batch_slot_drop(nrt_batch* b, int slot) {
+  b->slot(slot);
+  NRT_REQUIRE(!in_flight(b, slot), NRT_ERR_INVALID,
+              "slot " + std::to_string(slot) + " is in the launch in flight");
+  b->slots[slot].reset();  // its memory goes back to the pool, no device wait
+  b->dirty = true;
 }
 
 void batch_launch(nrt_batch* b, long long k) { b->launch(k); }

@@ -20,6 +20,11 @@ const nrt_options& batch_options(const nrt_batch* b);
 // batch_plan_threads) as a new slot; the batch owns it from then on.
 int batch_add_handle(nrt_batch* b, nrt_handle* h, int64_t num_steps, int64_t a0, int64_t a1,
                      int32_t trace_elem, int64_t trace_every);
+// Results of a completed slot that is not in the launch in flight, copied on the
+// caller's stream `side` without waiting for that launch; then its release.
+void batch_slot_collect(nrt_batch* b, int slot, hipStream_t side, std::vector<double>* V,
+                        std::vector<double>* trace, int* status, int64_t* steps_done);
+void batch_slot_drop(nrt_batch* b, int slot);
 // implicit.hip: steps of a single handle enqueued on its stream
 void implicit_steps_async(nrt_handle* h, int64_t first, int64_t num, int64_t a0, int64_t a1,
                           int32_t trace_elem, int64_t interval, int64_t gl0, double* trace_dev);

@@ -1329,20 +1329,24 @@ int implicit_calculate(nrt_handle* hnd) {
   return collect_stats(hnd);
 }
 
-void implicit_snapshot(nrt_handle* hnd, double* host_out) {
+// V in element order, copied on stream s (the handle's own stream, or another
+// one when the caller knows that no queued work writes V).
+void implicit_snapshot_on(nrt_handle* hnd, double* host_out, hipStream_t s) {
   Implicit* e = hnd->im;
   if (e->method == 0) {
-    NRT_HIP(hipMemcpyAsync(host_out, e->x.p, sizeof(double) * e->N, hipMemcpyDeviceToHost,
-                           hnd->stream));
-    NRT_HIP(hipStreamSynchronize(hnd->stream));
+    NRT_HIP(hipMemcpyAsync(host_out, e->x.p, sizeof(double) * e->N, hipMemcpyDeviceToHost, s));
+    NRT_HIP(hipStreamSynchronize(s));
     return;
   }
   std::vector<double> xs(e->ns0);  // slot order -> element order
-  NRT_HIP(hipMemcpyAsync(xs.data(), e->x.p, sizeof(double) * e->ns0, hipMemcpyDeviceToHost,
-                         hnd->stream));
-  NRT_HIP(hipStreamSynchronize(hnd->stream));
+  NRT_HIP(hipMemcpyAsync(xs.data(), e->x.p, sizeof(double) * e->ns0, hipMemcpyDeviceToHost, s));
+  NRT_HIP(hipStreamSynchronize(s));
   const int32_t* s0 = e->plan_h.slot0.data();
   for (int i = 0; i < e->N; i++) host_out[i] = xs[s0[i]];
+}
+
+void implicit_snapshot(nrt_handle* hnd, double* host_out) {
+  implicit_snapshot_on(hnd, host_out, hnd->stream);
 }
 
 int implicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
@@ -1460,13 +1464,17 @@ void implicit_steps_async(nrt_handle* h, int64_t first, int64_t num, int64_t a0,
 }
 
 // Status and inner-iteration count of a slot (its failure is reported, not cleared).
-void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner) {
+void implicit_slot_stats_on(nrt_handle* h, hipStream_t s, int* status, int64_t* inner) {
   Implicit* e = h->im;
   ImpStats st{};
-  NRT_HIP(hipMemcpyAsync(&st, e->st.p, sizeof(ImpStats), hipMemcpyDeviceToHost, h->stream));
-  NRT_HIP(hipStreamSynchronize(h->stream));
+  NRT_HIP(hipMemcpyAsync(&st, e->st.p, sizeof(ImpStats), hipMemcpyDeviceToHost, s));
+  NRT_HIP(hipStreamSynchronize(s));
   *status = st.status;
   *inner = (int64_t)st.iters_total;
+}
+
+void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner) {
+  implicit_slot_stats_on(h, h->stream, status, inner);
 }
 
 int64_t implicit_nnz(const Implicit* e) { return e->nnz; }

@@ -31,6 +31,16 @@ struct Error : public std::runtime_error {
     if (!(cond)) throw ::nrt::Error((status), (msg));                  \
   } while (0)
 
+// Device memory pool (pool.cpp): a caching allocator by size class, per device.
+// Freed blocks stay cached for the next allocation of their class instead of going
+// back through hipFree, which waits for the device and costs up to ~0.2 ms per
+// block; solvers, batch slots and served requests come and go at that rate. A
+// block must be returned only once no queued work uses it.
+void* pool_alloc(size_t bytes);                 // throws NRT_ERR_NOMEM
+void pool_free(void* p, size_t bytes);
+int64_t pool_cached_bytes(int device);          // cached, free for reuse
+int64_t pool_trim(int device);                  // returns cached blocks to HIP
+
 // Owning device buffer.
 template <class T>
 struct DBuf {
@@ -41,21 +51,15 @@ struct DBuf {
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) pool_free(p, n * sizeof(T));
     p = nullptr;
     n = 0;
   }
   void alloc(size_t count) {
     release();
-    n = count;
     if (count == 0) return;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      (void)hipGetLastError();
-      throw Error(NRT_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
-                                     " bytes failed: " + hipGetErrorString(e));
-    }
+    p = (T*)pool_alloc(count * sizeof(T));
+    n = count;
   }
   void upload(const T* src, size_t count, hipStream_t s) {
     alloc(count);

@@ -14,9 +14,9 @@
 //      cooperative launch per request
 //      (more elements than big_threshold, which would hold a whole batch launch
 //      back if one workgroup had to carry it), all on the batch stream,
-//   3. waits for the launch, then retires completed requests: final V and the
-//      probe trace are copied out and the slot is freed (which may let the FIFO
-//      head in).
+//   3. retires the requests completed by the previous launch while this one
+//      runs (final V and probe trace copied out over its own stream; the slot's
+//      memory back to the pool, which may let the FIFO head in), then waits.
 // Latency of a request = completion time - submission time (steady clock).
 #include <algorithm>
 #include <chrono>
@@ -148,6 +148,7 @@ struct nrt_server {
   // loop thread only
   std::map<int, Request> running;  // batch slot -> request
   std::vector<std::unique_ptr<BigRun>> big;
+  hipStream_t side = nullptr;      // result copies of retired slots
 
   bool is_big(const Request& r) const {
     return opt.kind == NRT_IMPLICIT && r.N() > cfg.big_threshold;
@@ -219,8 +220,9 @@ struct nrt_server {
             need = footprint(head);
             size_t free_b = 0, total_b = 0;
             (void)hipMemGetInfo(&free_b, &total_b);
+            const int64_t avail = (int64_t)free_b + pool_cached_bytes(opt.device) - reserved;
             const bool nothing_else = active == 0 && building == 0 && ready.empty();
-            const bool fits = (int64_t)free_b - reserved >= need + (512ll << 20);
+            const bool fits = avail >= need + (512ll << 20);
             const bool big_ok = !big_req || big_count < cfg.max_big;
             if ((fits && big_ok) || nothing_else) break;
           }
@@ -318,39 +320,41 @@ struct nrt_server {
     }
   }
 
-  void retire() {
-    int freed = 0;
-    for (auto it = running.begin(); it != running.end();) {
-      if (!batch_slot_complete(batch, it->first)) {
-        ++it;
-        continue;
-      }
-      nrt_slot_info si{};
-      if (nrt_batch_slot_info(batch, it->first, &si) != NRT_OK)
-        throw Error(NRT_ERR_HIP, nrt_batch_last_error(batch));
+  // Completed batch slots of the previous launch, while the next one runs: their
+  // results come over the server's own stream and their memory goes back to the
+  // pool; none of that waits for the launch in flight.
+  void retire_slots(std::vector<int>& done) {
+    for (int slot : done) {
+      auto it = running.find(slot);
+      if (it == running.end()) continue;
       Request& r = it->second;
       std::vector<double> V, tr;
-      if (cfg.keep_results) {
-        V.resize(r.N());
-        if (nrt_batch_snapshot(batch, it->first, V.data()) != NRT_OK)
-          throw Error(NRT_ERR_HIP, nrt_batch_last_error(batch));
-        int64_t n = 0;
-        nrt_batch_trace(batch, it->first, nullptr, 0, &n);
-        tr.resize((size_t)n);
-        if (n > 0) nrt_batch_trace(batch, it->first, tr.data(), n, &n);
-      }
-      finish_request(r, si.status, std::move(V), std::move(tr), si.steps_done);
-      nrt_batch_remove(batch, it->first);
-      it = running.erase(it);
-      freed++;
+      int status = NRT_OK;
+      int64_t steps = 0;
+      batch_slot_collect(batch, slot, side, cfg.keep_results ? &V : nullptr,
+                         cfg.keep_results ? &tr : nullptr, &status, &steps);
+      finish_request(r, status, std::move(V), std::move(tr), steps);
+      batch_slot_drop(batch, slot);
+      running.erase(it);
     }
+    if (!done.empty()) {
+      std::lock_guard<std::mutex> g(mu);
+      active -= (int64_t)done.size();
+      build_cv.notify_all();  // memory freed: the FIFO head may fit now
+    }
+    done.clear();
+  }
+
+  // Large requests run on the batch stream; after the launch has finished, the
+  // completed ones are retired in place.
+  void retire_big() {
+    int freed = 0;
     for (auto it = big.begin(); it != big.end();) {
       BigRun& b = **it;
       if (b.done < b.req.num_steps) {
         ++it;
         continue;
       }
-      NRT_HIP(hipStreamSynchronize(b.h->stream));
       std::vector<double> V, tr;
       if (cfg.keep_results) {
         V.resize(b.req.N());
@@ -365,26 +369,27 @@ struct nrt_server {
       implicit_slot_stats(b.h, &status, &inner);
       finish_request(b.req, status, std::move(V), std::move(tr), b.done);
       it = big.erase(it);
-      std::lock_guard<std::mutex> g(mu);
-      big_count--;
       freed++;
     }
     if (freed) {
       std::lock_guard<std::mutex> g(mu);
+      big_count -= freed;
       active -= freed;
-      build_cv.notify_all();  // memory freed: the FIFO head may fit now
+      build_cv.notify_all();
     }
   }
 
   void run() {
     try {
       NRT_HIP(hipSetDevice(opt.device));
+      NRT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
       std::deque<Built> got;
+      std::vector<int> done;  // batch slots completed by the last launch
       while (true) {
         {
           std::unique_lock<std::mutex> g(mu);
-          const auto has_work = [&] { return !running.empty() || !big.empty(); };
-          if (!has_work() && ready.empty()) {
+          const bool has_work = !running.empty() || !big.empty();
+          if (!has_work && ready.empty()) {
             if (to_build.empty() && building == 0) {
               busy = false;
               idle_cv.notify_all();
@@ -402,22 +407,30 @@ struct nrt_server {
         }
         const double t0 = now_s();
         join_ready(got);
-        if (running.empty() && big.empty()) continue;  // every joiner failed
         const long long k = cfg.steps_per_launch;
-        if (opt.kind == NRT_IMPLICIT) batch_launch_budget(batch, k, k * kBudgetElems, 8 * k);
-        else batch_launch(batch, k);
-        launch_big(k);
+        const bool stepping = running.size() > done.size() || !big.empty();
+        if (stepping) {
+          if (opt.kind == NRT_IMPLICIT) batch_launch_budget(batch, k, k * kBudgetElems, 8 * k);
+          else batch_launch(batch, k);
+          launch_big(k);
+        }
         const double t1 = now_s();
-        batch_finish(batch);
-        NRT_HIP(hipStreamSynchronize(batch_stream(batch)));
+        retire_slots(done);  // overlaps the launch
         const double t2 = now_s();
-        retire();
+        if (stepping) {
+          batch_finish(batch);
+          NRT_HIP(hipStreamSynchronize(batch_stream(batch)));
+        }
         const double t3 = now_s();
+        for (auto& kv : running)
+          if (batch_slot_complete(batch, kv.first)) done.push_back(kv.first);
+        retire_big();
+        const double t4 = now_s();
         std::lock_guard<std::mutex> g(mu);
-        stats.launches++;
+        if (stepping) stats.launches++;
         stats.launch_ms += (t1 - t0) * 1e3;
-        stats.wait_ms += (t2 - t1) * 1e3;
-        stats.retire_ms += (t3 - t2) * 1e3;
+        stats.retire_ms += (t2 - t1) * 1e3 + (t4 - t3) * 1e3;
+        stats.wait_ms += (t3 - t2) * 1e3;
       }
     } catch (const Error& e) {
       fail(e.status, e.what());
@@ -617,6 +630,7 @@ int nrt_server_destroy(nrt_server* s) {
     if (b.h) handle_destroy(b.h);
   s->ready.clear();
   s->running.clear();
+  if (s->side) (void)hipStreamDestroy(s->side);
   s->big.clear();
   if (s->batch) nrt_batch_destroy(s->batch);
   delete s;
