@@ -99,8 +99,15 @@ nrt_options resolve_options(const nrt_options* opt) {
   NRT_REQUIRE(o.precision == NRT_FP64 || (o.kind == NRT_IMPLICIT && o.krylov == NRT_DIRECT),
               NRT_ERR_UNSUPPORTED,
               "FP32 with FP64 refinement is available for the implicit tree solve");
-  NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB || o.krylov == NRT_DIRECT,
-              NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB or NRT_DIRECT");
+  NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB || o.krylov == NRT_GMRES ||
+                  o.krylov == NRT_DIRECT,
+              NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB, NRT_GMRES or NRT_DIRECT");
+  NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov != NRT_GMRES ||
+                  (o.restart >= 1 && o.restart <= 512),
+              NRT_ERR_INVALID, "GMRES restart must be in [1, 512]");
+  NRT_REQUIRE(o.x0_mode == NRT_X0_PREVIOUS || o.x0_mode == NRT_X0_RHS ||
+                  o.x0_mode == NRT_X0_ZERO,
+              NRT_ERR_INVALID, "bad x0_mode");
   NRT_REQUIRE(o.precond == NRT_PRECOND_NONE, NRT_ERR_UNSUPPORTED,
               "preconditioned Krylov is not built; use NRT_DIRECT for the tree solve");
   NRT_REQUIRE(o.chunk >= 0 && o.chunk <= kMaxChunk && o.chunk_hi >= 0 &&

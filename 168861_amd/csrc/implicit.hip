@@ -143,7 +143,10 @@ struct Implicit {
   int64_t trace_cap = 0;
   int64_t step = 1;
   // tree partition solve (method != 0)
-  int method = 0;        // 0 BiCGSTAB, 1 exact FP64 tree solve, 2 FP64 refinement over FP32
+  int method = 0;        // 0 Krylov, 1 exact FP64 tree solve, 2 FP64 refinement over FP32
+  bool gmres = false;    // Krylov method: GMRES(m) (else BiCGSTAB)
+  int gm = 0;            // GMRES restart length
+  DBuf<double> gV, gR, gpart;
   bool grid_mode = false;
   int threads = 256;
   TreePlanHost plan_h;
@@ -621,6 +624,272 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
 // residual b - A x and update x += d are the only FP64 vector work. Block mode:
 // one workgroup owns the whole tree and synchronises with workgroup barriers;
 // grid mode (large trees): every workgroup takes a share of the rows and chunks,
+// ------------------------------------------------------------------ GMRES(m)
+// Restarted GMRES (the method of the reference CPU solver, MKL FGMRES(150) with
+// x0 = 0 and an absolute 1e-14 stop, solverIm-cpu.cpp:192-272) as one persistent
+// cooperative kernel: per step the same update phase as the BiCGSTAB kernel, then
+// Arnoldi cycles of up to m steps. Orthogonalisation is classical Gram-Schmidt
+// with one re-orthogonalisation (CGS2): each Arnoldi step takes one SpMV and three
+// grid-wide reductions (the j+1 projections, the j+1 corrections, the norm), where
+// modified Gram-Schmidt would take j+1 sequential ones. Every workgroup receives
+// the same Hessenberg column (the reductions are bit-identical everywhere),
+// applies the Givens rotations itself and keeps its own copy of the triangular
+// factor, so the small least-squares problem needs no barrier of its own. The
+// SpMV reads a neighbour's entries of v_{k+1} = w_k / h_{k+1,k} by recomputing
+// them from w_k, double buffered by step parity, instead of waiting for them.
+constexpr int kGmresMaxRestart = 512;
+
+struct GmresDev {
+  int m;           // restart length
+  double* V;       // (m+1) x N basis (v_k at V + k N; only a block's own rows)
+  double* w0;      // Arnoldi vectors, even and odd steps
+  double* w1;
+  double* R;       // per workgroup: packed R (m(m+1)/2), cs, sn (m), g (m+1), y (m)
+  int64_t rstride;
+  double* part;    // 2 sets x nblocks x (m+1) partial sums
+};
+
+// h[0..cnt) = V_j . w over the whole vector (one grid barrier); every block ends
+// with the same values in hsh.
+__device__ __forceinline__ void gmres_dots(const ImpDev& d, const GmresDev& g, int cnt,
+                                           const double* __restrict__ w, int r_begin, int r_end,
+                                           double* hsh, double* lds, int& gslot) {
+  const int stride = g.m + 1;
+  double* part = g.part + (size_t)gslot * d.nblocks * stride;
+  gslot ^= 1;
+  const size_t N = (size_t)d.N;
+  for (int k0 = 0; k0 < cnt; k0 += 8) {
+    double acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = 0;
+    for (int i = r_begin + (int)threadIdx.x; i < r_end; i += (int)blockDim.x) {
+      const double wi = w[i];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (k0 + j < cnt) acc[j] = acc[j] + g.V[(size_t)(k0 + j) * N + i] * wi;
+    }
+    block_sum<8>(acc, lds);
+    if (threadIdx.x == 0)
+      for (int j = 0; j < 8 && k0 + j < cnt; j++) part[d.vb * stride + k0 + j] = acc[j];
+  }
+  grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    double sum = 0;
+    for (int b = 0; b < d.nblocks; b++) sum = sum + part[b * stride + t];
+    hsh[t] = sum;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, GmresDev g) {
+  __shared__ double lds[8 * 16];
+  __shared__ double hsh[kGmresMaxRestart + 1];
+  __shared__ double h2sh[kGmresMaxRestart + 1];
+  __shared__ double ysh[kGmresMaxRestart];
+  __shared__ double fsh[2];
+  d.vb = (int)blockIdx.x;
+  const int r_begin = blockIdx.x * d.rpb;
+  const int r_end = min(d.N, r_begin + d.rpb);
+  const int bs = kImpBlock;
+  const double* val = d.val;
+  const int32_t* __restrict__ rowptr = d.rowptr;
+  const int32_t* __restrict__ col = d.col;
+  const size_t N = (size_t)d.N;
+  double* Rb = g.R + (size_t)blockIdx.x * g.rstride;  // this block's least-squares data
+  const int m = g.m;
+  double* Rp = Rb;
+  double* cs = Rb + (size_t)m * (m + 1) / 2;
+  double* sn = cs + m;
+  double* gv = sn + m;
+  double* yv = gv + m + 1;
+  long long gl = 0, ksamp = 0;
+  int slot = 0, gslot = 0;
+  for (long long step = d.first; step < d.first + d.num; step++) {
+    const bool active = step >= d.a0 && step <= d.a1;
+    // ---- update phase: gating, HH rows, right-hand side, initial guess
+    for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
+      const double v = d.x[i];
+      const uint32_t c = d.code[i];
+      double kd;
+      if (c & kCodeHH) {
+        const int hh = (int)(c & kCodeIdx);
+        double W, K;
+        imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+        double diag;
+        (void)imp_update_row(d, hh, i, W, diag);
+        kd = K * d.dxv[i];
+      } else {
+        kd = d.kdx[i];
+      }
+      double bi = d.cmdx[i] * v / d.dT + kd;
+      if (active) bi = bi + d.input[i];
+      d.b[i] = bi;
+      if (d.x0_mode == NRT_X0_RHS) d.x[i] = bi;
+      else if (d.x0_mode == NRT_X0_ZERO) d.x[i] = 0.0;
+    }
+    grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+    int it = 0, status = NRT_ERR_NOT_CONVERGED, cycles = 0;
+    double rr = 0, bb = 0, thr = 0;
+    bool ok = true;
+    while (ok) {
+      // ---- cycle start: r0 = b - A x, beta = |r0|
+      double red[2] = {0.0, 0.0};
+      for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
+        double ax = 0;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * d.x[col[k]];
+        const double bi = d.b[i];
+        const double ri = bi - ax;
+        d.r0[i] = ri;
+        red[0] = red[0] + ri * ri;
+        red[1] = red[1] + bi * bi;
+      }
+      ok = reduce_all<2>(d, red, lds, slot);
+      if (!ok) break;
+      rr = red[0];
+      if (cycles == 0) {
+        bb = red[1];
+        thr = d.tol_mode == NRT_TOL_RELATIVE ? d.tol * sqrt(bb) : d.tol;
+      }
+      if (!isfinite(rr)) {
+        status = NRT_ERR_NUMERIC;
+        break;
+      }
+      const double beta = sqrt(rr);
+      if (beta <= thr) {
+        status = NRT_OK;
+        break;
+      }
+      if (it >= d.max_iter) break;
+      cycles++;
+      if (threadIdx.x == 0) {
+        gv[0] = beta;
+        for (int j = 1; j <= m; j++) gv[j] = 0.0;
+      }
+      const double* src = d.r0;  // v_k = src / div, neighbours' entries recomputed
+      double div = beta;
+      int kk = 0;
+      bool conv = false;
+      for (int k = 0; k < m; k++) {
+        double* __restrict__ wk = (k & 1) ? g.w1 : g.w0;
+        it++;
+        // v_k (own rows) and w = A v_k
+        for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
+          g.V[(size_t)k * N + i] = src[i] / div;
+          double acc = 0;
+          for (int q = rowptr[i]; q < rowptr[i + 1]; q++) acc = acc + val[q] * (src[col[q]] / div);
+          wk[i] = acc;
+        }
+        // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2; h += h2
+        gmres_dots(d, g, k + 1, wk, r_begin, r_end, hsh, lds, gslot);
+        for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
+          double wi = wk[i];
+          for (int j = 0; j <= k; j++) wi = wi - hsh[j] * g.V[(size_t)j * N + i];
+          wk[i] = wi;
+        }
+        gmres_dots(d, g, k + 1, wk, r_begin, r_end, h2sh, lds, gslot);
+        double nrm[1] = {0.0};
+        for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
+          double wi = wk[i];
+          for (int j = 0; j <= k; j++) wi = wi - h2sh[j] * g.V[(size_t)j * N + i];
+          wk[i] = wi;
+          nrm[0] = nrm[0] + wi * wi;
+        }
+        ok = reduce_all<1>(d, nrm, lds, slot);
+        if (!ok) break;
+        const double hn = sqrt(nrm[0]);
+        // Givens rotations on the new column (every block, identical values)
+        if (threadIdx.x == 0) {
+          double* Rk = Rp + (size_t)k * (k + 1) / 2;  // column k, rows 0..k
+          for (int j = 0; j <= k; j++) Rk[j] = hsh[j] + h2sh[j];
+          double hk1 = hn;
+          for (int j = 0; j < k; j++) {
+            const double a = Rk[j], b2 = Rk[j + 1];
+            Rk[j] = cs[j] * a + sn[j] * b2;
+            Rk[j + 1] = -sn[j] * a + cs[j] * b2;
+          }
+          const double a = Rk[k];
+          const double den = sqrt(a * a + hk1 * hk1);
+          cs[k] = den == 0 ? 1.0 : a / den;
+          sn[k] = den == 0 ? 0.0 : hk1 / den;
+          Rk[k] = den;
+          gv[k + 1] = -sn[k] * gv[k];
+          gv[k] = cs[k] * gv[k];
+          fsh[0] = fabs(gv[k + 1]);
+          fsh[1] = hn;
+        }
+        __syncthreads();
+        const double res = fsh[0];
+        kk = k + 1;
+        if (!isfinite(res)) {
+          status = NRT_ERR_NUMERIC;
+          ok = false;
+          break;
+        }
+        if (res <= thr || fsh[1] == 0.0) {
+          conv = true;
+          rr = res * res;
+          break;
+        }
+        if (it >= d.max_iter) break;
+        src = wk;
+        div = hn;
+      }
+      if (status == NRT_ERR_NUMERIC) {
+        ok = true;
+        break;
+      }
+      if (!ok) break;
+      // ---- x += V y, R y = g (back substitution, every block)
+      if (threadIdx.x == 0) {
+        for (int j = kk - 1; j >= 0; j--) {
+          double acc = gv[j];
+          for (int l = j + 1; l < kk; l++) acc = acc - Rp[(size_t)l * (l + 1) / 2 + j] * yv[l];
+          yv[j] = acc / Rp[(size_t)j * (j + 1) / 2 + j];
+        }
+        for (int j = 0; j < kk; j++) ysh[j] = yv[j];
+      }
+      __syncthreads();
+      for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
+        double xi = d.x[i];
+        for (int j = 0; j < kk; j++) xi = xi + ysh[j] * g.V[(size_t)j * N + i];
+        d.x[i] = xi;
+      }
+      grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);  // x before the next residual
+      if (__hip_atomic_load(d.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        ok = false;
+        break;
+      }
+      if (conv) {
+        status = NRT_OK;
+        break;
+      }
+      if (it >= d.max_iter) break;
+    }
+    if (!ok) status = NRT_ERR_HIP;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      d.st->iters_total += (unsigned long long)it;
+      d.st->steps += 1;
+      d.st->last_iters = it;
+      d.st->last_rr = rr;
+      d.st->last_bb = bb;
+      d.st->restarts += cycles > 0 ? cycles - 1 : 0;
+      if (d.st->status == NRT_OK && status != NRT_OK) {
+        d.st->status = status;
+        d.st->fail_step = step;
+        d.st->fail_iter = it;
+        d.st->fail_rr = rr;
+      }
+    }
+    if (!ok) return;
+    if (d.interval >= 0 && ++gl > d.interval) {
+      gl = 0;
+      if (d.trace && d.trace_elem >= r_begin && d.trace_elem < r_end && threadIdx.x == 0)
+        d.trace[ksamp] = d.x[d.trace_elem];
+      ksamp++;
+    }
+  }
+}
+
 // grid barriers separate the levels, and workgroup 0 solves the compressed forest.
 constexpr int kTreeBlock = 1024;
 constexpr int kTreeArrays = 10;  // d, up, dn, r, x, ca, cb, cg, cf, ci
@@ -1151,6 +1420,16 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
         e->tw32.alloc(kTreeArrays * T);
         NRT_HIP(hipMemsetAsync(e->tw32.p, 0, sizeof(float) * kTreeArrays * T, s));
       }
+    } else if (o.krylov == NRT_GMRES) {
+      e->gmres = true;
+      e->gm = std::min(o.restart, N);
+      e->nblocks = choose_blocks(hnd, N, (const void*)k_implicit_gmres_fp64);
+      e->threads = kImpBlock;
+      const int m = e->gm;
+      e->gV.alloc((size_t)(m + 1) * N);
+      const size_t rstride = (size_t)m * (m + 1) / 2 + 4 * (size_t)m + 1;
+      e->gR.alloc(rstride * e->nblocks);
+      e->gpart.alloc((size_t)2 * e->nblocks * (m + 1));
     } else {
       e->nblocks = choose_blocks(hnd, N);
       e->threads = kImpBlock;
@@ -1263,6 +1542,15 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.gl0 = gl0;
   d.trace_elem = probe_index(e, trace_elem);
   d.trace = trace_dev;
+  if (e->method == 0 && e->gmres) {
+    const int m = e->gm;
+    GmresDev g{m, e->gV.p, e->t.p, e->s.p, e->gR.p,
+               (int64_t)m * (m + 1) / 2 + 4 * (int64_t)m + 1, e->gpart.p};
+    void* args[] = {&d, &g};
+    NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_gmres_fp64, dim3(e->nblocks),
+                                       dim3(kImpBlock), args, 0, hnd->stream));
+    return NRT_OK;
+  }
   if (e->method == 0) {
     void* args[] = {&d};
     NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_bicgstab_fp64, dim3(e->nblocks),

@@ -23,7 +23,7 @@ FP64, FP32_REFINE = 0, 1
 BICGSTAB, GMRES, DIRECT = 0, 1, 2
 PRECOND_NONE, PRECOND_TREE = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
-X0_PREVIOUS, X0_RHS = 0, 1
+X0_PREVIOUS, X0_RHS, X0_ZERO = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

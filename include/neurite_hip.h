@@ -58,6 +58,7 @@ extern "C" {
 
 #define NRT_X0_PREVIOUS 0   /* Krylov starts from V of the previous step      */
 #define NRT_X0_RHS      1   /* Krylov starts from b (solverIm-gpu.cpp:169,769) */
+#define NRT_X0_ZERO     2   /* Krylov starts from 0 (solverIm-cpu.cpp:205-215) */
 
 /* Per-element material description (discretization.cpp:19-97 defaults). */
 typedef struct nrt_material {
@@ -160,11 +161,11 @@ typedef struct nrt_setup_out {
 typedef struct nrt_options {
   int32_t kind;         /* NRT_EXPLICIT / NRT_IMPLICIT                          */
   int32_t precision;    /* NRT_FP64 / NRT_FP32_REFINE                           */
-  int32_t krylov;       /* NRT_BICGSTAB / NRT_GMRES                             */
-  int32_t restart;      /* GMRES restart length (reference CPU: 150)           */
+  int32_t krylov;       /* NRT_BICGSTAB / NRT_GMRES / NRT_DIRECT                */
+  int32_t restart;      /* GMRES restart length, 1..512 (reference CPU: 150)   */
   int32_t tol_mode;     /* NRT_TOL_ABSOLUTE / NRT_TOL_RELATIVE                  */
-  int32_t x0_mode;      /* NRT_X0_PREVIOUS / NRT_X0_RHS (Krylov only; the tree
-                           solve's refinement always starts from V)             */
+  int32_t x0_mode;      /* NRT_X0_PREVIOUS / NRT_X0_RHS / NRT_X0_ZERO (Krylov
+                           only; the tree solve's refinement starts from V)     */
   int32_t check_every;  /* BiCGSTAB convergence test period (reference: 30)    */
   int32_t max_iter;     /* 0 -> N (reference MAX_ITER)                          */
   int32_t device;       /* -1 -> $idGPU or 0                                    */

@@ -87,3 +87,17 @@ def test_batch_rejects_krylov_and_bad_threads(nrt):
         st = L.nrt_batch_create(C.byref(nrt.default_options(**kw)), C.byref(b))
         assert st in (nrt.STATUS_CODES["unsupported"], nrt.STATUS_CODES["invalid"])
         assert not b.value
+
+
+def test_gmres_option_validation(nrt):
+    """Restart outside [1, 512] and unknown x0 modes are rejected before any device
+    work (so this runs without a GPU)."""
+    import ctypes as C
+    L = nrt.lib()
+    p = nrt.Problem.__new__(nrt.Problem)
+    h = C.c_void_p()
+    for kw, code in ((dict(restart=0), "invalid"), (dict(restart=513), "invalid"),
+                     (dict(restart=30, x0_mode=7), "invalid")):
+        o = nrt.default_options(kind=nrt.IMPLICIT, krylov=nrt.GMRES, **kw)
+        st = L.nrt_create(None, C.byref(o), C.byref(h))
+        assert st == nrt.STATUS_CODES[code], kw
