@@ -96,9 +96,10 @@ nrt_options resolve_options(const nrt_options* opt) {
   NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.kind == NRT_IMPLICIT, NRT_ERR_INVALID, "bad kind");
   NRT_REQUIRE(o.precision == NRT_FP64 || o.precision == NRT_FP32_REFINE, NRT_ERR_INVALID,
               "bad precision");
-  NRT_REQUIRE(o.precision == NRT_FP64 || (o.kind == NRT_IMPLICIT && o.krylov == NRT_DIRECT),
+  NRT_REQUIRE(o.precision == NRT_FP64 ||
+                  (o.kind == NRT_IMPLICIT && (o.krylov == NRT_DIRECT || o.krylov == NRT_GMRES)),
               NRT_ERR_UNSUPPORTED,
-              "FP32 with FP64 refinement is available for the implicit tree solve");
+              "FP32 with FP64 refinement is available for the implicit tree solve and GMRES");
   NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB || o.krylov == NRT_GMRES ||
                   o.krylov == NRT_DIRECT,
               NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB, NRT_GMRES or NRT_DIRECT");

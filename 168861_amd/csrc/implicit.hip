@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include "nrt_internal.hpp"
 #include "nrt_device.hpp"
 #include "tree.hpp"
@@ -58,6 +59,7 @@ struct ImpHH {
 
 struct ImpStats {
   unsigned long long iters_total;
+  unsigned long long sweeps_total;  // refinement sweeps (FP32 forms)
   unsigned long long steps;
   int last_iters;
   int status;
@@ -147,6 +149,7 @@ struct Implicit {
   bool gmres = false;    // Krylov method: GMRES(m) (else BiCGSTAB)
   int gm = 0;            // GMRES restart length
   DBuf<double> gV, gR, gpart;
+  DBuf<float> gV32, gW32, val32;  // mixed-precision GMRES: FP32 basis, vectors, matrix
   bool grid_mode = false;
   int threads = 256;
   TreePlanHost plan_h;
@@ -313,6 +316,18 @@ __global__ void k_fill(int N, double dT, const int32_t* __restrict__ rowptr,
     hh_mat[hh] = k;
     hh_dl[hh] = dlv;
   }
+}
+
+// FP32 copy of the matrix with every row divided by its diagonal entry.
+__global__ void k_rows_scaled_f32(int N, const int32_t* __restrict__ rowptr,
+                                  const int32_t* __restrict__ col, const double* __restrict__ val,
+                                  float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double dg = 1.0;
+  for (int k = rowptr[i]; k < rowptr[i + 1]; k++)
+    if (col[k] == i) dg = val[k];
+  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) out[k] = (float)(val[k] / dg);
 }
 
 // --------------------------------------------------------- persistent kernel
@@ -617,13 +632,6 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
   }
 }
 
-// --------------------------------------------- tree partition solve kernels
-// One persistent kernel per call, as the BiCGSTAB one, with the Krylov loop
-// replaced by the exact tree partition solve (tree.hpp): either one FP64 solve
-// per step (direct), or FP64 iterative refinement around FP32 solves whose
-// residual b - A x and update x += d are the only FP64 vector work. Block mode:
-// one workgroup owns the whole tree and synchronises with workgroup barriers;
-// grid mode (large trees): every workgroup takes a share of the rows and chunks,
 // ------------------------------------------------------------------ GMRES(m)
 // Restarted GMRES (the method of the reference CPU solver, MKL FGMRES(150) with
 // x0 = 0 and an absolute 1e-14 stop, solverIm-cpu.cpp:192-272) as one persistent
@@ -637,40 +645,85 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
 // factor, so the small least-squares problem needs no barrier of its own. The
 // SpMV reads a neighbour's entries of v_{k+1} = w_k / h_{k+1,k} by recomputing
 // them from w_k, double buffered by step parity, instead of waiting for them.
+//
+// T = float is the mixed-precision form: every cycle starts from the FP64
+// residual r = b - A x (FP64 matrix values) and ends with the FP64 update
+// x += V y, and the Arnoldi work between them (basis, SpMV, Gram-Schmidt) runs in
+// FP32 with compensated inner products. A cycle is then one sweep of iterative
+// refinement around an FP32 GMRES solve of the Jacobi-scaled system
+// D^-1 A d = D^-1 r (D = diag A; the FP32 matrix copy holds the scaled rows): the
+// unit rows of ghosts and sealed ends beside physical rows of ~1e-9 S give A a
+// condition number near 1e9, beyond what FP32 refinement converges on, and the
+// scaled system's is near 1e3. A cycle ends once its residual estimate has fallen
+// by kGmresInnerRel32 (the reach of FP32 arithmetic) or after m steps; only the
+// FP64 residual at the next cycle's start decides convergence, in the same scaled
+// norm: |D^-1 (b - A x)| <= refine_tol |D^-1 b|. The Hessenberg column, the Givens
+// rotations and the least-squares solve stay FP64 in both forms.
 constexpr int kGmresMaxRestart = 512;
+constexpr double kGmresInnerRel32 = 1e-6;
 
+template <class T>
 struct GmresDev {
   int m;           // restart length
-  double* V;       // (m+1) x N basis (v_k at V + k N; only a block's own rows)
-  double* w0;      // Arnoldi vectors, even and odd steps
-  double* w1;
+  T* V;            // (m+1) x N basis (v_k at V + k N; only a block's own rows)
+  T* w0;           // Arnoldi vectors, even and odd steps
+  T* w1;
   double* R;       // per workgroup: packed R (m(m+1)/2), cs, sn (m), g (m+1), y (m)
   int64_t rstride;
   double* part;    // 2 sets x nblocks x (m+1) partial sums
+  T* vals;         // matrix values of the Arnoldi SpMV (FP64: the matrix itself)
+  double refine_tol;  // FP32: convergence of the FP64 residual, relative to |b|
 };
+
+// Inner-product accumulators of one thread. FP64: plain sums. FP32: the product
+// error (one FMA) and the addition error (Knuth two-sum) are carried in a second
+// word, so a thread's partial is accurate to about twice FP32 precision before the
+// cross-thread sums, which run in FP64.
+struct DotAcc64 {
+  double s = 0;
+  __device__ __forceinline__ void add(double a, double b) { s = s + a * b; }
+  __device__ __forceinline__ double sum() const { return s; }
+};
+struct DotAcc32 {
+  float s = 0, c = 0;
+  __device__ __forceinline__ void add(float a, float b) {
+#pragma clang fp contract(off)
+    const float p = a * b;
+    const float pe = fmaf(a, b, -p);  // a*b - p exactly
+    float t, e;
+    two_sum(s, p, t, e);
+    s = t;
+    c = c + (e + pe);
+  }
+  __device__ __forceinline__ double sum() const { return (double)s + (double)c; }
+};
+template <class T>
+using DotAcc = typename std::conditional<std::is_same<T, float>::value, DotAcc32, DotAcc64>::type;
 
 // h[0..cnt) = V_j . w over the whole vector (one grid barrier); every block ends
 // with the same values in hsh.
-__device__ __forceinline__ void gmres_dots(const ImpDev& d, const GmresDev& g, int cnt,
-                                           const double* __restrict__ w, int r_begin, int r_end,
+template <class T>
+__device__ __forceinline__ void gmres_dots(const ImpDev& d, const GmresDev<T>& g, int cnt,
+                                           const T* __restrict__ w, int r_begin, int r_end,
                                            double* hsh, double* lds, int& gslot) {
   const int stride = g.m + 1;
   double* part = g.part + (size_t)gslot * d.nblocks * stride;
   gslot ^= 1;
   const size_t N = (size_t)d.N;
   for (int k0 = 0; k0 < cnt; k0 += 8) {
-    double acc[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) acc[j] = 0;
+    DotAcc<T> acc[8];
     for (int i = r_begin + (int)threadIdx.x; i < r_end; i += (int)blockDim.x) {
-      const double wi = w[i];
+      const T wi = w[i];
 #pragma unroll
       for (int j = 0; j < 8; j++)
-        if (k0 + j < cnt) acc[j] = acc[j] + g.V[(size_t)(k0 + j) * N + i] * wi;
+        if (k0 + j < cnt) acc[j].add(g.V[(size_t)(k0 + j) * N + i], wi);
     }
-    block_sum<8>(acc, lds);
+    double sums[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) sums[j] = acc[j].sum();
+    block_sum<8>(sums, lds);
     if (threadIdx.x == 0)
-      for (int j = 0; j < 8 && k0 + j < cnt; j++) part[d.vb * stride + k0 + j] = acc[j];
+      for (int j = 0; j < 8 && k0 + j < cnt; j++) part[d.vb * stride + k0 + j] = sums[j];
   }
   grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
   for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
@@ -681,7 +734,9 @@ __device__ __forceinline__ void gmres_dots(const ImpDev& d, const GmresDev& g, i
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, GmresDev g) {
+template <class T>
+__global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev<T> g) {
+  constexpr bool F32 = std::is_same<T, float>::value;
   __shared__ double lds[8 * 16];
   __shared__ double hsh[kGmresMaxRestart + 1];
   __shared__ double h2sh[kGmresMaxRestart + 1];
@@ -692,6 +747,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
   const int r_end = min(d.N, r_begin + d.rpb);
   const int bs = kImpBlock;
   const double* val = d.val;
+  const T* vals = g.vals;
   const int32_t* __restrict__ rowptr = d.rowptr;
   const int32_t* __restrict__ col = d.col;
   const size_t N = (size_t)d.N;
@@ -716,7 +772,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
         double W, K;
         imp_gate(d.hh, hh, v, d.dt_ms, W, K);
         double diag;
-        (void)imp_update_row(d, hh, i, W, diag);
+        if (imp_update_row(d, hh, i, W, diag) && F32)
+          for (int k = rowptr[i]; k < rowptr[i + 1]; k++) g.vals[k] = (T)(val[k] / diag);
         kd = K * d.dxv[i];
       } else {
         kd = d.kdx[i];
@@ -732,13 +789,21 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
     double rr = 0, bb = 0, thr = 0;
     bool ok = true;
     while (ok) {
-      // ---- cycle start: r0 = b - A x, beta = |r0|
+      // ---- cycle start: r0 = b - A x, beta = |r0| (FP64 in both forms; FP32
+      // cycles work on the Jacobi-scaled system, r0 = D^-1 (b - A x))
       double red[2] = {0.0, 0.0};
       for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
-        double ax = 0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * d.x[col[k]];
-        const double bi = d.b[i];
-        const double ri = bi - ax;
+        double ax = 0, dg = 1.0;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+          ax = ax + val[k] * d.x[col[k]];
+          if (F32 && col[k] == i) dg = val[k];
+        }
+        double bi = d.b[i];
+        double ri = bi - ax;
+        if (F32) {
+          ri = ri / dg;
+          bi = bi / dg;
+        }
         d.r0[i] = ri;
         red[0] = red[0] + ri * ri;
         red[1] = red[1] + bi * bi;
@@ -748,7 +813,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
       rr = red[0];
       if (cycles == 0) {
         bb = red[1];
-        thr = d.tol_mode == NRT_TOL_RELATIVE ? d.tol * sqrt(bb) : d.tol;
+        if (F32) thr = g.refine_tol * sqrt(bb);
+        else thr = d.tol_mode == NRT_TOL_RELATIVE ? d.tol * sqrt(bb) : d.tol;
       }
       if (!isfinite(rr)) {
         status = NRT_ERR_NUMERIC;
@@ -765,35 +831,42 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
         gv[0] = beta;
         for (int j = 1; j <= m; j++) gv[j] = 0.0;
       }
-      const double* src = d.r0;  // v_k = src / div, neighbours' entries recomputed
+      // an FP32 cycle stops where FP32 stops resolving its own residual
+      const double stop = F32 ? fmax(thr, kGmresInnerRel32 * beta) : thr;
+      // v_k = src / div, neighbours' entries recomputed: v_0 from the FP64
+      // residual, v_k (k > 0) from the previous step's Arnoldi vector
+      const T* prev = nullptr;
       double div = beta;
       int kk = 0;
       bool conv = false;
       for (int k = 0; k < m; k++) {
-        double* __restrict__ wk = (k & 1) ? g.w1 : g.w0;
+        T* __restrict__ wk = (k & 1) ? g.w1 : g.w0;
+        const T divT = (T)div;
+        auto vk = [&](int c) -> T { return k == 0 ? (T)(d.r0[c] / div) : prev[c] / divT; };
         it++;
         // v_k (own rows) and w = A v_k
         for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
-          g.V[(size_t)k * N + i] = src[i] / div;
-          double acc = 0;
-          for (int q = rowptr[i]; q < rowptr[i + 1]; q++) acc = acc + val[q] * (src[col[q]] / div);
+          g.V[(size_t)k * N + i] = vk(i);
+          T acc = 0;
+          for (int q = rowptr[i]; q < rowptr[i + 1]; q++) acc = acc + vals[q] * vk(col[q]);
           wk[i] = acc;
         }
         // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2; h += h2
-        gmres_dots(d, g, k + 1, wk, r_begin, r_end, hsh, lds, gslot);
+        gmres_dots<T>(d, g, k + 1, wk, r_begin, r_end, hsh, lds, gslot);
         for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
-          double wi = wk[i];
-          for (int j = 0; j <= k; j++) wi = wi - hsh[j] * g.V[(size_t)j * N + i];
+          T wi = wk[i];
+          for (int j = 0; j <= k; j++) wi = wi - (T)hsh[j] * g.V[(size_t)j * N + i];
           wk[i] = wi;
         }
-        gmres_dots(d, g, k + 1, wk, r_begin, r_end, h2sh, lds, gslot);
-        double nrm[1] = {0.0};
+        gmres_dots<T>(d, g, k + 1, wk, r_begin, r_end, h2sh, lds, gslot);
+        DotAcc<T> na;
         for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
-          double wi = wk[i];
-          for (int j = 0; j <= k; j++) wi = wi - h2sh[j] * g.V[(size_t)j * N + i];
+          T wi = wk[i];
+          for (int j = 0; j <= k; j++) wi = wi - (T)h2sh[j] * g.V[(size_t)j * N + i];
           wk[i] = wi;
-          nrm[0] = nrm[0] + wi * wi;
+          na.add(wi, wi);
         }
+        double nrm[1] = {na.sum()};
         ok = reduce_all<1>(d, nrm, lds, slot);
         if (!ok) break;
         const double hn = sqrt(nrm[0]);
@@ -825,13 +898,13 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
           ok = false;
           break;
         }
-        if (res <= thr || fsh[1] == 0.0) {
+        if (res <= stop || fsh[1] == 0.0) {
           conv = true;
           rr = res * res;
           break;
         }
         if (it >= d.max_iter) break;
-        src = wk;
+        prev = wk;
         div = hn;
       }
       if (status == NRT_ERR_NUMERIC) {
@@ -839,7 +912,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
         break;
       }
       if (!ok) break;
-      // ---- x += V y, R y = g (back substitution, every block)
+      // ---- x += V y, R y = g (back substitution, every block; FP64 update)
       if (threadIdx.x == 0) {
         for (int j = kk - 1; j >= 0; j--) {
           double acc = gv[j];
@@ -851,7 +924,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
       __syncthreads();
       for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
         double xi = d.x[i];
-        for (int j = 0; j < kk; j++) xi = xi + ysh[j] * g.V[(size_t)j * N + i];
+        for (int j = 0; j < kk; j++) xi = xi + ysh[j] * (double)g.V[(size_t)j * N + i];
         d.x[i] = xi;
       }
       grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);  // x before the next residual
@@ -859,7 +932,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
         ok = false;
         break;
       }
-      if (conv) {
+      // FP64: the cycle's residual estimate decides; FP32: the next FP64 residual
+      if (conv && !F32) {
         status = NRT_OK;
         break;
       }
@@ -868,6 +942,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
     if (!ok) status = NRT_ERR_HIP;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       d.st->iters_total += (unsigned long long)it;
+      d.st->sweeps_total += (unsigned long long)cycles;
       d.st->steps += 1;
       d.st->last_iters = it;
       d.st->last_rr = rr;
@@ -890,6 +965,13 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres_fp64(ImpDev d, Gmr
   }
 }
 
+// --------------------------------------------- tree partition solve kernels
+// One persistent kernel per call, as the BiCGSTAB one, with the Krylov loop
+// replaced by the exact tree partition solve (tree.hpp): either one FP64 solve
+// per step (direct), or FP64 iterative refinement around FP32 solves whose
+// residual b - A x and update x += d are the only FP64 vector work. Block mode:
+// one workgroup owns the whole tree and synchronises with workgroup barriers;
+// grid mode (large trees): every workgroup takes a share of the rows and chunks,
 // grid barriers separate the levels, and workgroup 0 solves the compressed forest.
 constexpr int kTreeBlock = 1024;
 constexpr int kTreeArrays = 10;  // d, up, dn, r, x, ca, cb, cg, cf, ci
@@ -1076,15 +1158,19 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     } else {
       while (true) {
         // FP64 residual r = b - A x into the level-0 right-hand side
+        // (its norm, and that of b, Jacobi-scaled: |D^-1 r| <= refine_tol |D^-1 b|)
         double red[2] = {0.0, 0.0};
         for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
-          double ax = 0;
-          for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * d.x[col[k]];
+          double ax = 0, dg = 1.0;
+          for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+            ax = ax + val[k] * d.x[col[k]];
+            if (d.srole[k] == 0) dg = val[k];
+          }
           const double bi = d.b[i];
           const double ri = bi - ax;
           t.w.r[i] = (T)ri;
-          red[0] = red[0] + ri * ri;
-          red[1] = red[1] + bi * bi;
+          red[0] = red[0] + (ri / dg) * (ri / dg);
+          red[1] = red[1] + (bi / dg) * (bi / dg);
         }
         if (GRID) ok = reduce_all<2>(d, red, lds, slot);
         else block_sum<2>(red, lds);
@@ -1109,6 +1195,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     if (!ok) status = NRT_ERR_HIP;
     if (d.vb == 0 && threadIdx.x == 0) {
       d.st->iters_total += (unsigned long long)sweeps;
+      if (!t.direct) d.st->sweeps_total += (unsigned long long)sweeps;
       d.st->steps += 1;
       d.st->last_iters = sweeps;
       d.st->last_rr = rr;
@@ -1291,9 +1378,14 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
     e->x.upload(p.V0, s);
     e->b.alloc(N);
     NRT_HIP(hipMemsetAsync(e->b.p, 0, sizeof(double) * N, s));
-    if (o.krylov != NRT_DIRECT) {
-      for (DBuf<double>* vbuf : {&e->r0, &e->r, &e->p[0], &e->p[1], &e->v[0], &e->v[1], &e->s,
-                                 &e->t}) {
+    const bool mixed = o.precision == NRT_FP32_REFINE;
+    if (o.krylov == NRT_BICGSTAB || o.krylov == NRT_GMRES) {
+      // GMRES: the residual (r0), and in FP64 its Arnoldi vectors (s, t)
+      std::vector<DBuf<double>*> vecs{&e->r0};
+      if (o.krylov == NRT_BICGSTAB) vecs = {&e->r0, &e->r, &e->p[0], &e->p[1], &e->v[0],
+                                            &e->v[1], &e->s, &e->t};
+      else if (!mixed) vecs = {&e->r0, &e->s, &e->t};
+      for (DBuf<double>* vbuf : vecs) {
         vbuf->alloc(N);
         NRT_HIP(hipMemsetAsync(vbuf->p, 0, sizeof(double) * N, s));
       }
@@ -1423,10 +1515,22 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
     } else if (o.krylov == NRT_GMRES) {
       e->gmres = true;
       e->gm = std::min(o.restart, N);
-      e->nblocks = choose_blocks(hnd, N, (const void*)k_implicit_gmres_fp64);
+      e->nblocks = choose_blocks(hnd, N, mixed ? (const void*)k_implicit_gmres<float>
+                                               : (const void*)k_implicit_gmres<double>);
       e->threads = kImpBlock;
       const int m = e->gm;
-      e->gV.alloc((size_t)(m + 1) * N);
+      if (mixed) {
+        e->gV32.alloc((size_t)(m + 1) * N);
+        e->gW32.alloc((size_t)2 * N);
+        NRT_HIP(hipMemsetAsync(e->gW32.p, 0, sizeof(float) * 2 * N, s));
+        e->val32.alloc(nnz);
+        hipLaunchKernelGGL(k_rows_scaled_f32, dim3(grid_for(N, 256)), dim3(256), 0, s, N,
+                           (const int32_t*)e->rowptr.p, (const int32_t*)e->col.p,
+                           (const double*)e->val.p, e->val32.p);
+        NRT_HIP(hipGetLastError());
+      } else {
+        e->gV.alloc((size_t)(m + 1) * N);
+      }
       const size_t rstride = (size_t)m * (m + 1) / 2 + 4 * (size_t)m + 1;
       e->gR.alloc(rstride * e->nblocks);
       e->gpart.alloc((size_t)2 * e->nblocks * (m + 1));
@@ -1544,11 +1648,20 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.trace = trace_dev;
   if (e->method == 0 && e->gmres) {
     const int m = e->gm;
-    GmresDev g{m, e->gV.p, e->t.p, e->s.p, e->gR.p,
-               (int64_t)m * (m + 1) / 2 + 4 * (int64_t)m + 1, e->gpart.p};
-    void* args[] = {&d, &g};
-    NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_gmres_fp64, dim3(e->nblocks),
-                                       dim3(kImpBlock), args, 0, hnd->stream));
+    const int64_t rstride = (int64_t)m * (m + 1) / 2 + 4 * (int64_t)m + 1;
+    if (o.precision == NRT_FP32_REFINE) {
+      GmresDev<float> g{m, e->gV32.p, e->gW32.p, e->gW32.p + e->N, e->gR.p, rstride,
+                        e->gpart.p, e->val32.p, o.refine_tol};
+      void* args[] = {&d, &g};
+      NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_gmres<float>, dim3(e->nblocks),
+                                         dim3(kImpBlock), args, 0, hnd->stream));
+    } else {
+      GmresDev<double> g{m, e->gV.p, e->t.p, e->s.p, e->gR.p, rstride, e->gpart.p, e->val.p,
+                         0.0};
+      void* args[] = {&d, &g};
+      NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_gmres<double>, dim3(e->nblocks),
+                                         dim3(kImpBlock), args, 0, hnd->stream));
+    }
     return NRT_OK;
   }
   if (e->method == 0) {
@@ -1583,6 +1696,7 @@ static int collect_stats(nrt_handle* hnd) {
   NRT_HIP(hipStreamSynchronize(hnd->stream));
   hnd->stats.steps = (int64_t)st.steps;
   hnd->stats.krylov_iters = (int64_t)st.iters_total;
+  hnd->stats.refine_sweeps = hnd->opt.precision == NRT_FP32_REFINE ? (int64_t)st.sweeps_total : 0;
   hnd->stats.last_iters = st.last_iters;
   hnd->stats.last_residual = sqrt(st.last_rr);
   hnd->stats.last_rhs_norm = sqrt(st.last_bb);

@@ -44,7 +44,11 @@ extern "C" {
 #define NRT_IMPLICIT 1   /* backward Euler + Krylov (solverIm-*.cpp)       */
 
 #define NRT_FP64          0  /* plain form: every step in FP64                      */
-#define NRT_FP32_REFINE   1  /* FP32 gates/SpMV/Krylov + FP64 residual & update     */
+#define NRT_FP32_REFINE   1  /* FP32 solves inside FP64 iterative refinement: the
+                                tree solve (NRT_DIRECT) or GMRES's Arnoldi work
+                                (SpMV, basis, compensated dots) in FP32; the
+                                residual b - A x, the update of V and the HH
+                                gates stay FP64                                   */
 
 #define NRT_BICGSTAB 0   /* solverIm-gpu.cpp:725-843 */
 #define NRT_GMRES    1   /* solverIm-cpu.cpp:192-272 (restarted GMRES)       */
@@ -170,9 +174,13 @@ typedef struct nrt_options {
   int32_t max_iter;     /* 0 -> N (reference MAX_ITER)                          */
   int32_t device;       /* -1 -> $idGPU or 0                                    */
   int32_t num_blocks;   /* 0 -> autotuned grid size for persistent kernels     */
-  double tol;           /* Krylov tolerance (see tol_mode)                      */
-  double refine_tol;    /* FP32_REFINE: outer FP64 residual tolerance (relative)*/
-  int32_t refine_max;   /* FP32_REFINE: maximum refinement sweeps               */
+  double tol;           /* Krylov tolerance (see tol_mode; FP64 forms)          */
+  double refine_tol;    /* FP32_REFINE: outer FP64 residual tolerance, relative
+                           and Jacobi-scaled: |D^-1 (b - A x)|_2 <=
+                           refine_tol |D^-1 b|_2 with D = diag(A)             */
+  int32_t refine_max;   /* FP32_REFINE tree solve: maximum refinement sweeps
+                           (GMRES: each restart cycle is one sweep; cycles are
+                           bounded by max_iter Arnoldi steps in total)          */
   int32_t precond;      /* NRT_PRECOND_NONE / NRT_PRECOND_TREE                  */
   int32_t chunk;        /* tree solve: chain chunk length at level 0 (0 -> auto) */
   int32_t chunk_hi;     /* tree solve: chunk length of the reduced levels
@@ -185,11 +193,12 @@ typedef struct nrt_options {
 typedef struct nrt_stats {
   int64_t steps;            /* time steps taken                                 */
   int64_t krylov_iters;     /* Krylov iterations summed over steps              */
-  int64_t refine_sweeps;    /* FP64 refinement sweeps summed over steps         */
+  int64_t refine_sweeps;    /* FP32_REFINE: refinement sweeps summed over steps */
   int32_t last_iters;       /* Krylov iterations of the last step               */
   int32_t last_status;      /* status of the last step                          */
-  double last_residual;     /* ||b - A x||_2 at exit of the last step           */
-  double last_rhs_norm;     /* ||b||_2 of the last step                         */
+  double last_residual;     /* ||b - A x||_2 at exit of the last step
+                               (FP32_REFINE: ||D^-1 (b - A x)||_2)              */
+  double last_rhs_norm;     /* ||b||_2 of the last step (FP32_REFINE: ||D^-1 b||) */
   double kernel_ms;         /* device time of the dominant kernel, last run     */
   int64_t kernel_launches;  /* launches of the dominant kernel, last run        */
   int64_t matrix_nnz;       /* implicit: non-zeros of the assembled CSR         */

@@ -101,3 +101,10 @@ def test_gmres_option_validation(nrt):
         o = nrt.default_options(kind=nrt.IMPLICIT, krylov=nrt.GMRES, **kw)
         st = L.nrt_create(None, C.byref(o), C.byref(h))
         assert st == nrt.STATUS_CODES[code], kw
+    # FP32 inside FP64 refinement: the tree solve and GMRES, not BiCGSTAB. GMRES's
+    # options pass and the NULL problem is what gets rejected.
+    o = nrt.default_options(kind=nrt.IMPLICIT, krylov=nrt.BICGSTAB, precision=nrt.FP32_REFINE)
+    assert L.nrt_create(None, C.byref(o), C.byref(h)) == nrt.STATUS_CODES["unsupported"]
+    o = nrt.default_options(kind=nrt.IMPLICIT, krylov=nrt.GMRES, precision=nrt.FP32_REFINE)
+    assert L.nrt_create(None, C.byref(o), C.byref(h)) == nrt.STATUS_CODES["invalid"]
+    assert "problem" in L.nrt_last_error(None).decode().lower()
