@@ -81,7 +81,31 @@ def run_form(nh, prob, opts, first, steps, a0, a1):
     s.close()
     return {"wall": t1 - t0, "kernel_ms": st1["kernel_ms"],
             "inner": st1["krylov_iters"] - st0["krylov_iters"], "nnz": st1["matrix_nnz"],
-            "blocks": st1["num_blocks"], "v": v}
+            "blocks": st1["num_blocks"], "v": v,
+            # GMRES restart cycles (FP32 forms: the refinement sweeps; FP64: the
+            # restarts plus one per step)
+            "cycles": (st1["refine_sweeps"] - st0["refine_sweeps"]) if st1["refine_sweeps"]
+            else (st1["restarts"] - st0["restarts"]) + steps}
+
+
+def gmres_bytes(N, H, nnz, steps, iters, cycles, vbytes):
+    """Algorithmic bytes of GMRES(m) steps (k_implicit_gmres<double|float>), with
+    the cycles of a run taken as equally long (iters / cycles Arnoldi steps):
+    update phase per step: 144 B per HH element + 40 B per element;
+    per cycle start: the FP64 residual (matrix 12 nnz + 4 (N+1), V, b, r0 written:
+      24 N) and at its end x += V y (kk basis vectors of vbytes N, x read and
+      written 16 N);
+    per Arnoldi step k: the SpMV (values vbytes nnz + columns 4 nnz + rowptr
+      4 (N+1), the previous vector read, v_k and w written: 3 vbytes N) and CGS2
+      (two projections and two corrections each read v_0..v_k and w, the
+      corrections write w: (4 (k+1) + 6) vbytes N)."""
+    if cycles <= 0:
+        return steps * (144 * H + 40 * N)
+    kk = iters / cycles
+    per_cycle = (12 * nnz + 4 * (N + 1) + 24 * N) + kk * vbytes * N + 16 * N
+    spmv = vbytes * nnz + 4 * nnz + 4 * (N + 1) + 3 * vbytes * N
+    cgs = (2 * kk * (kk + 1) + 6 * kk) * vbytes * N
+    return steps * (144 * H + 40 * N) + cycles * (per_cycle + kk * spmv + cgs)
 
 
 def explicit_bytes(N, H):
@@ -199,6 +223,70 @@ def served_stream(nh, reqs, precision):
                                            "batch_kernel_ms")}}
 
 
+def dendritic_tree(nh, wl, tree_steps, krylov_steps, max_iter):
+    """Config 3: the seeded depth-14 dendritic tree (16,383 segments, ~1.8M
+    compartments, dx = 2 um, per-segment strain U[0, 0.3], HH soma, dt = 25 us).
+    GMRES(30) in FP64 (relative 1e-10) and in FP32 inside FP64 refinement (scaled
+    relative 1e-12) for `krylov_steps` steps (each step takes hundreds of Arnoldi
+    steps), the tree solves for `tree_steps`; every form's error is taken against
+    the FP64 tree solve after the same steps."""
+    t0 = time.perf_counter()
+    geom, run = wl.random_tree(wl.seed_for(3), depth=14)
+    prob, _ = nh.setup(geom, dT=run.dT)
+    setup_s = time.perf_counter() - t0
+    N, H = prob.N, prob.H
+    a0, a1 = nh.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
+    gm = dict(krylov=nh.GMRES, restart=30, max_iter=max_iter)
+    specs = {
+        "fp64_tree_direct": (tree_steps, dict(krylov=nh.DIRECT, precision=nh.FP64)),
+        "mixed_tree_ir": (tree_steps, dict(krylov=nh.DIRECT, precision=nh.FP32_REFINE,
+                                           refine_tol=1e-10)),
+        "fp64_gmres30": (krylov_steps, dict(gm, precision=nh.FP64, tol=1e-10,
+                                            tol_mode=nh.TOL_RELATIVE)),
+        # the scaled 2-norm over ~900k rows: 1e-12 keeps every element within 1e-4 mV
+        "mixed_gmres30_ir": (krylov_steps, dict(gm, precision=nh.FP32_REFINE, refine_tol=1e-12)),
+    }
+    refs = {}
+    for st in {tree_steps, krylov_steps}:
+        refs[st] = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP64), 1, st, a0,
+                            a1)["v"]
+    mask = np.ones(N, bool)
+    mask[[0, N - 1]] = False
+    kernels = {"fp64_tree_direct": "k_implicit_tree<double>",
+               "mixed_tree_ir": "k_implicit_tree<float>",
+               "fp64_gmres30": "k_implicit_gmres<double>",
+               "mixed_gmres30_ir": "k_implicit_gmres<float>"}
+    out = {"workload": "seeded binary dendritic tree, depth 14 (16,383 segments of "
+                       "U[20,200] um), Rall 3/2 diameters from 4 um, dx 2 um, strain U[0,0.3] "
+                       "per segment, HH soma, dt 25 us, stimulus 1 nA x 0.5 ms at the soma",
+           "N": N, "hh_elements": H, "setup_s": setup_s}
+    for name, (steps, opts) in specs.items():
+        try:
+            f = run_form(nh, prob, opts, 1, steps, a0, a1)
+        except nh.NrtError as e:
+            out[name] = {"steps": steps, "error": str(e)}
+            continue
+        vb = 8 if name.startswith("fp64") else 4
+        if "gmres" in name:
+            nbytes = gmres_bytes(N, H, f["nnz"], steps, f["inner"], f["cycles"], vb)
+        else:
+            nbytes = tree_bytes(N, H, f["nnz"], steps, f["inner"], vb)
+        achieved = nbytes / (f["kernel_ms"] / 1e3) / 1e9
+        out[name] = {"compartment_steps_per_s": N * steps / f["wall"],
+                     "ms_per_step": f["wall"] / steps * 1e3, "steps": steps,
+                     "max_abs_dV_mV_vs_fp64": float(np.max(np.abs(f["v"] - refs[steps])[mask]))
+                     * 1e3,
+                     "inner_per_step": f["inner"] / steps,
+                     "cycles_per_step": f["cycles"] / steps, "workgroups": f["blocks"],
+                     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                  "kernel": kernels[name], "kernel_ms": f["kernel_ms"],
+                                  "algorithmic_bytes": nbytes}}
+        if "gmres" not in name:
+            del out[name]["cycles_per_step"]
+    return out
+
+
 def cpu_baseline(prob, budget_s=15.0, max_steps=3):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle  # checker library: timed here as the CPU baseline only
@@ -237,6 +325,12 @@ def main():
     ap.add_argument("--sweep-trees", type=int, default=4096)
     ap.add_argument("--sweep-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--tree3-steps", type=int, default=200,
+                    help="depth-14 dendritic tree: tree-solve steps (0 skips the leg)")
+    ap.add_argument("--krylov3-steps", type=int, default=4,
+                    help="depth-14 dendritic tree: GMRES(30) steps")
+    ap.add_argument("--krylov3-max-iter", type=int, default=4000,
+                    help="depth-14 dendritic tree: Arnoldi steps allowed per time step")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
@@ -323,6 +417,9 @@ def main():
     if "fp64_plain_bicgstab" in summary:
         out["speedup_vs_fp64_plain"] = (b["compartment_steps_per_s"] /
                                         summary["fp64_plain_bicgstab"]["compartment_steps_per_s"])
+    if a.tree3_steps > 0:
+        out["dendritic_tree"] = dendritic_tree(nh, wl, a.tree3_steps, a.krylov3_steps,
+                                               a.krylov3_max_iter)
     if a.sweep_trees > 0:
         out["batched_sweep"] = batched_sweep(nh, wl, a.sweep_trees, a.sweep_steps)
     if a.serve_requests > 0:
