@@ -38,7 +38,6 @@ namespace {
 constexpr int kExBlock = 256;            // elements per block of the explicit batch
 constexpr int kTuneThreads[] = {256, 512, 1024};
 constexpr int kPlanThreads = 512;        // workers assumed by a slot's chunk choice
-constexpr int kRowsPerWorkgroup = 24576;  // a slot gets a workgroup per this many rows
 // Footprint per element used for admission (bytes; the arrays a slot allocates,
 // rounded up): implicit tree solve ~ 7 FP64 + 3 int32 vectors, 3 CSR entries, the
 // plan and 8 tree work values over ~1.1 N level nodes; explicit ~ 6 FP64 + code.

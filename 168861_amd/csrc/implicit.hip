@@ -1021,15 +1021,40 @@ __device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>
     }
     tsync<GRID>(d);
   }
+  // Final forest. Split (grid launches): every workgroup eliminates the subtrees of
+  // its groups below the cut height, workgroup 0 the heights above, then back down;
+  // a subtree's nodes never leave their workgroup, so only the two hand-overs
+  // need grid barriers. Unsplit: workgroup 0 takes every height.
+  const int cut = GRID ? P.fin_cut : 0;
+  if (cut > 0) {
+    for (int h = 0; h < cut; h++) {
+      for (int g = d.vb; g < P.sub_groups; g += d.nblocks)
+        for (int k = P.sub_hoff[g * cut + h] + (int)threadIdx.x; k < P.sub_hoff[g * cut + h + 1];
+             k += blockDim.x)
+          tps_final_up(P, w, P.sub_order[k]);
+      __syncthreads();
+    }
+    tsync<GRID>(d);
+  }
   if (!GRID || d.vb == 0) {
-    for (int h = 0; h < P.heights; h++) {
+    for (int h = cut; h < P.heights; h++) {
       for (int k = P.fin_hoff[h] + (int)threadIdx.x; k < P.fin_hoff[h + 1]; k += blockDim.x)
         tps_final_up(P, w, P.fin_order[k]);
       __syncthreads();
     }
-    for (int h = P.heights - 1; h >= 0; h--) {
+    for (int h = P.heights - 1; h >= cut; h--) {
       for (int k = P.fin_hoff[h] + (int)threadIdx.x; k < P.fin_hoff[h + 1]; k += blockDim.x)
         tps_final_down(P, w, P.fin_order[k]);
+      __syncthreads();
+    }
+  }
+  if (cut > 0) {
+    tsync<GRID>(d);
+    for (int h = cut - 1; h >= 0; h--) {
+      for (int g = d.vb; g < P.sub_groups; g += d.nblocks)
+        for (int k = P.sub_hoff[g * cut + h] + (int)threadIdx.x; k < P.sub_hoff[g * cut + h + 1];
+             k += blockDim.x)
+          tps_final_down(P, w, P.sub_order[k]);
       __syncthreads();
     }
   }
@@ -1438,6 +1463,15 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       std::vector<uint8_t> dyn(N, 0);
       for (int hh = 0; hh < H; hh++) dyn[p.hh_elem[hh]] = 1;
       e->plan_h = build_tree_plan(par, chunk, chunk_hi, &dyn);
+      // grid launches split the final forest over their workgroups; a batch slot
+      // may run as a virtual grid of up to one workgroup per kRowsPerWorkgroup rows
+      if (e->grid_mode)
+        partition_final_forest(e->plan_h, e->nblocks, e->threads);
+      else if (batch_threads > 0)
+        partition_final_forest(e->plan_h,
+                               std::min(kMaxSlotBlocks, (N + kRowsPerWorkgroup - 1) /
+                                                            kRowsPerWorkgroup),
+                               batch_threads);
       const TreePlanHost& P = e->plan_h;
       // one device buffer holding every plan array
       std::vector<int32_t> flat;

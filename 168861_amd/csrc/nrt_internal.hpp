@@ -86,6 +86,7 @@ constexpr uint32_t kCodeSkip = 0x20000000u;  // not computed (terminal/ghost)
 constexpr uint32_t kCodeIdx = 0x1FFFFFFFu;
 constexpr int kMaxChunk = 4096;  // longest chain chunk of the tree solve
 constexpr int kMaxSlotBlocks = 64;  // workgroups of one batch slot's virtual grid
+constexpr int kRowsPerWorkgroup = 24576;  // a batch slot gets a workgroup per this many rows
 
 // Host-side view of an nrt_problem with validation helpers.
 struct HostProblem {

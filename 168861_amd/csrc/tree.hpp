@@ -53,6 +53,13 @@ struct TreePlanHost {
   std::vector<int32_t> fin_hoff;    // height boundaries into fin_order, H+1 entries
   std::vector<int32_t> fin_cptr, fin_child;  // children of final-forest nodes (CSR)
   int heights = 0;                  // H
+  // Final forest split for grid launches (partition_final_forest): the nodes below
+  // height fin_cut form whole subtrees, dealt to sub_groups groups that workgroups
+  // eliminate side by side; only heights >= fin_cut are left to one workgroup.
+  int fin_cut = 0;                  // 0: no split
+  int sub_groups = 0;
+  std::vector<int32_t> sub_order;   // nodes below fin_cut, by group, then height
+  std::vector<int32_t> sub_hoff;    // group g, height h: [sub_hoff[g*cut+h], ..+1)
   int64_t total_nodes() const { return node_off.empty() ? 0 : node_off.back(); }
 };
 
@@ -66,6 +73,13 @@ struct TreePlanHost {
 TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chunk_hi,
                              const std::vector<uint8_t>* dyn_rows = nullptr);
 
+// Splits the final forest of a plan for a launch of `groups` workgroups of
+// `threads` threads (any number of workgroups can run it: workgroup b takes the
+// groups b, b + nblocks, ...). The cut height minimises a latency model: one unit
+// per pass of a workgroup over a height (ceil(nodes / threads)), up and down,
+// plus two grid barriers when split. No split when groups <= 1.
+void partition_final_forest(TreePlanHost& P, int groups, int threads);
+
 // Parent of each row in an assembled CSR (the single column < row, or -1).
 std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* col);
 
@@ -77,6 +91,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
 // Device view of a plan (pointers into one device allocation).
 struct TreePlanDev {
   int levels, heights;
+  int fin_cut, sub_groups;
   const int32_t* nslots;
   const int32_t* nchunk;
   const int32_t* node_off;
@@ -91,6 +106,8 @@ struct TreePlanDev {
   const int32_t* fin_hoff;
   const int32_t* fin_cptr;
   const int32_t* fin_child;
+  const int32_t* sub_order;
+  const int32_t* sub_hoff;
 };
 
 // Per-value work arrays of one solve, over all levels (total_nodes each): d, up

@@ -197,10 +197,10 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
 // Device view over one flat int32 buffer holding every plan array; `flat`
 // receives the concatenation, the returned view has offsets relative to base.
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
-  const std::vector<int32_t>* parts[] = {&P.nslots,    &P.nchunk,   &P.node_off, &P.next,
+  const std::vector<int32_t>* parts[] = {&P.nslots,    &P.nchunk,    &P.node_off,  &P.next,
                                          &P.chunk_off, &P.chunk_len, &P.chunk_dyn, &P.slot0,
-                                         &P.row0,      &P.fin_par,  &P.fin_order, &P.fin_hoff,
-                                         &P.fin_cptr,  &P.fin_child};
+                                         &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
+                                         &P.fin_cptr,  &P.fin_child, &P.sub_order, &P.sub_hoff};
   constexpr int kParts = sizeof(parts) / sizeof(parts[0]);
   size_t off[kParts];
   size_t pos = 0;
@@ -212,10 +212,101 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
     }
     pos += parts[k]->size() + 1;
   }
-  return TreePlanDev{P.levels,       P.heights,      base + off[0],  base + off[1],
-                     base + off[2],  base + off[3],  base + off[4],  base + off[5],
-                     base + off[6],  base + off[7],  base + off[8],  base + off[9],
-                     base + off[10], base + off[11], base + off[12], base + off[13]};
+  return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
+                     base + off[0],  base + off[1],  base + off[2],  base + off[3],
+                     base + off[4],  base + off[5],  base + off[6],  base + off[7],
+                     base + off[8],  base + off[9],  base + off[10], base + off[11],
+                     base + off[12], base + off[13], base + off[14], base + off[15]};
+}
+
+void partition_final_forest(TreePlanHost& P, int groups, int threads) {
+  P.fin_cut = 0;
+  P.sub_groups = 0;
+  P.sub_order.clear();
+  P.sub_hoff.clear();
+  const int H = P.heights, n = (int)P.fin_par.size();
+  if (groups <= 1 || H <= 1 || n == 0) return;
+  const int T = std::max(1, threads);
+  std::vector<int32_t> h(n);
+  for (int k = 0; k < H; k++)
+    for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) h[P.fin_order[q]] = k;
+  auto rounds = [&](int64_t nodes) { return (nodes + T - 1) / T; };
+  // unsplit: one workgroup takes every height, up and down
+  int64_t best = 0;
+  for (int k = 0; k < H; k++) best += 2 * rounds(P.fin_hoff[k + 1] - P.fin_hoff[k]);
+  const int64_t kBarrierUnits = 4;  // two grid barriers, in height passes
+  int best_cut = 0;
+  std::vector<int32_t> root(n), bin;
+  std::vector<int32_t> best_root, best_bin;
+  const int max_cut = std::min(H, 64);
+  for (int cut = 1; cut <= max_cut; cut++) {
+    // subtree root of every node below the cut: nodes by decreasing height, so a
+    // parent's root is known before its children's
+    std::vector<int32_t> roots;
+    for (int k = std::min(cut, H) - 1; k >= 0; k--)
+      for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
+        const int u = P.fin_order[q], p = P.fin_par[u];
+        if (p < 0 || h[p] >= cut) {
+          root[u] = u;
+          roots.push_back(u);
+        } else {
+          root[u] = root[p];
+        }
+      }
+    if ((int)roots.size() < 2) continue;
+    // subtrees to groups, largest first onto the lightest group
+    std::vector<int64_t> size(n, 0);
+    for (int k = 0; k < cut; k++)
+      for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) size[root[P.fin_order[q]]]++;
+    std::sort(roots.begin(), roots.end(), [&](int a, int b) {
+      return size[a] != size[b] ? size[a] > size[b] : a < b;
+    });
+    const int G = std::min<int>(groups, (int)roots.size());
+    std::vector<int64_t> load(G, 0);
+    std::vector<int32_t> gof(n, -1);
+    for (int u : roots) {
+      int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      gof[u] = g;
+      load[g] += size[u];
+    }
+    std::vector<int64_t> cnt((size_t)G * cut, 0);
+    for (int k = 0; k < cut; k++)
+      for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++)
+        cnt[(size_t)gof[root[P.fin_order[q]]] * cut + k]++;
+    int64_t cost = kBarrierUnits;
+    for (int k = 0; k < cut; k++) {
+      int64_t mx = 0;
+      for (int g = 0; g < G; g++) mx = std::max(mx, cnt[(size_t)g * cut + k]);
+      cost += 2 * rounds(mx);
+    }
+    for (int k = cut; k < H; k++) cost += 2 * rounds(P.fin_hoff[k + 1] - P.fin_hoff[k]);
+    if (cost < best) {
+      best = cost;
+      best_cut = cut;
+      best_root = root;
+      best_bin.assign(n, -1);
+      for (int u : roots) best_bin[u] = gof[u];
+      bin.assign(1, G);
+    }
+  }
+  if (best_cut == 0) return;
+  const int cut = best_cut, G = bin[0];
+  P.fin_cut = cut;
+  P.sub_groups = G;
+  P.sub_hoff.assign((size_t)G * cut + 1, 0);
+  for (int k = 0; k < cut; k++)
+    for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
+      const int u = P.fin_order[q];
+      P.sub_hoff[(size_t)best_bin[best_root[u]] * cut + k + 1]++;
+    }
+  for (size_t i = 0; i + 1 < P.sub_hoff.size(); i++) P.sub_hoff[i + 1] += P.sub_hoff[i];
+  P.sub_order.assign(P.fin_hoff[cut], 0);
+  std::vector<int32_t> fill(P.sub_hoff.begin(), P.sub_hoff.end() - 1);
+  for (int k = 0; k < cut; k++)
+    for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
+      const int u = P.fin_order[q];
+      P.sub_order[fill[(size_t)best_bin[best_root[u]] * cut + k]++] = u;
+    }
 }
 
 }  // namespace nrt
@@ -229,14 +320,39 @@ using namespace nrt;
 extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int32_t* col,
                                         const double* val, const double* rhs, int chunk,
                                         int chunk_hi, const uint8_t* dyn_rows,
-                                        const double* rhs_factor, double* x, int* levels,
-                                        int* heights) {
+                                        const double* rhs_factor, int groups, double* x,
+                                        int* levels, int* heights, int* cut) {
   try {
     std::vector<int32_t> par = csr_parents(N, rowptr, col);
     std::vector<uint8_t> dyn;
     if (dyn_rows) dyn.assign(dyn_rows, dyn_rows + N);
     TreePlanHost H = build_tree_plan(par, chunk, chunk_hi > 0 ? chunk_hi : chunk,
                                      dyn_rows ? &dyn : nullptr);
+    partition_final_forest(H, groups, 256);
+    {
+      // the split's promise to the device: a subtree node's children sit in its own
+      // group at lower heights, and every node is placed exactly once
+      const int n = (int)H.fin_par.size(), cut = H.fin_cut, G = H.sub_groups;
+      std::vector<int32_t> grp(n, -1), hgt(n, -1), seen(n, 0);
+      for (int g = 0; g < G; g++)
+        for (int h = 0; h < cut; h++)
+          for (int k = H.sub_hoff[g * cut + h]; k < H.sub_hoff[g * cut + h + 1]; k++) {
+            grp[H.sub_order[k]] = g;
+            hgt[H.sub_order[k]] = h;
+            seen[H.sub_order[k]]++;
+          }
+      for (int h = cut; h < H.heights; h++)
+        for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++) seen[H.fin_order[k]]++;
+      for (int u = 0; u < n; u++) {
+        if (seen[u] != 1) throw std::runtime_error("final forest split: node placed twice or never");
+        if (grp[u] < 0) continue;
+        for (int k = H.fin_cptr[u]; k < H.fin_cptr[u + 1]; k++) {
+          const int c = H.fin_child[k];
+          if (grp[c] != grp[u] || hgt[c] >= hgt[u])
+            throw std::runtime_error("final forest split: a child outside its parent's group");
+        }
+      }
+    }
     std::vector<int32_t> flat;
     TreePlanDev tmp = plan_view(H, nullptr, &flat);
     (void)tmp;
@@ -273,12 +389,23 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           if (reuse && !H.chunk_dyn[ch]) tps_reduce_chunk_static(P, w, lv, ch);
           else tps_reduce_chunk(P, w, lv, ch);
         }
-      for (int h = 0; h < H.heights; h++)
+      // final forest in the order of a grid launch: the groups' subtrees below the
+      // cut, then the heights above it, then back down
+      const int cut = H.fin_cut, G = H.sub_groups;
+      for (int h = 0; h < cut; h++)
+        for (int g = 0; g < G; g++)
+          for (int k = H.sub_hoff[g * cut + h]; k < H.sub_hoff[g * cut + h + 1]; k++)
+            tps_final_up(P, w, H.sub_order[k]);
+      for (int h = cut; h < H.heights; h++)
         for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
           tps_final_up(P, w, H.fin_order[k]);
-      for (int h = H.heights - 1; h >= 0; h--)
+      for (int h = H.heights - 1; h >= cut; h--)
         for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
           tps_final_down(P, w, H.fin_order[k]);
+      for (int h = cut - 1; h >= 0; h--)
+        for (int g = 0; g < G; g++)
+          for (int k = H.sub_hoff[g * cut + h]; k < H.sub_hoff[g * cut + h + 1]; k++)
+            tps_final_down(P, w, H.sub_order[k]);
       for (int lv = H.levels - 1; lv >= 1; lv--)
         for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
           tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data());
@@ -292,6 +419,7 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
     for (int i = 0; i < N; i++) x[i] = xs[s0[i]];
     if (levels) *levels = H.levels;
     if (heights) *heights = H.heights;
+    if (cut) *cut = H.fin_cut;
     return NRT_OK;
   } catch (const std::exception&) {
     return NRT_ERR_TOPOLOGY;

@@ -247,7 +247,7 @@ def dendritic_tree(nh, wl, tree_steps, krylov_steps, max_iter):
         "mixed_gmres30_ir": (krylov_steps, dict(gm, precision=nh.FP32_REFINE, refine_tol=1e-12)),
     }
     refs = {}
-    for st in {tree_steps, krylov_steps}:
+    for st in {tree_steps, krylov_steps} - {0}:
         refs[st] = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP64), 1, st, a0,
                             a1)["v"]
     mask = np.ones(N, bool)
@@ -261,6 +261,8 @@ def dendritic_tree(nh, wl, tree_steps, krylov_steps, max_iter):
                        "per segment, HH soma, dt 25 us, stimulus 1 nA x 0.5 ms at the soma",
            "N": N, "hh_elements": H, "setup_s": setup_s}
     for name, (steps, opts) in specs.items():
+        if steps <= 0:
+            continue
         try:
             f = run_form(nh, prob, opts, 1, steps, a0, a1)
         except nh.NrtError as e:

@@ -15,24 +15,27 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
-def tps_host(rp, col, val, rhs, chunk, chunk_hi=0, dyn=None, rhs_factor=None):
+def tps_host(rp, col, val, rhs, chunk, chunk_hi=0, dyn=None, rhs_factor=None, groups=0,
+             cut_out=None):
     L = nh.lib()
     f = L.nrt_test_tree_solve_host
-    f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _dp,
-                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    f.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                  C.c_int, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     rp = np.ascontiguousarray(rp, np.int32)
     col = np.ascontiguousarray(col, np.int32)
     val = np.ascontiguousarray(val, np.float64)
     rhs = np.ascontiguousarray(rhs, np.float64)
     x = np.empty(len(rhs))
-    lv, h = C.c_int(0), C.c_int(0)
+    lv, h, cut = C.c_int(0), C.c_int(0), C.c_int(0)
     dyn = None if dyn is None else np.ascontiguousarray(dyn, np.uint8)
     rf = None if rhs_factor is None else np.ascontiguousarray(rhs_factor, np.float64)
     st = f(len(rhs), rp.ctypes.data_as(_ip), col.ctypes.data_as(_ip), val.ctypes.data_as(_dp),
            rhs.ctypes.data_as(_dp), chunk, chunk_hi,
            None if dyn is None else dyn.ctypes.data, None if rf is None else rf.ctypes.data,
-           x.ctypes.data_as(_dp), C.byref(lv), C.byref(h))
+           groups, x.ctypes.data_as(_dp), C.byref(lv), C.byref(h), C.byref(cut))
     assert st == 0
+    if cut_out is not None:
+        cut_out.append(cut.value)
     return x, lv.value, h.value
 
 
@@ -114,3 +117,50 @@ def test_tps_reused_factors_identical(name, chunk, chunk_hi, orc):
         assert np.array_equal(full, reuse), frac
         ref = orc.tree_solve(rp, col, val, rhs)
         assert np.max(np.abs(reuse - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("groups", [2, 7, 64, 256])
+def test_final_forest_split(groups, orc):
+    """Grid launches split the final forest into subtrees per workgroup group; the
+    hook checks the split (children stay in their parent's group, every node placed
+    once) and solves in the launch's order: the solution is bit-identical to the
+    unsplit one."""
+    for geom, dT in [(wl.random_tree(wl.seed_for(3), depth=10)[0], 2.5e-5),
+                     (wl.myelinated_axon()[0], 1e-5)]:
+        s, bufs, H = _setup(orc, geom)
+        p = _problem(geom, s, bufs, H, dT)
+        rp, col, val, rhs = _system(orc, p)
+        x0, _, h = tps_host(rp, col, val, rhs, 8, 8)
+        cuts = []
+        x1, _, _ = tps_host(rp, col, val, rhs, 8, 8, groups=groups, cut_out=cuts)
+        assert np.array_equal(x0, x1)
+        if geom.N > 100000:  # the bushy tree splits; a cable's final forest is small
+            assert 0 < cuts[0] <= h, (cuts, h)
+        ref = orc.tree_solve(rp, col, val, rhs)
+        assert np.max(np.abs(x1 - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+def test_final_forest_split_random_forests():
+    """Random forests of every shape, split for several group counts."""
+    rng = np.random.default_rng(11)
+    for trial in range(30):
+        N = int(rng.integers(2, 3000))
+        par = [-1] + [int(rng.integers(-1, i)) if rng.random() < 0.4 else i - 1
+                      for i in range(1, N)]
+        rp, col, val = [0], [], []
+        for i in range(N):
+            ent = {i: 10.0 + rng.random()}
+            if par[i] >= 0:
+                ent[par[i]] = -rng.random()
+            for j in range(i + 1, N):
+                if par[j] == i:
+                    ent[j] = -rng.random()
+            for j in sorted(ent):
+                col.append(j)
+                val.append(ent[j])
+            rp.append(len(col))
+        b = rng.standard_normal(N)
+        x0, _, _ = tps_host(rp, col, val, b, 4)
+        for groups in (2, 3, 16):
+            x1, _, _ = tps_host(rp, col, val, b, 4, groups=groups)
+            assert np.array_equal(x0, x1), (trial, groups)
