@@ -983,6 +983,7 @@ struct TreeArgs {
   int direct;
   double refine_tol;
   int refine_max;
+  int top_lds;  // the launch provides top_lds_bytes<T>() of dynamic LDS (grid launches)
 };
 
 template <bool GRID>
@@ -1010,7 +1011,7 @@ __device__ void tps_factor(const ImpDev& d, const TreePlanDev& P, const TreeWork
 // refinement sweep of the same step), every chunk reuses its factors.
 template <class T, bool GRID>
 __device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w, bool acc,
-                        bool all_static) {
+                        bool all_static, unsigned char* tl) {
   const int wid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
   const int nw = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
   for (int lv = 0; lv < P.levels; lv++) {
@@ -1036,7 +1037,24 @@ __device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>
     }
     tsync<GRID>(d);
   }
-  if (!GRID || d.vb == 0) {
+  if (GRID && tl != nullptr && P.top_n > 0) {
+    if (d.vb == 0) {
+      const TopLds<T> L = TopLds<T>::carve(tl);
+      const int ns = top_stage_count(P), nh = P.heights - cut;
+      for (int j = threadIdx.x; j < ns; j += blockDim.x) top_stage(P, w, L, j);
+      __syncthreads();
+      for (int h = 0; h < nh; h++) {
+        for (int j = L.hoff[h] + (int)threadIdx.x; j < L.hoff[h + 1]; j += blockDim.x) top_up(L, j);
+        __syncthreads();
+      }
+      for (int h = nh - 1; h >= 0; h--) {
+        for (int j = L.hoff[h] + (int)threadIdx.x; j < L.hoff[h + 1]; j += blockDim.x)
+          top_down(L, j);
+        __syncthreads();
+      }
+      for (int j = threadIdx.x; j < P.top_n; j += blockDim.x) top_store(P, w, L, j);
+    }
+  } else if (!GRID || d.vb == 0) {
     for (int h = cut; h < P.heights; h++) {
       for (int k = P.fin_hoff[h] + (int)threadIdx.x; k < P.fin_hoff[h + 1]; k += blockDim.x)
         tps_final_up(P, w, P.fin_order[k]);
@@ -1145,7 +1163,8 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
 // Steps d.first .. d.first+d.num-1 of one tree (GRID: the whole launch works on
 // it; else one workgroup does).
 template <class T, bool GRID>
-__device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t, double* lds) {
+__device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t, double* lds,
+                                           unsigned char* tl) {
   // slot ranges (the tree solve's vectors are in level-0 slot order)
   const int r_begin = GRID ? d.vb * d.rpb : 0;
   const int r_end = GRID ? min(d.ns0, r_begin + d.rpb) : d.ns0;
@@ -1178,7 +1197,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     double rr = 0, bb = 0;
     bool ok = true;
     if (t.direct) {
-      tps_run<T, GRID>(d, t.P, t.w, false, false);
+      tps_run<T, GRID>(d, t.P, t.w, false, false, tl);
       sweeps = 1;
     } else {
       while (true) {
@@ -1211,7 +1230,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
           status = NRT_ERR_NOT_CONVERGED;
           break;
         }
-        tps_run<T, GRID>(d, t.P, t.w, true, sweeps > 0);
+        tps_run<T, GRID>(d, t.P, t.w, true, sweeps > 0, tl);
         sweeps++;
       }
     }
@@ -1246,8 +1265,9 @@ template <class T, bool GRID>
 __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree(ImpDev d,
                                                                                  TreeArgs<T> t) {
   __shared__ double lds[4 * 16];
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
   d.vb = GRID ? (int)blockIdx.x : 0;
-  tree_steps<T, GRID>(d, t, lds);
+  tree_steps<T, GRID>(d, t, lds, GRID && t.top_lds ? dyn_lds : nullptr);
 }
 
 // Batched form: independent trees ("slots"), one workgroup each, no grid-wide
@@ -1289,9 +1309,9 @@ __global__ __launch_bounds__(BS) void k_batch_tree(const BatchSlot<T>* __restric
   d.nblocks = wk.vn;
   if (wk.vn > 1) {
     d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
-    tree_steps<T, true>(d, S.t, lds);
+    tree_steps<T, true>(d, S.t, lds, nullptr);
   } else {
-    tree_steps<T, false>(d, S.t, lds);
+    tree_steps<T, false>(d, S.t, lds, nullptr);
   }
 }
 
@@ -1454,9 +1474,11 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       }
       // Level 0: one chunk per worker covers an unbranched cable in one round;
       // reduced levels: short chunks (the serial sweep is the latency, and each
-      // level shrinks a chain by C/2).
+      // level shrinks a chain by C/2). Grid launches take 128-row chunks: every
+      // level costs grid barriers, and with chunks as long as a dendrite's
+      // segments one level is left (depth-14 tree: 0.47 -> 0.40 ms per step).
       const int workers = e->nblocks * e->threads;
-      const int auto0 = std::min(64, std::max(8, (N + workers - 1) / workers));
+      const int auto0 = e->grid_mode ? 128 : std::min(64, std::max(8, (N + workers - 1) / workers));
       const int chunk = o.chunk > 0 ? o.chunk : auto0;
       const int chunk_hi = o.chunk_hi > 0 ? o.chunk_hi : (o.chunk > 0 ? o.chunk : 8);
       // HH rows change their diagonal every step; every other chunk keeps its factors
@@ -1466,7 +1488,7 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // grid launches split the final forest over their workgroups; a batch slot
       // may run as a virtual grid of up to one workgroup per kRowsPerWorkgroup rows
       if (e->grid_mode)
-        partition_final_forest(e->plan_h, e->nblocks, e->threads);
+        partition_final_forest(e->plan_h, e->nblocks, e->threads, true);
       else if (batch_threads > 0)
         partition_final_forest(e->plan_h,
                                std::min(kMaxSlotBlocks, (N + kRowsPerWorkgroup - 1) /
@@ -1663,7 +1685,8 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b) {
   return TreeArgs<T>{e->plan,
                      TreeWork<T>{b, b + T_, b + 2 * T_, b + 3 * T_, b + 4 * T_, b + 5 * T_,
                                  b + 6 * T_, b + 7 * T_, b + 8 * T_, b + 9 * T_},
-                     e->method == 1 ? 1 : 0, o.refine_tol, o.refine_max};
+                     e->method == 1 ? 1 : 0, o.refine_tol, o.refine_max,
+                     e->grid_mode && e->plan_h.top_n > 0 ? 1 : 0};
 }
 
 static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
@@ -1706,8 +1729,10 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   }
   auto launch = [&](auto targs, const void* kgrid, const void* kblock) {
     void* args[] = {&d, &targs};
+    using TT = typename std::remove_pointer<decltype(targs.w.d)>::type;
+    const size_t shmem = targs.top_lds ? top_lds_bytes<TT>() : 0;
     if (e->grid_mode)
-      NRT_HIP(hipLaunchCooperativeKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, 0,
+      NRT_HIP(hipLaunchCooperativeKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, shmem,
                                          hnd->stream));
     else
       NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));

@@ -60,8 +60,19 @@ struct TreePlanHost {
   int sub_groups = 0;
   std::vector<int32_t> sub_order;   // nodes below fin_cut, by group, then height
   std::vector<int32_t> sub_hoff;    // group g, height h: [sub_hoff[g*cut+h], ..+1)
+  // The heights >= fin_cut staged in LDS by the one workgroup that solves them
+  // (top_n == 0: not staged). Local entries: the top nodes in height order, then
+  // their children below the cut (subtree roots, inputs only).
+  int top_n = 0, top_ext = 0;
+  std::vector<int32_t> top_node;    // final-local node of each local entry
+  std::vector<int32_t> top_hoff;    // height boundaries (heights fin_cut..H-1), local
+  std::vector<int32_t> top_cptr, top_child;  // children of top nodes, local indices
+  std::vector<int32_t> top_par;     // local parent of each top node, or -1
   int64_t total_nodes() const { return node_off.empty() ? 0 : node_off.back(); }
 };
+
+// LDS capacity for the staged top of the final forest (local entries).
+constexpr int kTopLdsNodes = 1024;
 
 // Builds the plan for a forest given by parent indices (par[i] < i or -1), with
 // chains cut into chunks of `chunk` rows at level 0 and `chunk_hi` rows above.
@@ -77,8 +88,10 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chu
 // `threads` threads (any number of workgroups can run it: workgroup b takes the
 // groups b, b + nblocks, ...). The cut height minimises a latency model: one unit
 // per pass of a workgroup over a height (ceil(nodes / threads)), up and down,
-// plus two grid barriers when split. No split when groups <= 1.
-void partition_final_forest(TreePlanHost& P, int groups, int threads);
+// plus two grid barriers when split; with `stage_top`, heights above a cut whose
+// top fits in kTopLdsNodes LDS entries cost a tenth (the lists for that are
+// built). No split when groups <= 1.
+void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage_top = false);
 
 // Parent of each row in an assembled CSR (the single column < row, or -1).
 std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* col);
@@ -92,6 +105,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
 struct TreePlanDev {
   int levels, heights;
   int fin_cut, sub_groups;
+  int top_n, top_ext;
   const int32_t* nslots;
   const int32_t* nchunk;
   const int32_t* node_off;
@@ -108,6 +122,11 @@ struct TreePlanDev {
   const int32_t* fin_child;
   const int32_t* sub_order;
   const int32_t* sub_hoff;
+  const int32_t* top_node;
+  const int32_t* top_hoff;
+  const int32_t* top_cptr;
+  const int32_t* top_child;
+  const int32_t* top_par;
 };
 
 // Per-value work arrays of one solve, over all levels (total_nodes each): d, up
@@ -373,6 +392,86 @@ NRT_HD void tps_final_down(const TreePlanDev& P, const TreeWork<T>& w, int v) {
   T acc = w.r[base + v];
   if (p >= 0) acc = acc - w.up[base + v] * w.x[base + p];
   w.x[base + v] = acc / w.ca[base + v];
+}
+
+// The top of the final forest staged in LDS (TreePlanHost::top_*): one workgroup
+// loads the top nodes and their children below the cut once, eliminates and
+// back-substitutes every top height there, and writes back only x. The arithmetic
+// is tps_final_up/down's, operand for operand.
+template <class T>
+struct TopLds {
+  T* d;  // d, then the pivot after elimination (children below the cut: their pivot)
+  T* r;
+  T* up;
+  T* dn;
+  T* x;
+  int32_t* cptr;
+  int32_t* child;
+  int32_t* par;
+  int32_t* hoff;
+  // carve from one buffer of top_lds_bytes<T>()
+  NRT_HD static TopLds carve(void* p) {
+    T* t = (T*)p;
+    int32_t* i = (int32_t*)(t + 5 * kTopLdsNodes);
+    return TopLds{t,     t + kTopLdsNodes,     t + 2 * kTopLdsNodes, t + 3 * kTopLdsNodes,
+                  t + 4 * kTopLdsNodes, i, i + kTopLdsNodes + 1, i + 2 * kTopLdsNodes + 1,
+                  i + 3 * kTopLdsNodes + 1};
+  }
+};
+template <class T>
+constexpr size_t top_lds_bytes() {
+  return 5 * kTopLdsNodes * sizeof(T) + (4 * kTopLdsNodes + 2) * sizeof(int32_t);
+}
+
+// Index j of every staged list: the values of entry j (j < top_n + top_ext) and
+// the plan lists' j-th items; run for j < top_stage_count(P).
+NRT_HD int top_stage_count(const TreePlanDev& P) {
+  const int ne = P.top_n + P.top_ext, nh = P.heights - P.fin_cut;
+  return ne > nh + 1 ? ne : nh + 1;
+}
+template <class T>
+NRT_HD void top_stage(const TreePlanDev& P, const TreeWork<T>& w, const TopLds<T>& L, int j) {
+  if (j < P.top_n + P.top_ext) {
+    const int g = P.node_off[P.levels] + P.top_node[j];
+    L.d[j] = j < P.top_n ? w.d[g] : w.ca[g];
+    L.r[j] = w.r[g];
+    L.up[j] = w.up[g];
+    L.dn[j] = w.dn[g];
+  }
+  if (j < P.top_n) {
+    L.par[j] = P.top_par[j];
+    L.cptr[j + 1] = P.top_cptr[j + 1];
+  }
+  if (j == 0) L.cptr[0] = P.top_cptr[0];
+  if (j < P.top_cptr[P.top_n]) L.child[j] = P.top_child[j];
+  if (j <= P.heights - P.fin_cut) L.hoff[j] = P.top_hoff[j];
+}
+
+template <class T>
+NRT_HD void top_up(const TopLds<T>& L, int j) {
+  T dv = L.d[j], rv = L.r[j];
+  for (int k = L.cptr[j]; k < L.cptr[j + 1]; k++) {
+    const int c = L.child[k];
+    const T f = L.dn[c] / L.d[c];
+    dv = dv - f * L.up[c];
+    rv = rv - f * L.r[c];
+  }
+  L.d[j] = dv;
+  L.r[j] = rv;
+}
+
+template <class T>
+NRT_HD void top_down(const TopLds<T>& L, int j) {
+  const int p = L.par[j];
+  T acc = L.r[j];
+  if (p >= 0) acc = acc - L.up[j] * L.x[p];
+  L.x[j] = acc / L.d[j];
+}
+
+// After the down sweep: x of top node j back to the final level.
+template <class T>
+NRT_HD void top_store(const TreePlanDev& P, const TreeWork<T>& w, const TopLds<T>& L, int j) {
+  w.x[P.node_off[P.levels] + P.top_node[j]] = L.x[j];
 }
 
 // Stores of the back-substitution: into the level's x (OUT_LEVEL), or at level 0

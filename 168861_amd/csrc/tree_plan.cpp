@@ -200,7 +200,9 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
   const std::vector<int32_t>* parts[] = {&P.nslots,    &P.nchunk,    &P.node_off,  &P.next,
                                          &P.chunk_off, &P.chunk_len, &P.chunk_dyn, &P.slot0,
                                          &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
-                                         &P.fin_cptr,  &P.fin_child, &P.sub_order, &P.sub_hoff};
+                                         &P.fin_cptr,  &P.fin_child, &P.sub_order, &P.sub_hoff,
+                                         &P.top_node,  &P.top_hoff,  &P.top_cptr,  &P.top_child,
+                                         &P.top_par};
   constexpr int kParts = sizeof(parts) / sizeof(parts[0]);
   size_t off[kParts];
   size_t pos = 0;
@@ -213,42 +215,100 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
     pos += parts[k]->size() + 1;
   }
   return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
-                     base + off[0],  base + off[1],  base + off[2],  base + off[3],
-                     base + off[4],  base + off[5],  base + off[6],  base + off[7],
-                     base + off[8],  base + off[9],  base + off[10], base + off[11],
-                     base + off[12], base + off[13], base + off[14], base + off[15]};
+                     P.top_n,        P.top_ext,      base + off[0],  base + off[1],
+                     base + off[2],  base + off[3],  base + off[4],  base + off[5],
+                     base + off[6],  base + off[7],  base + off[8],  base + off[9],
+                     base + off[10], base + off[11], base + off[12], base + off[13],
+                     base + off[14], base + off[15], base + off[16], base + off[17],
+                     base + off[18], base + off[19], base + off[20]};
 }
 
-void partition_final_forest(TreePlanHost& P, int groups, int threads) {
+// Local lists of the heights >= cut (see TreePlanHost::top_*); false when they
+// do not fit in kTopLdsNodes entries.
+static bool stage_final_top(TreePlanHost& P, int cut) {
+  const int H = P.heights, n = (int)P.fin_par.size();
+  const int t0 = P.fin_hoff[cut], nt = P.fin_hoff[H] - t0;
+  std::vector<int32_t> loc(n, -1);
+  std::vector<int32_t> node;
+  for (int j = 0; j < nt; j++) {
+    loc[P.fin_order[t0 + j]] = j;
+    node.push_back(P.fin_order[t0 + j]);
+  }
+  for (int j = 0; j < nt; j++) {
+    const int v = node[j];
+    for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) {
+      const int c = P.fin_child[k];
+      if (loc[c] < 0) {
+        loc[c] = (int)node.size();
+        node.push_back(c);
+      }
+    }
+  }
+  if (nt == 0 || (int)node.size() > kTopLdsNodes) return false;
+  P.top_n = nt;
+  P.top_ext = (int)node.size() - nt;
+  P.top_node = node;
+  P.top_hoff.clear();
+  for (int h = cut; h <= H; h++) P.top_hoff.push_back(P.fin_hoff[h] - t0);
+  P.top_cptr.assign(1, 0);
+  P.top_child.clear();
+  P.top_par.assign(nt, -1);
+  for (int j = 0; j < nt; j++) {
+    const int v = node[j];
+    for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) P.top_child.push_back(loc[P.fin_child[k]]);
+    P.top_cptr.push_back((int32_t)P.top_child.size());
+    if (P.fin_par[v] >= 0) P.top_par[j] = loc[P.fin_par[v]];
+  }
+  return true;
+}
+
+void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage_top) {
   P.fin_cut = 0;
   P.sub_groups = 0;
   P.sub_order.clear();
   P.sub_hoff.clear();
+  P.top_n = P.top_ext = 0;
+  P.top_node.clear();
+  P.top_hoff.clear();
+  P.top_cptr.clear();
+  P.top_child.clear();
+  P.top_par.clear();
   const int H = P.heights, n = (int)P.fin_par.size();
   if (groups <= 1 || H <= 1 || n == 0) return;
   const int T = std::max(1, threads);
   std::vector<int32_t> h(n);
   for (int k = 0; k < H; k++)
     for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) h[P.fin_order[q]] = k;
+  // Cost units: one pass of a workgroup over a height through global memory; a
+  // pass over LDS-staged heights costs kLdsPass of that, plus kStage to stage.
+  constexpr int64_t kScale = 10, kLdsPass = 1, kStage = 2 * kScale, kBarriers = 4 * kScale;
   auto rounds = [&](int64_t nodes) { return (nodes + T - 1) / T; };
-  // unsplit: one workgroup takes every height, up and down
-  int64_t best = 0;
-  for (int k = 0; k < H; k++) best += 2 * rounds(P.fin_hoff[k + 1] - P.fin_hoff[k]);
-  const int64_t kBarrierUnits = 4;  // two grid barriers, in height passes
+  auto top_cost = [&](int cut, int entries) {
+    int64_t c = 0;
+    const bool lds = stage_top && entries <= kTopLdsNodes;
+    for (int k = cut; k < H; k++) {
+      const int64_t r = rounds(P.fin_hoff[k + 1] - P.fin_hoff[k]);
+      c += 2 * r * (lds ? kLdsPass : kScale);
+    }
+    return c + (lds ? kStage : 0);
+  };
+  int64_t best = top_cost(0, n);  // unsplit
   int best_cut = 0;
-  std::vector<int32_t> root(n), bin;
-  std::vector<int32_t> best_root, best_bin;
+  std::vector<int32_t> root(n), best_root, best_bin;
+  int best_G = 0;
   const int max_cut = std::min(H, 64);
   for (int cut = 1; cut <= max_cut; cut++) {
     // subtree root of every node below the cut: nodes by decreasing height, so a
     // parent's root is known before its children's
     std::vector<int32_t> roots;
-    for (int k = std::min(cut, H) - 1; k >= 0; k--)
+    int ext = 0;
+    for (int k = cut - 1; k >= 0; k--)
       for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
         const int u = P.fin_order[q], p = P.fin_par[u];
         if (p < 0 || h[p] >= cut) {
           root[u] = u;
           roots.push_back(u);
+          if (p >= 0) ext++;
         } else {
           root[u] = root[p];
         }
@@ -273,40 +333,42 @@ void partition_final_forest(TreePlanHost& P, int groups, int threads) {
     for (int k = 0; k < cut; k++)
       for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++)
         cnt[(size_t)gof[root[P.fin_order[q]]] * cut + k]++;
-    int64_t cost = kBarrierUnits;
+    int64_t cost = kBarriers;
     for (int k = 0; k < cut; k++) {
       int64_t mx = 0;
       for (int g = 0; g < G; g++) mx = std::max(mx, cnt[(size_t)g * cut + k]);
-      cost += 2 * rounds(mx);
+      cost += 2 * rounds(mx) * kScale;
     }
-    for (int k = cut; k < H; k++) cost += 2 * rounds(P.fin_hoff[k + 1] - P.fin_hoff[k]);
+    cost += top_cost(cut, P.fin_hoff[H] - P.fin_hoff[cut] + ext);
     if (cost < best) {
       best = cost;
       best_cut = cut;
       best_root = root;
       best_bin.assign(n, -1);
       for (int u : roots) best_bin[u] = gof[u];
-      bin.assign(1, G);
+      best_G = G;
     }
   }
-  if (best_cut == 0) return;
-  const int cut = best_cut, G = bin[0];
-  P.fin_cut = cut;
-  P.sub_groups = G;
-  P.sub_hoff.assign((size_t)G * cut + 1, 0);
-  for (int k = 0; k < cut; k++)
-    for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
-      const int u = P.fin_order[q];
-      P.sub_hoff[(size_t)best_bin[best_root[u]] * cut + k + 1]++;
-    }
-  for (size_t i = 0; i + 1 < P.sub_hoff.size(); i++) P.sub_hoff[i + 1] += P.sub_hoff[i];
-  P.sub_order.assign(P.fin_hoff[cut], 0);
-  std::vector<int32_t> fill(P.sub_hoff.begin(), P.sub_hoff.end() - 1);
-  for (int k = 0; k < cut; k++)
-    for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
-      const int u = P.fin_order[q];
-      P.sub_order[fill[(size_t)best_bin[best_root[u]] * cut + k]++] = u;
-    }
+  if (best_cut > 0) {
+    const int cut = best_cut, G = best_G;
+    P.fin_cut = cut;
+    P.sub_groups = G;
+    P.sub_hoff.assign((size_t)G * cut + 1, 0);
+    for (int k = 0; k < cut; k++)
+      for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
+        const int u = P.fin_order[q];
+        P.sub_hoff[(size_t)best_bin[best_root[u]] * cut + k + 1]++;
+      }
+    for (size_t i = 0; i + 1 < P.sub_hoff.size(); i++) P.sub_hoff[i + 1] += P.sub_hoff[i];
+    P.sub_order.assign(P.fin_hoff[cut], 0);
+    std::vector<int32_t> fill(P.sub_hoff.begin(), P.sub_hoff.end() - 1);
+    for (int k = 0; k < cut; k++)
+      for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
+        const int u = P.fin_order[q];
+        P.sub_order[fill[(size_t)best_bin[best_root[u]] * cut + k]++] = u;
+      }
+  }
+  if (stage_top) stage_final_top(P, P.fin_cut);
 }
 
 }  // namespace nrt
@@ -328,7 +390,7 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
     if (dyn_rows) dyn.assign(dyn_rows, dyn_rows + N);
     TreePlanHost H = build_tree_plan(par, chunk, chunk_hi > 0 ? chunk_hi : chunk,
                                      dyn_rows ? &dyn : nullptr);
-    partition_final_forest(H, groups, 256);
+    partition_final_forest(H, groups, 256, groups > 0);
     {
       // the split's promise to the device: a subtree node's children sit in its own
       // group at lower heights, and every node is placed exactly once
@@ -396,12 +458,25 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
         for (int g = 0; g < G; g++)
           for (int k = H.sub_hoff[g * cut + h]; k < H.sub_hoff[g * cut + h + 1]; k++)
             tps_final_up(P, w, H.sub_order[k]);
-      for (int h = cut; h < H.heights; h++)
-        for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
-          tps_final_up(P, w, H.fin_order[k]);
-      for (int h = H.heights - 1; h >= cut; h--)
-        for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
-          tps_final_down(P, w, H.fin_order[k]);
+      if (H.top_n > 0) {
+        // the LDS-staged top, with host memory standing in for LDS
+        std::vector<unsigned char> lds(top_lds_bytes<double>());
+        const TopLds<double> L = TopLds<double>::carve(lds.data());
+        for (int j = 0; j < top_stage_count(P); j++) top_stage(P, w, L, j);
+        const int nh = H.heights - cut;
+        for (int h = 0; h < nh; h++)
+          for (int j = L.hoff[h]; j < L.hoff[h + 1]; j++) top_up(L, j);
+        for (int h = nh - 1; h >= 0; h--)
+          for (int j = L.hoff[h]; j < L.hoff[h + 1]; j++) top_down(L, j);
+        for (int j = 0; j < H.top_n; j++) top_store(P, w, L, j);
+      } else {
+        for (int h = cut; h < H.heights; h++)
+          for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
+            tps_final_up(P, w, H.fin_order[k]);
+        for (int h = H.heights - 1; h >= cut; h--)
+          for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
+            tps_final_down(P, w, H.fin_order[k]);
+      }
       for (int h = cut - 1; h >= 0; h--)
         for (int g = 0; g < G; g++)
           for (int k = H.sub_hoff[g * cut + h]; k < H.sub_hoff[g * cut + h + 1]; k++)
@@ -419,7 +494,7 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
     for (int i = 0; i < N; i++) x[i] = xs[s0[i]];
     if (levels) *levels = H.levels;
     if (heights) *heights = H.heights;
-    if (cut) *cut = H.fin_cut;
+    if (cut) *cut = H.fin_cut | (H.top_n > 0 ? (1 << 30) : 0);
     return NRT_OK;
   } catch (const std::exception&) {
     return NRT_ERR_TOPOLOGY;

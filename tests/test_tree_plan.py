@@ -125,8 +125,8 @@ def test_final_forest_split(groups, orc):
     hook checks the split (children stay in their parent's group, every node placed
     once) and solves in the launch's order: the solution is bit-identical to the
     unsplit one."""
-    for geom, dT in [(wl.random_tree(wl.seed_for(3), depth=10)[0], 2.5e-5),
-                     (wl.myelinated_axon()[0], 1e-5)]:
+    for bushy, geom, dT in [(True, wl.random_tree(wl.seed_for(3), depth=10)[0], 2.5e-5),
+                            (False, wl.myelinated_axon()[0], 1e-5)]:
         s, bufs, H = _setup(orc, geom)
         p = _problem(geom, s, bufs, H, dT)
         rp, col, val, rhs = _system(orc, p)
@@ -134,8 +134,9 @@ def test_final_forest_split(groups, orc):
         cuts = []
         x1, _, _ = tps_host(rp, col, val, rhs, 8, 8, groups=groups, cut_out=cuts)
         assert np.array_equal(x0, x1)
-        if geom.N > 100000:  # the bushy tree splits; a cable's final forest is small
-            assert 0 < cuts[0] <= h, (cuts, h)
+        cut, staged = cuts[0] & 0xFFFF, cuts[0] >> 30
+        if bushy:  # splits, and its top (LDS-staged) is small
+            assert 0 < cut <= h and staged, (cuts, h)
         ref = orc.tree_solve(rp, col, val, rhs)
         assert np.max(np.abs(x1 - ref)) <= 1e-11 * np.max(np.abs(ref))
 
