@@ -12,7 +12,8 @@ on the device.
 Forms measured on the same workload:
   fp64_tree_direct     exact tree partition solve in FP64
   mixed_tree_ir        FP32 tree solves inside FP64 iterative refinement
-                       (residual b - A x and x += d in FP64, relative 1e-10)
+                       (residual b - A x and x += d in FP64, Jacobi-scaled
+                       relative residual 1e-10)
   fp64_plain_bicgstab  the reference method: unpreconditioned BiCGSTAB,
                        relative residual 1e-10 (few steps: it is slow)
 `value` is the fastest form whose max |dV| against the FP64 exact solve after
@@ -409,7 +410,7 @@ def main():
                                "internodes (100 x 10 um), d=10 um, N=20202, backward Euler "
                                "dt=10 us, stimulus 10 nA x 0.5 ms",
                    "N": N, "hh_elements": H, "nnz": forms[best]["nnz"], "form": best,
-                   "tolerance": "relative 1e-10 (refinement and BiCGSTAB)"},
+                   "tolerance": "relative 1e-10 (refinement: Jacobi-scaled; BiCGSTAB: plain)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "launches": 1, "kernel_ms": b["kernel_ms"],
