@@ -65,6 +65,19 @@ class SetupOutStruct(C.Structure):
                 ("dT_crit", C.c_double), ("num_hh", C.c_int32)]
 
 
+class ScenarioInfo(C.Structure):
+    _fields_ = [("geometry", GeometryStruct), ("explicit_mode", C.c_int32),
+                ("implicit_mode", C.c_int32), ("viewer", C.c_int32), ("mes_point", C.c_int32),
+                ("tspan", C.c_double), ("xspan", C.c_double), ("dt_factor", C.c_double),
+                ("nu", C.c_double), ("epsilon", C.c_double), ("E_MAX", C.c_double),
+                ("init_time", C.c_double), ("end_input", C.c_double),
+                ("input_current", C.c_double), ("dist_mes", C.c_double),
+                ("epsilon_macro", C.c_double), ("out_path", C.c_char_p),
+                ("log_path", C.c_char_p), ("builder_log", C.c_char_p),
+                ("max_steps", C.c_int64), ("method", C.c_int32), ("precision", C.c_int32),
+                ("restart", C.c_int32), ("tol", C.c_double), ("vout_path", C.c_char_p)]
+
+
 class Options(C.Structure):
     _fields_ = [("kind", C.c_int32), ("precision", C.c_int32), ("krylov", C.c_int32),
                 ("restart", C.c_int32), ("tol_mode", C.c_int32), ("x0_mode", C.c_int32),
@@ -176,6 +189,11 @@ def lib():
         L.nrt_server_last_error.argtypes = [C.c_void_p]
         L.nrt_server_last_error.restype = C.c_char_p
         L.nrt_server_destroy.argtypes = [C.c_void_p]
+        L.nrt_scenario_create.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_char_p, C.c_char_p,
+                                          C.POINTER(C.c_void_p)]
+        L.nrt_scenario_get.argtypes = [C.c_void_p, C.POINTER(ScenarioInfo)]
+        L.nrt_scenario_last_error.restype = C.c_char_p
+        L.nrt_scenario_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -572,3 +590,41 @@ def active_range(dT, num_steps_T, init_time, end_input):
         return -1, -2
     idx = i[on]
     return int(idx[0]), int(idx[-1])
+
+
+def scenario(args, simulation=None, working_directory=None):
+    """The reference's pre-solver chain on a command line (nrt_scenario_*):
+    tokens, the builder named by `simulation`, the mechanical model, the
+    measurement point. Returns a dict: `geometry` (Geometry) and the run values."""
+    argv = [b"neurite_hip"] + [a.encode() for a in args]
+    arr = (C.c_char_p * len(argv))(*argv)
+    h = C.c_void_p()
+    L = lib()
+    st = L.nrt_scenario_create(len(argv), arr,
+                               simulation.encode() if simulation else None,
+                               working_directory.encode() if working_directory else None,
+                               C.byref(h))
+    if st != 0:
+        raise NrtError(st, L.nrt_scenario_last_error().decode())
+    try:
+        info = ScenarioInfo()
+        _check(L.nrt_scenario_get(h, C.byref(info)))
+        g = info.geometry
+        N = g.num_elem
+
+        def arr_(ptr, n, dt):
+            return np.ctypeslib.as_array(ptr, (n,)).astype(dt, copy=True) if n else np.zeros(0, dt)
+        mats = [Material.from_buffer_copy(g.materials[k]) for k in range(g.num_materials)]
+        geom = Geometry(arr_(g.mother, N, np.int32), arr_(g.dr, N, np.int32),
+                        arr_(g.dl, N, np.int32), arr_(g.branching, N, np.uint8),
+                        arr_(g.diameter, N, np.float64), arr_(g.dx0, N, np.float64),
+                        arr_(g.epsilon, N, np.float64), arr_(g.input, N, np.float64),
+                        arr_(g.material, N, np.int32), mats,
+                        arr_(g.terminals, g.num_term, np.int32), g.myelinate, g.E_MAX)
+        out = {f: getattr(info, f) for f, _ in ScenarioInfo._fields_ if f != "geometry"}
+        for k in ("out_path", "log_path", "builder_log", "vout_path"):
+            out[k] = out[k].decode() if out[k] is not None else None
+        out["geometry"] = geom
+        return out
+    finally:
+        L.nrt_scenario_destroy(h)

@@ -361,6 +361,50 @@ const char* nrt_server_last_error(const nrt_server* s);
 /* Finishes the queued and running requests, then stops the loop. */
 int nrt_server_destroy(nrt_server* s);
 
+/* ------------------------------------------------- scenarios (the driver)
+ * What the reference main does before a solver exists (neurite.cpp:84-184): the
+ * command-line tokens (input.cpp:25-149; solver flags init.cpp:40-55), the
+ * geometry builder named by the `simulation` environment variable
+ * (init.cpp:84-130; builders and the neuron loader space.cpp:37-798), the
+ * mechanical model's macroscopic strain from `input_data.txt`
+ * (mechanical_model.cpp:21-99, init.cpp:320-333) and the measurement point
+ * (init.cpp:339-358). Host work only. The driver `neurite_hip` (built beside the
+ * library) feeds the geometry to nrt_setup and runs the time loop.
+ *
+ * Tokens beyond the reference's (ignored by it): `steps N` caps the time loop,
+ * `method bicgstab|gmres|direct`, `prec fp64|mixed`, `restart M`, `tol T` choose
+ * the implicit solve, `vout FILE` writes the final V of every element. */
+typedef struct nrt_scenario nrt_scenario;
+typedef struct nrt_scenario_info {
+  nrt_geometry geometry;       /* raw geometry; arrays owned by the scenario      */
+  int32_t explicit_mode;       /* token flags e / i / g                           */
+  int32_t implicit_mode;
+  int32_t viewer;              /* g: the OpenGL viewer (not part of this build)   */
+  int32_t mes_point;           /* measurement element (init.cpp:339-358)          */
+  double tspan, xspan;         /* after the builder (xspan strained)              */
+  double dt_factor, nu, epsilon, E_MAX;
+  double init_time, end_input, input_current;  /* stimulus window and amplitude   */
+  double dist_mes;             /* measurement distance (after the strain)         */
+  double epsilon_macro;        /* mechanical model (0 without strain)             */
+  const char* out_path;        /* `out` token (NULL when absent)                  */
+  const char* log_path;        /* `log` token                                     */
+  const char* builder_log;     /* lines the builder writes to the log             */
+  int64_t max_steps;           /* `steps` (0: the full run)                       */
+  int32_t method;              /* `method` (-1: the reference's BiCGSTAB)         */
+  int32_t precision;           /* `prec`                                          */
+  int32_t restart;             /* `restart` (0: default)                          */
+  double tol;                  /* `tol` (0: default)                              */
+  const char* vout_path;       /* `vout`                                          */
+} nrt_scenario_info;
+
+/* argv as main's (argv[0] is the program); simulation / working_directory NULL
+   -> the environment variables of those names. */
+int nrt_scenario_create(int argc, const char* const* argv, const char* simulation,
+                        const char* working_directory, nrt_scenario** out);
+int nrt_scenario_get(const nrt_scenario* s, nrt_scenario_info* out);
+const char* nrt_scenario_last_error(void);
+void nrt_scenario_destroy(nrt_scenario* s);
+
 #ifdef __cplusplus
 }
 #endif
