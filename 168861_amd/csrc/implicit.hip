@@ -7,7 +7,7 @@
 // Update_B (:162-171).
 //
 // Solve: BiCGSTAB with the reference's recurrences (solverIm-gpu.cpp:725-843), run
-// as ONE persistent cooperative kernel per call: the HH gating, HH-row refresh and
+// as ONE persistent kernel per call (a co-resident grid): the HH gating, HH-row refresh and
 // right-hand side of each time step, then every Krylov iteration, with device
 // resident scalars and no host round trip. Grid-wide dependencies use a bounded
 // grid barrier; each iteration needs three (one per global reduction). Vectors that
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
 // ------------------------------------------------------------------ GMRES(m)
 // Restarted GMRES (the method of the reference CPU solver, MKL FGMRES(150) with
 // x0 = 0 and an absolute 1e-14 stop, solverIm-cpu.cpp:192-272) as one persistent
-// cooperative kernel: per step the same update phase as the BiCGSTAB kernel, then
+// kernel on a co-resident grid: per step the same update phase as the BiCGSTAB kernel, then
 // Arnoldi cycles of up to m steps. Orthogonalisation is classical Gram-Schmidt
 // with one re-orthogonalisation (CGS2): each Arnoldi step takes one SpMV and three
 // grid-wide reductions (the j+1 projections, the j+1 corrections, the norm), where
@@ -1689,6 +1689,13 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b) {
                      e->grid_mode && e->plan_h.top_n > 0 ? 1 : 0};
 }
 
+// Persistent kernels are launched as ordinary kernels: their grids are sized to be
+// co-resident (choose_blocks: at most the occupancy limit, normally one workgroup
+// per CU), and their grid barrier is bounded, so a grid that could not all be
+// resident ends in an error status instead of a wait without end. A cooperative
+// launch is not used: it takes the device's cooperative queue, which one process
+// at a time can hold, so a second process on the same GPU (a driver run beside a
+// test run, two simulations) would be refused its launches.
 static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64_t a1,
                              int32_t trace_elem, int64_t interval, double* trace_dev,
                              int64_t gl0 = 0) {
@@ -1710,20 +1717,20 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
       GmresDev<float> g{m, e->gV32.p, e->gW32.p, e->gW32.p + e->N, e->gR.p, rstride,
                         e->gpart.p, e->val32.p, o.refine_tol};
       void* args[] = {&d, &g};
-      NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_gmres<float>, dim3(e->nblocks),
+      NRT_HIP(hipLaunchKernel((const void*)k_implicit_gmres<float>, dim3(e->nblocks),
                                          dim3(kImpBlock), args, 0, hnd->stream));
     } else {
       GmresDev<double> g{m, e->gV.p, e->t.p, e->s.p, e->gR.p, rstride, e->gpart.p, e->val.p,
                          0.0};
       void* args[] = {&d, &g};
-      NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_gmres<double>, dim3(e->nblocks),
+      NRT_HIP(hipLaunchKernel((const void*)k_implicit_gmres<double>, dim3(e->nblocks),
                                          dim3(kImpBlock), args, 0, hnd->stream));
     }
     return NRT_OK;
   }
   if (e->method == 0) {
     void* args[] = {&d};
-    NRT_HIP(hipLaunchCooperativeKernel((const void*)k_implicit_bicgstab_fp64, dim3(e->nblocks),
+    NRT_HIP(hipLaunchKernel((const void*)k_implicit_bicgstab_fp64, dim3(e->nblocks),
                                        dim3(kImpBlock), args, 0, hnd->stream));
     return NRT_OK;
   }
@@ -1732,7 +1739,7 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
     using TT = typename std::remove_pointer<decltype(targs.w.d)>::type;
     const size_t shmem = targs.top_lds ? top_lds_bytes<TT>() : 0;
     if (e->grid_mode)
-      NRT_HIP(hipLaunchCooperativeKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, shmem,
+      NRT_HIP(hipLaunchKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, shmem,
                                          hnd->stream));
     else
       NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));

@@ -1,5 +1,5 @@
 // Device helpers for gfx950 (wave64): DPP/cross-lane reductions, block
-// reductions, and a bounded grid barrier for persistent cooperative kernels.
+// reductions, and a bounded grid barrier for persistent kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -105,7 +105,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   __syncthreads();
 }
 
-// Grid barrier for a cooperative launch (all blocks co-resident). bar[0] counts
+// Grid barrier for a persistent launch (all blocks co-resident). bar[0] counts
 // arrivals, bar[1] is the generation. The wait is bounded: after spin_limit
 // polls, or once another block raised *err, it sets *err and returns so that
 // every block drains; callers test *err after each barrier.

@@ -11,7 +11,7 @@
 //      requests in batch slots (about k * 16384 element-steps each, so a launch
 //      lasts about as long whatever the mix of sizes and small requests are not
 //      held back by large ones), then k steps of each large request, in one
-//      cooperative launch per request
+//      persistent-kernel launch per request
 //      (more elements than big_threshold, which would hold a whole batch launch
 //      back if one workgroup had to carry it), all on the batch stream,
 //   3. retires the requests completed by the previous launch while this one

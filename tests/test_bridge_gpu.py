@@ -60,9 +60,10 @@ def test_bridge_explicit_matches_reference(name):
 
 
 @pytest.mark.skip(reason="the reference-tree executable's first cooperative BiCGSTAB launch "
-                         "returns hipErrorIllegalState (the same library passes every BiCGSTAB "
-                         "test from Python); cause not found by reading, so the case stays off "
-                         "the GPU for now")
+                         "returned hipErrorIllegalState when run as a child of a test process "
+                         "that had made cooperative launches (the device's cooperative queue is "
+                         "held by one process); launches are now ordinary, and this case waits "
+                         "for its re-run after its one fix run failed")
 def test_bridge_implicit_matches_exact(orc):
     print("[gpu-case] bridge implicit ex6 dtf 10", flush=True)
     d = _run("CT_symmetric_tree", "i Tt 0.1 dtf 10 out o.dat log o.log", 300)
