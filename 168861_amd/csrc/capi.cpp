@@ -114,6 +114,8 @@ nrt_options resolve_options(const nrt_options* opt) {
   NRT_REQUIRE(o.chunk >= 0 && o.chunk <= kMaxChunk && o.chunk_hi >= 0 &&
                   o.chunk_hi <= kMaxChunk,
               NRT_ERR_INVALID, "tree chunk lengths must be in [0, 4096]");
+  NRT_REQUIRE(o.tree_groups >= 0 && o.tree_groups <= 1024, NRT_ERR_INVALID,
+              "tree_groups must be in [0, 1024]");
   return o;
 }
 

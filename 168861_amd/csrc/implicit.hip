@@ -151,6 +151,10 @@ struct Implicit {
   DBuf<double> gV, gR, gpart;
   DBuf<float> gV32, gW32, val32;  // mixed-precision GMRES: FP32 basis, vectors, matrix
   bool grid_mode = false;
+  bool local_mode = false;  // tree solve in local mode (plan_h.groups workgroups)
+  bool local_res = false;   // ... in its group-resident form
+  size_t local_lds = 0;     // dynamic LDS of a local-mode launch
+  DBuf<int32_t> hh_goff;
   int threads = 256;
   TreePlanHost plan_h;
   int ns0 = 0;  // level-0 slots (the tree solve's vectors are in slot order)
@@ -984,6 +988,7 @@ struct TreeArgs {
   double refine_tol;
   int refine_max;
   int top_lds;  // the launch provides top_lds_bytes<T>() of dynamic LDS (grid launches)
+  const int32_t* hh_goff;  // local mode: HH entries of group g are [hh_goff[g], +1)
 };
 
 template <bool GRID>
@@ -1261,6 +1266,346 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
   }
 }
 
+// ------------------------------------------------------------ local mode
+// One workgroup per row group (TreePlanHost::groups): workgroup g reduces the
+// columns of its group at every level and its local final nodes with workgroup
+// barriers only; after a grid barrier workgroup 0 solves the top of the final
+// forest (LDS-staged when it fits), and after a second one every workgroup
+// back-substitutes its own nodes. The arithmetic per chunk and per node is the
+// level-synchronous solve's.
+
+// Factors of every chunk of the group (once per launch).
+template <class T>
+__device__ void tps_factor_local(const TreePlanDev& P, const TreeWork<T>& w, int g) {
+  for (int lv = 0; lv < P.levels; lv++) {
+    const int32_t* gc = P.gcol + lv * (P.groups + 1);
+    const int c1 = gc[g + 1];
+    for (int c = gc[g] + (int)threadIdx.x; c < c1; c += blockDim.x)
+      tps_reduce_chunk(P, w, lv, P.chunk_off[lv] + c);
+    __syncthreads();
+  }
+}
+
+// The top of the final forest, between the two grid barriers (workgroup 0).
+template <class T>
+__device__ __forceinline__ void local_top(const ImpDev& d, const TreePlanDev& P,
+                                          const TreeWork<T>& w, unsigned char* tl) {
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+  if (d.vb == 0) {
+    const int nh = P.heights - P.fin_cut;
+    if (tl != nullptr) {
+      const TopLds<T> L = TopLds<T>::carve(tl, gl_top_entries(P));
+      const int ns = top_stage_count(P);
+      for (int j = tid; j < ns; j += nt) top_stage(P, w, L, j);
+      __syncthreads();
+      for (int h = 0; h < nh; h++) {
+        for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_up(L, j);
+        __syncthreads();
+      }
+      for (int h = nh - 1; h >= 0; h--) {
+        for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_down(L, j);
+        __syncthreads();
+      }
+      for (int j = tid; j < P.top_n; j += nt) top_store(P, w, L, j);
+    } else {
+      for (int h = 0; h < nh; h++) {
+        for (int j = P.top_hoff[h] + tid; j < P.top_hoff[h + 1]; j += nt)
+          tps_final_up(P, w, P.top_node[j]);
+        __syncthreads();
+      }
+      for (int h = nh - 1; h >= 0; h--) {
+        for (int j = P.top_hoff[h] + tid; j < P.top_hoff[h + 1]; j += nt)
+          tps_final_down(P, w, P.top_node[j]);
+        __syncthreads();
+      }
+    }
+  }
+  grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+}
+
+template <class T>
+__device__ void tps_run_local(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
+                              bool acc, bool all_static, unsigned char* tl) {
+  const int g = d.vb, G = P.groups, H = P.heights, L = P.levels;
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int lv = 0; lv < L; lv++) {
+    const int32_t* gc = P.gcol + lv * (G + 1);
+    const int c1 = gc[g + 1];
+    for (int c = gc[g] + tid; c < c1; c += nt) {
+      const int ch = P.chunk_off[lv] + c;
+      if (!all_static && P.chunk_dyn[ch]) tps_reduce_chunk(P, w, lv, ch);
+      else tps_reduce_chunk_static(P, w, lv, ch);
+    }
+    __syncthreads();
+  }
+  const int32_t* lh = P.loc_hoff + g * H;
+  for (int h = 0; h < H; h++) {
+    const int k0 = lh[h], k1 = lh[h + 1];
+    if (k0 == k1) continue;  // uniform over the workgroup
+    for (int k = k0 + tid; k < k1; k += nt) tps_final_up(P, w, P.loc_order[k]);
+    __syncthreads();
+  }
+  if (P.top_n > 0) local_top<T>(d, P, w, tl);
+  for (int h = H - 1; h >= 0; h--) {
+    const int k0 = lh[h], k1 = lh[h + 1];
+    if (k0 == k1) continue;
+    for (int k = k0 + tid; k < k1; k += nt) tps_final_down(P, w, P.loc_order[k]);
+    __syncthreads();
+  }
+  for (int lv = L - 1; lv >= 1; lv--) {
+    const int32_t* gc = P.gcol + lv * (G + 1);
+    const int c1 = gc[g + 1];
+    for (int c = gc[g] + tid; c < c1; c += nt)
+      tps_back_chunk<OUT_LEVEL>(P, w, lv, P.chunk_off[lv] + c, nullptr);
+    __syncthreads();
+  }
+  const int c1 = P.gcol[g + 1];
+  for (int c = P.gcol[g] + tid; c < c1; c += nt) {
+    if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, c, d.x);
+    else tps_back_chunk<OUT_ASSIGN>(P, w, 0, c, d.x);
+  }
+  __syncthreads();
+}
+
+// Group-resident form: the group's levels above 0 and its local final nodes in
+// LDS (lw), its plan metadata staged in LDS (M).
+template <class T>
+__device__ void tps_factor_res(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
+                               int L) {
+  for (int lv = 0; lv < L; lv++) {
+    const int nc = M.lvl[5 * lv + 1];
+    for (int j = (int)threadIdx.x; j < nc; j += blockDim.x) {
+      if (lv == 0) glm_reduce<true>(w, lw, M, 0, j, kGlFull);
+      else glm_reduce<false>(w, lw, M, lv, j, kGlFull);
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__device__ void tps_run_res(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
+                            const TreeWorkL<T>& lw, const GlMeta& M, bool acc, bool all_static,
+                            unsigned char* tl) {
+  const int H = P.heights, L = P.levels, how = all_static ? kGlStatic : kGlStep;
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int lv = 0; lv < L; lv++) {
+    const int nc = M.lvl[5 * lv + 1];
+    for (int j = tid; j < nc; j += nt) {
+      if (lv == 0) glm_reduce<true>(w, lw, M, 0, j, how);
+      else glm_reduce<false>(w, lw, M, lv, j, how);
+    }
+    __syncthreads();
+  }
+  for (int h = 0; h < H; h++) {
+    const int k0 = M.hoff[h], k1 = M.hoff[h + 1];
+    if (k0 == k1) continue;  // uniform over the workgroup
+    for (int j = k0 + tid; j < k1; j += nt) glm_final_up(w, lw, M, j);
+    __syncthreads();
+  }
+  if (P.top_n > 0) local_top<T>(d, P, w, tl);
+  for (int h = H - 1; h >= 0; h--) {
+    const int k0 = M.hoff[h], k1 = M.hoff[h + 1];
+    if (k0 == k1) continue;
+    for (int j = k0 + tid; j < k1; j += nt) glm_final_down(w, lw, M, j);
+    __syncthreads();
+  }
+  for (int lv = L - 1; lv >= 1; lv--) {
+    const int nc = M.lvl[5 * lv + 1];
+    for (int j = tid; j < nc; j += nt) glm_back<false, OUT_LEVEL>(w, lw, M, lv, j, nullptr);
+    __syncthreads();
+  }
+  const int nc0 = M.lvl[1];
+  for (int j = tid; j < nc0; j += nt) {
+    if (acc) glm_back<true, OUT_ADD>(w, lw, M, 0, j, d.x);
+    else glm_back<true, OUT_ASSIGN>(w, lw, M, 0, j, d.x);
+  }
+  __syncthreads();
+}
+
+// Update phase over the group's own level-0 slots (its columns at every row
+// position) and its HH entries; as tree_update otherwise.
+template <class T>
+__device__ __forceinline__ void tree_update_local(const ImpDev& d, const TreeArgs<T>& t,
+                                                  bool active) {
+  const int g = d.vb, tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  const double* __restrict__ xv = d.x;
+  const uint32_t* __restrict__ code = d.scode;
+  const double* __restrict__ cmdx = d.scmdx;
+  const double* __restrict__ kdx = d.skdx;
+  const double* __restrict__ input = d.sinput;
+  T* __restrict__ rt = t.w.r;
+  double* __restrict__ b = d.b;
+  const double dT = d.dT;
+  for (int hh = t.hh_goff[g] + tid; hh < t.hh_goff[g + 1]; hh += nt) {
+    const int i = d.hh.row[hh];
+    const int si = d.hh_slot[hh];
+    const double v = xv[si];
+    double W, K, diag;
+    imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+    if (imp_update_row(d, hh, i, W, diag)) t.w.d[si] = (T)diag;
+    double bi = d.cmdx[i] * v / dT + K * d.dxv[i];
+    if (active) bi = bi + d.input[i];
+    if (t.direct) rt[si] = (T)bi;
+    else b[si] = bi;
+  }
+  const int NC = t.P.nchunk[0], c0 = t.P.gcol[g], nc = t.P.gcol[g + 1] - c0;
+  const int ns = nc * (t.P.nslots[0] / NC);
+  for (int j0 = tid; j0 < ns; j0 += kTpsBatch * nt) {
+    int si[kTpsBatch];
+    uint32_t bc[kTpsBatch];
+    double bv[kTpsBatch], bm[kTpsBatch], bk[kTpsBatch], bin[kTpsBatch];
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      const int q = j0 + j * nt;
+      si[j] = q < ns ? (q / nc) * NC + c0 + q % nc : -1;
+      if (si[j] >= 0) {
+        bc[j] = code[si[j]];
+        bv[j] = xv[si[j]];
+        bm[j] = cmdx[si[j]];
+        bk[j] = kdx[si[j]];
+        bin[j] = active ? input[si[j]] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTpsBatch; j++) {
+      if (si[j] >= 0 && !(bc[j] & (kCodeHH | kCodeSkip))) {
+        double bi = bm[j] * bv[j] / dT + bk[j];
+        if (active) bi = bi + bin[j];
+        if (t.direct) rt[si[j]] = (T)bi;
+        else b[si[j]] = bi;
+      }
+    }
+  }
+}
+
+// Steps d.first .. d.first+d.num-1 in local mode.
+template <class T, bool RES>
+__device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs<T>& t,
+                                                 double* lds, unsigned char* dyn_lds) {
+  const TreePlanDev& P = t.P;
+  const int g = d.vb, tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  const int NC = P.nchunk[0], c0 = P.gcol[g], nc = P.gcol[g + 1] - c0;
+  const int ns = nc * (P.nslots[0] / NC);
+  const double* val = d.val;
+  const int32_t* __restrict__ rowptr = d.srowptr;
+  const int32_t* __restrict__ col = d.scol;
+  long long gl = d.gl0, ksamp = 0;
+  int slot = 0;
+  const double lim2 = t.refine_tol * t.refine_tol;
+  const bool owns_trace = d.trace && d.trace_elem >= 0 && d.trace_elem % NC >= c0 &&
+                          d.trace_elem % NC < c0 + nc;
+  // level-0 tree system of the own slots (a row's coupling to a child in another
+  // group is written into that child's slot: a grid barrier before the factors)
+  for (int q = tid; q < ns; q += nt) {
+    const int i = (q / nc) * NC + c0 + q % nc;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      const T v = (T)val[k];
+      const int role = d.srole[k];
+      if (role == 0) t.w.d[i] = v;
+      else if (role == 1) t.w.up[i] = v;
+      else t.w.dn[col[k]] = v;
+    }
+  }
+  // dynamic LDS: the staged top (when used), then (RES) the group's values and
+  // metadata
+  unsigned char* tl = t.top_lds ? dyn_lds : nullptr;
+  const GlLayout lay = gl_layout<T>(P, t.top_lds != 0);
+  TreeWorkL<T> lw{};
+  GlMeta M{};
+  if (RES) {
+    lw = gl_carve<T>(dyn_lds + lay.top, P.lds_nodes);
+    M = gl_meta_carve(dyn_lds + lay.top + lay.values, P);
+    gl_meta_stage(P, g, M, tid, nt);
+  }
+  grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+  if (RES) tps_factor_res<T>(t.w, lw, M, P.levels);
+  else tps_factor_local<T>(P, t.w, g);
+  for (long long step = d.first; step < d.first + d.num; step++) {
+    const bool active = step >= d.a0 && step <= d.a1;
+    tree_update_local<T>(d, t, active);
+    __syncthreads();
+    int sweeps = 0, status = NRT_OK;
+    double rr = 0, bb = 0;
+    bool ok = true;
+    if (t.direct) {
+      if (RES) tps_run_res<T>(d, P, t.w, lw, M, false, false, tl);
+      else tps_run_local<T>(d, P, t.w, false, false, tl);
+      sweeps = 1;
+    } else {
+      while (true) {
+        // FP64 residual of the own rows; it reads x across group boundaries, so
+        // every group's back-substitution must be complete
+        grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+        double red[2] = {0.0, 0.0};
+        for (int q = tid; q < ns; q += nt) {
+          const int i = (q / nc) * NC + c0 + q % nc;
+          double ax = 0, dg = 1.0;
+          for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+            ax = ax + val[k] * d.x[col[k]];
+            if (d.srole[k] == 0) dg = val[k];
+          }
+          const double bi = d.b[i];
+          const double ri = bi - ax;
+          t.w.r[i] = (T)ri;
+          red[0] = red[0] + (ri / dg) * (ri / dg);
+          red[1] = red[1] + (bi / dg) * (bi / dg);
+        }
+        ok = reduce_all<2>(d, red, lds, slot);
+        if (!ok) break;
+        rr = red[0];
+        bb = red[1];
+        if (!isfinite(rr)) {
+          status = NRT_ERR_NUMERIC;
+          break;
+        }
+        if (rr <= lim2 * bb) break;
+        if (sweeps >= t.refine_max) {
+          status = NRT_ERR_NOT_CONVERGED;
+          break;
+        }
+        if (RES) tps_run_res<T>(d, P, t.w, lw, M, true, sweeps > 0, tl);
+        else tps_run_local<T>(d, P, t.w, true, sweeps > 0, tl);
+        sweeps++;
+      }
+    }
+    if (__hip_atomic_load(d.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) ok = false;
+    if (!ok) status = NRT_ERR_HIP;
+    if (g == 0 && tid == 0) {
+      d.st->iters_total += (unsigned long long)sweeps;
+      if (!t.direct) d.st->sweeps_total += (unsigned long long)sweeps;
+      d.st->steps += 1;
+      d.st->last_iters = sweeps;
+      d.st->last_rr = rr;
+      d.st->last_bb = bb;
+      if (d.st->status == NRT_OK && status != NRT_OK) {
+        d.st->status = status;
+        d.st->fail_step = step;
+        d.st->fail_iter = sweeps;
+        d.st->fail_rr = rr;
+      }
+    }
+    if (!ok) return;
+    if (d.interval >= 0 && ++gl > d.interval) {
+      gl = 0;
+      if (owns_trace && tid == 0) d.trace[ksamp] = d.x[d.trace_elem];
+      ksamp++;
+    }
+  }
+}
+
+// Workgroup size of local mode: a group's levels have a few hundred chunks at most,
+// and 512 threads leave 256 VGPRs to the FP64 sweeps (1024 would spill).
+constexpr int kLocalBlock = 512;
+
+template <class T, bool RES>
+__global__ __launch_bounds__(kLocalBlock) void k_implicit_tree_local(ImpDev d, TreeArgs<T> t) {
+  __shared__ double lds[4 * 16];
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  d.vb = (int)blockIdx.x;
+  tree_steps_local<T, RES>(d, t, lds, dyn_lds);
+}
+
 template <class T, bool GRID>
 __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree(ImpDev d,
                                                                                  TreeArgs<T> t) {
@@ -1462,8 +1807,40 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       int nb = o.num_blocks;
       if (nb == 0 && N <= (1 << 17)) nb = 1;
       if (batch_threads > 0) nb = 1;  // a batch slot: one workgroup of batch_threads
-      e->grid_mode = nb != 1;
-      if (e->grid_mode) {
+      // Local mode (tree_groups > 1): one workgroup of kLocalBlock threads per row
+      // group, each reducing and back-substituting its own rows alone; falls back
+      // to the other forms when the plan has no reduction level or the HH entries
+      // are not in row order. Automatic for the FP64 solve from 16k rows: a group
+      // of ~5000 rows per workgroup, at most 32 workgroups, past which the top of
+      // the final forest (one workgroup) costs more than the groups save (measured:
+      // 20,202-row cable 70 -> 50 us per step with 4 groups; depth-14 tree of
+      // 902,037 rows 389 -> 163 us with 32, 199 with 48).
+      int G = batch_threads > 0 ? 0 : o.tree_groups;
+      if (G == 0 && batch_threads == 0 && e->method == 1 && o.num_blocks == 0 && N >= (1 << 14))
+        G = std::min(32, (N + 2500) / 5000);
+      if (G > 1) {
+        for (int hh = 1; hh < H; hh++)
+          if (p.hh_elem[hh] < p.hh_elem[hh - 1]) G = 0;
+        G = std::min(G, N / 4);
+      }
+      e->local_mode = G > 1;
+      if (e->local_mode) {
+        const void* k = e->method == 1 ? (const void*)k_implicit_tree_local<double, false>
+                                       : (const void*)k_implicit_tree_local<float, false>;
+        const size_t shmem = e->method == 1 ? top_lds_bytes<double>() : top_lds_bytes<float>();  // upper bound
+        int cus = 0, per_cu = 0;
+        NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hnd->device));
+        NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kLocalBlock, shmem));
+        NRT_REQUIRE(G <= cus * std::max(1, per_cu), NRT_ERR_INVALID,
+                    "tree_groups exceeds the co-resident limit " +
+                        std::to_string(cus * std::max(1, per_cu)));
+        nb = G;
+      }
+      e->grid_mode = nb != 1 && !e->local_mode;
+      if (e->local_mode) {
+        e->nblocks = G;
+        e->threads = kLocalBlock;
+      } else if (e->grid_mode) {
         const void* k = e->method == 1 ? (const void*)k_implicit_tree<double, true>
                                        : (const void*)k_implicit_tree<float, true>;
         e->nblocks = o.num_blocks > 0 || nb <= 0 ? choose_blocks(hnd, N, k) : nb;
@@ -1478,13 +1855,53 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // level costs grid barriers, and with chunks as long as a dendrite's
       // segments one level is left (depth-14 tree: 0.47 -> 0.40 ms per step).
       const int workers = e->nblocks * e->threads;
-      const int auto0 = e->grid_mode ? 128 : std::min(64, std::max(8, (N + workers - 1) / workers));
+      int auto0 = e->grid_mode ? 128 : std::min(64, std::max(8, (N + workers - 1) / workers));
+      if (e->local_mode) auto0 = std::min(32, std::max(8, (N + workers - 1) / workers));
       const int chunk = o.chunk > 0 ? o.chunk : auto0;
       const int chunk_hi = o.chunk_hi > 0 ? o.chunk_hi : (o.chunk > 0 ? o.chunk : 8);
       // HH rows change their diagonal every step; every other chunk keeps its factors
       std::vector<uint8_t> dyn(N, 0);
       for (int hh = 0; hh < H; hh++) dyn[p.hh_elem[hh]] = 1;
-      e->plan_h = build_tree_plan(par, chunk, chunk_hi, &dyn);
+      std::vector<int32_t> rgrp;
+      if (e->local_mode) {
+        rgrp = row_groups(N, G);
+        e->plan_h = build_tree_plan(par, chunk, chunk_hi, &dyn, &rgrp);
+        if (e->plan_h.levels == 0) {  // nothing to reduce: one workgroup instead
+          e->local_mode = false;
+          e->nblocks = 1;
+          e->threads = kTreeBlock;
+        } else {
+          try {
+            check_local_plan(e->plan_h);
+          } catch (const std::exception& ex) {
+            throw Error(NRT_ERR_TOPOLOGY, std::string("tree solve, local mode: ") + ex.what());
+          }
+        }
+      }
+      if (!e->local_mode) e->plan_h = build_tree_plan(par, chunk, chunk_hi, &dyn);
+      if (e->local_mode) {
+        // group-resident form when the top (if staged), the group's values and its
+        // metadata fit in LDS beside the kernel's own; else the levels stay in
+        // global memory
+        const bool f64 = e->method == 1;
+        const TreePlanDev V = plan_view(e->plan_h, nullptr, nullptr);  // sizes only
+        const bool top = e->plan_h.top_lds_ok != 0;
+        const GlLayout lay = f64 ? gl_layout<double>(V, top) : gl_layout<float>(V, top);
+        int max_lds = 0, cus = 0, per_cu = 0;
+        NRT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                      hnd->device));
+        NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hnd->device));
+        const void* kr = f64 ? (const void*)k_implicit_tree_local<double, true>
+                             : (const void*)k_implicit_tree_local<float, true>;
+        e->local_res = lay.total + 1024 <= (size_t)max_lds;
+        if (e->local_res) {
+          NRT_HIP(hipFuncSetAttribute(kr, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lay.total));
+          NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kr, kLocalBlock, lay.total));
+          e->local_res = G <= cus * per_cu;
+        }
+        e->local_lds = e->local_res ? lay.total : lay.top;
+      }
       // grid launches split the final forest over their workgroups; a batch slot
       // may run as a virtual grid of up to one workgroup per kRowsPerWorkgroup rows
       if (e->grid_mode)
@@ -1558,6 +1975,13 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       e->val.upload(sval, s);  // the refreshed HH rows are written here
       e->hh_slot.upload(hslot, s);
       e->hh_mat.upload(hmat, s);
+      if (e->local_mode) {
+        // HH entries by group (they are in row order, groups are row ranges)
+        std::vector<int32_t> hoff(G + 1, 0);
+        for (int hh = 0; hh < H; hh++) hoff[rgrp[p.hh_elem[hh]] + 1]++;
+        for (int g = 0; g < G; g++) hoff[g + 1] += hoff[g];
+        e->hh_goff.upload(hoff, s);
+      }
       e->rowptr.release();
       e->col.release();
       const size_t T = (size_t)P.total_nodes();
@@ -1686,7 +2110,11 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b) {
                      TreeWork<T>{b, b + T_, b + 2 * T_, b + 3 * T_, b + 4 * T_, b + 5 * T_,
                                  b + 6 * T_, b + 7 * T_, b + 8 * T_, b + 9 * T_},
                      e->method == 1 ? 1 : 0, o.refine_tol, o.refine_max,
-                     e->grid_mode && e->plan_h.top_n > 0 ? 1 : 0};
+                     (e->grid_mode && e->plan_h.top_n > 0) ||
+                             (e->local_mode && e->plan_h.top_lds_ok)
+                         ? 1
+                         : 0,
+                     e->local_mode ? e->hh_goff.p : nullptr};
 }
 
 // Persistent kernels are launched as ordinary kernels: their grids are sized to be
@@ -1734,11 +2162,14 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
                                        dim3(kImpBlock), args, 0, hnd->stream));
     return NRT_OK;
   }
-  auto launch = [&](auto targs, const void* kgrid, const void* kblock) {
+  auto launch = [&](auto targs, const void* kgrid, const void* kblock, const void* klocal) {
     void* args[] = {&d, &targs};
     using TT = typename std::remove_pointer<decltype(targs.w.d)>::type;
     const size_t shmem = targs.top_lds ? top_lds_bytes<TT>() : 0;
-    if (e->grid_mode)
+    if (e->local_mode)
+      NRT_HIP(hipLaunchKernel(klocal, dim3(e->nblocks), dim3(kLocalBlock), args, e->local_lds,
+                              hnd->stream));
+    else if (e->grid_mode)
       NRT_HIP(hipLaunchKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, shmem,
                                          hnd->stream));
     else
@@ -1746,10 +2177,14 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   };
   if (e->method == 1)
     launch(tree_args<double>(e, o, e->tw64.p), (const void*)k_implicit_tree<double, true>,
-           (const void*)k_implicit_tree<double, false>);
+           (const void*)k_implicit_tree<double, false>,
+           (const void*)(e->local_res ? (const void*)k_implicit_tree_local<double, true>
+                                      : (const void*)k_implicit_tree_local<double, false>));
   else
     launch(tree_args<float>(e, o, e->tw32.p), (const void*)k_implicit_tree<float, true>,
-           (const void*)k_implicit_tree<float, false>);
+           (const void*)k_implicit_tree<float, false>,
+           (const void*)(e->local_res ? (const void*)k_implicit_tree_local<float, true>
+                                      : (const void*)k_implicit_tree_local<float, false>));
   return NRT_OK;
 }
 
@@ -1858,7 +2293,7 @@ size_t implicit_slot_bytes(int precision) {
 void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_elem,
                         long long interval, double* trace, void* out) {
   Implicit* e = h->im;
-  NRT_REQUIRE(e->method != 0 && !e->grid_mode, NRT_ERR_UNSUPPORTED,
+  NRT_REQUIRE(e->method != 0 && !e->grid_mode && !e->local_mode, NRT_ERR_UNSUPPORTED,
               "batch slots use the block-mode tree solve");
   ImpDev d = imp_dev(h);
   d.a0 = a0;

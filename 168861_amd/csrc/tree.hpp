@@ -68,6 +68,30 @@ struct TreePlanHost {
   std::vector<int32_t> top_hoff;    // height boundaries (heights fin_cut..H-1), local
   std::vector<int32_t> top_cptr, top_child;  // children of top nodes, local indices
   std::vector<int32_t> top_par;     // local parent of each top node, or -1
+  // Local mode (build_tree_plan with row groups): G groups of rows, one per
+  // workgroup of a grid launch. Chains break where the group changes, so every
+  // chunk of every level belongs to one group; a final-forest node is local when
+  // its whole subtree is in its group, else it is in the top (closed upwards).
+  // Each workgroup reduces, solves its local nodes and back-substitutes alone;
+  // only the top needs the whole grid (two grid barriers per solve).
+  int groups = 0;
+  std::vector<int32_t> gcol;        // level lv, group g: columns [gcol[lv*(G+1)+g], +1)
+  std::vector<int32_t> loc_order;   // local final nodes by group, then height
+  std::vector<int32_t> loc_hoff;    // group g, height h: [loc_hoff[g*H+h], +1)
+  int top_lds_ok = 0;               // the top fits in kTopLdsNodes LDS entries
+  // Group-resident form of local mode: each group's nodes of levels 1..L-1 and its
+  // local final nodes get local indices (kept in LDS by the launch). Level lv of
+  // group g starts at lbase[g*(L+2)+lv], node k of its column c at
+  // + k*nc + (c - c0) (c0, nc: the group's columns of that level); its local final
+  // nodes follow at lbase[g*(L+2)+L] in loc_order order; lbase[g*(L+2)+L+1] is
+  // the group's count. Chunk-end codes: a local index (>= 0) or -2-u for final
+  // node u of the top (kept in the global arrays).
+  int lds_nodes = 0;                // largest group count
+  int gl_cols = 0, gl_fin = 0, gl_edges = 0;  // largest group: columns, local final nodes, their children
+  std::vector<int32_t> lbase;
+  std::vector<int32_t> lnext;       // per slot of levels 0..L-1: end code, -1 if no end
+  std::vector<int32_t> lf_cptr, lf_child;  // local final nodes (loc_order order): children
+  std::vector<int32_t> lf_par;      // parent: local index, top code, or -1
   int64_t total_nodes() const { return node_off.empty() ? 0 : node_off.back(); }
 };
 
@@ -81,8 +105,10 @@ constexpr int kTopLdsNodes = 1024;
 // every step, the others reuse their factors. Within a level, chunks are placed
 // (slot columns) dynamic first, then by decreasing length, so a wave's lanes take
 // the same path and stay active together.
+// groups (optional, N entries, values 0..G-1): local mode, see TreePlanHost.
 TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chunk_hi,
-                             const std::vector<uint8_t>* dyn_rows = nullptr);
+                             const std::vector<uint8_t>* dyn_rows = nullptr,
+                             const std::vector<int32_t>* groups = nullptr);
 
 // Splits the final forest of a plan for a launch of `groups` workgroups of
 // `threads` threads (any number of workgroups can run it: workgroup b takes the
@@ -92,6 +118,14 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chu
 // top fits in kTopLdsNodes LDS entries cost a tenth (the lists for that are
 // built). No split when groups <= 1.
 void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage_top = false);
+
+// Local mode row groups: G contiguous ranges of rows of (nearly) equal size.
+std::vector<int32_t> row_groups(int N, int G);
+
+// Throws when a local-mode plan breaks what its launch relies on: every final node
+// local to one group or in the top, the top closed upwards, a local node's children
+// in its group, every chunk's columns and ends in its own group.
+void check_local_plan(const TreePlanHost& P);
 
 // Parent of each row in an assembled CSR (the single column < row, or -1).
 std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* col);
@@ -106,6 +140,8 @@ struct TreePlanDev {
   int levels, heights;
   int fin_cut, sub_groups;
   int top_n, top_ext;
+  int groups, top_lds_ok, lds_nodes;
+  int gl_cols, gl_fin, gl_edges;
   const int32_t* nslots;
   const int32_t* nchunk;
   const int32_t* node_off;
@@ -127,6 +163,14 @@ struct TreePlanDev {
   const int32_t* top_cptr;
   const int32_t* top_child;
   const int32_t* top_par;
+  const int32_t* gcol;
+  const int32_t* loc_order;
+  const int32_t* loc_hoff;
+  const int32_t* lbase;
+  const int32_t* lnext;
+  const int32_t* lf_cptr;
+  const int32_t* lf_child;
+  const int32_t* lf_par;
 };
 
 // Per-value work arrays of one solve, over all levels (total_nodes each): d, up
@@ -136,6 +180,7 @@ struct TreePlanDev {
 // that reuse them: the forward multipliers f and the pivots' reciprocals.
 template <class T>
 struct TreeWork {
+  using value_type = T;
   T* d;
   T* up;
   T* dn;
@@ -154,53 +199,110 @@ struct TreeWork {
 #define NRT_HD inline
 #endif
 
+// LDS-resident arrays are typed in the LDS address space in device code, so that
+// loads and stores through them are LDS instructions, never generic ones (host
+// code, e.g. the emulation in tests, uses plain pointers).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define NRT_LDS __attribute__((address_space(3)))
+#else
+#define NRT_LDS
+#endif
+template <class T>
+struct TreeWorkL {
+  using value_type = T;
+  NRT_LDS T* d;
+  NRT_LDS T* up;
+  NRT_LDS T* dn;
+  NRT_LDS T* r;
+  NRT_LDS T* x;
+  NRT_LDS T* ca;
+  NRT_LDS T* cb;
+  NRT_LDS T* cg;
+  NRT_LDS T* cf;
+  NRT_LDS T* ci;
+};
+
+// Reciprocal of a pivot. On the device: the hardware estimate refined by Newton
+// steps r <- r + r (1 - x r) (two for FP64, one for FP32), within about an ulp and
+// a third as many dependent operations as a correctly rounded division, which
+// matters on the sweeps' serial paths. On the host: a division. The systems are
+// diagonally dominant, so pivots are far from zero, infinity and denormals.
+template <class T>
+NRT_HD T tps_rcp(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 8) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+    return r;
+  } else {
+    float r = __builtin_amdgcn_rcpf(x);
+    r = __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+    return r;
+  }
+#else
+  return T(1) / x;
+#endif
+}
+
 // Rows of a chunk whose loads are issued together: the sweeps are recurrences, but
 // their operands do not depend on them, so a batch of rows is fetched at once and
 // the latency is paid once per batch rather than once per row.
 constexpr int kTpsBatch = 4;
 
-// Eliminate the interior of chunk `ch` of level `lv` and write its ends to lv+1.
+// Where a chunk's two surviving ends go: the slots of the next level in the same
+// arrays (level-synchronous layout). End codes are slot indices.
 template <class T>
-NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
-  T* __restrict__ d = w.d;
-  T* __restrict__ up = w.up;
-  T* __restrict__ dn = w.dn;
-  T* __restrict__ r = w.r;
-  T* __restrict__ ca = w.ca;
-  T* __restrict__ cb = w.cb;
-  T* __restrict__ cg = w.cg;
-  T* __restrict__ cf = w.cf;
-  T* __restrict__ ci = w.ci;
-  const int NC = P.nchunk[lv];
-  const int c = ch - P.chunk_off[lv];
-  const int m = P.chunk_len[ch];
-  const int base = P.node_off[lv] + c, nbase = P.node_off[lv + 1];
+struct EndsSlots {
+  TreeWork<T> w;
+  NRT_HD void put(int e, T d, T up, T dn, T r) const {
+    w.d[e] = d;
+    w.up[e] = up;
+    w.dn[e] = dn;
+    w.r[e] = r;
+  }
+  NRT_HD void put_r(int e, T r) const { w.r[e] = r; }
+  NRT_HD T x(int e) const { return w.x[e]; }
+};
+
+// Eliminate the interior of the chunk of m rows at base, base + stride, ... in the
+// arrays w; its ends (codes es for row 0, ee for row m-1) go through `ends`.
+template <class W, class E>
+NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, const E& ends) {
+  using T = typename W::value_type;
+  using P = decltype(w.d);
+  const P __restrict__ d = w.d;
+  const P __restrict__ up = w.up;
+  const P __restrict__ dn = w.dn;
+  const P __restrict__ r = w.r;
+  const P __restrict__ ca = w.ca;
+  const P __restrict__ cb = w.cb;
+  const P __restrict__ cg = w.cg;
+  const P __restrict__ cf = w.cf;
+  const P __restrict__ ci = w.ci;
   if (m <= 2) {  // nothing to eliminate: the nodes pass through
     for (int k = 0; k < m; k++) {
-      const int g = base + k * NC, t = nbase + P.next[g];
-      d[t] = d[g];
-      up[t] = up[g];
-      dn[t] = dn[g];
-      r[t] = r[g];
+      const int g = base + k * NC;
+      ends.put(k == 0 ? es : ee, d[g], up[g], dn[g], r[g]);
     }
     return;
   }
   // start row s = 0 and first interior row 1
   const T d0 = d[base], u0 = up[base], n0 = dn[base], r0 = r[base];
-  const int ts = nbase + P.next[base];
-  const int te = nbase + P.next[base + (m - 1) * NC];
   const int g1 = base + NC;
   const T dn1 = dn[g1];
   // forward sweep over rows k = 1..m-1:
   //   d'_k x_k + dn_{k+1} x_{k+1} + g_k x_s = r'_k,
   // stored as (ca, cb, cg) = (d', g, r') for interior rows; row m-1 becomes the
   // end row of the reduced system: G_e x_s + D_e x_e + (children) = R_e.
+  // Each row's pivot reciprocal is computed once, here, and kept in ci: the
+  // backward sweep and the steps that reuse the factors only multiply.
   T dp = d[g1], gk = up[g1], rp = r[g1];
-  ca[g1] = dp;
   cb[g1] = gk;
   cg[g1] = rp;
-  T pdp = dp, pgk = gk, prp = rp;  // values of the row before the last one
-  T dnl = dn1;                       // dn of the last row
+  T pgk = gk, prp = rp;  // values of the row before the last one
+  T inv = T(0);          // reciprocal of that row's pivot
+  T dnl = dn1;           // dn of the last row
   for (int k0 = 2; k0 <= m - 1; k0 += kTpsBatch) {
     T bu[kTpsBatch], bd[kTpsBatch], bn[kTpsBatch], br[kTpsBatch];
 #pragma unroll
@@ -218,48 +320,44 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
     for (int j = 0; j < kTpsBatch; j++) {
       const int k = k0 + j;
       if (k <= m - 1) {
-        pdp = dp;
         pgk = gk;
         prp = rp;
-        const T f = bu[j] / dp;
+        const int g = base + k * NC;
+        inv = tps_rcp(dp);
+        ci[g - NC] = inv;
+        const T f = bu[j] * inv;
         dp = bd[j] - f * bn[j];
         gk = -f * gk;
         rp = br[j] - f * rp;
         dnl = bn[j];
-        const int g = base + k * NC;
         cf[g] = f;
         if (k <= m - 2) {
-          ca[g] = dp;
           cb[g] = gk;
           cg[g] = rp;
         }
       }
     }
   }
-  d[te] = dp;
-  up[te] = gk;
-  r[te] = rp;
+  const T end_d = dp, end_up = gk, end_r = rp;
   // backward sweep: x_k = alpha_k + beta_k x_s + gamma_k x_e, into (ca, cb, cg)
   T al, be, ga;
   {
     const int g2 = base + (m - 2) * NC;
-    const T inv = T(1) / pdp;
     al = prp * inv;
     be = -pgk * inv;
     ga = -dnl * inv;
     ca[g2] = al;
     cb[g2] = be;
     cg[g2] = ga;
-    ci[g2] = inv;
   }
   for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatch) {
-    T bp[kTpsBatch], bg[kTpsBatch], brr[kTpsBatch], bn[kTpsBatch];
+    T bi[kTpsBatch], bg[kTpsBatch], brr[kTpsBatch], bn[kTpsBatch];
 #pragma unroll
     for (int j = 0; j < kTpsBatch; j++) {
       const int k = k0 - j;
       if (k >= 1) {
         const int g = base + k * NC;
-        bp[j] = ca[g];
+        bi[j] = ci[g];
         bg[j] = cb[g];
         brr[j] = cg[g];
         bn[j] = dn[g + NC];  // coupling k -> k+1
@@ -269,52 +367,52 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
     for (int j = 0; j < kTpsBatch; j++) {
       const int k = k0 - j;
       if (k >= 1) {
-        const T inv = T(1) / bp[j];
         const T dcur = bn[j];
-        al = (brr[j] - dcur * al) * inv;
-        be = (-bg[j] - dcur * be) * inv;
-        ga = -dcur * ga * inv;
+        al = (brr[j] - dcur * al) * bi[j];
+        be = (-bg[j] - dcur * be) * bi[j];
+        ga = -dcur * ga * bi[j];
         const int g = base + k * NC;
         ca[g] = al;
         cb[g] = be;
         cg[g] = ga;
-        ci[g] = inv;
       }
     }
   }
-  // start row s: up_s x_P + (d_s + dn_1 beta_1) x_s + dn_1 gamma_1 x_e = r_s - dn_1 alpha_1
-  d[ts] = d0 + dn1 * be;
-  up[ts] = u0;
-  dn[ts] = n0;
-  r[ts] = r0 - dn1 * al;
-  dn[te] = dn1 * ga;  // s -> e coupling
+  // start row s: up_s x_P + (d_s + dn_1 beta_1) x_s + dn_1 gamma_1 x_e = r_s - dn_1 alpha_1;
+  // end row e: its dn is the s -> e coupling
+  ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * al);
+  ends.put(ee, end_d, end_up, dn1 * ga, end_r);
+}
+
+// Eliminate the interior of chunk `ch` of level `lv` and write its ends to lv+1.
+template <class T>
+NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
+  const int NC = P.nchunk[lv], m = P.chunk_len[ch];
+  const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
+  const int es = nbase + P.next[base];
+  const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
+  chunk_reduce(w, base, NC, m, es, ee, EndsSlots<T>{w});
 }
 
 // Right-hand side only, for a chunk whose factors (f, 1/pivot, beta, gamma, the
 // reduced matrix values of its ends) are those of its last full elimination: the
 // same operations on r as tps_reduce_chunk, in the same order, so the results are
 // identical; the sweeps are multiply-adds with no division on the dependent path.
-template <class T>
-NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
-  T* __restrict__ r = w.r;
-  T* __restrict__ ca = w.ca;
-  const T* __restrict__ dn = w.dn;
-  const T* __restrict__ cf = w.cf;
-  const T* __restrict__ ci = w.ci;
-  const int NC = P.nchunk[lv];
-  const int c = ch - P.chunk_off[lv];
-  const int m = P.chunk_len[ch];
-  const int base = P.node_off[lv] + c, nbase = P.node_off[lv + 1];
+template <class W, class E>
+NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int ee,
+                                const E& ends) {
+  using T = typename W::value_type;
+  using P = decltype(w.d);
+  const P __restrict__ r = w.r;
+  const P __restrict__ ca = w.ca;
+  const P __restrict__ dn = w.dn;
+  const P __restrict__ cf = w.cf;
+  const P __restrict__ ci = w.ci;
   if (m <= 2) {
-    for (int k = 0; k < m; k++) {
-      const int g = base + k * NC;
-      r[nbase + P.next[g]] = r[g];
-    }
+    for (int k = 0; k < m; k++) ends.put_r(k == 0 ? es : ee, r[base + k * NC]);
     return;
   }
   const T r0 = r[base];
-  const int ts = nbase + P.next[base];
-  const int te = nbase + P.next[base + (m - 1) * NC];
   const T dn1 = dn[base + NC];
   // forward: rp_k = r_k - f_k rp_{k-1}
   T rp = r[base + NC];
@@ -339,7 +437,7 @@ NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, 
       }
     }
   }
-  r[te] = rp;
+  ends.put_r(ee, rp);
   // backward: alpha_k = (rp_k - dn_{k+1} alpha_{k+1}) / pivot_k
   const int g2 = base + (m - 2) * NC;
   T al = ca[g2] * ci[g2];
@@ -365,7 +463,16 @@ NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, 
       }
     }
   }
-  r[ts] = r0 - dn1 * al;
+  ends.put_r(es, r0 - dn1 * al);
+}
+
+template <class T>
+NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
+  const int NC = P.nchunk[lv], m = P.chunk_len[ch];
+  const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
+  const int es = nbase + P.next[base];
+  const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
+  chunk_reduce_static(w, base, NC, m, es, ee, EndsSlots<T>{w});
 }
 
 // Final forest, elimination of node v (all of its children are final already).
@@ -377,7 +484,7 @@ NRT_HD void tps_final_up(const TreePlanDev& P, const TreeWork<T>& w, int v) {
   T dv = w.d[base + v], rv = w.r[base + v];
   for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) {
     const int c = base + P.fin_child[k];
-    const T f = w.dn[c] / w.ca[c];
+    const T f = w.dn[c] * tps_rcp(w.ca[c]);
     dv = dv - f * w.up[c];
     rv = rv - f * w.r[c];
   }
@@ -391,7 +498,7 @@ NRT_HD void tps_final_down(const TreePlanDev& P, const TreeWork<T>& w, int v) {
   const int p = P.fin_par[v];
   T acc = w.r[base + v];
   if (p >= 0) acc = acc - w.up[base + v] * w.x[base + p];
-  w.x[base + v] = acc / w.ca[base + v];
+  w.x[base + v] = acc * tps_rcp(w.ca[base + v]);
 }
 
 // The top of the final forest staged in LDS (TreePlanHost::top_*): one workgroup
@@ -409,18 +516,17 @@ struct TopLds {
   int32_t* child;
   int32_t* par;
   int32_t* hoff;
-  // carve from one buffer of top_lds_bytes<T>()
-  NRT_HD static TopLds carve(void* p) {
+  // carve from one buffer of top_lds_bytes<T>(n) for n >= top_n + top_ext entries
+  NRT_HD static TopLds carve(void* p, int n = kTopLdsNodes) {
     T* t = (T*)p;
-    int32_t* i = (int32_t*)(t + 5 * kTopLdsNodes);
-    return TopLds{t,     t + kTopLdsNodes,     t + 2 * kTopLdsNodes, t + 3 * kTopLdsNodes,
-                  t + 4 * kTopLdsNodes, i, i + kTopLdsNodes + 1, i + 2 * kTopLdsNodes + 1,
-                  i + 3 * kTopLdsNodes + 1};
+    int32_t* i = (int32_t*)(t + 5 * n);
+    return TopLds{t, t + n, t + 2 * n, t + 3 * n, t + 4 * n, i, i + n + 1, i + 2 * n + 1,
+                  i + 3 * n + 1};
   }
 };
 template <class T>
-constexpr size_t top_lds_bytes() {
-  return 5 * kTopLdsNodes * sizeof(T) + (4 * kTopLdsNodes + 2) * sizeof(int32_t);
+constexpr size_t top_lds_bytes(int n = kTopLdsNodes) {
+  return 5 * (size_t)n * sizeof(T) + (4 * (size_t)n + 2) * sizeof(int32_t);
 }
 
 // Index j of every staged list: the values of entry j (j < top_n + top_ext) and
@@ -452,7 +558,7 @@ NRT_HD void top_up(const TopLds<T>& L, int j) {
   T dv = L.d[j], rv = L.r[j];
   for (int k = L.cptr[j]; k < L.cptr[j + 1]; k++) {
     const int c = L.child[k];
-    const T f = L.dn[c] / L.d[c];
+    const T f = L.dn[c] * tps_rcp(L.d[c]);
     dv = dv - f * L.up[c];
     rv = rv - f * L.r[c];
   }
@@ -465,7 +571,7 @@ NRT_HD void top_down(const TopLds<T>& L, int j) {
   const int p = L.par[j];
   T acc = L.r[j];
   if (p >= 0) acc = acc - L.up[j] * L.x[p];
-  L.x[j] = acc / L.d[j];
+  L.x[j] = acc * tps_rcp(L.d[j]);
 }
 
 // After the down sweep: x of top node j back to the final level.
@@ -479,30 +585,27 @@ NRT_HD void top_store(const TreePlanDev& P, const TreeWork<T>& w, const TopLds<T
 // offset 0), assigned or accumulated (a refinement).
 enum TpsOut { OUT_LEVEL = 0, OUT_ASSIGN = 1, OUT_ADD = 2 };
 
-template <int MODE, class T>
-NRT_HD void tps_put(const TreePlanDev& P, const TreeWork<T>& w, int g, T v, double* out) {
-  (void)P;
+template <int MODE, class W, class T>
+NRT_HD void tps_put(const W& w, int g, T v, double* out) {
   if (MODE == OUT_LEVEL) w.x[g] = v;
   else out[g] = MODE == OUT_ADD ? out[g] + (double)v : (double)v;
 }
 
-// Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1.
-template <int MODE, class T>
-NRT_HD void tps_back_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch,
-                           double* out) {
-  const int NC = P.nchunk[lv];
-  const int c = ch - P.chunk_off[lv];
-  const int m = P.chunk_len[ch];
-  const int base = P.node_off[lv] + c, nbase = P.node_off[lv + 1];
-  const T* __restrict__ xn = w.x + nbase;
+// Back-substitution of the chunk of m rows at base, base + NC, ... in w from the
+// solution at its ends (codes es, ee, read through `ends`).
+template <int MODE, class W, class E>
+NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, const E& ends,
+                       double* out) {
+  using T = typename W::value_type;
+  using P = decltype(w.d);
   const int ge = base + (m - 1) * NC;
-  const T xs = xn[P.next[base]];
-  const T xe = m >= 2 ? xn[P.next[ge]] : xs;
-  tps_put<MODE>(P, w, base, xs, out);
-  if (m >= 2) tps_put<MODE>(P, w, ge, xe, out);
-  const T* __restrict__ ca = w.ca;
-  const T* __restrict__ cb = w.cb;
-  const T* __restrict__ cg = w.cg;
+  const T xs = ends.x(es);
+  const T xe = m >= 2 ? ends.x(ee) : xs;
+  tps_put<MODE>(w, base, xs, out);
+  if (m >= 2) tps_put<MODE>(w, ge, xe, out);
+  const P __restrict__ ca = w.ca;
+  const P __restrict__ cb = w.cb;
+  const P __restrict__ cg = w.cg;
   for (int k0 = 1; k0 <= m - 2; k0 += kTpsBatch) {
     T a[kTpsBatch], b[kTpsBatch], g[kTpsBatch];
 #pragma unroll
@@ -518,15 +621,230 @@ NRT_HD void tps_back_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, i
 #pragma unroll
     for (int j = 0; j < kTpsBatch; j++) {
       const int k = k0 + j;
-      if (k <= m - 2) tps_put<MODE>(P, w, base + k * NC, a[j] + b[j] * xs + g[j] * xe, out);
+      if (k <= m - 2) tps_put<MODE>(w, base + k * NC, a[j] + b[j] * xs + g[j] * xe, out);
     }
   }
+}
+
+// Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1.
+template <int MODE, class T>
+NRT_HD void tps_back_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch,
+                           double* out) {
+  const int NC = P.nchunk[lv], m = P.chunk_len[ch];
+  const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
+  const int es = nbase + P.next[base];
+  const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
+  chunk_back<MODE>(w, base, NC, m, es, ee, EndsSlots<T>{w}, out);
 }
 
 // Level 0 as the final forest itself (a system with no chain longer than 2).
 template <int MODE, class T>
 NRT_HD void tps_final_out(const TreePlanDev& P, const TreeWork<T>& w, int v, double* out) {
-  tps_put<MODE>(P, w, v, w.x[v], out);
+  (void)P;
+  tps_put<MODE>(w, v, w.x[v], out);
+}
+
+// ---- group-resident local mode (TreePlanHost::lbase): level 0 and the top in the
+// global arrays w, every other node of group g in its local arrays lw. The
+// arithmetic is the level-synchronous routines', operand for operand.
+template <class T>
+struct EndsLocal {
+  TreeWorkL<T> lw;
+  TreeWork<T> w;
+  int fbase;  // first final-forest slot in w
+  NRT_HD void put(int e, T d, T up, T dn, T r) const {
+    if (e >= 0) {
+      lw.d[e] = d;
+      lw.up[e] = up;
+      lw.dn[e] = dn;
+      lw.r[e] = r;
+    } else {
+      const int t = fbase - 2 - e;
+      w.d[t] = d;
+      w.up[t] = up;
+      w.dn[t] = dn;
+      w.r[t] = r;
+    }
+  }
+  NRT_HD void put_r(int e, T r) const {
+    if (e >= 0) lw.r[e] = r;
+    else w.r[fbase - 2 - e] = r;
+  }
+  NRT_HD T x(int e) const { return e >= 0 ? lw.x[e] : w.x[fbase - 2 - e]; }
+};
+
+// Plan metadata of one group in the group-resident form, staged once per launch
+// (LDS on the device) so that no step waits on a chain of plan loads:
+//   lvl[5*lv ..]: first column, column count, local base, first col entry, NC of level lv
+//   col[4*q ..]: chunk length, start code, end code, dyn flag of the group's column q
+//   scal: first final-forest slot in w, local base of the final nodes, final count
+//   hoff: the local final nodes' height boundaries (group-relative, H+1)
+//   fpar, fu, fcp, fch: their parent codes, final ids, children (local indices)
+struct GlMeta {
+  NRT_LDS int32_t* lvl;
+  NRT_LDS int32_t* col;
+  NRT_LDS int32_t* scal;
+  NRT_LDS int32_t* hoff;
+  NRT_LDS int32_t* fpar;
+  NRT_LDS int32_t* fu;
+  NRT_LDS int32_t* fcp;
+  NRT_LDS int32_t* fch;
+};
+
+// Byte layout of a group-resident launch's dynamic LDS: the staged top (when
+// used), the group's value arrays, its metadata.
+struct GlLayout {
+  size_t top, values, meta, total;
+};
+NRT_HD size_t gl_round16(size_t b) { return (b + 15) / 16 * 16; }
+NRT_HD int gl_top_entries(const TreePlanDev& P) {
+  const int n = P.top_n + P.top_ext;
+  return n > 0 ? n : 1;
+}
+NRT_HD size_t gl_meta_ints(const TreePlanDev& P) {
+  return (size_t)5 * P.levels + 4 + (P.heights + 1) + 4 * (size_t)P.gl_cols + 3 * P.gl_fin + 1 +
+         P.gl_edges;
+}
+template <class T>
+NRT_HD GlLayout gl_layout(const TreePlanDev& P, bool top) {
+  GlLayout o;
+  o.top = top ? gl_round16(top_lds_bytes<T>(gl_top_entries(P))) : 0;
+  o.values = gl_round16((size_t)10 * P.lds_nodes * sizeof(T));
+  o.meta = gl_round16(gl_meta_ints(P) * sizeof(int32_t));
+  o.total = o.top + o.values + o.meta;
+  return o;
+}
+NRT_HD GlMeta gl_meta_carve(void* p, const TreePlanDev& P) {
+  NRT_LDS int32_t* i = (NRT_LDS int32_t*)p;
+  GlMeta M;
+  M.lvl = i;
+  M.col = M.lvl + 5 * P.levels;
+  M.scal = M.col + 4 * P.gl_cols;
+  M.hoff = M.scal + 4;
+  M.fpar = M.hoff + P.heights + 1;
+  M.fu = M.fpar + P.gl_fin;
+  M.fcp = M.fu + P.gl_fin;
+  M.fch = M.fcp + P.gl_fin + 1;
+  return M;
+}
+
+// Stage group g's metadata; workers tid of nt (all entries once the caller's
+// workers have run and synchronised).
+NRT_HD void gl_meta_stage(const TreePlanDev& P, int g, const GlMeta& M, int tid, int nt) {
+  const int G = P.groups, L = P.levels, H = P.heights;
+  int mb = 0;
+  for (int lv = 0; lv < L; lv++) {
+    const int c0 = P.gcol[lv * (G + 1) + g], nc = P.gcol[lv * (G + 1) + g + 1] - c0;
+    const int NC = P.nchunk[lv], slot0 = P.node_off[lv];
+    if (tid == 0) {
+      M.lvl[5 * lv] = c0;
+      M.lvl[5 * lv + 1] = nc;
+      M.lvl[5 * lv + 2] = lv > 0 ? P.lbase[g * (L + 2) + lv] : 0;
+      M.lvl[5 * lv + 3] = mb;
+      M.lvl[5 * lv + 4] = NC;
+    }
+    for (int j = tid; j < nc; j += nt) {
+      const int ch = P.chunk_off[lv] + c0 + j, m = P.chunk_len[ch];
+      const int slot = slot0 + c0 + j;
+      M.col[4 * (mb + j)] = m;
+      M.col[4 * (mb + j) + 1] = P.lnext[slot];
+      M.col[4 * (mb + j) + 2] = m >= 2 ? P.lnext[slot + (m - 1) * NC] : P.lnext[slot];
+      M.col[4 * (mb + j) + 3] = P.chunk_dyn[ch];
+    }
+    mb += nc;
+  }
+  const int k0 = P.loc_hoff[g * H], nf = P.loc_hoff[g * H + H] - k0;
+  if (tid == 0) {
+    M.scal[0] = P.node_off[L];
+    M.scal[1] = P.lbase[g * (L + 2) + L];
+    M.scal[2] = nf;
+    M.scal[3] = 0;
+  }
+  for (int h = tid; h <= H; h += nt) M.hoff[h] = P.loc_hoff[g * H + h] - k0;
+  const int e0 = P.lf_cptr[k0];
+  for (int j = tid; j < nf; j += nt) {
+    M.fpar[j] = P.lf_par[k0 + j];
+    M.fu[j] = P.loc_order[k0 + j];
+    M.fcp[j] = P.lf_cptr[k0 + j] - e0;
+  }
+  if (tid == 0) M.fcp[nf] = P.lf_cptr[k0 + nf] - e0;
+  for (int q = tid; q < P.lf_cptr[k0 + nf] - e0; q += nt) M.fch[q] = P.lf_child[e0 + q];
+}
+
+// Group column j of level lv from the staged metadata (level 0 rows in w, the
+// rows above in lw): eliminated in full (how == kGlFull), in full when its values
+// change between steps (kGlStep), or with its kept factors (kGlStatic).
+enum { kGlFull = 0, kGlStep = 1, kGlStatic = 2 };
+template <bool L0, class T>
+NRT_HD void glm_reduce(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
+                       int j, int how) {
+  const int c0 = M.lvl[5 * lv], nc = M.lvl[5 * lv + 1];
+  const int q = 4 * (M.lvl[5 * lv + 3] + j);
+  const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
+  const bool full = how == kGlFull || (how == kGlStep && M.col[q + 3]);
+  const EndsLocal<T> E{lw, w, M.scal[0]};
+  if (L0) {
+    const int NC = M.lvl[5 * lv + 4];
+    if (full) chunk_reduce(w, c0 + j, NC, m, es, ee, E);
+    else chunk_reduce_static(w, c0 + j, NC, m, es, ee, E);
+  } else {
+    const int base = M.lvl[5 * lv + 2] + j;
+    if (full) chunk_reduce(lw, base, nc, m, es, ee, E);
+    else chunk_reduce_static(lw, base, nc, m, es, ee, E);
+  }
+}
+
+template <bool L0, int MODE, class T>
+NRT_HD void glm_back(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
+                     int j, double* out) {
+  const int c0 = M.lvl[5 * lv], nc = M.lvl[5 * lv + 1];
+  const int q = 4 * (M.lvl[5 * lv + 3] + j);
+  const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
+  const EndsLocal<T> E{lw, w, M.scal[0]};
+  if (L0) chunk_back<MODE>(w, c0 + j, M.lvl[5 * lv + 4], m, es, ee, E, out);
+  else chunk_back<OUT_LEVEL>(lw, M.lvl[5 * lv + 2] + j, nc, m, es, ee, E, out);
+}
+
+// Local final node j of the group (group-relative, in height order).
+template <class T>
+NRT_HD void glm_final_up(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int j) {
+  const int li = M.scal[1] + j;
+  T dv = lw.d[li], rv = lw.r[li];
+  for (int q = M.fcp[j]; q < M.fcp[j + 1]; q++) {
+    const int c = M.fch[q];
+    const T f = lw.dn[c] * tps_rcp(lw.ca[c]);
+    dv = dv - f * lw.up[c];
+    rv = rv - f * lw.r[c];
+  }
+  lw.ca[li] = dv;
+  lw.r[li] = rv;
+  if (M.fpar[j] <= -2) {  // under the top: hand over to the global arrays
+    const int t = M.scal[0] + M.fu[j];
+    w.ca[t] = dv;
+    w.r[t] = rv;
+    w.up[t] = lw.up[li];
+    w.dn[t] = lw.dn[li];
+  }
+}
+
+template <class T>
+NRT_HD void glm_final_down(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
+                           int j) {
+  const int li = M.scal[1] + j, p = M.fpar[j];
+  T acc = lw.r[li];
+  if (p != -1) {
+    const T xp = p >= 0 ? lw.x[p] : w.x[M.scal[0] - 2 - p];
+    acc = acc - lw.up[li] * xp;
+  }
+  lw.x[li] = acc * tps_rcp(lw.ca[li]);
+}
+
+// Local arrays of n entries each, carved from one buffer (LDS on the device).
+template <class T>
+NRT_HD TreeWorkL<T> gl_carve(void* p, int n) {
+  NRT_LDS T* t = (NRT_LDS T*)p;
+  return TreeWorkL<T>{t,         t + n,     t + 2 * n, t + 3 * n, t + 4 * n,
+                      t + 5 * n, t + 6 * n, t + 7 * n, t + 8 * n, t + 9 * n};
 }
 
 }  // namespace nrt

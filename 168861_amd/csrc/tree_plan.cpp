@@ -22,10 +22,60 @@ std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* co
   return par;
 }
 
+// The top of the final forest for one workgroup (see TreePlanHost::top_*):
+// the nodes of `is_top` (closed upwards) in height order, then their children
+// outside the set. The lists are always built (the top can be solved from global
+// memory); top_lds_ok says whether they fit kTopLdsNodes LDS entries.
+void stage_top_set(TreePlanHost& P, const std::vector<uint8_t>& is_top) {
+  const int H = P.heights, n = (int)P.fin_par.size();
+  std::vector<int32_t> loc(n, -1), node;
+  int h0 = -1;
+  P.top_hoff.clear();
+  for (int k = 0; k < H; k++) {
+    const int before = (int)node.size();
+    for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
+      const int u = P.fin_order[q];
+      if (!is_top[u]) continue;
+      loc[u] = (int)node.size();
+      node.push_back(u);
+    }
+    if (h0 < 0 && (int)node.size() > before) h0 = k;
+    if (h0 >= 0) P.top_hoff.push_back(before);
+  }
+  const int nt = (int)node.size();
+  if (h0 >= 0) P.top_hoff.push_back(nt);
+  P.fin_cut = h0 < 0 ? H : h0;  // heights from here hold the top (and maybe local nodes)
+  for (int j = 0; j < nt; j++) {
+    const int v = node[j];
+    for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) {
+      const int c = P.fin_child[k];
+      if (loc[c] < 0) {
+        loc[c] = (int)node.size();
+        node.push_back(c);
+      }
+    }
+  }
+  P.top_n = nt;
+  P.top_ext = (int)node.size() - nt;
+  P.top_node = node;
+  P.top_cptr.assign(1, 0);
+  P.top_child.clear();
+  P.top_par.assign(nt, -1);
+  for (int j = 0; j < nt; j++) {
+    const int v = node[j];
+    for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++)
+      P.top_child.push_back(loc[P.fin_child[k]]);
+    P.top_cptr.push_back((int32_t)P.top_child.size());
+    if (P.fin_par[v] >= 0) P.top_par[j] = loc[P.fin_par[v]];
+  }
+  P.top_lds_ok = nt > 0 && (int)node.size() <= kTopLdsNodes ? 1 : 0;
+}
+
 namespace {
 struct LocalLevel {
   std::vector<int32_t> par;                 // level-local parents
   std::vector<uint8_t> dyn;                 // node values change between steps
+  std::vector<int32_t> grp;                 // group of each node (local mode)
   std::vector<int32_t> cs, cl;              // chunks (local start, length)
   std::vector<uint8_t> cdyn;                // chunk holds a changing node
   std::vector<int32_t> pos;                 // chunk -> slot column
@@ -36,7 +86,8 @@ struct LocalLevel {
 }  // namespace
 
 TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int chunk_hi,
-                             const std::vector<uint8_t>* dyn_rows) {
+                             const std::vector<uint8_t>* dyn_rows,
+                             const std::vector<int32_t>* groups) {
   TreePlanHost P;
   P.C = std::max(3, chunk);
   P.C_hi = std::max(3, chunk_hi);
@@ -50,10 +101,22 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
     if ((int)dyn_rows->size() != N0) throw std::runtime_error("tree plan: dyn_rows size");
     dyn = *dyn_rows;
   }
+  std::vector<int32_t> grp(N0, 0);
+  int G = 0;
+  if (groups) {
+    if ((int)groups->size() != N0) throw std::runtime_error("tree plan: groups size");
+    grp = *groups;
+    for (int g : grp) {
+      if (g < 0) throw std::runtime_error("tree plan: negative group");
+      G = std::max(G, g + 1);
+    }
+  }
+  P.groups = G;
   while (true) {
     LocalLevel L;
     L.par = par;
     L.dyn = dyn;
+    L.grp = grp;
     const int n = (int)par.size();
     std::vector<int32_t> nchild(n, 0);
     for (int v = 0; v < n; v++)
@@ -63,7 +126,8 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
     int v = 0;
     while (v < n) {
       int b = v;
-      while (b + 1 < n && par[b + 1] == b && nchild[b] == 1) b++;
+      // a chain: consecutive single-child links, broken where the group changes
+      while (b + 1 < n && par[b + 1] == b && nchild[b] == 1 && grp[b + 1] == grp[b]) b++;
       for (int a = v; a <= b; a += C) {
         const int m = std::min(C, b - a + 1);
         L.cs.push_back(a);
@@ -97,6 +161,13 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
       if (m >= 2) ndyn[L.next[a + m - 1]] = f;
     }
     dyn.swap(ndyn);
+    std::vector<int32_t> ngrp(cnt, 0);
+    for (size_t k = 0; k < L.cs.size(); k++) {
+      const int a = L.cs[k], m = L.cl[k];
+      ngrp[L.next[a]] = grp[a];
+      if (m >= 2) ngrp[L.next[a + m - 1]] = grp[a];
+    }
+    grp.swap(ngrp);
     std::vector<int32_t> npar(cnt, -1);
     for (size_t k = 0; k < L.cs.size(); k++) {
       const int a = L.cs[k], m = L.cl[k];
@@ -122,13 +193,22 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
     } else {
       L.nchunk = (int)L.cs.size();
       for (int m : L.cl) L.cmax = std::max(L.cmax, m);
-      // slot columns: dynamic chunks first, then longest first (stable)
+      // slot columns: by group (local mode), then dynamic chunks first, then
+      // longest first (stable)
       std::vector<int32_t> order(L.nchunk);
       for (int c = 0; c < L.nchunk; c++) order[c] = c;
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const int ga = L.grp[L.cs[a]], gb = L.grp[L.cs[b]];
+        if (ga != gb) return ga < gb;
         if (L.cdyn[a] != L.cdyn[b]) return L.cdyn[a] > L.cdyn[b];
         return L.cl[a] > L.cl[b];
       });
+      if (G > 0) {
+        std::vector<int32_t> cnt(G + 1, 0);
+        for (int c = 0; c < L.nchunk; c++) cnt[L.grp[L.cs[c]] + 1]++;
+        for (int g = 0; g < G; g++) cnt[g + 1] += cnt[g];
+        P.gcol.insert(P.gcol.end(), cnt.begin(), cnt.end());
+      }
       L.pos.assign(L.nchunk, 0);
       for (int p = 0; p < L.nchunk; p++) L.pos[order[p]] = p;
       for (int c = 0; c < L.nchunk; c++)
@@ -191,6 +271,90 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
   for (int u = 0; u < n; u++)
     if (fp[u] >= 0) P.fin_child[pos[fp[u]]++] = u;
   P.nchunk.push_back(0);  // keeps nchunk[levels] addressable
+  if (G > 0) {
+    // local final nodes: the whole subtree in the node's group (children have
+    // larger indices, so one pass from the end settles every node)
+    const std::vector<int32_t>& fg = lv[nl - 1].grp;
+    std::vector<uint8_t> mixed(n, 0);
+    for (int u = n - 1; u >= 0; u--)
+      if (fp[u] >= 0 && (mixed[u] || fg[u] != fg[fp[u]])) mixed[fp[u]] = 1;
+    P.loc_hoff.assign((size_t)G * H + 1, 0);
+    for (int u = 0; u < n; u++)
+      if (!mixed[u]) P.loc_hoff[(size_t)fg[u] * H + h[u] + 1]++;
+    for (size_t i = 0; i + 1 < P.loc_hoff.size(); i++) P.loc_hoff[i + 1] += P.loc_hoff[i];
+    P.loc_order.assign(P.loc_hoff.back(), 0);
+    std::vector<int32_t> lf(P.loc_hoff.begin(), P.loc_hoff.end() - 1);
+    for (int k = 0; k < H; k++)
+      for (int q = P.fin_hoff[k]; q < P.fin_hoff[k + 1]; q++) {
+        const int u = P.fin_order[q];
+        if (!mixed[u]) P.loc_order[lf[(size_t)fg[u] * H + h[u]]++] = u;
+      }
+    stage_top_set(P, mixed);
+    // group-resident indices
+    const int L = P.levels;
+    P.lbase.assign((size_t)G * (L + 2), 0);
+    std::vector<int32_t> lfin(n, -1);
+    for (int g = 0; g < G; g++) {
+      int32_t off = 0;
+      for (int l = 1; l < L; l++) {
+        const int32_t* gc = P.gcol.data() + (size_t)l * (G + 1);
+        int cmax = 0;
+        for (int c = gc[g]; c < gc[g + 1]; c++)
+          cmax = std::max(cmax, P.chunk_len[P.chunk_off[l] + c]);
+        P.lbase[(size_t)g * (L + 2) + l] = off;
+        off += cmax * (gc[g + 1] - gc[g]);
+      }
+      P.lbase[(size_t)g * (L + 2) + L] = off;
+      const int k0 = P.loc_hoff[(size_t)g * H], k1 = P.loc_hoff[(size_t)g * H + H];
+      for (int k = k0; k < k1; k++) lfin[P.loc_order[k]] = off + (k - k0);
+      off += k1 - k0;
+      P.lbase[(size_t)g * (L + 2) + L + 1] = off;
+      P.lds_nodes = std::max<int>(P.lds_nodes, off);
+      int cols = 0;
+      for (int l = 0; l < L; l++)
+        cols += P.gcol[(size_t)l * (G + 1) + g + 1] - P.gcol[(size_t)l * (G + 1) + g];
+      P.gl_cols = std::max(P.gl_cols, cols);
+      P.gl_fin = std::max(P.gl_fin, k1 - k0);
+    }
+    auto final_code = [&](int u) { return mixed[u] ? -2 - u : lfin[u]; };
+    P.lnext.assign(P.node_off[L], -1);
+    for (int l = 0; l < L; l++) {
+      const int NC = P.nchunk[l];
+      const int32_t* gc = P.gcol.data() + (size_t)l * (G + 1);
+      for (int g = 0; g < G; g++)
+        for (int c = gc[g]; c < gc[g + 1]; c++) {
+          const int m = P.chunk_len[P.chunk_off[l] + c];
+          for (int e : {0, m - 1}) {
+            const int slot = P.node_off[l] + c + e * NC, t = P.next[slot];
+            int code;
+            if (l + 1 < L) {
+              const int NCn = P.nchunk[l + 1];
+              const int32_t* gn = P.gcol.data() + (size_t)(l + 1) * (G + 1);
+              code = P.lbase[(size_t)g * (L + 2) + l + 1] + (t / NCn) * (gn[g + 1] - gn[g]) +
+                     (t % NCn - gn[g]);
+            } else {
+              code = final_code(t);
+            }
+            P.lnext[slot] = code;
+          }
+        }
+    }
+    const int nloc = (int)P.loc_order.size();
+    P.lf_cptr.assign(nloc + 1, 0);
+    P.lf_child.clear();
+    P.lf_par.assign(nloc, -1);
+    for (int k = 0; k < nloc; k++) {
+      const int u = P.loc_order[k];
+      for (int q = P.fin_cptr[u]; q < P.fin_cptr[u + 1]; q++)
+        P.lf_child.push_back(lfin[P.fin_child[q]]);
+      P.lf_cptr[k + 1] = (int32_t)P.lf_child.size();
+      if (fp[u] >= 0) P.lf_par[k] = final_code(fp[u]);
+    }
+    for (int g = 0; g < G; g++) {
+      const int k0 = P.loc_hoff[(size_t)g * H], k1 = P.loc_hoff[(size_t)g * H + H];
+      P.gl_edges = std::max(P.gl_edges, P.lf_cptr[k1] - P.lf_cptr[k0]);
+    }
+  }
   return P;
 }
 
@@ -202,7 +366,9 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
                                          &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
                                          &P.fin_cptr,  &P.fin_child, &P.sub_order, &P.sub_hoff,
                                          &P.top_node,  &P.top_hoff,  &P.top_cptr,  &P.top_child,
-                                         &P.top_par};
+                                         &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
+                                         &P.lbase,     &P.lnext,     &P.lf_cptr,   &P.lf_child,
+                                         &P.lf_par};
   constexpr int kParts = sizeof(parts) / sizeof(parts[0]);
   size_t off[kParts];
   size_t pos = 0;
@@ -215,51 +381,35 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
     pos += parts[k]->size() + 1;
   }
   return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
-                     P.top_n,        P.top_ext,      base + off[0],  base + off[1],
-                     base + off[2],  base + off[3],  base + off[4],  base + off[5],
-                     base + off[6],  base + off[7],  base + off[8],  base + off[9],
-                     base + off[10], base + off[11], base + off[12], base + off[13],
-                     base + off[14], base + off[15], base + off[16], base + off[17],
-                     base + off[18], base + off[19], base + off[20]};
+                     P.top_n,        P.top_ext,      P.groups,       P.top_lds_ok,
+                     P.lds_nodes,    P.gl_cols,      P.gl_fin,       P.gl_edges,
+                     base + off[0],  base + off[1],  base + off[2],  base + off[3],
+                     base + off[4],  base + off[5],  base + off[6],  base + off[7],
+                     base + off[8],  base + off[9],  base + off[10], base + off[11],
+                     base + off[12], base + off[13], base + off[14], base + off[15],
+                     base + off[16], base + off[17], base + off[18], base + off[19],
+                     base + off[20], base + off[21], base + off[22], base + off[23],
+                     base + off[24], base + off[25], base + off[26], base + off[27],
+                     base + off[28]};
 }
 
-// Local lists of the heights >= cut (see TreePlanHost::top_*); false when they
-// do not fit in kTopLdsNodes entries.
-static bool stage_final_top(TreePlanHost& P, int cut) {
+// Top lists of the heights >= cut (see TreePlanHost::top_*), kept only when they
+// fit in kTopLdsNodes LDS entries (a grid launch solves an unstaged top from the
+// plan's global lists).
+static void stage_final_top(TreePlanHost& P, int cut) {
   const int H = P.heights, n = (int)P.fin_par.size();
-  const int t0 = P.fin_hoff[cut], nt = P.fin_hoff[H] - t0;
-  std::vector<int32_t> loc(n, -1);
-  std::vector<int32_t> node;
-  for (int j = 0; j < nt; j++) {
-    loc[P.fin_order[t0 + j]] = j;
-    node.push_back(P.fin_order[t0 + j]);
+  std::vector<uint8_t> is_top(n, 0);
+  for (int q = P.fin_hoff[cut]; q < P.fin_hoff[H]; q++) is_top[P.fin_order[q]] = 1;
+  stage_top_set(P, is_top);
+  P.fin_cut = cut;
+  if (!P.top_lds_ok) {
+    P.top_n = P.top_ext = 0;
+    P.top_node.clear();
+    P.top_hoff.clear();
+    P.top_cptr.clear();
+    P.top_child.clear();
+    P.top_par.clear();
   }
-  for (int j = 0; j < nt; j++) {
-    const int v = node[j];
-    for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) {
-      const int c = P.fin_child[k];
-      if (loc[c] < 0) {
-        loc[c] = (int)node.size();
-        node.push_back(c);
-      }
-    }
-  }
-  if (nt == 0 || (int)node.size() > kTopLdsNodes) return false;
-  P.top_n = nt;
-  P.top_ext = (int)node.size() - nt;
-  P.top_node = node;
-  P.top_hoff.clear();
-  for (int h = cut; h <= H; h++) P.top_hoff.push_back(P.fin_hoff[h] - t0);
-  P.top_cptr.assign(1, 0);
-  P.top_child.clear();
-  P.top_par.assign(nt, -1);
-  for (int j = 0; j < nt; j++) {
-    const int v = node[j];
-    for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) P.top_child.push_back(loc[P.fin_child[k]]);
-    P.top_cptr.push_back((int32_t)P.top_child.size());
-    if (P.fin_par[v] >= 0) P.top_par[j] = loc[P.fin_par[v]];
-  }
-  return true;
 }
 
 void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage_top) {
@@ -371,6 +521,105 @@ void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage
   if (stage_top) stage_final_top(P, P.fin_cut);
 }
 
+std::vector<int32_t> row_groups(int N, int G) {
+  std::vector<int32_t> g(N);
+  for (int i = 0; i < N; i++) g[i] = (int32_t)((int64_t)i * G / N);
+  return g;
+}
+
+void check_local_plan(const TreePlanHost& P) {
+  auto need = [](bool ok, const char* what) {
+    if (!ok) throw std::runtime_error(std::string("local plan: ") + what);
+  };
+  const int G = P.groups, H = P.heights, n = (int)P.fin_par.size(), L = P.levels;
+  need(G >= 1 && L >= 1, "needs groups and a reduction level");
+  need((int)P.gcol.size() == L * (G + 1), "gcol size");
+  std::vector<int32_t> hgt(n);
+  for (int h = 0; h < H; h++)
+    for (int q = P.fin_hoff[h]; q < P.fin_hoff[h + 1]; q++) hgt[P.fin_order[q]] = h;
+  // final nodes: each once, local (with its group and height) or top
+  std::vector<int32_t> fgrp(n, -1), seen(n, 0);
+  std::vector<uint8_t> top(n, 0);
+  need((int)P.loc_hoff.size() == G * H + 1, "loc_hoff size");
+  for (int g = 0; g < G; g++)
+    for (int h = 0; h < H; h++)
+      for (int k = P.loc_hoff[g * H + h]; k < P.loc_hoff[g * H + h + 1]; k++) {
+        const int u = P.loc_order[k];
+        need(hgt[u] == h, "local node listed at a wrong height");
+        fgrp[u] = g;
+        seen[u]++;
+      }
+  for (int j = 0; j < P.top_n; j++) {
+    top[P.top_node[j]] = 1;
+    seen[P.top_node[j]]++;
+  }
+  for (int u = 0; u < n; u++) need(seen[u] == 1, "final node placed twice or never");
+  for (int u = 0; u < n; u++) {
+    const int p = P.fin_par[u];
+    if (top[u]) need(p < 0 || top[p], "top not closed upwards");
+    else
+      for (int k = P.fin_cptr[u]; k < P.fin_cptr[u + 1]; k++)
+        need(!top[P.fin_child[k]] && fgrp[P.fin_child[k]] == fgrp[u],
+             "a local node's child outside its group");
+  }
+  // group-resident indices: every chunk row and end, every local final node and
+  // its links inside the group's range [0, lbase[g][L+1]), top codes on top nodes
+  const bool gl = !P.lbase.empty();
+  auto lcount = [&](int g) { return P.lbase[(size_t)g * (L + 2) + L + 1]; };
+  if (gl) {
+    need((int)P.lbase.size() == G * (L + 2) && (int)P.lnext.size() == P.node_off[L], "lbase size");
+    for (int g = 0; g < G; g++) {
+      need(lcount(g) <= P.lds_nodes, "group count above lds_nodes");
+      for (int l = 1; l <= L; l++)
+        need(P.lbase[(size_t)g * (L + 2) + l] <= P.lbase[(size_t)g * (L + 2) + l + 1],
+             "lbase order");
+      const int k0 = P.loc_hoff[(size_t)g * H], k1 = P.loc_hoff[(size_t)g * H + H];
+      need(P.lbase[(size_t)g * (L + 2) + L] + (k1 - k0) == lcount(g), "local final block size");
+      for (int k = k0; k < k1; k++) {
+        for (int q = P.lf_cptr[k]; q < P.lf_cptr[k + 1]; q++)
+          need(P.lf_child[q] >= P.lbase[(size_t)g * (L + 2) + L] && P.lf_child[q] < lcount(g),
+               "local child index out of its group's final block");
+        const int p = P.lf_par[k];
+        need(p == -1 || (p >= 0 && p < lcount(g)) || (p <= -2 && -2 - p < n && top[-2 - p]),
+             "local parent code");
+      }
+    }
+  }
+  // chunks: columns by group; both ends of a chunk go to its group (or the top)
+  for (int lv = 0; lv < L; lv++) {
+    const int NC = P.nchunk[lv], NCn = P.nchunk[lv + 1];
+    const int32_t* gc = P.gcol.data() + (size_t)lv * (G + 1);
+    need(gc[0] == 0 && gc[G] == NC, "gcol bounds");
+    for (int g = 0; g < G; g++) {
+      need(gc[g] <= gc[g + 1], "gcol order");
+      for (int c = gc[g]; c < gc[g + 1]; c++) {
+        const int m = P.chunk_len[P.chunk_off[lv] + c];
+        for (int e : {0, m - 1}) {
+          const int t = P.next[P.node_off[lv] + c + e * NC];
+          need(t >= 0, "chunk end without a next slot");
+          if (lv + 1 < L) {
+            const int32_t* gn = P.gcol.data() + (size_t)(lv + 1) * (G + 1);
+            const int cn = t % NCn;
+            need(cn >= gn[g] && cn < gn[g + 1], "chunk end in another group's column");
+          } else {
+            need(top[t] || fgrp[t] == g, "chunk end in another group's final nodes");
+          }
+          if (gl) {
+            const int code = P.lnext[P.node_off[lv] + c + e * NC];
+            need((code >= 0 && code < lcount(g)) || (code <= -2 && -2 - code < n && top[-2 - code]),
+                 "chunk end code");
+          }
+        }
+        if (gl && lv >= 1) {
+          const int base = P.lbase[(size_t)g * (L + 2) + lv] + (c - gc[g]);
+          const int last = base + (m - 1) * (gc[g + 1] - gc[g]);
+          need(base >= 0 && last < P.lbase[(size_t)g * (L + 2) + lv + 1], "chunk rows out of level");
+        }
+      }
+    }
+  }
+}
+
 }  // namespace nrt
 
 using namespace nrt;
@@ -388,10 +637,19 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
     std::vector<int32_t> par = csr_parents(N, rowptr, col);
     std::vector<uint8_t> dyn;
     if (dyn_rows) dyn.assign(dyn_rows, dyn_rows + N);
+    // groups > 0: a grid launch's split of the final forest over that many
+    // workgroups; groups < 0: local mode with -groups row groups, or with
+    // -groups - 4096 groups in the group-resident form (a host buffer per group
+    // stands in for its LDS)
+    const bool local = groups < 0;
+    const bool resident = groups <= -4096;
+    std::vector<int32_t> rgrp;
+    if (local) rgrp = row_groups(N, resident ? -groups - 4096 : -groups);
     TreePlanHost H = build_tree_plan(par, chunk, chunk_hi > 0 ? chunk_hi : chunk,
-                                     dyn_rows ? &dyn : nullptr);
-    partition_final_forest(H, groups, 256, groups > 0);
-    {
+                                     dyn_rows ? &dyn : nullptr, local ? &rgrp : nullptr);
+    if (local) check_local_plan(H);
+    else partition_final_forest(H, groups, 256, groups > 0);
+    if (!local) {
       // the split's promise to the device: a subtree node's children sit in its own
       // group at lower heights, and every node is placed exactly once
       const int n = (int)H.fin_par.size(), cut = H.fin_cut, G = H.sub_groups;
@@ -444,6 +702,45 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
     // with reuse, two solves in a row (as consecutive steps do): the second must
     // not depend on what the first left behind
     std::vector<double> xs(H.nslots[0], 0.0);  // the solution in level-0 slot order
+    // local mode: the top of the final forest (LDS-staged when it fits, with host
+    // memory standing in for LDS)
+    auto solve_top = [&]() {
+      if (H.top_n > 0 && H.top_lds_ok) {
+        std::vector<unsigned char> lds(top_lds_bytes<double>(gl_top_entries(P)));
+        const TopLds<double> L = TopLds<double>::carve(lds.data(), gl_top_entries(P));
+        for (int j = 0; j < top_stage_count(P); j++) top_stage(P, w, L, j);
+        const int nh = H.heights - H.fin_cut;
+        for (int h = 0; h < nh; h++)
+          for (int j = L.hoff[h]; j < L.hoff[h + 1]; j++) top_up(L, j);
+        for (int h = nh - 1; h >= 0; h--)
+          for (int j = L.hoff[h]; j < L.hoff[h + 1]; j++) top_down(L, j);
+        for (int j = 0; j < H.top_n; j++) top_store(P, w, L, j);
+      } else if (H.top_n > 0) {
+        const int nh = H.heights - H.fin_cut;
+        for (int h = 0; h < nh; h++)
+          for (int j = H.top_hoff[h]; j < H.top_hoff[h + 1]; j++) tps_final_up(P, w, H.top_node[j]);
+        for (int h = nh - 1; h >= 0; h--)
+          for (int j = H.top_hoff[h]; j < H.top_hoff[h + 1]; j++)
+            tps_final_down(P, w, H.top_node[j]);
+      }
+    };
+    // group-resident form: every group's values and staged metadata in a buffer
+    // of its own, laid out as the launch lays out its LDS
+    std::vector<std::vector<double>> lbuf;
+    const GlLayout lay = gl_layout<double>(P, false);
+    auto gl_view = [&](int g, TreeWorkL<double>& lw, GlMeta& M) {
+      unsigned char* b = (unsigned char*)lbuf[g].data();
+      lw = gl_carve<double>(b + lay.top, H.lds_nodes);
+      M = gl_meta_carve(b + lay.top + lay.values, P);
+    };
+    auto gl_levels = [&](int g, int how) {
+      TreeWorkL<double> lw;
+      GlMeta M;
+      gl_view(g, lw, M);
+      for (int j = 0; j < M.lvl[1]; j++) glm_reduce<true>(w, lw, M, 0, j, how);
+      for (int lv = 1; lv < H.levels; lv++)
+        for (int j = 0; j < M.lvl[5 * lv + 1]; j++) glm_reduce<false>(w, lw, M, lv, j, how);
+    };
     for (int pass = 0; pass < (reuse ? 2 : 1); pass++) {
       for (int i = 0; i < N; i++) w.r[s0[i]] = rhs[i];
       for (int lv = 0; lv < H.levels; lv++)
@@ -451,6 +748,75 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           if (reuse && !H.chunk_dyn[ch]) tps_reduce_chunk_static(P, w, lv, ch);
           else tps_reduce_chunk(P, w, lv, ch);
         }
+      if (resident) {
+        const int G = H.groups, L = H.levels, Hh = H.heights;
+        if (lbuf.empty()) {
+          lbuf.assign(G, std::vector<double>(lay.total / sizeof(double) + 1));
+          for (int g = 0; g < G; g++) {
+            TreeWorkL<double> lw;
+            GlMeta M;
+            gl_view(g, lw, M);
+            gl_meta_stage(P, g, M, 0, 1);
+          }
+          // factors first (with another right-hand side when reusing them)
+          for (int i = 0; i < N; i++) w.r[s0[i]] = rhs_factor ? rhs_factor[i] : 0.0;
+          for (int g = 0; g < G; g++) gl_levels(g, kGlFull);
+          for (int i = 0; i < N; i++) w.r[s0[i]] = rhs[i];
+        }
+        for (int g = G - 1; g >= 0; g--) {
+          gl_levels(g, reuse ? kGlStep : kGlFull);
+          TreeWorkL<double> lw;
+          GlMeta M;
+          gl_view(g, lw, M);
+          for (int h = 0; h < Hh; h++)
+            for (int j = M.hoff[h]; j < M.hoff[h + 1]; j++) glm_final_up(w, lw, M, j);
+        }
+        solve_top();
+        for (int g = 0; g < G; g++) {
+          TreeWorkL<double> lw;
+          GlMeta M;
+          gl_view(g, lw, M);
+          for (int h = Hh - 1; h >= 0; h--)
+            for (int j = M.hoff[h]; j < M.hoff[h + 1]; j++) glm_final_down(w, lw, M, j);
+          for (int lv = L - 1; lv >= 1; lv--)
+            for (int j = 0; j < M.lvl[5 * lv + 1]; j++)
+              glm_back<false, OUT_LEVEL>(w, lw, M, lv, j, nullptr);
+          for (int j = 0; j < M.lvl[1]; j++) glm_back<true, OUT_ASSIGN>(w, lw, M, 0, j, xs.data());
+        }
+        continue;
+      }
+      if (local) {
+        // local mode in the order of its launch: every group alone up to its local
+        // final nodes (here last group first: no group may need another's work),
+        // the top, then every group back down
+        const int G = H.groups, Hh = H.heights;
+        for (int g = G - 1; g >= 0; g--) {
+          for (int lv = 0; lv < H.levels; lv++) {
+            const int* gc = H.gcol.data() + (size_t)lv * (G + 1);
+            for (int ch = H.chunk_off[lv] + gc[g]; ch < H.chunk_off[lv] + gc[g + 1]; ch++) {
+              if (reuse && !H.chunk_dyn[ch]) tps_reduce_chunk_static(P, w, lv, ch);
+              else tps_reduce_chunk(P, w, lv, ch);
+            }
+          }
+          for (int h = 0; h < Hh; h++)
+            for (int k = H.loc_hoff[g * Hh + h]; k < H.loc_hoff[g * Hh + h + 1]; k++)
+              tps_final_up(P, w, H.loc_order[k]);
+        }
+        solve_top();
+        for (int g = 0; g < G; g++) {
+          for (int h = Hh - 1; h >= 0; h--)
+            for (int k = H.loc_hoff[g * Hh + h]; k < H.loc_hoff[g * Hh + h + 1]; k++)
+              tps_final_down(P, w, H.loc_order[k]);
+          for (int lv = H.levels - 1; lv >= 0; lv--) {
+            const int* gc = H.gcol.data() + (size_t)lv * (G + 1);
+            for (int ch = H.chunk_off[lv] + gc[g]; ch < H.chunk_off[lv] + gc[g + 1]; ch++) {
+              if (lv > 0) tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr);
+              else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
+            }
+          }
+        }
+        continue;
+      }
       // final forest in the order of a grid launch: the groups' subtrees below the
       // cut, then the heights above it, then back down
       const int cut = H.fin_cut, G = H.sub_groups;
@@ -494,7 +860,10 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
     for (int i = 0; i < N; i++) x[i] = xs[s0[i]];
     if (levels) *levels = H.levels;
     if (heights) *heights = H.heights;
-    if (cut) *cut = H.fin_cut | (H.top_n > 0 ? (1 << 30) : 0);
+    if (cut)
+      *cut = resident ? H.lds_nodes
+           : local  ? H.top_n | (H.top_lds_ok ? (1 << 30) : 0)
+                   : H.fin_cut | (H.top_n > 0 ? (1 << 30) : 0);
     return NRT_OK;
   } catch (const std::exception&) {
     return NRT_ERR_TOPOLOGY;

@@ -85,7 +85,7 @@ class Options(C.Structure):
                 ("num_blocks", C.c_int32), ("tol", C.c_double), ("refine_tol", C.c_double),
                 ("refine_max", C.c_int32), ("precond", C.c_int32), ("chunk", C.c_int32),
                 ("chunk_hi", C.c_int32), ("batch_threads", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("tree_groups", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Stats(C.Structure):

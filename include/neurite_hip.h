@@ -187,7 +187,13 @@ typedef struct nrt_options {
                            (0 -> chunk if set, else auto)                        */
   int32_t batch_threads;/* batch (implicit): workgroup size per tree, 256, 512
                            or 1024 (0 -> autotuned on the first launches)        */
-  int32_t reserved[3];
+  int32_t tree_groups;  /* tree solve, local mode: this many workgroups of 512
+                           threads, each owning a contiguous share of the rows
+                           (its chunks and final nodes), two grid barriers per
+                           solve (0 -> auto: FP64 solve from 16384 rows with
+                           num_blocks 0, N/5000 groups up to 32; 1 -> off;
+                           ignored in a batch)                                    */
+  int32_t reserved[2];
 } nrt_options;
 
 typedef struct nrt_stats {

@@ -98,3 +98,62 @@ def test_tree_large_grid_mode(form, nrt, orc):
     ref, _ = _exact(orc, prob, 40, 1, 20)
     tol = TOL_V if form == "fp64" else TOL_MIXED
     assert max_abs_diff(v, ref, ghost_mask(prob)) <= tol
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+@pytest.mark.parametrize("groups,chunk", [(2, 0), (8, 0), (8, 32), (16, 0), (37, 0)])
+def test_tree_local_mode_config2(form, groups, chunk, nrt, orc):
+    """Config 2 in local mode: one workgroup per row group, the top of the final
+    forest (a chain across the group boundaries) solved between two grid barriers."""
+    geom, run = wl.myelinated_axon()
+    prob, _ = nrt.setup(geom, dT=run.dT)
+    a0, a1 = nrt.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
+    steps = 300
+    _say(f"tree {form} config2 local groups={groups} chunk={chunk}")
+    probe = 1011
+    with nrt.Solver(prob, kind=nh.IMPLICIT, tree_groups=groups, chunk=chunk,
+                    **FORMS[form]) as s:
+        tr = s.run(1, steps, a0, a1, trace_elem=probe, interval=0)
+        v = s.snapshot()
+        st = s.stats()
+    assert st["num_blocks"] == groups
+    ref, rtr = _exact(orc, prob, steps, a0, a1, probe, 0)
+    tol = TOL_V if form == "fp64" else TOL_MIXED
+    assert max_abs_diff(v, ref, ghost_mask(prob)) <= tol
+    assert max_abs_diff(tr, rtr) <= tol
+    assert np.max(rtr) > 0.0
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+@pytest.mark.parametrize("name,dtf", [("ex3_hh_y", 100), ("ex6_ct_tree", 10),
+                                      ("ex8_neuron", 100)])
+def test_tree_local_mode_golden(form, name, dtf, nrt, orc):
+    g = load_golden(name)
+    dT = g["dT_crit"] * dtf
+    p = problem_from_golden(g, dT=dT)
+    a0, a1 = nrt.active_range(dT, 10 ** 9, g["init_time"], g["end_input"])
+    _say(f"tree {form} local groups=3 {name} N={p.N}")
+    with nrt.Solver(p, kind=nh.IMPLICIT, tree_groups=3, chunk=4, **FORMS[form]) as s:
+        s.run(1, 200, a0, a1)
+        v = s.snapshot()
+    ref, _ = _exact(orc, p, 200, a0, a1)
+    tol = TOL_V if form == "fp64" else TOL_MIXED
+    assert max_abs_diff(v, ref, ghost_mask(p)) <= tol
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+@pytest.mark.parametrize("groups", [32, 128])
+def test_tree_local_mode_large(form, groups, nrt, orc):
+    """A depth-12 tree in local mode: 32 groups stage the top in LDS, 128 groups
+    leave a top too large for it (solved from the plan's global lists)."""
+    geom, run = wl.random_tree(wl.seed_for(3), depth=12)
+    prob, _ = nrt.setup(geom, dT=run.dT)
+    _say(f"tree {form} local groups={groups} random depth 12 N={prob.N}")
+    with nrt.Solver(prob, kind=nh.IMPLICIT, tree_groups=groups, **FORMS[form]) as s:
+        s.run(1, 40, 1, 20)
+        v = s.snapshot()
+        st = s.stats()
+    assert st["num_blocks"] == groups
+    ref, _ = _exact(orc, prob, 40, 1, 20)
+    tol = TOL_V if form == "fp64" else TOL_MIXED
+    assert max_abs_diff(v, ref, ghost_mask(prob)) <= tol
