@@ -127,6 +127,7 @@ nrt_handle* handle_create(const nrt_problem* p, const nrt_options& o, hipStream_
     h->opt = o;
     h->device = pick_device(o.device);
     h->N = q.N;
+    h->stats.tree_form = -1;
     NRT_HIP(hipSetDevice(h->device));
     if (shared) {
       h->stream = shared;

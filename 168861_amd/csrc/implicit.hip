@@ -2204,6 +2204,10 @@ static int collect_stats(nrt_handle* hnd) {
   hnd->stats.last_status = st.status;
   hnd->stats.matrix_nnz = e->nnz;
   hnd->stats.num_blocks = e->nblocks;
+  hnd->stats.tree_form = e->method == 0      ? -1
+                         : e->local_mode     ? (e->local_res ? NRT_TREE_LOCAL_LDS : NRT_TREE_LOCAL)
+                         : e->grid_mode      ? NRT_TREE_GRID
+                                             : NRT_TREE_BLOCK;
   hnd->stats.restarts = st.restarts;
   if (err) throw Error(NRT_ERR_HIP, "grid barrier timed out inside the persistent Krylov kernel");
   // a reported failure is returned once, then cleared

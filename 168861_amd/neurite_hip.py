@@ -24,6 +24,11 @@ BICGSTAB, GMRES, DIRECT = 0, 1, 2
 PRECOND_NONE, PRECOND_TREE = 0, 1
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 X0_PREVIOUS, X0_RHS, X0_ZERO = 0, 1, 2
+TREE_BLOCK, TREE_GRID, TREE_LOCAL, TREE_LOCAL_LDS = 0, 1, 2, 3
+# demangled name of the tree-solve kernel of each launch form (nrt_stats.tree_form)
+TREE_KERNELS = {TREE_BLOCK: "k_implicit_tree<{T}, false>", TREE_GRID: "k_implicit_tree<{T}, true>",
+                TREE_LOCAL: "k_implicit_tree_local<{T}, false>",
+                TREE_LOCAL_LDS: "k_implicit_tree_local<{T}, true>"}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -93,7 +98,8 @@ class Stats(C.Structure):
                 ("last_iters", C.c_int32), ("last_status", C.c_int32),
                 ("last_residual", C.c_double), ("last_rhs_norm", C.c_double),
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64),
-                ("matrix_nnz", C.c_int64), ("num_blocks", C.c_int32), ("restarts", C.c_int32)]
+                ("matrix_nnz", C.c_int64), ("num_blocks", C.c_int32), ("restarts", C.c_int32),
+                ("tree_form", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SlotInfo(C.Structure):

@@ -67,6 +67,15 @@ def tree_bytes(N, H, nnz, steps, solves, vbytes):
     return steps * per_step + solves * per_solve + (solves + steps) * per_resid
 
 
+def tree_kernel(nh, name, f):
+    """Name of the kernel a form's launches ran (as rocprofv3 lists it)."""
+    fixed = {"fp64_plain_bicgstab": "k_implicit_bicgstab_fp64",
+             "fp64_gmres30": "k_implicit_gmres<double>", "mixed_gmres30_ir": "k_implicit_gmres<float>"}
+    if name in fixed:
+        return fixed[name]
+    return nh.TREE_KERNELS[f["tree_form"]].format(T="double" if name.startswith("fp64") else "float")
+
+
 def run_form(nh, prob, opts, first, steps, a0, a1):
     s = nh.Solver(prob, kind=nh.IMPLICIT, device=0, **opts)
     if first > 1:
@@ -82,7 +91,7 @@ def run_form(nh, prob, opts, first, steps, a0, a1):
     s.close()
     return {"wall": t1 - t0, "kernel_ms": st1["kernel_ms"],
             "inner": st1["krylov_iters"] - st0["krylov_iters"], "nnz": st1["matrix_nnz"],
-            "blocks": st1["num_blocks"], "v": v,
+            "blocks": st1["num_blocks"], "v": v, "tree_form": st1["tree_form"],
             # GMRES restart cycles (FP32 forms: the refinement sweeps; FP64: the
             # restarts plus one per step)
             "cycles": (st1["refine_sweeps"] - st0["refine_sweeps"]) if st1["refine_sweeps"]
@@ -253,9 +262,7 @@ def dendritic_tree(nh, wl, tree_steps, krylov_steps, max_iter):
                             a1)["v"]
     mask = np.ones(N, bool)
     mask[[0, N - 1]] = False
-    kernels = {"fp64_tree_direct": "k_implicit_tree<double>",
-               "mixed_tree_ir": "k_implicit_tree<float>",
-               "fp64_gmres30": "k_implicit_gmres<double>",
+    kernels = {"fp64_gmres30": "k_implicit_gmres<double>",
                "mixed_gmres30_ir": "k_implicit_gmres<float>"}
     out = {"workload": "seeded binary dendritic tree, depth 14 (16,383 segments of "
                        "U[20,200] um), Rall 3/2 diameters from 4 um, dx 2 um, strain U[0,0.3] "
@@ -283,7 +290,7 @@ def dendritic_tree(nh, wl, tree_steps, krylov_steps, max_iter):
                      "cycles_per_step": f["cycles"] / steps, "workgroups": f["blocks"],
                      "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                                  "kernel": kernels[name], "kernel_ms": f["kernel_ms"],
+                                  "kernel": tree_kernel(nh, name, f), "kernel_ms": f["kernel_ms"],
                                   "algorithmic_bytes": nbytes}}
         if "gmres" not in name:
             del out[name]["cycles_per_step"]
@@ -380,9 +387,7 @@ def main():
     ok = {k: v for k, v in summary.items() if v["max_abs_dV_mV_vs_fp64"] <= TOL_MV}
     best = max(ok, key=lambda k: ok[k]["compartment_steps_per_s"])
     b = summary[best]
-    kernel = {"fp64_tree_direct": "k_implicit_tree<double>",
-              "mixed_tree_ir": "k_implicit_tree<float>",
-              "fp64_plain_bicgstab": "k_implicit_bicgstab_fp64"}[best]
+    kernel = tree_kernel(nh, best, forms[best])
     achieved = b["algorithmic_bytes"] / (b["kernel_ms"] / 1e3) / 1e9
     # HBM bytes measured with rocprofv3 counters for this kernel and workload (per step,
     # calibrated with a copy), scaled to this launch's steps

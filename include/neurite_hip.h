@@ -210,7 +210,16 @@ typedef struct nrt_stats {
   int64_t matrix_nnz;       /* implicit: non-zeros of the assembled CSR         */
   int32_t num_blocks;       /* persistent kernels: workgroups in the grid       */
   int32_t restarts;         /* Krylov breakdown restarts summed over steps      */
+  int32_t tree_form;        /* tree solve launch: NRT_TREE_BLOCK, _GRID, _LOCAL,
+                               _LOCAL_LDS (group-resident), -1 otherwise        */
+  int32_t reserved;
 } nrt_stats;
+
+/* nrt_stats.tree_form */
+#define NRT_TREE_BLOCK      0  /* one workgroup                                  */
+#define NRT_TREE_GRID       1  /* level-synchronous grid (a barrier per level)   */
+#define NRT_TREE_LOCAL      2  /* local mode, levels in global memory            */
+#define NRT_TREE_LOCAL_LDS  3  /* local mode, each group's levels in its LDS     */
 
 typedef struct nrt_handle nrt_handle;
 
