@@ -339,57 +339,84 @@ constexpr int kImpBlock = 256;
 // breakdown restarts allowed per time step before the step reports NOT_CONVERGED
 constexpr int kMaxRestarts = 64;
 
-// HH gating (solverIm-gpu.cpp:61-98) at potential v; returns W and K.
-__device__ __forceinline__ void imp_gate(const ImpHH& hd, int i, double v, double dt_ms,
-                                         double& W, double& K) {
-  const double rest = hd.rest[i];
-  double vna = v + hd.ls_Na[i];
-  double vk = v + hd.ls_K[i];
-  vna = (vna - rest) / 0.001;
-  vk = (vk - rest) / 0.001;
+// HH gating (solverIm-gpu.cpp:61-98): the per-entry parameters, and one step of
+// the gates at potential v (updates m, h, n; returns W and K).
+struct HHGate {
+  double rest, ls_Na, ls_K, g_Na, g_K, G_L, E_Na, E_K, E_L;
+};
+__device__ __forceinline__ HHGate hh_gate_load(const ImpHH& hd, int i) {
+  return HHGate{hd.rest[i], hd.ls_Na[i], hd.ls_K[i], hd.g_Na[i], hd.g_K[i],
+                hd.G_L[i],  hd.E_Na[i],  hd.E_K[i],  hd.E_L[i]};
+}
+__device__ __forceinline__ void hh_gate_step(const HHGate& q, double v, double dt_ms, double& m,
+                                             double& h, double& n, double& W, double& K) {
+  double vna = v + q.ls_Na;
+  double vk = v + q.ls_K;
+  vna = (vna - q.rest) / 0.001;
+  vk = (vk - q.rest) / 0.001;
   const double am = (2.5 - 0.1 * vna) / (exp(2.5 - 0.1 * vna) - 1);
   const double bm = 4 * exp(-vna / 18);
   const double ah = 0.07 * exp(-vna / 20);
   const double bh = 1 / (exp(3 - 0.1 * vna) + 1);
   const double an = (0.1 - 0.01 * vk) / (exp(1 - 0.1 * vk) - 1);
   const double bn = 0.125 * exp(-vk / 80);
-  const double m0 = hd.m[i], h0 = hd.h[i], n0 = hd.n[i];
-  const double m = m0 + dt_ms * (am * (1 - m0) - bm * m0);
-  const double h = h0 + dt_ms * (ah * (1 - h0) - bh * h0);
-  const double n = n0 + dt_ms * (an * (1 - n0) - bn * n0);
+  const double m0 = m, h0 = h, n0 = n;
+  m = m0 + dt_ms * (am * (1 - m0) - bm * m0);
+  h = h0 + dt_ms * (ah * (1 - h0) - bh * h0);
+  n = n0 + dt_ms * (an * (1 - n0) - bn * n0);
+  const double gna = q.g_Na * m * m * m * h;
+  const double gk = q.g_K * n * n * n * n;
+  const double gl = q.G_L;
+  W = -(gna + gk + gl);
+  K = gna * q.E_Na + gk * q.E_K + gl * q.E_L;
+}
+__device__ __forceinline__ void imp_gate(const ImpHH& hd, int i, double v, double dt_ms,
+                                         double& W, double& K) {
+  double m = hd.m[i], h = hd.h[i], n = hd.n[i];
+  hh_gate_step(hh_gate_load(hd, i), v, dt_ms, m, h, n, W, K);
   hd.m[i] = m;
   hd.h[i] = h;
   hd.n[i] = n;
-  const double gna = hd.g_Na[i] * m * m * m * h;
-  const double gk = hd.g_K[i] * n * n * n * n;
-  const double gl = hd.G_L[i];
-  W = -(gna + gk + gl);
-  K = gna * hd.E_Na[i] + gk * hd.E_K[i] + gl * hd.E_L[i];
 }
 
-// Update_Mat (solverIm-gpu.cpp:103-156) for the HH row of element iv. Returns
-// false for identity rows (left as they are), else true with the new diagonal.
-__device__ __forceinline__ bool imp_update_row(const ImpDev& d, int hh, int iv, double W,
-                                               double& diag) {
-  const int type = d.hh.type[hh];
+// Update_Mat (solverIm-gpu.cpp:103-156) for the HH row of element iv: the row's
+// fixed coefficients, and its refresh for a new W (returns false for identity
+// rows, left as they are, else true with the new diagonal).
+struct HHRow {
+  int type, im;
+  double alpha, inv_rdx, betaR, betaL, dxv, c_dt;
+};
+__device__ __forceinline__ HHRow hh_row_load(const ImpDev& d, int hh, int iv) {
+  HHRow q{};
+  q.type = d.hh.type[hh];
   // identity rows (ghosts, sealed ends) keep their 1 and read no neighbour
-  if (type == RT_OTHER || type == RT_TERM) return false;
-  const int im = d.hh.mat[hh];
-  double* v = d.val;
-  const double alpha = -1 / d.rdx[iv];
-  const double betaR = 1 / d.rdx[iv + 1];
-  const double betaMe = -(W * d.dxv[iv]) + 1 / d.rdx[iv];
-  const double beta = d.cmdx[iv] / d.dT + betaMe + betaR;
+  if (q.type == RT_OTHER || q.type == RT_TERM) return q;
+  q.im = d.hh.mat[hh];
+  q.alpha = -1 / d.rdx[iv];
+  q.inv_rdx = 1 / d.rdx[iv];
+  q.betaR = 1 / d.rdx[iv + 1];
+  q.dxv = d.dxv[iv];
+  q.c_dt = d.cmdx[iv] / d.dT;
+  if (q.type == RT_BRANCH || q.type == RT_FIRST_BRANCH) q.betaL = 1 / d.rdx[d.hh.dl[hh]];
+  return q;
+}
+__device__ __forceinline__ bool hh_row_refresh(const HHRow& q, double W, double* v,
+                                               double& diag) {
+  if (q.type == RT_OTHER || q.type == RT_TERM) return false;
+  const int im = q.im;
+  const double alpha = q.alpha;
+  const double betaR = q.betaR;
+  const double betaMe = -(W * q.dxv) + q.inv_rdx;
+  const double beta = q.c_dt + betaMe + betaR;
   const double delta = -betaR;
-  double betaL;
-  switch (type) {
+  const double betaL = q.betaL;
+  switch (q.type) {
     case RT_NEXT_TERM:
       diag = beta + delta;
       v[im] = alpha;
       v[im + 1] = diag;
       return true;
     case RT_BRANCH:
-      betaL = 1 / d.rdx[d.hh.dl[hh]];
       diag = beta + betaL;
       v[im] = alpha;
       v[im + 1] = diag;
@@ -408,7 +435,6 @@ __device__ __forceinline__ bool imp_update_row(const ImpDev& d, int hh, int iv, 
       v[im + 1] = delta;
       return true;
     case RT_FIRST_BRANCH:
-      betaL = 1 / d.rdx[d.hh.dl[hh]];
       diag = alpha + beta + betaL;
       v[im] = diag;
       v[im + 1] = delta;
@@ -417,6 +443,10 @@ __device__ __forceinline__ bool imp_update_row(const ImpDev& d, int hh, int iv, 
     default:
       return false;
   }
+}
+__device__ __forceinline__ bool imp_update_row(const ImpDev& d, int hh, int iv, double W,
+                                               double& diag) {
+  return hh_row_refresh(hh_row_load(d, hh, iv), W, d.val, diag);
 }
 
 // Grid-wide sum of NV values: block partials go to one of two alternating slots,
@@ -1299,13 +1329,31 @@ __device__ __forceinline__ void local_top(const ImpDev& d, const TreePlanDev& P,
       const int ns = top_stage_count(P);
       for (int j = tid; j < ns; j += nt) top_stage(P, w, L, j);
       __syncthreads();
-      for (int h = 0; h < nh; h++) {
-        for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_up(L, j);
+      if (P.top_wide <= kWave) {
+        // a narrow top: wave 0 alone, its heights ordered by the wave's own LDS
+        // order (LDS operations of one wave complete in issue order), no barrier
+        if (tid < kWave) {
+          for (int h = 0; h < nh; h++) {
+            const int j = L.hoff[h] + tid;
+            if (j < L.hoff[h + 1]) top_up(L, j);
+            wave_lds_sync();
+          }
+          for (int h = nh - 1; h >= 0; h--) {
+            const int j = L.hoff[h] + tid;
+            if (j < L.hoff[h + 1]) top_down(L, j);
+            wave_lds_sync();
+          }
+        }
         __syncthreads();
-      }
-      for (int h = nh - 1; h >= 0; h--) {
-        for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_down(L, j);
-        __syncthreads();
+      } else {
+        for (int h = 0; h < nh; h++) {
+          for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_up(L, j);
+          __syncthreads();
+        }
+        for (int h = nh - 1; h >= 0; h--) {
+          for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_down(L, j);
+          __syncthreads();
+        }
       }
       for (int j = tid; j < P.top_n; j += nt) top_store(P, w, L, j);
     } else {

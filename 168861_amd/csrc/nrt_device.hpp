@@ -105,6 +105,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   __syncthreads();
 }
 
+// Orders LDS accesses between the lanes of one wave: the LDS executes a wave's
+// operations in issue order, so keeping the compiler from moving accesses across
+// this point (a wavefront-scope fence, which emits no instruction) is enough.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Grid barrier for a persistent launch (all blocks co-resident). bar[0] counts
 // arrivals, bar[1] is the generation. The wait is bounded: after spin_limit
 // polls, or once another block raised *err, it sets *err and returns so that

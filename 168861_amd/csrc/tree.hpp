@@ -79,6 +79,7 @@ struct TreePlanHost {
   std::vector<int32_t> loc_order;   // local final nodes by group, then height
   std::vector<int32_t> loc_hoff;    // group g, height h: [loc_hoff[g*H+h], +1)
   int top_lds_ok = 0;               // the top fits in kTopLdsNodes LDS entries
+  int top_wide = 0;                 // most top nodes at one height
   // Group-resident form of local mode: each group's nodes of levels 1..L-1 and its
   // local final nodes get local indices (kept in LDS by the launch). Level lv of
   // group g starts at lbase[g*(L+2)+lv], node k of its column c at
@@ -140,7 +141,7 @@ struct TreePlanDev {
   int levels, heights;
   int fin_cut, sub_groups;
   int top_n, top_ext;
-  int groups, top_lds_ok, lds_nodes;
+  int groups, top_lds_ok, lds_nodes, top_wide;
   int gl_cols, gl_fin, gl_edges;
   const int32_t* nslots;
   const int32_t* nchunk;

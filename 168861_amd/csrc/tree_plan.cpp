@@ -69,6 +69,9 @@ void stage_top_set(TreePlanHost& P, const std::vector<uint8_t>& is_top) {
     if (P.fin_par[v] >= 0) P.top_par[j] = loc[P.fin_par[v]];
   }
   P.top_lds_ok = nt > 0 && (int)node.size() <= kTopLdsNodes ? 1 : 0;
+  P.top_wide = 0;
+  for (size_t k = 0; k + 1 < P.top_hoff.size(); k++)
+    P.top_wide = std::max(P.top_wide, P.top_hoff[k + 1] - P.top_hoff[k]);
 }
 
 namespace {
@@ -382,7 +385,8 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
   }
   return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
                      P.top_n,        P.top_ext,      P.groups,       P.top_lds_ok,
-                     P.lds_nodes,    P.gl_cols,      P.gl_fin,       P.gl_edges,
+                     P.lds_nodes,    P.top_wide,     P.gl_cols,      P.gl_fin,
+                     P.gl_edges,
                      base + off[0],  base + off[1],  base + off[2],  base + off[3],
                      base + off[4],  base + off[5],  base + off[6],  base + off[7],
                      base + off[8],  base + off[9],  base + off[10], base + off[11],
