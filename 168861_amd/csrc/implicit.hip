@@ -1859,13 +1859,13 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // group, each reducing and back-substituting its own rows alone; falls back
       // to the other forms when the plan has no reduction level or the HH entries
       // are not in row order. Automatic for the FP64 solve from 16k rows: a group
-      // of ~5000 rows per workgroup, at most 32 workgroups, past which the top of
-      // the final forest (one workgroup) costs more than the groups save (measured:
-      // 20,202-row cable 70 -> 50 us per step with 4 groups; depth-14 tree of
-      // 902,037 rows 389 -> 163 us with 32, 199 with 48).
+      // of ~5000 rows per workgroup, at most 64 workgroups, past which the top of
+      // the final forest (one workgroup, height by height) costs more than the
+      // groups save (measured: 20,202-row cable 70 -> 45 us per step with 4
+      // groups; depth-14 tree of 902,037 rows 389 -> 129 us with 64, 146 with 128).
       int G = batch_threads > 0 ? 0 : o.tree_groups;
       if (G == 0 && batch_threads == 0 && e->method == 1 && o.num_blocks == 0 && N >= (1 << 14))
-        G = std::min(32, (N + 2500) / 5000);
+        G = std::min(64, (N + 2500) / 5000);
       if (G > 1) {
         for (int hh = 1; hh < H; hh++)
           if (p.hh_elem[hh] < p.hh_elem[hh - 1]) G = 0;
@@ -1949,6 +1949,17 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
           e->local_res = G <= cus * per_cu;
         }
         e->local_lds = e->local_res ? lay.total : lay.top;
+        if (!e->local_res && lay.top > 0) {
+          if (lay.top + 1024 > (size_t)max_lds) {  // solve the top from global memory
+            e->plan_h.top_lds_ok = 0;
+            e->local_lds = 0;
+          } else {
+            const void* kl = f64 ? (const void*)k_implicit_tree_local<double, false>
+                                 : (const void*)k_implicit_tree_local<float, false>;
+            NRT_HIP(hipFuncSetAttribute(kl, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lay.top));
+          }
+        }
       }
       // grid launches split the final forest over their workgroups; a batch slot
       // may run as a virtual grid of up to one workgroup per kRowsPerWorkgroup rows

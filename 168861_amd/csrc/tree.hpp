@@ -78,7 +78,8 @@ struct TreePlanHost {
   std::vector<int32_t> gcol;        // level lv, group g: columns [gcol[lv*(G+1)+g], +1)
   std::vector<int32_t> loc_order;   // local final nodes by group, then height
   std::vector<int32_t> loc_hoff;    // group g, height h: [loc_hoff[g*H+h], +1)
-  int top_lds_ok = 0;               // the top fits in kTopLdsNodes LDS entries
+  int top_lds_ok = 0;               // the top fits in LDS (grid mode: kTopLdsNodes
+                                    // entries; local mode, sized to the top: kTopLdsLocal)
   int top_wide = 0;                 // most top nodes at one height
   // Group-resident form of local mode: each group's nodes of levels 1..L-1 and its
   // local final nodes get local indices (kept in LDS by the launch). Level lv of
@@ -98,6 +99,9 @@ struct TreePlanHost {
 
 // LDS capacity for the staged top of the final forest (local entries).
 constexpr int kTopLdsNodes = 1024;
+// Local mode stages the top in an LDS region sized to it, up to this many entries
+// (56 B each in FP64: 140 KB of the 160 KB per workgroup).
+constexpr int kTopLdsLocal = 2560;
 
 // Builds the plan for a forest given by parent indices (par[i] < i or -1), with
 // chains cut into chunks of `chunk` rows at level 0 and `chunk_hi` rows above.

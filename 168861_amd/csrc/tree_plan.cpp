@@ -25,8 +25,8 @@ std::vector<int32_t> csr_parents(int N, const int32_t* rowptr, const int32_t* co
 // The top of the final forest for one workgroup (see TreePlanHost::top_*):
 // the nodes of `is_top` (closed upwards) in height order, then their children
 // outside the set. The lists are always built (the top can be solved from global
-// memory); top_lds_ok says whether they fit kTopLdsNodes LDS entries.
-void stage_top_set(TreePlanHost& P, const std::vector<uint8_t>& is_top) {
+// memory); top_lds_ok says whether they fit max_entries LDS entries.
+void stage_top_set(TreePlanHost& P, const std::vector<uint8_t>& is_top, int max_entries) {
   const int H = P.heights, n = (int)P.fin_par.size();
   std::vector<int32_t> loc(n, -1), node;
   int h0 = -1;
@@ -68,7 +68,7 @@ void stage_top_set(TreePlanHost& P, const std::vector<uint8_t>& is_top) {
     P.top_cptr.push_back((int32_t)P.top_child.size());
     if (P.fin_par[v] >= 0) P.top_par[j] = loc[P.fin_par[v]];
   }
-  P.top_lds_ok = nt > 0 && (int)node.size() <= kTopLdsNodes ? 1 : 0;
+  P.top_lds_ok = nt > 0 && (int)node.size() <= max_entries ? 1 : 0;
   P.top_wide = 0;
   for (size_t k = 0; k + 1 < P.top_hoff.size(); k++)
     P.top_wide = std::max(P.top_wide, P.top_hoff[k + 1] - P.top_hoff[k]);
@@ -292,7 +292,7 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
         const int u = P.fin_order[q];
         if (!mixed[u]) P.loc_order[lf[(size_t)fg[u] * H + h[u]]++] = u;
       }
-    stage_top_set(P, mixed);
+    stage_top_set(P, mixed, kTopLdsLocal);
     // group-resident indices
     const int L = P.levels;
     P.lbase.assign((size_t)G * (L + 2), 0);
@@ -404,7 +404,7 @@ static void stage_final_top(TreePlanHost& P, int cut) {
   const int H = P.heights, n = (int)P.fin_par.size();
   std::vector<uint8_t> is_top(n, 0);
   for (int q = P.fin_hoff[cut]; q < P.fin_hoff[H]; q++) is_top[P.fin_order[q]] = 1;
-  stage_top_set(P, is_top);
+  stage_top_set(P, is_top, kTopLdsNodes);
   P.fin_cut = cut;
   if (!P.top_lds_ok) {
     P.top_n = P.top_ext = 0;

@@ -142,10 +142,10 @@ def test_tree_local_mode_golden(form, name, dtf, nrt, orc):
 
 
 @pytest.mark.parametrize("form", sorted(FORMS))
-@pytest.mark.parametrize("groups", [32, 128])
+@pytest.mark.parametrize("groups", [32, 128, 256])
 def test_tree_local_mode_large(form, groups, nrt, orc):
-    """A depth-12 tree in local mode: 32 groups stage the top in LDS, 128 groups
-    leave a top too large for it (solved from the plan's global lists)."""
+    """A depth-12 tree in local mode: 32 and 128 groups stage the top in LDS, 256
+    groups leave a top too large for it (solved from the plan's global lists)."""
     geom, run = wl.random_tree(wl.seed_for(3), depth=12)
     prob, _ = nrt.setup(geom, dT=run.dT)
     _say(f"tree {form} local groups={groups} random depth 12 N={prob.N}")
