@@ -1329,31 +1329,31 @@ __device__ __forceinline__ void local_top(const ImpDev& d, const TreePlanDev& P,
       const int ns = top_stage_count(P);
       for (int j = tid; j < ns; j += nt) top_stage(P, w, L, j);
       __syncthreads();
-      if (P.top_wide <= kWave) {
-        // a narrow top: wave 0 alone, its heights ordered by the wave's own LDS
-        // order (LDS operations of one wave complete in issue order), no barrier
-        if (tid < kWave) {
-          for (int h = 0; h < nh; h++) {
-            const int j = L.hoff[h] + tid;
-            if (j < L.hoff[h + 1]) top_up(L, j);
-            wave_lds_sync();
-          }
-          for (int h = nh - 1; h >= 0; h--) {
-            const int j = L.hoff[h] + tid;
-            if (j < L.hoff[h + 1]) top_down(L, j);
-            wave_lds_sync();
-          }
-        }
+      // Heights below top_narrow (wide) take the workgroup and a barrier each;
+      // from top_narrow up (at most a wave's nodes per height) wave 0 works alone,
+      // its heights ordered by the wave's own LDS order (LDS operations of one wave
+      // complete in issue order), with no barrier.
+      const int hn = P.top_narrow;
+      for (int h = 0; h < hn; h++) {
+        for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_up(L, j);
         __syncthreads();
-      } else {
-        for (int h = 0; h < nh; h++) {
-          for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_up(L, j);
-          __syncthreads();
+      }
+      if (tid < kWave) {
+        for (int h = hn; h < nh; h++) {
+          const int j = L.hoff[h] + tid;
+          if (j < L.hoff[h + 1]) top_up(L, j);
+          wave_lds_sync();
         }
-        for (int h = nh - 1; h >= 0; h--) {
-          for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_down(L, j);
-          __syncthreads();
+        for (int h = nh - 1; h >= hn; h--) {
+          const int j = L.hoff[h] + tid;
+          if (j < L.hoff[h + 1]) top_down(L, j);
+          wave_lds_sync();
         }
+      }
+      __syncthreads();
+      for (int h = hn - 1; h >= 0; h--) {
+        for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_down(L, j);
+        __syncthreads();
       }
       for (int j = tid; j < P.top_n; j += nt) top_store(P, w, L, j);
     } else {

@@ -81,6 +81,8 @@ struct TreePlanHost {
   int top_lds_ok = 0;               // the top fits in LDS (grid mode: kTopLdsNodes
                                     // entries; local mode, sized to the top: kTopLdsLocal)
   int top_wide = 0;                 // most top nodes at one height
+  int top_narrow = 0;               // first top height (relative) from which no height
+                                    // holds more than a wave's 64 nodes
   // Group-resident form of local mode: each group's nodes of levels 1..L-1 and its
   // local final nodes get local indices (kept in LDS by the launch). Level lv of
   // group g starts at lbase[g*(L+2)+lv], node k of its column c at
@@ -145,7 +147,7 @@ struct TreePlanDev {
   int levels, heights;
   int fin_cut, sub_groups;
   int top_n, top_ext;
-  int groups, top_lds_ok, lds_nodes, top_wide;
+  int groups, top_lds_ok, lds_nodes, top_wide, top_narrow;
   int gl_cols, gl_fin, gl_edges;
   const int32_t* nslots;
   const int32_t* nchunk;

@@ -70,8 +70,12 @@ void stage_top_set(TreePlanHost& P, const std::vector<uint8_t>& is_top, int max_
   }
   P.top_lds_ok = nt > 0 && (int)node.size() <= max_entries ? 1 : 0;
   P.top_wide = 0;
-  for (size_t k = 0; k + 1 < P.top_hoff.size(); k++)
-    P.top_wide = std::max(P.top_wide, P.top_hoff[k + 1] - P.top_hoff[k]);
+  P.top_narrow = 0;
+  for (size_t k = 0; k + 1 < P.top_hoff.size(); k++) {
+    const int wdt = P.top_hoff[k + 1] - P.top_hoff[k];
+    P.top_wide = std::max(P.top_wide, wdt);
+    if (wdt > 64) P.top_narrow = (int)k + 1;
+  }
 }
 
 namespace {
@@ -385,7 +389,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
   }
   return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
                      P.top_n,        P.top_ext,      P.groups,       P.top_lds_ok,
-                     P.lds_nodes,    P.top_wide,     P.gl_cols,      P.gl_fin,
+                     P.lds_nodes,    P.top_wide,     P.top_narrow,   P.gl_cols,      P.gl_fin,
                      P.gl_edges,
                      base + off[0],  base + off[1],  base + off[2],  base + off[3],
                      base + off[4],  base + off[5],  base + off[6],  base + off[7],
