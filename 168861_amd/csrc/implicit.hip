@@ -1306,7 +1306,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
 
 // Factors of every chunk of the group (once per launch).
 template <class T>
-__device__ void tps_factor_local(const TreePlanDev& P, const TreeWork<T>& w, int g) {
+__device__ __forceinline__ void tps_factor_local(const TreePlanDev& P, const TreeWork<T>& w, int g) {
   for (int lv = 0; lv < P.levels; lv++) {
     const int32_t* gc = P.gcol + lv * (P.groups + 1);
     const int c1 = gc[g + 1];
@@ -1373,7 +1373,7 @@ __device__ __forceinline__ void local_top(const ImpDev& d, const TreePlanDev& P,
 }
 
 template <class T>
-__device__ void tps_run_local(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
+__device__ __forceinline__ void tps_run_local(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
                               bool acc, bool all_static, unsigned char* tl) {
   const int g = d.vb, G = P.groups, H = P.heights, L = P.levels;
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
@@ -1418,33 +1418,76 @@ __device__ void tps_run_local(const ImpDev& d, const TreePlanDev& P, const TreeW
 
 // Group-resident form: the group's levels above 0 and its local final nodes in
 // LDS (lw), its plan metadata staged in LDS (M).
+
+// A wide chunk (column j of level lv) eliminated by the whole workgroup: parallel
+// cyclic reduction of its interior, each step reading every row before any row is
+// written (a barrier between), interior row 1 + tid for thread tid.
 template <class T>
-__device__ void tps_factor_res(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
-                               int L) {
-  for (int lv = 0; lv < L; lv++) {
-    const int nc = M.lvl[5 * lv + 1];
-    for (int j = (int)threadIdx.x; j < nc; j += blockDim.x) {
-      if (lv == 0) glm_reduce<true>(w, lw, M, 0, j, kGlFull);
-      else glm_reduce<false>(w, lw, M, lv, j, kGlFull);
-    }
+__device__ __forceinline__ void wide_reduce(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
+                            int lv, int j) {
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  const int q = 4 * (M.lvl[7 * lv + 3] + j);
+  const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
+  const int base = M.lvl[7 * lv + 2] + j, st = M.lvl[7 * lv + 1];
+  for (int k = 1 + tid; k <= m - 2; k += nt) wide_init(lw, base, st, m, k);
+  __syncthreads();
+  const int ns = wide_steps(m - 2);
+  const int k0 = 1 + tid;  // the plan keeps m - 2 <= 512 = blockDim
+  for (int s = 0, h = 1; s < ns; s++, h *= 2) {
+    WideRow<T> o0{};
+    if (k0 <= m - 2) o0 = wide_step(lw, base, st, m, k0, h);
+    __syncthreads();
+    if (k0 <= m - 2) wide_put(lw, base, st, k0, o0);
     __syncthreads();
   }
+  for (int k = 1 + tid; k <= m - 2; k += nt) wide_finish(lw, base, st, k);
+  __syncthreads();
+  if (tid == 0) wide_ends(lw, base, st, m, es, ee, EndsLocal<T>{lw, w, M.scal[0]});
 }
 
 template <class T>
-__device__ void tps_run_res(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
-                            const TreeWorkL<T>& lw, const GlMeta& M, bool acc, bool all_static,
-                            unsigned char* tl) {
-  const int H = P.heights, L = P.levels, how = all_static ? kGlStatic : kGlStep;
+__device__ __forceinline__ void wide_back(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
+                          int j) {
+  const int q = 4 * (M.lvl[7 * lv + 3] + j);
+  const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
+  const int base = M.lvl[7 * lv + 2] + j, st = M.lvl[7 * lv + 1];
+  const EndsLocal<T> E{lw, w, M.scal[0]};
+  wide_back_rows(lw, base, st, m, E.x(es), E.x(ee), (int)threadIdx.x, (int)blockDim.x);
+}
+
+// Levels 0 .. L-1 of the group: per-thread chunks, then its wide chunks.
+template <class T>
+__device__ __forceinline__ void res_levels(const TreeWork<T>& w, const TreeWorkL<T>& lw,
+                                           const GlMeta& M, int L, int how) {
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   for (int lv = 0; lv < L; lv++) {
-    const int nc = M.lvl[5 * lv + 1];
+    const int nc = M.lvl[7 * lv + 1];
     for (int j = tid; j < nc; j += nt) {
       if (lv == 0) glm_reduce<true>(w, lw, M, 0, j, how);
       else glm_reduce<false>(w, lw, M, lv, j, how);
     }
     __syncthreads();
+    const int wn = M.lvl[7 * lv + 6];
+    for (int i = 0; i < wn; i++) {
+      wide_reduce(w, lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
+      __syncthreads();
+    }
   }
+}
+
+template <class T>
+__device__ __forceinline__ void tps_factor_res(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
+                               int L) {
+  res_levels<T>(w, lw, M, L, kGlFull);
+}
+
+template <class T>
+__device__ __forceinline__ void tps_run_res(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
+                            const TreeWorkL<T>& lw, const GlMeta& M, bool acc, bool all_static,
+                            unsigned char* tl) {
+  const int H = P.heights, L = P.levels, how = all_static ? kGlStatic : kGlStep;
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  res_levels<T>(w, lw, M, L, how);
   for (int h = 0; h < H; h++) {
     const int k0 = M.hoff[h], k1 = M.hoff[h + 1];
     if (k0 == k1) continue;  // uniform over the workgroup
@@ -1459,8 +1502,10 @@ __device__ void tps_run_res(const ImpDev& d, const TreePlanDev& P, const TreeWor
     __syncthreads();
   }
   for (int lv = L - 1; lv >= 1; lv--) {
-    const int nc = M.lvl[5 * lv + 1];
+    const int nc = M.lvl[7 * lv + 1];
     for (int j = tid; j < nc; j += nt) glm_back<false, OUT_LEVEL>(w, lw, M, lv, j, nullptr);
+    const int wn = M.lvl[7 * lv + 6];
+    for (int i = 0; i < wn; i++) wide_back(w, lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
     __syncthreads();
   }
   const int nc0 = M.lvl[1];
@@ -1930,15 +1975,19 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       if (e->local_mode) {
         // group-resident form when the top (if staged), the group's values and its
         // metadata fit in LDS beside the kernel's own; else the levels stay in
-        // global memory
+        // global memory, where the plan keeps no wide chunk (their cooperative
+        // elimination needs the group's LDS)
         const bool f64 = e->method == 1;
-        const TreePlanDev V = plan_view(e->plan_h, nullptr, nullptr);  // sizes only
-        const bool top = e->plan_h.top_lds_ok != 0;
-        const GlLayout lay = f64 ? gl_layout<double>(V, top) : gl_layout<float>(V, top);
         int max_lds = 0, cus = 0, per_cu = 0;
         NRT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                       hnd->device));
         NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hnd->device));
+        auto layout = [&]() {
+          const TreePlanDev V = plan_view(e->plan_h, nullptr, nullptr);  // sizes only
+          const bool top = e->plan_h.top_lds_ok != 0;
+          return f64 ? gl_layout<double>(V, top) : gl_layout<float>(V, top);
+        };
+        GlLayout lay = layout();
         const void* kr = f64 ? (const void*)k_implicit_tree_local<double, true>
                              : (const void*)k_implicit_tree_local<float, true>;
         e->local_res = lay.total + 1024 <= (size_t)max_lds;
@@ -1947,6 +1996,15 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
                                       (int)lay.total));
           NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kr, kLocalBlock, lay.total));
           e->local_res = G <= cus * per_cu;
+        }
+        if (!e->local_res && !e->plan_h.wide_col.empty()) {
+          e->plan_h = build_tree_plan(par, chunk, chunk_hi, &dyn, &rgrp, false);
+          try {
+            check_local_plan(e->plan_h);
+          } catch (const std::exception& ex) {
+            throw Error(NRT_ERR_TOPOLOGY, std::string("tree solve, local mode: ") + ex.what());
+          }
+          lay = layout();
         }
         e->local_lds = e->local_res ? lay.total : lay.top;
         if (!e->local_res && lay.top > 0) {

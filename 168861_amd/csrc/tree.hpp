@@ -45,7 +45,8 @@ struct TreePlanHost {
   std::vector<int32_t> next;        // flat: slot in level+1 of a surviving node (-1 otherwise)
   std::vector<int32_t> chunk_off;   // per level, L+1 entries, into chunk_len
   std::vector<int32_t> chunk_len;
-  std::vector<int32_t> chunk_dyn;   // 1: the chunk's matrix values change between steps
+  std::vector<int32_t> chunk_dyn;   // bit 0: the chunk's matrix values change between
+                                    // steps; bit 1: a wide chunk (local mode)
   std::vector<int32_t> slot0;       // level-0 slot of each row (N entries)
   std::vector<int32_t> row0;        // row of each level-0 slot (-1 for padding)
   std::vector<int32_t> fin_par;     // final forest: parent (final-local) or -1
@@ -92,6 +93,9 @@ struct TreePlanHost {
   // node u of the top (kept in the global arrays).
   int lds_nodes = 0;                // largest group count
   int gl_cols = 0, gl_fin = 0, gl_edges = 0;  // largest group: columns, local final nodes, their children
+  int gl_wide = 0;                  // largest group: wide chunks
+  std::vector<int32_t> wide_off;    // group g, level lv: wide chunks [wide_off[g*(L+1)+lv], +1)
+  std::vector<int32_t> wide_col;    // ... their columns (group-relative)
   std::vector<int32_t> lbase;
   std::vector<int32_t> lnext;       // per slot of levels 0..L-1: end code, -1 if no end
   std::vector<int32_t> lf_cptr, lf_child;  // local final nodes (loc_order order): children
@@ -101,6 +105,13 @@ struct TreePlanHost {
 
 // LDS capacity for the staged top of the final forest (local entries).
 constexpr int kTopLdsNodes = 1024;
+// Local mode keeps a chain of kWideMin+1 .. kWideMax nodes whole above level 0 (a
+// wide chunk); the group-resident launch eliminates it with the whole workgroup by
+// parallel cyclic reduction, one interior row per thread. Longer chains are cut
+// into short chunks as usual, so the next level's chain is shorter: PCR moves
+// every row at every step (n log n LDS traffic), the serial sweep n.
+constexpr int kWideMin = 64;
+constexpr int kWideMax = 2 + 512;
 // Local mode stages the top in an LDS region sized to it, up to this many entries
 // (56 B each in FP64: 140 KB of the 160 KB per workgroup).
 constexpr int kTopLdsLocal = 2560;
@@ -113,9 +124,10 @@ constexpr int kTopLdsLocal = 2560;
 // (slot columns) dynamic first, then by decreasing length, so a wave's lanes take
 // the same path and stay active together.
 // groups (optional, N entries, values 0..G-1): local mode, see TreePlanHost.
+// wide: in local mode, keep long chains whole above level 0 (kWideMin).
 TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chunk_hi,
                              const std::vector<uint8_t>* dyn_rows = nullptr,
-                             const std::vector<int32_t>* groups = nullptr);
+                             const std::vector<int32_t>* groups = nullptr, bool wide = true);
 
 // Splits the final forest of a plan for a launch of `groups` workgroups of
 // `threads` threads (any number of workgroups can run it: workgroup b takes the
@@ -148,7 +160,7 @@ struct TreePlanDev {
   int fin_cut, sub_groups;
   int top_n, top_ext;
   int groups, top_lds_ok, lds_nodes, top_wide, top_narrow;
-  int gl_cols, gl_fin, gl_edges;
+  int gl_cols, gl_fin, gl_edges, gl_wide;
   const int32_t* nslots;
   const int32_t* nchunk;
   const int32_t* node_off;
@@ -178,6 +190,8 @@ struct TreePlanDev {
   const int32_t* lf_cptr;
   const int32_t* lf_child;
   const int32_t* lf_par;
+  const int32_t* wide_off;
+  const int32_t* wide_col;
 };
 
 // Per-value work arrays of one solve, over all levels (total_nodes each): d, up
@@ -682,11 +696,13 @@ struct EndsLocal {
 
 // Plan metadata of one group in the group-resident form, staged once per launch
 // (LDS on the device) so that no step waits on a chain of plan loads:
-//   lvl[5*lv ..]: first column, column count, local base, first col entry, NC of level lv
+//   lvl[7*lv ..]: first column, column count, local base, first col entry, NC of
+//                 level lv, first and count of its wide chunks in wcol
 //   col[4*q ..]: chunk length, start code, end code, dyn flag of the group's column q
 //   scal: first final-forest slot in w, local base of the final nodes, final count
 //   hoff: the local final nodes' height boundaries (group-relative, H+1)
 //   fpar, fu, fcp, fch: their parent codes, final ids, children (local indices)
+//   wcol: the group's wide chunks (columns), level by level
 struct GlMeta {
   NRT_LDS int32_t* lvl;
   NRT_LDS int32_t* col;
@@ -696,6 +712,7 @@ struct GlMeta {
   NRT_LDS int32_t* fu;
   NRT_LDS int32_t* fcp;
   NRT_LDS int32_t* fch;
+  NRT_LDS int32_t* wcol;
 };
 
 // Byte layout of a group-resident launch's dynamic LDS: the staged top (when
@@ -709,8 +726,8 @@ NRT_HD int gl_top_entries(const TreePlanDev& P) {
   return n > 0 ? n : 1;
 }
 NRT_HD size_t gl_meta_ints(const TreePlanDev& P) {
-  return (size_t)5 * P.levels + 4 + (P.heights + 1) + 4 * (size_t)P.gl_cols + 3 * P.gl_fin + 1 +
-         P.gl_edges;
+  return (size_t)7 * P.levels + 4 + (P.heights + 1) + 4 * (size_t)P.gl_cols + 3 * P.gl_fin + 1 +
+         P.gl_edges + P.gl_wide;
 }
 template <class T>
 NRT_HD GlLayout gl_layout(const TreePlanDev& P, bool top) {
@@ -725,13 +742,14 @@ NRT_HD GlMeta gl_meta_carve(void* p, const TreePlanDev& P) {
   NRT_LDS int32_t* i = (NRT_LDS int32_t*)p;
   GlMeta M;
   M.lvl = i;
-  M.col = M.lvl + 5 * P.levels;
+  M.col = M.lvl + 7 * P.levels;
   M.scal = M.col + 4 * P.gl_cols;
   M.hoff = M.scal + 4;
   M.fpar = M.hoff + P.heights + 1;
   M.fu = M.fpar + P.gl_fin;
   M.fcp = M.fu + P.gl_fin;
   M.fch = M.fcp + P.gl_fin + 1;
+  M.wcol = M.fch + P.gl_edges;
   return M;
 }
 
@@ -743,13 +761,18 @@ NRT_HD void gl_meta_stage(const TreePlanDev& P, int g, const GlMeta& M, int tid,
   for (int lv = 0; lv < L; lv++) {
     const int c0 = P.gcol[lv * (G + 1) + g], nc = P.gcol[lv * (G + 1) + g + 1] - c0;
     const int NC = P.nchunk[lv], slot0 = P.node_off[lv];
+    const int w0 = P.wide_off[g * (L + 1)], wb = P.wide_off[g * (L + 1) + lv];
+    const int wn = P.wide_off[g * (L + 1) + lv + 1] - wb;
     if (tid == 0) {
-      M.lvl[5 * lv] = c0;
-      M.lvl[5 * lv + 1] = nc;
-      M.lvl[5 * lv + 2] = lv > 0 ? P.lbase[g * (L + 2) + lv] : 0;
-      M.lvl[5 * lv + 3] = mb;
-      M.lvl[5 * lv + 4] = NC;
+      M.lvl[7 * lv] = c0;
+      M.lvl[7 * lv + 1] = nc;
+      M.lvl[7 * lv + 2] = lv > 0 ? P.lbase[g * (L + 2) + lv] : 0;
+      M.lvl[7 * lv + 3] = mb;
+      M.lvl[7 * lv + 4] = NC;
+      M.lvl[7 * lv + 5] = wb - w0;
+      M.lvl[7 * lv + 6] = wn;
     }
+    for (int i = tid; i < wn; i += nt) M.wcol[wb - w0 + i] = P.wide_col[wb + i];
     for (int j = tid; j < nc; j += nt) {
       const int ch = P.chunk_off[lv] + c0 + j, m = P.chunk_len[ch];
       const int slot = slot0 + c0 + j;
@@ -785,17 +808,18 @@ enum { kGlFull = 0, kGlStep = 1, kGlStatic = 2 };
 template <bool L0, class T>
 NRT_HD void glm_reduce(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
                        int j, int how) {
-  const int c0 = M.lvl[5 * lv], nc = M.lvl[5 * lv + 1];
-  const int q = 4 * (M.lvl[5 * lv + 3] + j);
+  const int c0 = M.lvl[7 * lv], nc = M.lvl[7 * lv + 1];
+  const int q = 4 * (M.lvl[7 * lv + 3] + j);
   const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
-  const bool full = how == kGlFull || (how == kGlStep && M.col[q + 3]);
+  if (M.col[q + 3] & 2) return;  // a wide chunk: the workgroup's (wide_reduce)
+  const bool full = how == kGlFull || (how == kGlStep && (M.col[q + 3] & 1));
   const EndsLocal<T> E{lw, w, M.scal[0]};
   if (L0) {
-    const int NC = M.lvl[5 * lv + 4];
+    const int NC = M.lvl[7 * lv + 4];
     if (full) chunk_reduce(w, c0 + j, NC, m, es, ee, E);
     else chunk_reduce_static(w, c0 + j, NC, m, es, ee, E);
   } else {
-    const int base = M.lvl[5 * lv + 2] + j;
+    const int base = M.lvl[7 * lv + 2] + j;
     if (full) chunk_reduce(lw, base, nc, m, es, ee, E);
     else chunk_reduce_static(lw, base, nc, m, es, ee, E);
   }
@@ -804,12 +828,13 @@ NRT_HD void glm_reduce(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMet
 template <bool L0, int MODE, class T>
 NRT_HD void glm_back(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
                      int j, double* out) {
-  const int c0 = M.lvl[5 * lv], nc = M.lvl[5 * lv + 1];
-  const int q = 4 * (M.lvl[5 * lv + 3] + j);
+  const int c0 = M.lvl[7 * lv], nc = M.lvl[7 * lv + 1];
+  const int q = 4 * (M.lvl[7 * lv + 3] + j);
   const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
+  if (M.col[q + 3] & 2) return;  // a wide chunk: the workgroup's (wide_back)
   const EndsLocal<T> E{lw, w, M.scal[0]};
-  if (L0) chunk_back<MODE>(w, c0 + j, M.lvl[5 * lv + 4], m, es, ee, E, out);
-  else chunk_back<OUT_LEVEL>(lw, M.lvl[5 * lv + 2] + j, nc, m, es, ee, E, out);
+  if (L0) chunk_back<MODE>(w, c0 + j, M.lvl[7 * lv + 4], m, es, ee, E, out);
+  else chunk_back<OUT_LEVEL>(lw, M.lvl[7 * lv + 2] + j, nc, m, es, ee, E, out);
 }
 
 // Local final node j of the group (group-relative, in height order).
@@ -844,6 +869,113 @@ NRT_HD void glm_final_down(const TreeWork<T>& w, const TreeWorkL<T>& lw, const G
     acc = acc - lw.up[li] * xp;
   }
   lw.x[li] = acc * tps_rcp(lw.ca[li]);
+}
+
+// ---- wide chunks: parallel cyclic reduction (PCR) of a chain's interior.
+// The m rows of the chunk are at base + k*st in lw (k = 0 .. m-1); the interior
+// rows 1 .. m-2 are the unknowns, rows 0 and m-1 (s, e) parameters, as for
+// chunk_reduce: x_k = alpha_k + beta_k x_s + gamma_k x_e. Row k reads
+// a_k x_{k-1} + d_k x_k + c_k x_{k+1} = r_k with a_k = up[k], c_k = dn[k+1]. Three
+// right-hand sides go through the reduction: r, rb (x_s's coefficient moved to
+// the right: -a_1 at row 1) and rg (-c_{m-2} at row m-2). Working arrays: a in
+// cf, c in ci, d in x, r in r (a per-solve value), rb in cb, rg in cg; the
+// matrix arrays (d, up, dn) are left as they are. Each step combines row i with
+// rows i - h and i + h (h = 1, 2, 4, ...) until no coupling is left; then
+// alpha = r/d, beta = rb/d, gamma = rg/d land in ca, cb, cg.
+template <class T>
+struct WideRow {
+  T a, d, c, r, rb, rg;
+};
+
+template <class T>
+NRT_HD void wide_init(const TreeWorkL<T>& lw, int base, int st, int m, int k) {
+  const int g = base + k * st;
+  lw.cf[g] = k == 1 ? T(0) : lw.up[g];
+  lw.ci[g] = k == m - 2 ? T(0) : lw.dn[g + st];
+  lw.x[g] = lw.d[g];
+  lw.cb[g] = k == 1 ? -lw.up[g] : T(0);
+  lw.cg[g] = k == m - 2 ? -lw.dn[g + st] : T(0);
+}
+
+// New values of interior row k at stride h (read phase of a step).
+template <class T>
+NRT_HD WideRow<T> wide_step(const TreeWorkL<T>& lw, int base, int st, int m, int k, int h) {
+  const int g = base + k * st;
+  WideRow<T> o{lw.cf[g], lw.x[g], lw.ci[g], lw.r[g], lw.cb[g], lw.cg[g]};
+  if (k - h >= 1) {
+    const int q = g - h * st;
+    const T k1 = o.a * tps_rcp(lw.x[q]);
+    o.d = o.d - k1 * lw.ci[q];
+    o.r = o.r - k1 * lw.r[q];
+    o.rb = o.rb - k1 * lw.cb[q];
+    o.rg = o.rg - k1 * lw.cg[q];
+    o.a = -k1 * lw.cf[q];
+  } else {
+    o.a = T(0);
+  }
+  if (k + h <= m - 2) {
+    const int q = g + h * st;
+    const T k2 = o.c * tps_rcp(lw.x[q]);
+    o.d = o.d - k2 * lw.cf[q];
+    o.r = o.r - k2 * lw.r[q];
+    o.rb = o.rb - k2 * lw.cb[q];
+    o.rg = o.rg - k2 * lw.cg[q];
+    o.c = -k2 * lw.ci[q];
+  } else {
+    o.c = T(0);
+  }
+  return o;
+}
+
+template <class T>
+NRT_HD void wide_put(const TreeWorkL<T>& lw, int base, int st, int k, const WideRow<T>& o) {
+  const int g = base + k * st;
+  lw.cf[g] = o.a;
+  lw.x[g] = o.d;
+  lw.ci[g] = o.c;
+  lw.r[g] = o.r;
+  lw.cb[g] = o.rb;
+  lw.cg[g] = o.rg;
+}
+
+// After the last step: alpha, beta, gamma of interior row k.
+template <class T>
+NRT_HD void wide_finish(const TreeWorkL<T>& lw, int base, int st, int k) {
+  const int g = base + k * st;
+  const T inv = tps_rcp(lw.x[g]);
+  lw.ca[g] = lw.r[g] * inv;
+  lw.cb[g] = lw.cb[g] * inv;
+  lw.cg[g] = lw.cg[g] * inv;
+}
+
+// The two ends of the reduced chunk (one worker): as chunk_reduce's, from the
+// interior's alpha, beta, gamma next to them. r of rows 1 and m-2 were reduced in
+// place, so the end rows take r from the row's own (untouched) slots.
+template <class T, class E>
+NRT_HD void wide_ends(const TreeWorkL<T>& lw, int base, int st, int m, int es, int ee,
+                      const E& ends) {
+  const int g0 = base, g1 = base + st, gl = base + (m - 2) * st, ge = base + (m - 1) * st;
+  const T dn1 = lw.dn[g1], ae = lw.up[ge];
+  ends.put(es, lw.d[g0] + dn1 * lw.cb[g1], lw.up[g0], lw.dn[g0], lw.r[g0] - dn1 * lw.ca[g1]);
+  ends.put(ee, lw.d[ge] + ae * lw.cg[gl], ae * lw.cb[gl], dn1 * lw.cg[g1],
+           lw.r[ge] - ae * lw.ca[gl]);
+}
+
+// Back-substitution of a wide chunk from its ends' solution, rows k = k0, k0+dk, ...
+template <class T>
+NRT_HD void wide_back_rows(const TreeWorkL<T>& lw, int base, int st, int m, T xs, T xe, int k0,
+                           int dk) {
+  for (int k = k0; k < m; k += dk) {
+    const int g = base + k * st;
+    lw.x[g] = k == 0 ? xs : (k == m - 1 ? xe : lw.ca[g] + lw.cb[g] * xs + lw.cg[g] * xe);
+  }
+}
+
+// Steps of a PCR over n interior rows: strides 1, 2, 4, ... below n.
+NRT_HD int wide_steps(int n) {
+  int s = 0;
+  for (int h = 1; h < n; h *= 2) s++;
+  return s;
 }
 
 // Local arrays of n entries each, carved from one buffer (LDS on the device).

@@ -85,6 +85,7 @@ struct LocalLevel {
   std::vector<int32_t> grp;                 // group of each node (local mode)
   std::vector<int32_t> cs, cl;              // chunks (local start, length)
   std::vector<uint8_t> cdyn;                // chunk holds a changing node
+  std::vector<uint8_t> cwide;               // wide chunk (a whole chain above level 0)
   std::vector<int32_t> pos;                 // chunk -> slot column
   std::vector<int32_t> next;                // local next-level index
   std::vector<int32_t> slot;                // local node -> slot
@@ -94,7 +95,7 @@ struct LocalLevel {
 
 TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int chunk_hi,
                              const std::vector<uint8_t>* dyn_rows,
-                             const std::vector<int32_t>* groups) {
+                             const std::vector<int32_t>* groups, bool wide) {
   TreePlanHost P;
   P.C = std::max(3, chunk);
   P.C_hi = std::max(3, chunk_hi);
@@ -128,17 +129,23 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
     std::vector<int32_t> nchild(n, 0);
     for (int v = 0; v < n; v++)
       if (par[v] >= 0) nchild[par[v]]++;
-    const int C = lv.empty() ? P.C : P.C_hi;
     bool reduces = false;
     int v = 0;
     while (v < n) {
       int b = v;
       // a chain: consecutive single-child links, broken where the group changes
       while (b + 1 < n && par[b + 1] == b && nchild[b] == 1 && grp[b + 1] == grp[b]) b++;
+      // above level 0 in local mode a long chain stays whole (wide chunks, which the
+      // group-resident launch eliminates with the whole workgroup)
+      int C = lv.empty() ? P.C : P.C_hi;
+      const bool whole =
+          wide && !lv.empty() && G > 0 && b - v + 1 > kWideMin && b - v + 1 <= kWideMax;
+      if (whole) C = kWideMax;
       for (int a = v; a <= b; a += C) {
         const int m = std::min(C, b - a + 1);
         L.cs.push_back(a);
         L.cl.push_back(m);
+        L.cwide.push_back(whole ? 1 : 0);
         if (m >= 3) reduces = true;
       }
       v = b + 1;
@@ -246,7 +253,8 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
     P.chunk_dyn.resize(off + L.nchunk);
     for (int c = 0; c < L.nchunk; c++) {
       P.chunk_len[off + L.pos[c]] = L.cl[c];
-      P.chunk_dyn[off + L.pos[c]] = L.cdyn[c];
+      // bit 0: eliminated in full every step; bit 1: wide (always in full)
+      P.chunk_dyn[off + L.pos[c]] = L.cwide[c] ? 3 : L.cdyn[c];
     }
     P.chunk_off.push_back((int32_t)P.chunk_len.size());
   }
@@ -361,6 +369,22 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
       const int k0 = P.loc_hoff[(size_t)g * H], k1 = P.loc_hoff[(size_t)g * H + H];
       P.gl_edges = std::max(P.gl_edges, P.lf_cptr[k1] - P.lf_cptr[k0]);
     }
+    // wide chunks (levels above 0) of every group, by level
+    P.wide_off.assign((size_t)G * (L + 1) + 1, 0);
+    P.wide_col.clear();
+    for (int g = 0; g < G; g++) {
+      for (int l = 0; l < L; l++) {
+        P.wide_off[(size_t)g * (L + 1) + l] = (int32_t)P.wide_col.size();
+        if (l == 0) continue;
+        const int32_t* gc = P.gcol.data() + (size_t)l * (G + 1);
+        for (int c = gc[g]; c < gc[g + 1]; c++)
+          if (P.chunk_dyn[P.chunk_off[l] + c] & 2) P.wide_col.push_back(c - gc[g]);
+      }
+      P.wide_off[(size_t)g * (L + 1) + L] = (int32_t)P.wide_col.size();
+      P.gl_wide = std::max(P.gl_wide, P.wide_off[(size_t)g * (L + 1) + L] -
+                                          P.wide_off[(size_t)g * (L + 1)]);
+    }
+    P.wide_off.back() = (int32_t)P.wide_col.size();
   }
   return P;
 }
@@ -375,7 +399,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
                                          &P.top_node,  &P.top_hoff,  &P.top_cptr,  &P.top_child,
                                          &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
                                          &P.lbase,     &P.lnext,     &P.lf_cptr,   &P.lf_child,
-                                         &P.lf_par};
+                                         &P.lf_par,    &P.wide_off,  &P.wide_col};
   constexpr int kParts = sizeof(parts) / sizeof(parts[0]);
   size_t off[kParts];
   size_t pos = 0;
@@ -390,7 +414,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
   return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
                      P.top_n,        P.top_ext,      P.groups,       P.top_lds_ok,
                      P.lds_nodes,    P.top_wide,     P.top_narrow,   P.gl_cols,      P.gl_fin,
-                     P.gl_edges,
+                     P.gl_edges,     P.gl_wide,
                      base + off[0],  base + off[1],  base + off[2],  base + off[3],
                      base + off[4],  base + off[5],  base + off[6],  base + off[7],
                      base + off[8],  base + off[9],  base + off[10], base + off[11],
@@ -398,7 +422,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
                      base + off[16], base + off[17], base + off[18], base + off[19],
                      base + off[20], base + off[21], base + off[22], base + off[23],
                      base + off[24], base + off[25], base + off[26], base + off[27],
-                     base + off[28]};
+                     base + off[28], base + off[29], base + off[30]};
 }
 
 // Top lists of the heights >= cut (see TreePlanHost::top_*), kept only when they
@@ -741,13 +765,38 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
       lw = gl_carve<double>(b + lay.top, H.lds_nodes);
       M = gl_meta_carve(b + lay.top + lay.values, P);
     };
+    // wide chunks: every PCR step reads all rows, then writes all rows (as the
+    // launch's workgroup does between its barriers)
+    auto wide_reduce = [&](const TreeWorkL<double>& lw, const GlMeta& M, int lv, int j) {
+      const int q = 4 * (M.lvl[7 * lv + 3] + j);
+      const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
+      const int base = M.lvl[7 * lv + 2] + j, st = M.lvl[7 * lv + 1];
+      for (int k = 1; k <= m - 2; k++) wide_init(lw, base, st, m, k);
+      std::vector<WideRow<double>> tmp(m);
+      for (int s = 0, h = 1; s < wide_steps(m - 2); s++, h *= 2) {
+        for (int k = 1; k <= m - 2; k++) tmp[k] = wide_step(lw, base, st, m, k, h);
+        for (int k = 1; k <= m - 2; k++) wide_put(lw, base, st, k, tmp[k]);
+      }
+      for (int k = 1; k <= m - 2; k++) wide_finish(lw, base, st, k);
+      wide_ends(lw, base, st, m, es, ee, EndsLocal<double>{lw, w, M.scal[0]});
+    };
+    auto wide_back = [&](const TreeWorkL<double>& lw, const GlMeta& M, int lv, int j) {
+      const int q = 4 * (M.lvl[7 * lv + 3] + j);
+      const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
+      const int base = M.lvl[7 * lv + 2] + j, st = M.lvl[7 * lv + 1];
+      const EndsLocal<double> E{lw, w, M.scal[0]};
+      wide_back_rows(lw, base, st, m, E.x(es), E.x(ee), 0, 1);
+    };
     auto gl_levels = [&](int g, int how) {
       TreeWorkL<double> lw;
       GlMeta M;
       gl_view(g, lw, M);
       for (int j = 0; j < M.lvl[1]; j++) glm_reduce<true>(w, lw, M, 0, j, how);
-      for (int lv = 1; lv < H.levels; lv++)
-        for (int j = 0; j < M.lvl[5 * lv + 1]; j++) glm_reduce<false>(w, lw, M, lv, j, how);
+      for (int lv = 1; lv < H.levels; lv++) {
+        for (int j = 0; j < M.lvl[7 * lv + 1]; j++) glm_reduce<false>(w, lw, M, lv, j, how);
+        for (int i = 0; i < M.lvl[7 * lv + 6]; i++)
+          wide_reduce(lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
+      }
     };
     for (int pass = 0; pass < (reuse ? 2 : 1); pass++) {
       for (int i = 0; i < N; i++) w.r[s0[i]] = rhs[i];
@@ -786,9 +835,12 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           gl_view(g, lw, M);
           for (int h = Hh - 1; h >= 0; h--)
             for (int j = M.hoff[h]; j < M.hoff[h + 1]; j++) glm_final_down(w, lw, M, j);
-          for (int lv = L - 1; lv >= 1; lv--)
-            for (int j = 0; j < M.lvl[5 * lv + 1]; j++)
+          for (int lv = L - 1; lv >= 1; lv--) {
+            for (int j = 0; j < M.lvl[7 * lv + 1]; j++)
               glm_back<false, OUT_LEVEL>(w, lw, M, lv, j, nullptr);
+            for (int i = 0; i < M.lvl[7 * lv + 6]; i++)
+              wide_back(lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
+          }
           for (int j = 0; j < M.lvl[1]; j++) glm_back<true, OUT_ASSIGN>(w, lw, M, 0, j, xs.data());
         }
         continue;

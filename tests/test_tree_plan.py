@@ -195,11 +195,14 @@ def test_local_mode(G, orc):
             assert np.max(np.abs(x - ref)) <= 1e-11 * np.max(np.abs(ref)), (chunk, G)
             top_n, staged = tops[0] & 0xFFFF, tops[0] >> 30
             assert top_n > 0 and staged, tops
-            # group-resident form: the same operations on per-group buffers
+            # group-resident form: the same operations on per-group buffers, except
+            # that chains longer than 128 nodes above level 0 (the cable's) are
+            # eliminated by parallel cyclic reduction instead of a serial sweep
             xr, _, _ = tps_host(rp, col, val, rhs, chunk, chunk_hi, dyn=dyn,
                                 rhs_factor=rng.standard_normal(len(rhs)),
                                 groups=-(4096 + G), cut_out=tops)
-            assert np.array_equal(xr, x), (chunk, G)
+            assert np.max(np.abs(xr - ref)) <= 1e-11 * np.max(np.abs(ref)), (chunk, G)
+            assert np.max(np.abs(xr - x)) <= 1e-12 * np.max(np.abs(ref)), (chunk, G)
             assert 0 < tops[1] < len(rhs)
 
 
@@ -215,7 +218,7 @@ def test_local_mode_golden_systems(name, orc):
             continue
         assert np.max(np.abs(x[0] - ref)) <= 1e-12 * np.max(np.abs(ref)), (name, G)
         xr = tps_host(rp, col, val, rhs, 4, 4, groups=-(4096 + G))
-        assert np.array_equal(xr[0], x[0]), (name, G)
+        assert np.max(np.abs(xr[0] - x[0])) <= 1e-12 * np.max(np.abs(ref)), (name, G)
 
 
 def test_local_mode_random_forests():
@@ -252,5 +255,29 @@ def test_local_mode_random_forests():
             solved += 1
             assert np.max(np.abs(r[0] - x0)) <= 1e-12 * np.max(np.abs(x0)), (trial, G)
             xr = tps_host(rp, col, val, b, 4, groups=-(4096 + G))
-            assert np.array_equal(xr[0], r[0]), (trial, G)
+            assert np.max(np.abs(xr[0] - r[0])) <= 1e-12 * np.max(np.abs(x0)), (trial, G)
     assert solved >= 30
+
+
+def test_wide_chunks_pcr():
+    """A long unbranched cable: above level 0 each group's chain stays one wide chunk
+    (plan levels drop), and its parallel cyclic reduction, emulated with every step
+    reading all rows before writing any, matches the sequential elimination."""
+    rng = np.random.default_rng(5)
+    for N, G in ((3000, 2), (9000, 3), (5000, 1)):
+        rp, col, val = [0], [], []
+        for i in range(N):
+            ent = {i: 2.5 + rng.random()}
+            if i > 0:
+                ent[i - 1] = -rng.random()
+            if i + 1 < N:
+                ent[i + 1] = -rng.random()
+            for j in sorted(ent):
+                col.append(j)
+                val.append(ent[j])
+            rp.append(len(col))
+        b = rng.standard_normal(N)
+        x0, lv0, _ = tps_host(rp, col, val, b, 8, 8)
+        xr, lv, _ = tps_host(rp, col, val, b, 8, 8, groups=-(4096 + G))
+        assert lv < lv0, (lv, lv0)
+        assert np.max(np.abs(xr - x0)) <= 1e-12 * np.max(np.abs(x0)), (N, G)
