@@ -164,8 +164,8 @@ struct Implicit {
   DBuf<uint8_t> srole;
   DBuf<int32_t> plan_buf;
   TreePlanDev plan{};
-  DBuf<double> tw64;
-  DBuf<float> tw32;
+  DBuf<double> tw64, tw64alt;  // alt: the second top-input buffer (group-resident form)
+  DBuf<float> tw32, tw32alt;
 };
 
 // ------------------------------------------------------------------ assembly
@@ -1019,6 +1019,7 @@ struct TreeArgs {
   int refine_max;
   int top_lds;  // the launch provides top_lds_bytes<T>() of dynamic LDS (grid launches)
   const int32_t* hh_goff;  // local mode: HH entries of group g are [hh_goff[g], +1)
+  TreeWork<T> walt;        // group-resident form: the second top-input buffer
 };
 
 template <bool GRID>
@@ -1423,8 +1424,8 @@ __device__ __forceinline__ void tps_run_local(const ImpDev& d, const TreePlanDev
 // cyclic reduction of its interior, each step reading every row before any row is
 // written (a barrier between), interior row 1 + tid for thread tid.
 template <class T>
-__device__ __forceinline__ void wide_reduce(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
-                            int lv, int j) {
+__device__ __forceinline__ void wide_reduce(const GlIO<T>& io, const TreeWorkL<T>& lw,
+                                            const GlMeta& M, int lv, int j) {
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   const int q = 4 * (M.lvl[7 * lv + 3] + j);
   const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
@@ -1442,76 +1443,122 @@ __device__ __forceinline__ void wide_reduce(const TreeWork<T>& w, const TreeWork
   }
   for (int k = 1 + tid; k <= m - 2; k += nt) wide_finish(lw, base, st, k);
   __syncthreads();
-  if (tid == 0) wide_ends(lw, base, st, m, es, ee, EndsLocal<T>{lw, w, M.scal[0]});
+  if (tid == 0) wide_ends(lw, base, st, m, es, ee, io.ends(lw, M.scal[0]));
 }
 
 template <class T>
-__device__ __forceinline__ void wide_back(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
-                          int j) {
+__device__ __forceinline__ void wide_back(const GlIO<T>& io, const TreeWorkL<T>& lw,
+                                          const GlMeta& M, int lv, int j) {
   const int q = 4 * (M.lvl[7 * lv + 3] + j);
   const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
   const int base = M.lvl[7 * lv + 2] + j, st = M.lvl[7 * lv + 1];
-  const EndsLocal<T> E{lw, w, M.scal[0]};
+  const EndsLocal<T> E = io.ends(lw, M.scal[0]);
   wide_back_rows(lw, base, st, m, E.x(es), E.x(ee), (int)threadIdx.x, (int)blockDim.x);
 }
 
 // Levels 0 .. L-1 of the group: per-thread chunks, then its wide chunks.
 template <class T>
-__device__ __forceinline__ void res_levels(const TreeWork<T>& w, const TreeWorkL<T>& lw,
+__device__ __forceinline__ void res_levels(const GlIO<T>& io, const TreeWorkL<T>& lw,
                                            const GlMeta& M, int L, int how) {
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   for (int lv = 0; lv < L; lv++) {
     const int nc = M.lvl[7 * lv + 1];
     for (int j = tid; j < nc; j += nt) {
-      if (lv == 0) glm_reduce<true>(w, lw, M, 0, j, how);
-      else glm_reduce<false>(w, lw, M, lv, j, how);
+      if (lv == 0) glm_reduce<true>(io, lw, M, 0, j, how);
+      else glm_reduce<false>(io, lw, M, lv, j, how);
     }
     __syncthreads();
     const int wn = M.lvl[7 * lv + 6];
     for (int i = 0; i < wn; i++) {
-      wide_reduce(w, lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
+      wide_reduce(io, lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
       __syncthreads();
     }
   }
 }
 
 template <class T>
-__device__ __forceinline__ void tps_factor_res(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
-                               int L) {
-  res_levels<T>(w, lw, M, L, kGlFull);
+__device__ __forceinline__ void tps_factor_res(const GlIO<T>& io, const TreeWorkL<T>& lw,
+                                               const GlMeta& M, int L) {
+  res_levels<T>(io, lw, M, L, kGlFull);
+}
+
+// The top of the final forest in the group-resident form: after the grid barrier
+// every workgroup stages this solve's top inputs (io.wt) and solves the top itself
+// in its LDS, all writing the same solution to io.w.x; no second barrier. A
+// workgroup cannot write the next-but-one solve's inputs into the buffer read
+// here before every workgroup has passed the next solve's barrier, that is, has
+// finished this solve.
+template <class T>
+__device__ __forceinline__ void local_top_res(const ImpDev& d, const TreePlanDev& P,
+                                              const GlIO<T>& io, unsigned char* tl) {
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+  const int nh = P.heights - P.fin_cut;
+  if (tl != nullptr) {
+    const TopLds<T> L = TopLds<T>::carve(tl, gl_top_entries(P));
+    const int ns = top_stage_count(P);
+    for (int j = tid; j < ns; j += nt) top_stage(P, io.wt, L, j);
+    __syncthreads();
+    const int hn = P.top_narrow;
+    for (int h = 0; h < hn; h++) {
+      for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_up(L, j);
+      __syncthreads();
+    }
+    if (tid < kWave) {
+      for (int h = hn; h < nh; h++) {
+        const int j = L.hoff[h] + tid;
+        if (j < L.hoff[h + 1]) top_up(L, j);
+        wave_lds_sync();
+      }
+      for (int h = nh - 1; h >= hn; h--) {
+        const int j = L.hoff[h] + tid;
+        if (j < L.hoff[h + 1]) top_down(L, j);
+        wave_lds_sync();
+      }
+    }
+    __syncthreads();
+    for (int h = hn - 1; h >= 0; h--) {
+      for (int j = L.hoff[h] + tid; j < L.hoff[h + 1]; j += nt) top_down(L, j);
+      __syncthreads();
+    }
+    for (int j = tid; j < P.top_n; j += nt) top_store(P, io.w, L, j);
+  }
+  // (the group-resident form is only chosen with its top staged: tl is set)
+  __syncthreads();
 }
 
 template <class T>
-__device__ __forceinline__ void tps_run_res(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
-                            const TreeWorkL<T>& lw, const GlMeta& M, bool acc, bool all_static,
-                            unsigned char* tl) {
+__device__ __forceinline__ void tps_run_res(const ImpDev& d, const TreePlanDev& P,
+                                            const GlIO<T>& io, const TreeWorkL<T>& lw,
+                                            const GlMeta& M, bool acc, bool all_static,
+                                            unsigned char* tl) {
   const int H = P.heights, L = P.levels, how = all_static ? kGlStatic : kGlStep;
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
-  res_levels<T>(w, lw, M, L, how);
+  res_levels<T>(io, lw, M, L, how);
   for (int h = 0; h < H; h++) {
     const int k0 = M.hoff[h], k1 = M.hoff[h + 1];
     if (k0 == k1) continue;  // uniform over the workgroup
-    for (int j = k0 + tid; j < k1; j += nt) glm_final_up(w, lw, M, j);
+    for (int j = k0 + tid; j < k1; j += nt) glm_final_up(io, lw, M, j);
     __syncthreads();
   }
-  if (P.top_n > 0) local_top<T>(d, P, w, tl);
+  if (P.top_n > 0) local_top_res<T>(d, P, io, tl);
   for (int h = H - 1; h >= 0; h--) {
     const int k0 = M.hoff[h], k1 = M.hoff[h + 1];
     if (k0 == k1) continue;
-    for (int j = k0 + tid; j < k1; j += nt) glm_final_down(w, lw, M, j);
+    for (int j = k0 + tid; j < k1; j += nt) glm_final_down(io, lw, M, j);
     __syncthreads();
   }
   for (int lv = L - 1; lv >= 1; lv--) {
     const int nc = M.lvl[7 * lv + 1];
-    for (int j = tid; j < nc; j += nt) glm_back<false, OUT_LEVEL>(w, lw, M, lv, j, nullptr);
+    for (int j = tid; j < nc; j += nt) glm_back<false, OUT_LEVEL>(io, lw, M, lv, j, nullptr);
     const int wn = M.lvl[7 * lv + 6];
-    for (int i = 0; i < wn; i++) wide_back(w, lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
+    for (int i = 0; i < wn; i++) wide_back(io, lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
     __syncthreads();
   }
   const int nc0 = M.lvl[1];
   for (int j = tid; j < nc0; j += nt) {
-    if (acc) glm_back<true, OUT_ADD>(w, lw, M, 0, j, d.x);
-    else glm_back<true, OUT_ASSIGN>(w, lw, M, 0, j, d.x);
+    if (acc) glm_back<true, OUT_ADD>(io, lw, M, 0, j, d.x);
+    else glm_back<true, OUT_ASSIGN>(io, lw, M, 0, j, d.x);
   }
   __syncthreads();
 }
@@ -1612,8 +1659,11 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
     gl_meta_stage(P, g, M, tid, nt);
   }
   grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
-  if (RES) tps_factor_res<T>(t.w, lw, M, P.levels);
+  // group-resident form: the top inputs of solve k go to buffer k & 1 (the factor
+  // phase fills both)
+  if (RES) tps_factor_res<T>(GlIO<T>{t.w, t.w, t.walt, true}, lw, M, P.levels);
   else tps_factor_local<T>(P, t.w, g);
+  int nsolve = 0;
   for (long long step = d.first; step < d.first + d.num; step++) {
     const bool active = step >= d.a0 && step <= d.a1;
     tree_update_local<T>(d, t, active);
@@ -1622,7 +1672,8 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
     double rr = 0, bb = 0;
     bool ok = true;
     if (t.direct) {
-      if (RES) tps_run_res<T>(d, P, t.w, lw, M, false, false, tl);
+      if (RES) tps_run_res<T>(d, P, GlIO<T>{t.w, (nsolve++ & 1) ? t.walt : t.w, t.w, false}, lw, M,
+                              false, false, tl);
       else tps_run_local<T>(d, P, t.w, false, false, tl);
       sweeps = 1;
     } else {
@@ -1657,7 +1708,9 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
           status = NRT_ERR_NOT_CONVERGED;
           break;
         }
-        if (RES) tps_run_res<T>(d, P, t.w, lw, M, true, sweeps > 0, tl);
+        if (RES)
+          tps_run_res<T>(d, P, GlIO<T>{t.w, (nsolve++ & 1) ? t.walt : t.w, t.w, false}, lw, M, true,
+                         sweeps > 0, tl);
         else tps_run_local<T>(d, P, t.w, true, sweeps > 0, tl);
         sweeps++;
       }
@@ -1990,7 +2043,9 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
         GlLayout lay = layout();
         const void* kr = f64 ? (const void*)k_implicit_tree_local<double, true>
                              : (const void*)k_implicit_tree_local<float, true>;
-        e->local_res = lay.total + 1024 <= (size_t)max_lds;
+        // (its top, when it has one, must be staged: every workgroup solves it)
+        e->local_res = lay.total + 1024 <= (size_t)max_lds &&
+                       (e->plan_h.top_n == 0 || e->plan_h.top_lds_ok);
         if (e->local_res) {
           NRT_HIP(hipFuncSetAttribute(kr, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lay.total));
@@ -2105,9 +2160,17 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       if (e->method == 1) {
         e->tw64.alloc(kTreeArrays * T);
         NRT_HIP(hipMemsetAsync(e->tw64.p, 0, sizeof(double) * kTreeArrays * T, s));
+        if (e->local_res) {
+          e->tw64alt.alloc(kTreeArrays * T);
+          NRT_HIP(hipMemsetAsync(e->tw64alt.p, 0, sizeof(double) * kTreeArrays * T, s));
+        }
       } else {
         e->tw32.alloc(kTreeArrays * T);
         NRT_HIP(hipMemsetAsync(e->tw32.p, 0, sizeof(float) * kTreeArrays * T, s));
+        if (e->local_res) {
+          e->tw32alt.alloc(kTreeArrays * T);
+          NRT_HIP(hipMemsetAsync(e->tw32alt.p, 0, sizeof(float) * kTreeArrays * T, s));
+        }
       }
     } else if (o.krylov == NRT_GMRES) {
       e->gmres = true;
@@ -2221,17 +2284,26 @@ static int32_t probe_index(const Implicit* e, int32_t elem) {
 }
 
 template <class T>
-static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b) {
+static TreeWork<T> tree_work(T* b, size_t n) {
+  return TreeWork<T>{b,         b + n,     b + 2 * n, b + 3 * n, b + 4 * n,
+                     b + 5 * n, b + 6 * n, b + 7 * n, b + 8 * n, b + 9 * n};
+}
+
+template <class T>
+static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b, T* alt = nullptr) {
   const size_t T_ = (size_t)e->plan_h.total_nodes();
+  const TreeWork<T> w = tree_work(b, T_);
   return TreeArgs<T>{e->plan,
-                     TreeWork<T>{b, b + T_, b + 2 * T_, b + 3 * T_, b + 4 * T_, b + 5 * T_,
-                                 b + 6 * T_, b + 7 * T_, b + 8 * T_, b + 9 * T_},
-                     e->method == 1 ? 1 : 0, o.refine_tol, o.refine_max,
+                     w,
+                     e->method == 1 ? 1 : 0,
+                     o.refine_tol,
+                     o.refine_max,
                      (e->grid_mode && e->plan_h.top_n > 0) ||
                              (e->local_mode && e->plan_h.top_lds_ok)
                          ? 1
                          : 0,
-                     e->local_mode ? e->hh_goff.p : nullptr};
+                     e->local_mode ? e->hh_goff.p : nullptr,
+                     alt ? tree_work(alt, T_) : w};
 }
 
 // Persistent kernels are launched as ordinary kernels: their grids are sized to be
@@ -2293,12 +2365,12 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
       NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));
   };
   if (e->method == 1)
-    launch(tree_args<double>(e, o, e->tw64.p), (const void*)k_implicit_tree<double, true>,
+    launch(tree_args<double>(e, o, e->tw64.p, e->tw64alt.p), (const void*)k_implicit_tree<double, true>,
            (const void*)k_implicit_tree<double, false>,
            (const void*)(e->local_res ? (const void*)k_implicit_tree_local<double, true>
                                       : (const void*)k_implicit_tree_local<double, false>));
   else
-    launch(tree_args<float>(e, o, e->tw32.p), (const void*)k_implicit_tree<float, true>,
+    launch(tree_args<float>(e, o, e->tw32.p, e->tw32alt.p), (const void*)k_implicit_tree<float, true>,
            (const void*)k_implicit_tree<float, false>,
            (const void*)(e->local_res ? (const void*)k_implicit_tree_local<float, true>
                                       : (const void*)k_implicit_tree_local<float, false>));

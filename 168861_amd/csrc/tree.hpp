@@ -668,11 +668,15 @@ NRT_HD void tps_final_out(const TreePlanDev& P, const TreeWork<T>& w, int v, dou
 // ---- group-resident local mode (TreePlanHost::lbase): level 0 and the top in the
 // global arrays w, every other node of group g in its local arrays lw. The
 // arithmetic is the level-synchronous routines', operand for operand.
+// Values of top nodes go to wt (and wt2 as well when both: the factor phase fills
+// both of a launch's alternating top-input buffers); the top's solution is read
+// from w.x.
 template <class T>
 struct EndsLocal {
   TreeWorkL<T> lw;
-  TreeWork<T> w;
-  int fbase;  // first final-forest slot in w
+  TreeWork<T> w, wt, wt2;
+  int fbase;  // first final-forest slot
+  bool both;
   NRT_HD void put(int e, T d, T up, T dn, T r) const {
     if (e >= 0) {
       lw.d[e] = d;
@@ -681,17 +685,38 @@ struct EndsLocal {
       lw.r[e] = r;
     } else {
       const int t = fbase - 2 - e;
-      w.d[t] = d;
-      w.up[t] = up;
-      w.dn[t] = dn;
-      w.r[t] = r;
+      wt.d[t] = d;
+      wt.up[t] = up;
+      wt.dn[t] = dn;
+      wt.r[t] = r;
+      if (both) {
+        wt2.d[t] = d;
+        wt2.up[t] = up;
+        wt2.dn[t] = dn;
+        wt2.r[t] = r;
+      }
     }
   }
   NRT_HD void put_r(int e, T r) const {
-    if (e >= 0) lw.r[e] = r;
-    else w.r[fbase - 2 - e] = r;
+    if (e >= 0) {
+      lw.r[e] = r;
+    } else {
+      wt.r[fbase - 2 - e] = r;
+      if (both) wt2.r[fbase - 2 - e] = r;
+    }
   }
   NRT_HD T x(int e) const { return e >= 0 ? lw.x[e] : w.x[fbase - 2 - e]; }
+};
+
+// Global arrays of the group-resident form: level 0 and the top's solution (w),
+// the top inputs of this solve (wt; wt2 too when both).
+template <class T>
+struct GlIO {
+  TreeWork<T> w, wt, wt2;
+  bool both;
+  NRT_HD EndsLocal<T> ends(const TreeWorkL<T>& lw, int fbase) const {
+    return EndsLocal<T>{lw, w, wt, wt2, fbase, both};
+  }
 };
 
 // Plan metadata of one group in the group-resident form, staged once per launch
@@ -806,14 +831,15 @@ NRT_HD void gl_meta_stage(const TreePlanDev& P, int g, const GlMeta& M, int tid,
 // change between steps (kGlStep), or with its kept factors (kGlStatic).
 enum { kGlFull = 0, kGlStep = 1, kGlStatic = 2 };
 template <bool L0, class T>
-NRT_HD void glm_reduce(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
+NRT_HD void glm_reduce(const GlIO<T>& io, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
                        int j, int how) {
+  const TreeWork<T>& w = io.w;
   const int c0 = M.lvl[7 * lv], nc = M.lvl[7 * lv + 1];
   const int q = 4 * (M.lvl[7 * lv + 3] + j);
   const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
   if (M.col[q + 3] & 2) return;  // a wide chunk: the workgroup's (wide_reduce)
   const bool full = how == kGlFull || (how == kGlStep && (M.col[q + 3] & 1));
-  const EndsLocal<T> E{lw, w, M.scal[0]};
+  const EndsLocal<T> E = io.ends(lw, M.scal[0]);
   if (L0) {
     const int NC = M.lvl[7 * lv + 4];
     if (full) chunk_reduce(w, c0 + j, NC, m, es, ee, E);
@@ -826,20 +852,22 @@ NRT_HD void glm_reduce(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMet
 }
 
 template <bool L0, int MODE, class T>
-NRT_HD void glm_back(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
+NRT_HD void glm_back(const GlIO<T>& io, const TreeWorkL<T>& lw, const GlMeta& M, int lv,
                      int j, double* out) {
+  const TreeWork<T>& w = io.w;
   const int c0 = M.lvl[7 * lv], nc = M.lvl[7 * lv + 1];
   const int q = 4 * (M.lvl[7 * lv + 3] + j);
   const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
   if (M.col[q + 3] & 2) return;  // a wide chunk: the workgroup's (wide_back)
-  const EndsLocal<T> E{lw, w, M.scal[0]};
+  const EndsLocal<T> E = io.ends(lw, M.scal[0]);
   if (L0) chunk_back<MODE>(w, c0 + j, M.lvl[7 * lv + 4], m, es, ee, E, out);
   else chunk_back<OUT_LEVEL>(lw, M.lvl[7 * lv + 2] + j, nc, m, es, ee, E, out);
 }
 
 // Local final node j of the group (group-relative, in height order).
 template <class T>
-NRT_HD void glm_final_up(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M, int j) {
+NRT_HD void glm_final_up(const GlIO<T>& io, const TreeWorkL<T>& lw, const GlMeta& M, int j) {
+  const TreeWork<T>& w = io.wt;
   const int li = M.scal[1] + j;
   T dv = lw.d[li], rv = lw.r[li];
   for (int q = M.fcp[j]; q < M.fcp[j + 1]; q++) {
@@ -860,8 +888,8 @@ NRT_HD void glm_final_up(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlM
 }
 
 template <class T>
-NRT_HD void glm_final_down(const TreeWork<T>& w, const TreeWorkL<T>& lw, const GlMeta& M,
-                           int j) {
+NRT_HD void glm_final_down(const GlIO<T>& io, const TreeWorkL<T>& lw, const GlMeta& M, int j) {
+  const TreeWork<T>& w = io.w;
   const int li = M.scal[1] + j, p = M.fpar[j];
   T acc = lw.r[li];
   if (p != -1) {

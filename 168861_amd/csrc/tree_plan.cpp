@@ -757,7 +757,8 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
       }
     };
     // group-resident form: every group's values and staged metadata in a buffer
-    // of its own, laid out as the launch lays out its LDS
+    // of its own, laid out as the launch lays out its LDS (one top-input buffer)
+    const GlIO<double> io{w, w, w, false};
     std::vector<std::vector<double>> lbuf;
     const GlLayout lay = gl_layout<double>(P, false);
     auto gl_view = [&](int g, TreeWorkL<double>& lw, GlMeta& M) {
@@ -778,22 +779,22 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
         for (int k = 1; k <= m - 2; k++) wide_put(lw, base, st, k, tmp[k]);
       }
       for (int k = 1; k <= m - 2; k++) wide_finish(lw, base, st, k);
-      wide_ends(lw, base, st, m, es, ee, EndsLocal<double>{lw, w, M.scal[0]});
+      wide_ends(lw, base, st, m, es, ee, io.ends(lw, M.scal[0]));
     };
     auto wide_back = [&](const TreeWorkL<double>& lw, const GlMeta& M, int lv, int j) {
       const int q = 4 * (M.lvl[7 * lv + 3] + j);
       const int m = M.col[q], es = M.col[q + 1], ee = M.col[q + 2];
       const int base = M.lvl[7 * lv + 2] + j, st = M.lvl[7 * lv + 1];
-      const EndsLocal<double> E{lw, w, M.scal[0]};
+      const EndsLocal<double> E = io.ends(lw, M.scal[0]);
       wide_back_rows(lw, base, st, m, E.x(es), E.x(ee), 0, 1);
     };
     auto gl_levels = [&](int g, int how) {
       TreeWorkL<double> lw;
       GlMeta M;
       gl_view(g, lw, M);
-      for (int j = 0; j < M.lvl[1]; j++) glm_reduce<true>(w, lw, M, 0, j, how);
+      for (int j = 0; j < M.lvl[1]; j++) glm_reduce<true>(io, lw, M, 0, j, how);
       for (int lv = 1; lv < H.levels; lv++) {
-        for (int j = 0; j < M.lvl[7 * lv + 1]; j++) glm_reduce<false>(w, lw, M, lv, j, how);
+        for (int j = 0; j < M.lvl[7 * lv + 1]; j++) glm_reduce<false>(io, lw, M, lv, j, how);
         for (int i = 0; i < M.lvl[7 * lv + 6]; i++)
           wide_reduce(lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
       }
@@ -826,7 +827,7 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           GlMeta M;
           gl_view(g, lw, M);
           for (int h = 0; h < Hh; h++)
-            for (int j = M.hoff[h]; j < M.hoff[h + 1]; j++) glm_final_up(w, lw, M, j);
+            for (int j = M.hoff[h]; j < M.hoff[h + 1]; j++) glm_final_up(io, lw, M, j);
         }
         solve_top();
         for (int g = 0; g < G; g++) {
@@ -834,14 +835,14 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           GlMeta M;
           gl_view(g, lw, M);
           for (int h = Hh - 1; h >= 0; h--)
-            for (int j = M.hoff[h]; j < M.hoff[h + 1]; j++) glm_final_down(w, lw, M, j);
+            for (int j = M.hoff[h]; j < M.hoff[h + 1]; j++) glm_final_down(io, lw, M, j);
           for (int lv = L - 1; lv >= 1; lv--) {
             for (int j = 0; j < M.lvl[7 * lv + 1]; j++)
-              glm_back<false, OUT_LEVEL>(w, lw, M, lv, j, nullptr);
+              glm_back<false, OUT_LEVEL>(io, lw, M, lv, j, nullptr);
             for (int i = 0; i < M.lvl[7 * lv + 6]; i++)
               wide_back(lw, M, lv, M.wcol[M.lvl[7 * lv + 5] + i]);
           }
-          for (int j = 0; j < M.lvl[1]; j++) glm_back<true, OUT_ASSIGN>(w, lw, M, 0, j, xs.data());
+          for (int j = 0; j < M.lvl[1]; j++) glm_back<true, OUT_ASSIGN>(io, lw, M, 0, j, xs.data());
         }
         continue;
       }
