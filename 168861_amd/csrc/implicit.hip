@@ -1482,6 +1482,62 @@ __device__ __forceinline__ void tps_factor_res(const GlIO<T>& io, const TreeWork
   res_levels<T>(io, lw, M, L, kGlFull);
 }
 
+// A small top (at most a wave's nodes, at most kLaneKids children each) solved by
+// wave 0 with one node per lane: the node's height, children and parent (local
+// top indices) stay in registers for the launch, so a height costs one round of
+// LDS reads. The arithmetic is top_up/top_down's.
+constexpr int kLaneKids = 4;
+struct TopLane {
+  int h, nk, p;
+  int k[kLaneKids];
+};
+__device__ __forceinline__ TopLane top_lane_load(const TreePlanDev& P, int j) {
+  TopLane t{};
+  t.h = -1;
+  if (j >= P.top_n) return t;
+  const int nh = P.heights - P.fin_cut;
+  for (int h = 0; h < nh; h++)
+    if (j >= P.top_hoff[h] && j < P.top_hoff[h + 1]) t.h = h;
+  t.nk = P.top_cptr[j + 1] - P.top_cptr[j];
+#pragma unroll
+  for (int q = 0; q < kLaneKids; q++) t.k[q] = q < t.nk ? P.top_child[P.top_cptr[j] + q] : 0;
+  t.p = P.top_par[j];
+  return t;
+}
+// the top qualifies for the lane form
+__device__ __forceinline__ bool top_lanes_ok(const TreePlanDev& P) {
+  return P.top_n <= kWave && P.top_kids <= kLaneKids;
+}
+template <class T>
+__device__ __forceinline__ void top_lane_solve(const TopLds<T>& L, const TopLane& t, int j,
+                                               int nh) {
+  for (int h = 0; h < nh; h++) {
+    if (t.h == h) {
+      T dv = L.d[j], rv = L.r[j];
+#pragma unroll
+      for (int q = 0; q < kLaneKids; q++) {
+        if (q < t.nk) {
+          const int c = t.k[q];
+          const T f = L.dn[c] * tps_rcp(L.d[c]);
+          dv = dv - f * L.up[c];
+          rv = rv - f * L.r[c];
+        }
+      }
+      L.d[j] = dv;
+      L.r[j] = rv;
+    }
+    wave_lds_sync();
+  }
+  for (int h = nh - 1; h >= 0; h--) {
+    if (t.h == h) {
+      T acc = L.r[j];
+      if (t.p >= 0) acc = acc - L.up[j] * L.x[t.p];
+      L.x[j] = acc * tps_rcp(L.d[j]);
+    }
+    wave_lds_sync();
+  }
+}
+
 // The top of the final forest in the group-resident form: after the grid barrier
 // every workgroup stages this solve's top inputs (io.wt) and solves the top itself
 // in its LDS, all writing the same solution to io.w.x; no second barrier. A
@@ -1490,11 +1546,19 @@ __device__ __forceinline__ void tps_factor_res(const GlIO<T>& io, const TreeWork
 // finished this solve.
 template <class T>
 __device__ __forceinline__ void local_top_res(const ImpDev& d, const TreePlanDev& P,
-                                              const GlIO<T>& io, unsigned char* tl) {
+                                              const GlIO<T>& io, unsigned char* tl,
+                                              const TopLane& lane) {
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
   const int nh = P.heights - P.fin_cut;
-  if (tl != nullptr) {
+  if (tl != nullptr && top_lanes_ok(P)) {
+    const TopLds<T> L = TopLds<T>::carve(tl, gl_top_entries(P));
+    for (int j = tid; j < P.top_n + P.top_ext; j += nt) top_stage_values(P, io.wt, L, j);
+    __syncthreads();
+    if (tid < kWave) top_lane_solve(L, lane, tid, nh);
+    __syncthreads();
+    for (int j = tid; j < P.top_n; j += nt) top_store(P, io.w, L, j);
+  } else if (tl != nullptr) {
     const TopLds<T> L = TopLds<T>::carve(tl, gl_top_entries(P));
     const int ns = top_stage_count(P);
     for (int j = tid; j < ns; j += nt) top_stage(P, io.wt, L, j);
@@ -1531,7 +1595,7 @@ template <class T>
 __device__ __forceinline__ void tps_run_res(const ImpDev& d, const TreePlanDev& P,
                                             const GlIO<T>& io, const TreeWorkL<T>& lw,
                                             const GlMeta& M, bool acc, bool all_static,
-                                            unsigned char* tl) {
+                                            unsigned char* tl, const TopLane& lane) {
   const int H = P.heights, L = P.levels, how = all_static ? kGlStatic : kGlStep;
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   res_levels<T>(io, lw, M, L, how);
@@ -1541,7 +1605,7 @@ __device__ __forceinline__ void tps_run_res(const ImpDev& d, const TreePlanDev& 
     for (int j = k0 + tid; j < k1; j += nt) glm_final_up(io, lw, M, j);
     __syncthreads();
   }
-  if (P.top_n > 0) local_top_res<T>(d, P, io, tl);
+  if (P.top_n > 0) local_top_res<T>(d, P, io, tl, lane);
   for (int h = H - 1; h >= 0; h--) {
     const int k0 = M.hoff[h], k1 = M.hoff[h + 1];
     if (k0 == k1) continue;
@@ -1664,6 +1728,7 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
   if (RES) tps_factor_res<T>(GlIO<T>{t.w, t.w, t.walt, true}, lw, M, P.levels);
   else tps_factor_local<T>(P, t.w, g);
   int nsolve = 0;
+  const TopLane lane = RES && top_lanes_ok(P) && tid < kWave ? top_lane_load(P, tid) : TopLane{};
   for (long long step = d.first; step < d.first + d.num; step++) {
     const bool active = step >= d.a0 && step <= d.a1;
     tree_update_local<T>(d, t, active);
@@ -1673,7 +1738,7 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
     bool ok = true;
     if (t.direct) {
       if (RES) tps_run_res<T>(d, P, GlIO<T>{t.w, (nsolve++ & 1) ? t.walt : t.w, t.w, false}, lw, M,
-                              false, false, tl);
+                              false, false, tl, lane);
       else tps_run_local<T>(d, P, t.w, false, false, tl);
       sweeps = 1;
     } else {
@@ -1710,7 +1775,7 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
         }
         if (RES)
           tps_run_res<T>(d, P, GlIO<T>{t.w, (nsolve++ & 1) ? t.walt : t.w, t.w, false}, lw, M, true,
-                         sweeps > 0, tl);
+                         sweeps > 0, tl, lane);
         else tps_run_local<T>(d, P, t.w, true, sweeps > 0, tl);
         sweeps++;
       }

@@ -84,6 +84,7 @@ struct TreePlanHost {
   int top_wide = 0;                 // most top nodes at one height
   int top_narrow = 0;               // first top height (relative) from which no height
                                     // holds more than a wave's 64 nodes
+  int top_kids = 0;                 // most children of one top node
   // Group-resident form of local mode: each group's nodes of levels 1..L-1 and its
   // local final nodes get local indices (kept in LDS by the launch). Level lv of
   // group g starts at lbase[g*(L+2)+lv], node k of its column c at
@@ -159,7 +160,7 @@ struct TreePlanDev {
   int levels, heights;
   int fin_cut, sub_groups;
   int top_n, top_ext;
-  int groups, top_lds_ok, lds_nodes, top_wide, top_narrow;
+  int groups, top_lds_ok, lds_nodes, top_wide, top_narrow, top_kids;
   int gl_cols, gl_fin, gl_edges, gl_wide;
   const int32_t* nslots;
   const int32_t* nchunk;
@@ -572,6 +573,17 @@ NRT_HD void top_stage(const TreePlanDev& P, const TreeWork<T>& w, const TopLds<T
   if (j == 0) L.cptr[0] = P.top_cptr[0];
   if (j < P.top_cptr[P.top_n]) L.child[j] = P.top_child[j];
   if (j <= P.heights - P.fin_cut) L.hoff[j] = P.top_hoff[j];
+}
+
+// Values only (the lists of a launch are fixed): entry j < top_n + top_ext.
+template <class T>
+NRT_HD void top_stage_values(const TreePlanDev& P, const TreeWork<T>& w, const TopLds<T>& L,
+                             int j) {
+  const int g = P.node_off[P.levels] + P.top_node[j];
+  L.d[j] = j < P.top_n ? w.d[g] : w.ca[g];
+  L.r[j] = w.r[g];
+  L.up[j] = w.up[g];
+  L.dn[j] = w.dn[g];
 }
 
 template <class T>

@@ -71,6 +71,8 @@ void stage_top_set(TreePlanHost& P, const std::vector<uint8_t>& is_top, int max_
   P.top_lds_ok = nt > 0 && (int)node.size() <= max_entries ? 1 : 0;
   P.top_wide = 0;
   P.top_narrow = 0;
+  P.top_kids = 0;
+  for (int j = 0; j < nt; j++) P.top_kids = std::max(P.top_kids, P.top_cptr[j + 1] - P.top_cptr[j]);
   for (size_t k = 0; k + 1 < P.top_hoff.size(); k++) {
     const int wdt = P.top_hoff[k + 1] - P.top_hoff[k];
     P.top_wide = std::max(P.top_wide, wdt);
@@ -413,7 +415,8 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
   }
   return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
                      P.top_n,        P.top_ext,      P.groups,       P.top_lds_ok,
-                     P.lds_nodes,    P.top_wide,     P.top_narrow,   P.gl_cols,      P.gl_fin,
+                     P.lds_nodes,    P.top_wide,     P.top_narrow,   P.top_kids,     P.gl_cols,
+                     P.gl_fin,
                      P.gl_edges,     P.gl_wide,
                      base + off[0],  base + off[1],  base + off[2],  base + off[3],
                      base + off[4],  base + off[5],  base + off[6],  base + off[7],
