@@ -1641,7 +1641,11 @@ __device__ __forceinline__ void tree_update_local(const ImpDev& d, const TreeArg
   T* __restrict__ rt = t.w.r;
   double* __restrict__ b = d.b;
   const double dT = d.dT;
-  for (int hh = t.hh_goff[g] + tid; hh < t.hh_goff[g + 1]; hh += nt) {
+  // HH rows are long dependent chains (gating); when they fill at most half of
+  // the workgroup they get whole waves of their own and the plain slots the rest
+  const int hh0 = t.hh_goff[g], nhh = t.hh_goff[g + 1] - hh0;
+  const int hthr = nhh > 0 && 2 * nhh <= nt ? (nhh + kWave - 1) / kWave * kWave : 0;
+  for (int hh = hh0 + tid; hh < hh0 + nhh && (hthr == 0 || tid < hthr); hh += hthr ? hthr : nt) {
     const int i = d.hh.row[hh];
     const int si = d.hh_slot[hh];
     const double v = xv[si];
@@ -1655,13 +1659,14 @@ __device__ __forceinline__ void tree_update_local(const ImpDev& d, const TreeArg
   }
   const int NC = t.P.nchunk[0], c0 = t.P.gcol[g], nc = t.P.gcol[g + 1] - c0;
   const int ns = nc * (t.P.nslots[0] / NC);
-  for (int j0 = tid; j0 < ns; j0 += kTpsBatch * nt) {
+  const int ptid = tid - hthr, pnt = nt - hthr;
+  for (int j0 = ptid; ptid >= 0 && j0 < ns; j0 += kTpsBatch * pnt) {
     int si[kTpsBatch];
     uint32_t bc[kTpsBatch];
     double bv[kTpsBatch], bm[kTpsBatch], bk[kTpsBatch], bin[kTpsBatch];
 #pragma unroll
     for (int j = 0; j < kTpsBatch; j++) {
-      const int q = j0 + j * nt;
+      const int q = j0 + j * pnt;
       si[j] = q < ns ? (q / nc) * NC + c0 + q % nc : -1;
       if (si[j] >= 0) {
         bc[j] = code[si[j]];
@@ -2021,14 +2026,15 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // Local mode (tree_groups > 1): one workgroup of kLocalBlock threads per row
       // group, each reducing and back-substituting its own rows alone; falls back
       // to the other forms when the plan has no reduction level or the HH entries
-      // are not in row order. Automatic for the FP64 solve from 16k rows: a group
-      // of ~5000 rows per workgroup, at most 64 workgroups, past which the top of
-      // the final forest (one workgroup, height by height) costs more than the
-      // groups save (measured: 20,202-row cable 70 -> 45 us per step with 4
-      // groups; depth-14 tree of 902,037 rows 389 -> 129 us with 64, 146 with 128).
+      // are not in row order. Automatic for the FP64 solve from 16k rows: ~3,400
+      // rows per workgroup, at most 64 workgroups, past which the top of the final
+      // forest costs more than the groups save (measured per step: 20,202-row
+      // cable 70 us in one workgroup, 34.8 with 4 groups, 33.2 with 6, 34.5 with
+      // 8; depth-14 tree of 902,037 rows 389 us in grid mode, 129 with 64
+      // groups, 146 with 128).
       int G = batch_threads > 0 ? 0 : o.tree_groups;
       if (G == 0 && batch_threads == 0 && e->method == 1 && o.num_blocks == 0 && N >= (1 << 14))
-        G = std::min(64, (N + 2500) / 5000);
+        G = std::min(64, (N + 1700) / 3400);
       if (G > 1) {
         for (int hh = 1; hh < H; hh++)
           if (p.hh_elem[hh] < p.hh_elem[hh - 1]) G = 0;
