@@ -2026,14 +2026,14 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // Local mode (tree_groups > 1): one workgroup of kLocalBlock threads per row
       // group, each reducing and back-substituting its own rows alone; falls back
       // to the other forms when the plan has no reduction level or the HH entries
-      // are not in row order. Automatic for the FP64 solve from 16k rows: ~3,400
+      // are not in row order. Automatic for the tree solve from 16k rows: ~3,400
       // rows per workgroup, at most 64 workgroups, past which the top of the final
-      // forest costs more than the groups save (measured per step: 20,202-row
+      // forest costs more than the groups save (measured per step, FP64: 20,202-row
       // cable 70 us in one workgroup, 34.8 with 4 groups, 33.2 with 6, 34.5 with
       // 8; depth-14 tree of 902,037 rows 389 us in grid mode, 129 with 64
-      // groups, 146 with 128).
+      // groups, 146 with 128; FP32 + refinement: 263 -> 145 us and 799 -> 327 us).
       int G = batch_threads > 0 ? 0 : o.tree_groups;
-      if (G == 0 && batch_threads == 0 && e->method == 1 && o.num_blocks == 0 && N >= (1 << 14))
+      if (G == 0 && batch_threads == 0 && e->method != 0 && o.num_blocks == 0 && N >= (1 << 14))
         G = std::min(64, (N + 1700) / 3400);
       if (G > 1) {
         for (int hh = 1; hh < H; hh++)
