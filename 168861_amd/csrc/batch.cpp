@@ -33,9 +33,9 @@ void explicit_batch_step(const void* slots, const int2* blk, int nblk, int nslot
                          long long* gstep, hipStream_t s);
 const double* explicit_slot_v(nrt_handle* h, long long done);
 int64_t explicit_device_bytes(const Explicit* e);
+int explicit_batch_block_elems();
 
 namespace {
-constexpr int kExBlock = 256;            // elements per block of the explicit batch
 constexpr int kTuneThreads[] = {256, 512, 1024};
 constexpr int kPlanThreads = 512;        // workers assumed by a slot's chunk choice
 // Footprint per element used for admission (bytes; the arrays a slot allocates,
@@ -149,7 +149,7 @@ struct nrt_batch {
         Slot& s = *slots[order[k]];
         explicit_slot_fill(s.h, s.s0, s.num_steps, s.a0, s.a1, s.trace_elem, s.interval,
                            s.trace.p, host.data() + k * sb);
-        for (int e0 = 0; e0 < s.N; e0 += kExBlock) bl.push_back(make_int2((int)k, e0));
+        for (int e0 = 0; e0 < s.N; e0 += explicit_batch_block_elems()) bl.push_back(make_int2((int)k, e0));
       }
       table.upload(host, stream);
       blk.upload(bl, stream);

@@ -93,15 +93,79 @@ __device__ __forceinline__ void hh_gate(const HHDev& hd, int i, double v, double
   kdx = K * dx;
 }
 
-// Explicit step of element j: reads Vo, writes Vn[j] and its sealed-end copies.
-__device__ __forceinline__ void explicit_element(
-    int j, const double* __restrict__ Vo, double* __restrict__ Vn,
-    const double* __restrict__ coefA, const double* __restrict__ rdx,
-    const double* __restrict__ wdx, const double* __restrict__ kdx,
-    const uint32_t* __restrict__ code, const ExRec* __restrict__ recs, const HHDev& hd,
-    double dt_ms, bool active) {
-  const uint32_t c = code[j];
-  const double v = Vo[j];
+#define NRT_GLB __attribute__((address_space(1)))
+template <class P>
+__device__ __forceinline__ NRT_GLB P* glb(P* p) {
+  return (NRT_GLB P*)p;
+}
+// Empty asm statements that take a value: the value (and so the load that makes it)
+// is complete at this point, which keeps the compiler from sinking a load below a
+// later branch.
+template <class X>
+__device__ __forceinline__ void ready_v(X x) {
+  asm volatile("" ::"v"(x));
+}
+template <class X>
+__device__ __forceinline__ void ready_s(X x) {
+  asm volatile("" ::"s"(x));
+}
+
+// Explicit step of element j of N: reads Vo, writes Vn[j] and its sealed-end copies.
+// A plain chain element (no record) couples to j - 1 and j + 1; its operands are
+// fetched with its code (ex_load), before the code is known, so a thread waits for
+// memory once (the clamped neighbour indices equal j -/+ 1 wherever the element is
+// plain). Pointers go through the global address space: the batch kernel reads them
+// from its slot records, where the compiler cannot tell that they are global.
+struct ExOps {
+  uint32_t c;
+  double v, vm, vp, rj, rp, aj, w0, k0;
+};
+
+__device__ __forceinline__ ExOps ex_load(int j, int N, const double* __restrict__ Vo_,
+                                         const double* __restrict__ coefA_,
+                                         const double* __restrict__ rdx_,
+                                         const double* __restrict__ wdx_,
+                                         const double* __restrict__ kdx_,
+                                         const uint32_t* __restrict__ code_) {
+  const NRT_GLB double* __restrict__ Vo = glb(Vo_);
+  const NRT_GLB double* __restrict__ rdx = glb(rdx_);
+  const int jm = j > 0 ? j - 1 : j, jp = j + 1 < N ? j + 1 : j;
+  ExOps o;
+  o.c = glb(code_)[j];
+  o.v = Vo[j];
+  o.vm = Vo[jm];
+  o.vp = Vo[jp];
+  o.rj = rdx[j];
+  o.rp = rdx[jp];
+  o.aj = glb(coefA_)[j];
+  o.w0 = glb(wdx_)[j];
+  o.k0 = glb(kdx_)[j];
+  return o;
+}
+
+// The loads of ex_load complete here (not later, under the branches on the code).
+__device__ __forceinline__ void ex_ready(const ExOps& o) {
+  ready_v(o.c);
+  ready_v(o.v);
+  ready_v(o.vm);
+  ready_v(o.vp);
+  ready_v(o.rj);
+  ready_v(o.rp);
+  ready_v(o.aj);
+  ready_v(o.w0);
+  ready_v(o.k0);
+}
+
+__device__ __forceinline__ void ex_finish(int j, const ExOps& o, const double* __restrict__ Vo_,
+                                          double* __restrict__ Vn_,
+                                          const double* __restrict__ rdx_,
+                                          const ExRec* __restrict__ recs_, const HHDev& hd,
+                                          double dt_ms, bool active) {
+  const NRT_GLB double* __restrict__ Vo = glb(Vo_);
+  NRT_GLB double* __restrict__ Vn = glb(Vn_);
+  const NRT_GLB double* __restrict__ rdx = glb(rdx_);
+  const uint32_t c = o.c;
+  const double v = o.v;
   if (c & kCodeSkip) {
     // Terminal / ghost: its potential is a sealed-end copy written by its source,
     // but an HH terminal still advances its gates (solverEx-cpu.cpp:62).
@@ -111,38 +175,45 @@ __device__ __forceinline__ void explicit_element(
     }
     return;
   }
-  int mo = j - 1, dr = j + 1, dl = -1, hh = -1;
-  const ExRec* rec = nullptr;
-  if (c & kCodeRec) {
-    rec = &recs[c & kCodeIdx];
-    mo = rec->mother;
-    dr = rec->dr;
-    dl = (rec->flags & 1) ? rec->dl : -1;
-    hh = rec->hh;
-  } else if (c & kCodeHH) {
-    hh = (int)(c & kCodeIdx);
+  if (!(c & kCodeRec)) {
+    double wd = o.w0, kd = o.k0;
+    if (c & kCodeHH) hh_gate(hd, (int)(c & kCodeIdx), v, dt_ms, wd, kd);
+    // DR term + mother term + W*dX*V + K*dX
+    double s = (o.vp - v) / o.rp;
+    s = s + (o.vm - v) / o.rj;
+    s = s + wd * v;
+    s = s + kd;
+    Vn[j] = v + o.aj * s;
+    return;
   }
-  double wd, kd;
-  if (hh >= 0) {
-    hh_gate(hd, hh, v, dt_ms, wd, kd);
-  } else {
-    wd = wdx[j];
-    kd = kdx[j];
-  }
+  const NRT_GLB ExRec& rec = glb(recs_)[c & kCodeIdx];
+  const int mo = rec.mother, dr = rec.dr, hh = rec.hh, fl = rec.flags;
+  const int dl = (fl & 1) ? rec.dl : -1;
+  double wd = o.w0, kd = o.k0;
+  if (hh >= 0) hh_gate(hd, hh, v, dt_ms, wd, kd);
   // branching*(DL term) + DR term + mother term + W*dX*V + K*dX + input
   const double tdr = (Vo[dr] - v) / rdx[dr];
   double s = (dl >= 0) ? ((Vo[dl] - v) / rdx[dl] + tdr) : tdr;
-  s = s + (Vo[mo] - v) / rdx[j];
+  s = s + (Vo[mo] - v) / o.rj;
   s = s + wd * v;
   s = s + kd;
-  if (rec && (rec->flags & 2) && active) s = s + rec->input;
-  const double vn = v + coefA[j] * s;
+  if ((fl & 2) && active) s = s + rec.input;
+  const double vn = v + o.aj * s;
   Vn[j] = vn;
-  if (rec) {
 #pragma unroll
-    for (int t = 0; t < 3; t++)
-      if (rec->tgt[t] >= 0) Vn[rec->tgt[t]] = vn;
-  }
+  for (int t = 0; t < 3; t++)
+    if (rec.tgt[t] >= 0) Vn[rec.tgt[t]] = vn;
+}
+
+__device__ __forceinline__ void explicit_element(
+    int j, int N, const double* __restrict__ Vo, double* __restrict__ Vn,
+    const double* __restrict__ coefA, const double* __restrict__ rdx,
+    const double* __restrict__ wdx, const double* __restrict__ kdx,
+    const uint32_t* __restrict__ code, const ExRec* __restrict__ recs, const HHDev& hd,
+    double dt_ms, bool active) {
+  const ExOps o = ex_load(j, N, Vo, coefA, rdx, wdx, kdx, code);
+  ex_ready(o);
+  ex_finish(j, o, Vo, Vn, rdx, recs, hd, dt_ms, active);
 }
 
 // One explicit step for every element (thread per element).
@@ -154,7 +225,7 @@ __global__ __launch_bounds__(256) void k_explicit_fp64(
     int active) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
-  explicit_element(j, Vo, Vn, coefA, rdx, wdx, kdx, code, recs, hd, dt_ms, active != 0);
+  explicit_element(j, N, Vo, Vn, coefA, rdx, wdx, kdx, code, recs, hd, dt_ms, active != 0);
 }
 
 Explicit* explicit_create(nrt_handle* hnd, const HostProblem& p) {
@@ -330,6 +401,8 @@ void explicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64
 // Slots of an explicit batch: one thread per element over all slots, one launch
 // per step; blocks map to (slot, first element). The batch step counter lives on
 // the device so a captured sequence of steps can be replayed.
+constexpr int kExEPT = 1;  // elements per thread of the batch kernel (256 threads a block)
+
 struct ExSlot {
   int N;
   int cur0;  // buffer holding V when the slot joined
@@ -352,15 +425,51 @@ __global__ __launch_bounds__(256) void k_explicit_batch(const ExSlot* __restrict
                                                         const long long* __restrict__ gstep) {
   const int2 b = blk[blockIdx.x];
   const ExSlot& S = slots[b.x];
-  const long long done = *gstep - S.s0;
-  if (done < 0 || done >= S.num_steps) return;
-  const int j = b.y + (int)threadIdx.x;
-  if (j >= S.N) return;
-  const int cur = (S.cur0 + (int)(done & 1)) & 1;
-  const long long step = done + 1;
-  explicit_element(j, S.V[cur], S.V[1 - cur], S.coefA, S.rdx, S.wdx, S.kdx, S.code, S.recs,
-                   S.hd, S.dt_ms, step >= S.a0 && step <= S.a1);
+  // the slot's fields are read before the one exit test (not short-circuit), so the
+  // scalar loads are in flight together
+  const long long s0 = S.s0, nst = S.num_steps, g = *gstep, a0 = S.a0, a1 = S.a1;
+  const int N = S.N, cur0 = S.cur0;
+  const double* V0 = S.V[0];
+  const double* V1 = S.V[1];
+  const double* coefA = S.coefA;
+  const double* rdx = S.rdx;
+  const double* wdx = S.wdx;
+  const double* kdx = S.kdx;
+  const uint32_t* code = S.code;
+  const ExRec* recs = S.recs;
+  ready_s(V0);
+  ready_s(V1);
+  ready_s(coefA);
+  ready_s(rdx);
+  ready_s(wdx);
+  ready_s(kdx);
+  ready_s(code);
+  ready_s(recs);
+  const long long done = g - s0;
+  const int j0 = b.y + (int)threadIdx.x;
+  if ((done < 0) | (done >= nst) | (j0 >= N)) return;
+  const bool odd = ((cur0 + (int)(done & 1)) & 1) != 0;
+  const double* Vo = odd ? V1 : V0;
+  double* Vn = odd ? (double*)V0 : (double*)V1;
+  const bool active = done + 1 >= a0 && done + 1 <= a1;
+  // kExEPT elements per thread, 256 apart: every element's loads are issued before
+  // any is waited for (an element past the slot's end loads element j0 again)
+  ExOps o[kExEPT];
+#pragma unroll
+  for (int u = 0; u < kExEPT; u++) {
+    const int j = j0 + u * 256;
+    o[u] = ex_load(j < N ? j : j0, N, Vo, coefA, rdx, wdx, kdx, code);
+  }
+#pragma unroll
+  for (int u = 0; u < kExEPT; u++) ex_ready(o[u]);
+#pragma unroll
+  for (int u = 0; u < kExEPT; u++) {
+    const int j = j0 + u * 256;
+    if (j < N) ex_finish(j, o[u], Vo, Vn, rdx, recs, S.hd, S.dt_ms, active);
+  }
 }
+
+int explicit_batch_block_elems() { return 256 * kExEPT; }
 
 // After a step: the probe samples due at it, then the step counter advances.
 __global__ void k_explicit_batch_trace(const ExSlot* __restrict__ slots, int nslots,
