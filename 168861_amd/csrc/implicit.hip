@@ -1046,7 +1046,8 @@ __device__ void tps_factor(const ImpDev& d, const TreePlanDev& P, const TreeWork
 // factors; all_static: the matrix did not change since the last solve (a later
 // refinement sweep of the same step), every chunk reuses its factors.
 template <class T, bool GRID>
-__device__ void tps_run(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w, bool acc,
+__device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
+                                        const TreeWork<T>& w, bool acc,
                         bool all_static, unsigned char* tl) {
   const int wid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
   const int nw = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
@@ -1851,28 +1852,35 @@ struct SlotWork {
 };
 
 template <class T, int BS>
-__global__ __launch_bounds__(BS) void k_batch_tree(const BatchSlot<T>* __restrict__ slots,
+__global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __restrict__ slots,
                                                   const SlotWork* __restrict__ work,
                                                   const int2* __restrict__ wgmap) {
   __shared__ double lds[4 * 16];
   const int2 m = wgmap[blockIdx.x];  // (slot, workgroup index in its grid)
   const SlotWork wk = work[m.x];
   if (wk.steps <= 0) return;
-  const BatchSlot<T>& S = slots[m.x];
-  ImpDev d = S.d;
+  // The slot records are read through the constant address space (the kernel never
+  // writes them): pointers loaded from there are known to be global, so the solve's
+  // loads and stores compile to global, not generic (flat), instructions.
+  const __attribute__((address_space(4))) BatchSlot<T>* cs =
+      (const __attribute__((address_space(4))) BatchSlot<T>*)slots;
+  ImpDev d;
+  __builtin_memcpy(&d, (const void*)&cs[m.x].d, sizeof(ImpDev));
+  const TreeArgs<T>& t = *(const TreeArgs<T>*)&cs[m.x].t;
+  double* const trace_base = cs[m.x].trace_base;
   d.first = wk.done + 1;
   d.num = wk.steps;
   if (d.interval >= 0) {
     d.gl0 = wk.done % (d.interval + 1);
-    d.trace = S.trace_base ? S.trace_base + wk.done / (d.interval + 1) : nullptr;
+    d.trace = trace_base ? trace_base + wk.done / (d.interval + 1) : nullptr;
   }
   d.vb = m.y;
   d.nblocks = wk.vn;
   if (wk.vn > 1) {
     d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
-    tree_steps<T, true>(d, S.t, lds, nullptr);
+    tree_steps<T, true>(d, t, lds, nullptr);
   } else {
-    tree_steps<T, false>(d, S.t, lds, nullptr);
+    tree_steps<T, false>(d, t, lds, nullptr);
   }
 }
 
