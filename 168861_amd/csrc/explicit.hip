@@ -115,7 +115,9 @@ __device__ __forceinline__ void ready_s(X x) {
 // fetched with its code (ex_load), before the code is known, so a thread waits for
 // memory once (the clamped neighbour indices equal j -/+ 1 wherever the element is
 // plain). Pointers go through the global address space: the batch kernel reads them
-// from its slot records, where the compiler cannot tell that they are global.
+// from its slot records, where the compiler cannot tell that they are global. The
+// per-element constants and the new potential of a plain element stream through
+// once a step (nontemporal); the potentials are read by the neighbours too (cached).
 struct ExOps {
   uint32_t c;
   double v, vm, vp, rj, rp, aj, w0, k0;
@@ -131,15 +133,15 @@ __device__ __forceinline__ ExOps ex_load(int j, int N, const double* __restrict_
   const NRT_GLB double* __restrict__ rdx = glb(rdx_);
   const int jm = j > 0 ? j - 1 : j, jp = j + 1 < N ? j + 1 : j;
   ExOps o;
-  o.c = glb(code_)[j];
+  o.c = __builtin_nontemporal_load(glb(code_) + j);
   o.v = Vo[j];
   o.vm = Vo[jm];
   o.vp = Vo[jp];
   o.rj = rdx[j];
   o.rp = rdx[jp];
-  o.aj = glb(coefA_)[j];
-  o.w0 = glb(wdx_)[j];
-  o.k0 = glb(kdx_)[j];
+  o.aj = __builtin_nontemporal_load(glb(coefA_) + j);
+  o.w0 = __builtin_nontemporal_load(glb(wdx_) + j);
+  o.k0 = __builtin_nontemporal_load(glb(kdx_) + j);
   return o;
 }
 
@@ -183,7 +185,7 @@ __device__ __forceinline__ void ex_finish(int j, const ExOps& o, const double* _
     s = s + (o.vm - v) / o.rj;
     s = s + wd * v;
     s = s + kd;
-    Vn[j] = v + o.aj * s;
+    __builtin_nontemporal_store(v + o.aj * s, Vn + j);
     return;
   }
   const NRT_GLB ExRec& rec = glb(recs_)[c & kCodeIdx];
