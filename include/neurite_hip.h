@@ -191,7 +191,7 @@ typedef struct nrt_options {
                            threads, each owning a contiguous share of the rows
                            (its chunks and final nodes), two grid barriers per
                            solve (0 -> auto: tree solve from 16384 rows with
-                           num_blocks 0, N/3400 groups up to 64; 1 -> off;
+                           num_blocks 0, N/2900 groups up to 64; 1 -> off;
                            ignored in a batch)                                    */
   int32_t reserved[2];
 } nrt_options;
