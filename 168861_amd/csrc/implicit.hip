@@ -2035,7 +2035,7 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // group, each reducing and back-substituting its own rows alone; falls back
       // to the other forms when the plan has no reduction level or the HH entries
       // are not in row order. Automatic for the tree solve from 16k rows: ~2,900
-      // rows per workgroup, at most 64 workgroups, past which the top of the final
+      // rows per workgroup (FP32: ~3,400), at most 64 workgroups, past which the top of the final
       // forest costs more than the groups save (measured per step, FP64: 20,202-row
       // cable 70 us in one workgroup; with level-0 chunks of one per thread, 4
       // groups 33.7 us, 7 groups 31.7, 8 groups 32.0, 12 groups 33.9; depth-14
@@ -2043,7 +2043,7 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // 146 with 128; FP32 + refinement: 263 -> 138 us and 799 -> 327 us).
       int G = batch_threads > 0 ? 0 : o.tree_groups;
       if (G == 0 && batch_threads == 0 && e->method != 0 && o.num_blocks == 0 && N >= (1 << 14))
-        G = std::min(64, (N + 1450) / 2900);
+        G = std::min(64, e->method == 1 ? (N + 1450) / 2900 : (N + 1700) / 3400);
       if (G > 1) {
         for (int hh = 1; hh < H; hh++)
           if (p.hh_elem[hh] < p.hh_elem[hh - 1]) G = 0;
@@ -2080,12 +2080,15 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // level shrinks a chain by C/2). Grid launches take 128-row chunks: every
       // level costs grid barriers, and with chunks as long as a dendrite's
       // segments one level is left (depth-14 tree: 0.47 -> 0.40 ms per step).
-      // Local mode: one level-0 chunk per thread of a group, down to 5 rows (a
-      // group with more chunks than threads sweeps two in a row: 20,202-row cable
-      // with 8 groups and 4-row chunks 74 us per step, 5-row chunks 32 us).
+      // Local mode: one level-0 chunk per thread of a group, down to 5 rows for
+      // FP64 (a group with more chunks than threads sweeps two in a row: 20,202-row
+      // cable with 8 groups and 4-row chunks 74 us per step, 5-row chunks 32 us),
+      // 8 for FP32 (its refinement took 4 sweeps a step instead of 3 with 6- and
+      // 7-row chunks on that cable: 176 us per step against 142).
       const int workers = e->nblocks * e->threads;
       int auto0 = e->grid_mode ? 128 : std::min(64, std::max(8, (N + workers - 1) / workers));
-      if (e->local_mode) auto0 = std::min(32, std::max(5, (N + workers - 1) / workers));
+      if (e->local_mode)
+        auto0 = std::min(32, std::max(e->method == 1 ? 5 : 8, (N + workers - 1) / workers));
       const int chunk = o.chunk > 0 ? o.chunk : auto0;
       const int chunk_hi = o.chunk_hi > 0 ? o.chunk_hi : (o.chunk > 0 ? o.chunk : 8);
       // HH rows change their diagonal every step; every other chunk keeps its factors

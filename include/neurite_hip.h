@@ -189,9 +189,9 @@ typedef struct nrt_options {
                            or 1024 (0 -> autotuned on the first launches)        */
   int32_t tree_groups;  /* tree solve, local mode: this many workgroups of 512
                            threads, each owning a contiguous share of the rows
-                           (its chunks and final nodes), two grid barriers per
-                           solve (0 -> auto: tree solve from 16384 rows with
-                           num_blocks 0, N/2900 groups up to 64; 1 -> off;
+                           (its chunks and final nodes), one or two grid
+                           barriers per solve (0 -> auto: tree solve from 16384 rows with
+                           num_blocks 0, N/2900 groups (FP32: N/3400) up to 64; 1 -> off;
                            ignored in a batch)                                    */
   int32_t reserved[2];
 } nrt_options;
