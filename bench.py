@@ -343,8 +343,8 @@ def main():
                     help="depth-14 dendritic tree: Arnoldi steps allowed per time step")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU "
-              "decomposition is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the solvers have "
+              "no multi-GPU path (each neurite or batch runs on one device)", file=sys.stderr)
         sys.exit(2)
 
     import neurite_hip as nh
