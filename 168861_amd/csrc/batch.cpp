@@ -37,7 +37,10 @@ int explicit_batch_block_elems();
 
 namespace {
 constexpr int kTuneThreads[] = {256, 512, 1024};
-constexpr int kPlanThreads = 512;        // workers assumed by a slot's chunk choice
+// workers assumed by a slot's chunk choice: the size the workgroup autotuner
+// picks for the FP64 sweep (4,096 trees: planned for 512, 2.16 ms per step; for
+// 256, 2.06; with level-0 chunks of at most 16 rows, 2.00)
+constexpr int kPlanThreads = 256;
 // Footprint per element used for admission (bytes; the arrays a slot allocates,
 // rounded up): implicit tree solve ~ 7 FP64 + 3 int32 vectors, 3 CSR entries, the
 // plan and 8 tree work values over ~1.1 N level nodes; explicit ~ 6 FP64 + code.

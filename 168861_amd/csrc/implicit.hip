@@ -2089,6 +2089,7 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       int auto0 = e->grid_mode ? 128 : std::min(64, std::max(8, (N + workers - 1) / workers));
       if (e->local_mode)
         auto0 = std::min(32, std::max(e->method == 1 ? 5 : 8, (N + workers - 1) / workers));
+      if (batch_threads > 0) auto0 = std::min(16, auto0);  // batch slots: at most 16 rows
       const int chunk = o.chunk > 0 ? o.chunk : auto0;
       const int chunk_hi = o.chunk_hi > 0 ? o.chunk_hi : (o.chunk > 0 ? o.chunk : 8);
       // HH rows change their diagonal every step; every other chunk keeps its factors
