@@ -7,7 +7,7 @@ import pytest
 
 import neurite_hip as nh
 import workloads as wl
-from common import geometry_from_golden, load_golden, problem_from_golden
+from common import geometry_from_golden, ghost_mask, load_golden, problem_from_golden
 
 
 def _setup(orc, geom):
@@ -231,3 +231,22 @@ def test_tree_generator_numbering():
             assert dr[j] == j + 1 and mo[j + 1] == j
         if br[j]:
             assert mo[dl[j]] == j and dl[j] > j
+
+
+def test_bicgstab_from_rhs_converges_on_hh_y(orc):
+    """The configuration smoke() runs on the GPU: BiCGSTAB (the reference GPU solver's
+    method) from x0 = b at relative 1e-12, tested every iteration, on the HH Y-branch at
+    100x the critical step. Every step converges within max_iter = N, and V stays within
+    1e-9 V of the exact tree solve after 50 steps."""
+    g = load_golden("ex3_hh_y")
+    p = problem_from_golden(g, dT=g["dT_crit"] * 100)
+    ex = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    ex.run(1, 50, 1, 50)
+    it = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_BICGSTAB, tol=1e-12, tol_mode=1,
+                      x0_mode=1, check_every=1)
+    for step in range(1, 51):
+        it.update(True)
+        assert it.calculate() == 0, f"step {step}"
+        assert it.stats()[0] < p.N
+    m = ghost_mask(p)
+    assert np.max(np.abs(it.snapshot() - ex.snapshot())[m]) <= 1e-9
