@@ -206,6 +206,33 @@ def test_local_mode(G, orc):
             assert 0 < tops[1] < len(rhs)
 
 
+@pytest.mark.parametrize("G,chunk,chunk_hi", [(7, 6, 5), (7, 6, 8), (6, 8, 8), (8, 5, 5)])
+def test_local_mode_cable_default_plans(G, chunk, chunk_hi, orc):
+    """The plans the automatic choice gives the 20,202-row cable (FP64: 7 groups, 6-row
+    level-0 chunks, 5-row reduced chunks; FP32: 6 groups, 8 and 8) and their neighbours:
+    one level-0 chunk per thread (at most 512 per group), and the local and
+    group-resident forms both match the oracle's sequential elimination."""
+    geom = wl.myelinated_axon()[0]
+    s, bufs, H = _setup(orc, geom)
+    p = _problem(geom, s, bufs, H, 1e-5)
+    rp, col, val, rhs = _system(orc, p)
+    assert (len(rhs) + G - 1) // G <= 512 * chunk
+    dyn = np.zeros(len(rhs), np.uint8)
+    dyn[np.asarray(p.hh_elem)] = 1
+    rng = np.random.default_rng(G * 100 + chunk)
+    ref = orc.tree_solve(rp, col, val, rhs)
+    tops = []
+    x, lv, _ = tps_host(rp, col, val, rhs, chunk, chunk_hi, dyn=dyn,
+                        rhs_factor=rng.standard_normal(len(rhs)), groups=-G, cut_out=tops)
+    assert lv >= 1
+    assert np.max(np.abs(x - ref)) <= 1e-11 * np.max(np.abs(ref))
+    xr, _, _ = tps_host(rp, col, val, rhs, chunk, chunk_hi, dyn=dyn,
+                        rhs_factor=rng.standard_normal(len(rhs)), groups=-(4096 + G),
+                        cut_out=tops)
+    assert np.max(np.abs(xr - ref)) <= 1e-11 * np.max(np.abs(ref))
+    assert 0 < tops[1] < len(rhs)
+
+
 @pytest.mark.parametrize("name", ["ex3_hh_y", "ex5_ct_y", "ex6_ct_tree", "ex8_neuron"])
 def test_local_mode_golden_systems(name, orc):
     g = load_golden(name)
