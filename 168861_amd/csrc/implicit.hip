@@ -70,6 +70,8 @@ struct ImpStats {
   int fail_iter;
   int restarts;
   double fail_rho, fail_apr, fail_w1, fail_w2, fail_rr;
+  double fail_bb;    // |b|^2 of the failing step (refinement: |D^-1 b|^2)
+  int fail_cycles;   // GMRES restart cycles / refinement sweeps of the failing step
 };
 
 struct ImpDev {
@@ -652,6 +654,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
         d.st->fail_w1 = f_w1;
         d.st->fail_w2 = f_w2;
         d.st->fail_rr = rr;
+        d.st->fail_bb = bb;
+        d.st->fail_cycles = restarts;
       }
     }
     if (!ok) return;
@@ -987,6 +991,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev
         d.st->fail_step = step;
         d.st->fail_iter = it;
         d.st->fail_rr = rr;
+        d.st->fail_bb = bb;
+        d.st->fail_cycles = cycles;
       }
     }
     if (!ok) return;
@@ -1286,6 +1292,8 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
         d.st->fail_step = step;
         d.st->fail_iter = sweeps;
         d.st->fail_rr = rr;
+        d.st->fail_bb = bb;
+        d.st->fail_cycles = sweeps;
       }
     }
     if (!ok) return;
@@ -1800,6 +1808,8 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
         d.st->fail_step = step;
         d.st->fail_iter = sweeps;
         d.st->fail_rr = rr;
+        d.st->fail_bb = bb;
+        d.st->fail_cycles = sweeps;
       }
     }
     if (!ok) return;
@@ -2203,28 +2213,8 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
         sin_[si] = p.input[i];
         scode[si] = code[i];
       }
-      std::vector<int32_t> srp(ns0 + 1, 0), scol(nnz), hslot(H), hmat(H);
-      std::vector<double> sval(nnz);
-      std::vector<uint8_t> srole(nnz);
-      for (int si = 0; si < ns0; si++) {
-        const int row = P.row0[si];
-        srp[si + 1] = srp[si] + (row >= 0 ? hrp[row + 1] - hrp[row] : 0);
-      }
-      for (int si = 0; si < ns0; si++) {
-        const int row = P.row0[si];
-        if (row < 0) continue;
-        for (int k = hrp[row], t = srp[si]; k < hrp[row + 1]; k++, t++) {
-          const int c = hcol[k];
-          scol[t] = P.slot0[c];
-          sval[t] = hval[k];
-          srole[t] = c == row ? 0 : (c < row ? 1 : 2);
-        }
-      }
-      for (int hh = 0; hh < H; hh++) {
-        const int row = p.hh_elem[hh];
-        hslot[hh] = P.slot0[row];
-        hmat[hh] = srp[hslot[hh]] + (hhm[hh] - hrp[row]);  // same position within the row
-      }
+      const SlotCsr sc = slot_order_csr(P, N, hrp.data(), hcol.data(), hval.data(), H,
+                                        p.hh_elem.data(), hhm.data());
       e->x.upload(sx, s);
       e->b.alloc(ns0);
       NRT_HIP(hipMemsetAsync(e->b.p, 0, sizeof(double) * ns0, s));
@@ -2232,12 +2222,12 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       e->skdx.upload(skd, s);
       e->sinput.upload(sin_, s);
       e->scode.upload(scode, s);
-      e->srowptr.upload(srp, s);
-      e->scol.upload(scol, s);
-      e->srole.upload(srole, s);
-      e->val.upload(sval, s);  // the refreshed HH rows are written here
-      e->hh_slot.upload(hslot, s);
-      e->hh_mat.upload(hmat, s);
+      e->srowptr.upload(sc.rowptr, s);
+      e->scol.upload(sc.col, s);
+      e->srole.upload(sc.role, s);
+      e->val.upload(sc.val, s);  // the refreshed HH rows are written here
+      e->hh_slot.upload(sc.hh_slot, s);
+      e->hh_mat.upload(sc.hh_mat, s);
       if (e->local_mode) {
         // HH entries by group (they are in row order, groups are row ranges)
         std::vector<int32_t> hoff(G + 1, 0);
@@ -2370,7 +2360,8 @@ static ImpDev imp_dev(nrt_handle* hnd) {
 
 // The probe's index in the vectors a launch works on: the element itself, or its
 // level-0 slot for the tree solve.
-static int32_t probe_index(const Implicit* e, int32_t elem) {
+static int32_t probe_index(const Implicit* e, int32_t elem, int64_t interval) {
+  if (interval < 0 || elem < 0 || elem >= e->N) return 0;  // no probe: never read
   return e->method != 0 ? e->plan_h.slot0[elem] : elem;
 }
 
@@ -2416,7 +2407,7 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   d.a1 = a1;
   d.interval = interval;
   d.gl0 = gl0;
-  d.trace_elem = probe_index(e, trace_elem);
+  d.trace_elem = probe_index(e, trace_elem, interval);
   d.trace = trace_dev;
   if (e->method == 0 && e->gmres) {
     const int m = e->gm;
@@ -2468,6 +2459,44 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
   return NRT_OK;
 }
 
+// Zero the grid barrier's words (arrivals, generation, error flag) after a launch
+// that timed out in it; the copy follows the launch on the stream.
+static void reset_barrier(Implicit* e, hipStream_t s) {
+  static const unsigned zeros[4] = {0u, 0u, 0u, 0u};
+  NRT_HIP(hipMemcpyAsync(e->bar.p, zeros, sizeof(zeros), hipMemcpyHostToDevice, s));
+  NRT_HIP(hipStreamSynchronize(s));
+}
+
+// What a failed step reports, in the terms of its method.
+static std::string failure_message(const Implicit* e, const nrt_options& o, const ImpStats& st) {
+  char msg[320];
+  const double rel = st.fail_bb > 0 ? sqrt(st.fail_rr / st.fail_bb) : 0.0;
+  const char* what = st.status == NRT_ERR_NUMERIC ? "non-finite value" : "not converged";
+  if (e->method == 0 && !e->gmres) {
+    snprintf(msg, sizeof(msg),
+             "BiCGSTAB step %lld, %s after %d iterations: |r| = %.3e (relative %.3e, tol %.1e "
+             "%s); rho=%.3e (r,r0)-term=%.3e (t,s)=%.3e (t,t)=%.3e; breakdown restarts %d",
+             st.fail_step, what, st.fail_iter, sqrt(st.fail_rr), rel, o.tol,
+             o.tol_mode == NRT_TOL_RELATIVE ? "relative" : "absolute", st.fail_rho, st.fail_apr,
+             st.fail_w1, st.fail_w2, st.fail_cycles);
+  } else if (e->method == 0) {
+    snprintf(msg, sizeof(msg),
+             "GMRES(%d) step %lld, %s after %d Arnoldi steps in %d restart cycles: |r| = %.3e "
+             "(relative %.3e; %s)",
+             e->gm, st.fail_step, what, st.fail_iter, st.fail_cycles, sqrt(st.fail_rr), rel,
+             o.precision == NRT_FP32_REFINE ? "Jacobi-scaled, refine_tol" : "tol");
+  } else if (e->method == 2) {
+    snprintf(msg, sizeof(msg),
+             "FP32 tree solve in FP64 refinement, step %lld, %s after %d sweeps: scaled "
+             "residual %.3e (relative %.3e, refine_tol %.1e, refine_max %d)",
+             st.fail_step, what, st.fail_cycles, sqrt(st.fail_rr), rel, o.refine_tol,
+             o.refine_max);
+  } else {
+    snprintf(msg, sizeof(msg), "FP64 tree solve, step %lld: %s", st.fail_step, what);
+  }
+  return msg;
+}
+
 static int collect_stats(nrt_handle* hnd) {
   Implicit* e = hnd->im;
   ImpStats st{};
@@ -2489,15 +2518,18 @@ static int collect_stats(nrt_handle* hnd) {
                          : e->grid_mode      ? NRT_TREE_GRID
                                              : NRT_TREE_BLOCK;
   hnd->stats.restarts = st.restarts;
-  if (err) throw Error(NRT_ERR_HIP, "grid barrier timed out inside the persistent Krylov kernel");
+  if (err) {
+    // A timed-out barrier leaves its arrival count and the error flag set; the
+    // launch has drained (every workgroup leaves at the flag), so clear them for
+    // the handle's next launch.
+    reset_barrier(e, hnd->stream);
+    throw Error(NRT_ERR_HIP,
+                "grid barrier timed out inside the persistent kernel (its workgroups were not "
+                "all resident at once: another launch held part of the device)");
+  }
   // a reported failure is returned once, then cleared
   if (st.status != NRT_OK) {
-    char msg[256];
-    snprintf(msg, sizeof(msg),
-             "step %lld iteration %d: rho=%.3e apr=%.3e w1=%.3e w2=%.3e rr=%.3e (restarts %d)",
-             st.fail_step, st.fail_iter, st.fail_rho, st.fail_apr, st.fail_w1, st.fail_w2,
-             st.fail_rr, st.restarts);
-    hnd->last_error = msg;
+    hnd->last_error = failure_message(e, hnd->opt, st);
     const int zero = 0;
     NRT_HIP(hipMemcpyAsync(&e->st.p->status, &zero, sizeof(int), hipMemcpyHostToDevice,
                            hnd->stream));
@@ -2583,7 +2615,7 @@ void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_ele
   d.a0 = a0;
   d.a1 = a1;
   d.interval = interval;
-  d.trace_elem = probe_index(e, trace_elem);
+  d.trace_elem = probe_index(e, trace_elem, interval);
   if (e->method == 1) {
     BatchSlot<double> S{d, tree_args<double>(e, h->opt, e->tw64.p), trace};
     std::memcpy(out, &S, sizeof(S));

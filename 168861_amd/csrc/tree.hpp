@@ -154,6 +154,24 @@ struct TreePlanDev;
 // Device view of a plan whose arrays are concatenated in one int32 buffer at
 // `base`; with `flat` non-null the concatenation is appended there.
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat);
+// Host view with every plan array in an allocation of its own (`store`), each of
+// exactly its length: an access past the end of any one array leaves its
+// allocation (host emulation under a bounds-checking build).
+TreePlanDev plan_view_parts(const TreePlanHost& P, std::vector<std::vector<int32_t>>& store);
+constexpr int kPlanParts = 31;
+
+// The matrix with its rows in level-0 slot order (padding slots are empty rows):
+// columns as slots, each entry's role (0 diagonal, 1 coupling to the parent, 2 to
+// a child), and for HH entry hh the slot of its row and the position of its first
+// entry there (hh_mat gives that position in the element-order matrix; the row's
+// entries keep their order).
+struct SlotCsr {
+  std::vector<int32_t> rowptr, col, hh_slot, hh_mat;
+  std::vector<double> val;
+  std::vector<uint8_t> role;
+};
+SlotCsr slot_order_csr(const TreePlanHost& P, int N, const int32_t* rowptr, const int32_t* col,
+                       const double* val, int H, const int32_t* hh_elem, const int32_t* hh_mat);
 
 // Device view of a plan (pointers into one device allocation).
 struct TreePlanDev {

@@ -393,39 +393,90 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
 
 // Device view over one flat int32 buffer holding every plan array; `flat`
 // receives the concatenation, the returned view has offsets relative to base.
+// The plan's arrays in the order of the device buffer.
+static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&parts)[kPlanParts]) {
+  const std::vector<int32_t>* p[kPlanParts] = {
+      &P.nslots,    &P.nchunk,    &P.node_off,  &P.next,      &P.chunk_off, &P.chunk_len,
+      &P.chunk_dyn, &P.slot0,     &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
+      &P.fin_cptr,  &P.fin_child, &P.sub_order, &P.sub_hoff,  &P.top_node,  &P.top_hoff,
+      &P.top_cptr,  &P.top_child, &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
+      &P.lbase,     &P.lnext,     &P.lf_cptr,   &P.lf_child,  &P.lf_par,    &P.wide_off,
+      &P.wide_col};
+  for (int k = 0; k < kPlanParts; k++) parts[k] = p[k];
+}
+
+static TreePlanDev plan_from(const TreePlanHost& P, const int32_t* const* a) {
+  return TreePlanDev{P.levels,     P.heights,    P.fin_cut,   P.sub_groups, P.top_n,
+                     P.top_ext,    P.groups,     P.top_lds_ok, P.lds_nodes, P.top_wide,
+                     P.top_narrow, P.top_kids,   P.gl_cols,   P.gl_fin,     P.gl_edges,
+                     P.gl_wide,    a[0],         a[1],        a[2],         a[3],
+                     a[4],         a[5],         a[6],        a[7],         a[8],
+                     a[9],         a[10],        a[11],       a[12],        a[13],
+                     a[14],        a[15],        a[16],       a[17],        a[18],
+                     a[19],        a[20],        a[21],       a[22],        a[23],
+                     a[24],        a[25],        a[26],       a[27],        a[28],
+                     a[29],        a[30]};
+}
+
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
-  const std::vector<int32_t>* parts[] = {&P.nslots,    &P.nchunk,    &P.node_off,  &P.next,
-                                         &P.chunk_off, &P.chunk_len, &P.chunk_dyn, &P.slot0,
-                                         &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
-                                         &P.fin_cptr,  &P.fin_child, &P.sub_order, &P.sub_hoff,
-                                         &P.top_node,  &P.top_hoff,  &P.top_cptr,  &P.top_child,
-                                         &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
-                                         &P.lbase,     &P.lnext,     &P.lf_cptr,   &P.lf_child,
-                                         &P.lf_par,    &P.wide_off,  &P.wide_col};
-  constexpr int kParts = sizeof(parts) / sizeof(parts[0]);
-  size_t off[kParts];
+  const std::vector<int32_t>* parts[kPlanParts];
+  plan_parts(P, parts);
+  const int32_t* a[kPlanParts];
   size_t pos = 0;
-  for (int k = 0; k < kParts; k++) {
-    off[k] = pos;
+  for (int k = 0; k < kPlanParts; k++) {
+    a[k] = base + pos;
     if (flat) {
       flat->insert(flat->end(), parts[k]->begin(), parts[k]->end());
       flat->push_back(0);
     }
     pos += parts[k]->size() + 1;
   }
-  return TreePlanDev{P.levels,       P.heights,      P.fin_cut,      P.sub_groups,
-                     P.top_n,        P.top_ext,      P.groups,       P.top_lds_ok,
-                     P.lds_nodes,    P.top_wide,     P.top_narrow,   P.top_kids,     P.gl_cols,
-                     P.gl_fin,
-                     P.gl_edges,     P.gl_wide,
-                     base + off[0],  base + off[1],  base + off[2],  base + off[3],
-                     base + off[4],  base + off[5],  base + off[6],  base + off[7],
-                     base + off[8],  base + off[9],  base + off[10], base + off[11],
-                     base + off[12], base + off[13], base + off[14], base + off[15],
-                     base + off[16], base + off[17], base + off[18], base + off[19],
-                     base + off[20], base + off[21], base + off[22], base + off[23],
-                     base + off[24], base + off[25], base + off[26], base + off[27],
-                     base + off[28], base + off[29], base + off[30]};
+  return plan_from(P, a);
+}
+
+TreePlanDev plan_view_parts(const TreePlanHost& P, std::vector<std::vector<int32_t>>& store) {
+  const std::vector<int32_t>* parts[kPlanParts];
+  plan_parts(P, parts);
+  store.clear();
+  store.reserve(kPlanParts);
+  const int32_t* a[kPlanParts];
+  for (int k = 0; k < kPlanParts; k++) {
+    store.emplace_back(parts[k]->begin(), parts[k]->end());
+    a[k] = store.back().data();
+  }
+  return plan_from(P, a);
+}
+
+SlotCsr slot_order_csr(const TreePlanHost& P, int N, const int32_t* rowptr, const int32_t* col,
+                       const double* val, int H, const int32_t* hh_elem, const int32_t* hh_mat) {
+  const int ns0 = P.nslots[0], nnz = rowptr[N];
+  SlotCsr c;
+  c.rowptr.assign(ns0 + 1, 0);
+  c.col.assign(nnz, 0);
+  c.val.assign(nnz, 0.0);
+  c.role.assign(nnz, 0);
+  c.hh_slot.assign(H, 0);
+  c.hh_mat.assign(H, 0);
+  for (int si = 0; si < ns0; si++) {
+    const int row = P.row0[si];
+    c.rowptr[si + 1] = c.rowptr[si] + (row >= 0 ? rowptr[row + 1] - rowptr[row] : 0);
+  }
+  for (int si = 0; si < ns0; si++) {
+    const int row = P.row0[si];
+    if (row < 0) continue;
+    for (int k = rowptr[row], t = c.rowptr[si]; k < rowptr[row + 1]; k++, t++) {
+      const int cc = col[k];
+      c.col[t] = P.slot0[cc];
+      c.val[t] = val[k];
+      c.role[t] = cc == row ? 0 : (cc < row ? 1 : 2);
+    }
+  }
+  for (int hh = 0; hh < H; hh++) {
+    const int row = hh_elem[hh];
+    c.hh_slot[hh] = P.slot0[row];
+    c.hh_mat[hh] = c.rowptr[c.hh_slot[hh]] + (hh_mat[hh] - rowptr[row]);
+  }
+  return c;
 }
 
 // Top lists of the heights >= cut (see TreePlanHost::top_*), kept only when they
@@ -708,15 +759,15 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
         }
       }
     }
-    std::vector<int32_t> flat;
-    TreePlanDev tmp = plan_view(H, nullptr, &flat);
-    (void)tmp;
-    TreePlanDev P = plan_view(H, flat.data(), nullptr);
+    // every plan array and every work array in an allocation of its own, of exactly
+    // its length, so that a bounds-checking build of this hook catches an access
+    // past any one of them (the device keeps them in one buffer each)
+    std::vector<std::vector<int32_t>> parts;
+    TreePlanDev P = plan_view_parts(H, parts);
     const size_t T = (size_t)H.total_nodes();
-    std::vector<double> buf(10 * T, 0.0);
-    double* b = buf.data();
-    TreeWork<double> w{b,         b + T,     b + 2 * T, b + 3 * T, b + 4 * T,
-                       b + 5 * T, b + 6 * T, b + 7 * T, b + 8 * T, b + 9 * T};
+    std::vector<std::vector<double>> wa(10, std::vector<double>(T, 0.0));
+    TreeWork<double> w{wa[0].data(), wa[1].data(), wa[2].data(), wa[3].data(), wa[4].data(),
+                       wa[5].data(), wa[6].data(), wa[7].data(), wa[8].data(), wa[9].data()};
     const int32_t* s0 = H.slot0.data();
     for (int i = 0; i < N; i++) {
       for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {

@@ -108,3 +108,33 @@ def test_gmres_option_validation(nrt):
     o = nrt.default_options(kind=nrt.IMPLICIT, krylov=nrt.GMRES, precision=nrt.FP32_REFINE)
     assert L.nrt_create(None, C.byref(o), C.byref(h)) == nrt.STATUS_CODES["invalid"]
     assert "problem" in L.nrt_last_error(None).decode().lower()
+
+
+def _undefined_symbols(path):
+    import shutil
+    import subprocess
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    r = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return {ln.split()[-1].split("@")[0] for ln in r.stdout.splitlines() if ln.strip()}
+
+
+def test_no_cooperative_launches():
+    """Every persistent kernel is an ordinary launch on a grid sized to be co-resident,
+    with bounded barriers: the built library imports no cooperative-launch entry point
+    (a cooperative launch takes the device's cooperative queue, which one process at a
+    time can hold, so a second process on the GPU, such as the drop-in driver or the
+    reference-tree bridge executable run beside a test process, was refused its
+    launches). The bridge executable itself makes no HIP call: it reaches the device
+    through the library only."""
+    lib = os.path.join(REPO, "168861_amd", "lib", "libneurite_hip.so")
+    und = _undefined_symbols(lib)
+    coop = sorted(s for s in und if "cooperative" in s.lower())
+    assert not coop, coop
+    assert "hipLaunchKernel" in und
+    exe = os.path.join(REPO, "oracle", "_ref", "neurite_ref_hip")
+    if os.path.exists(exe):
+        hip = sorted(s for s in _undefined_symbols(exe) if s.startswith("hip"))
+        assert not hip, hip
+        assert any(s.startswith("nrt_") for s in _undefined_symbols(exe))
