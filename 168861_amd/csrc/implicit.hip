@@ -455,6 +455,22 @@ __device__ __forceinline__ bool imp_update_row(const ImpDev& d, int hh, int iv, 
 // one barrier, then every block sums all partials in the same order. A block can
 // only refill a slot after passing the next reduction's barrier, by which time
 // every block has read that slot, so one barrier per reduction suffices.
+// Grid-wide maximum of NV non-negative values, as reduce_all.
+template <int NV>
+__device__ __forceinline__ bool reduce_all_max(const ImpDev& d, double (&v)[NV], double* lds,
+                                               int& slot) {
+  block_max<NV>(v, lds);
+  double* part = d.part + (size_t)slot * d.nblocks * 4;
+  slot ^= 1;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) part[d.vb * 4 + k] = v[k];
+  }
+  grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
+  grid_partials_max<NV>(part, 4, d.nblocks, v, lds);
+  return __hip_atomic_load(d.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+}
+
 template <int NV>
 __device__ __forceinline__ bool reduce_all(const ImpDev& d, double (&v)[NV], double* lds,
                                            int& slot) {
@@ -1244,8 +1260,9 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
       sweeps = 1;
     } else {
       while (true) {
-        // FP64 residual r = b - A x into the level-0 right-hand side
-        // (its norm, and that of b, Jacobi-scaled: |D^-1 r| <= refine_tol |D^-1 b|)
+        // FP64 residual r = b - A x into the level-0 right-hand side; the test is on
+        // max norms, Jacobi-scaled: |D^-1 r|_inf <= refine_tol |D^-1 b|_inf, a bound
+        // per element whatever the number of rows (rr, bb hold their squares)
         double red[2] = {0.0, 0.0};
         for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
           double ax = 0, dg = 1.0;
@@ -1256,14 +1273,14 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
           const double bi = d.b[i];
           const double ri = bi - ax;
           t.w.r[i] = (T)ri;
-          red[0] = red[0] + (ri / dg) * (ri / dg);
-          red[1] = red[1] + (bi / dg) * (bi / dg);
+          red[0] = fmax(red[0], fabs(ri / dg));
+          red[1] = fmax(red[1], fabs(bi / dg));
         }
-        if (GRID) ok = reduce_all<2>(d, red, lds, slot);
-        else block_sum<2>(red, lds);
+        if (GRID) ok = reduce_all_max<2>(d, red, lds, slot);
+        else block_max<2>(red, lds);
         if (!ok) break;
-        rr = red[0];
-        bb = red[1];
+        rr = red[0] * red[0];
+        bb = red[1] * red[1];
         if (!isfinite(rr)) {
           status = NRT_ERR_NUMERIC;
           break;
@@ -1771,13 +1788,13 @@ __device__ __forceinline__ void tree_steps_local(const ImpDev& d, const TreeArgs
           const double bi = d.b[i];
           const double ri = bi - ax;
           t.w.r[i] = (T)ri;
-          red[0] = red[0] + (ri / dg) * (ri / dg);
-          red[1] = red[1] + (bi / dg) * (bi / dg);
+          red[0] = fmax(red[0], fabs(ri / dg));
+          red[1] = fmax(red[1], fabs(bi / dg));
         }
-        ok = reduce_all<2>(d, red, lds, slot);
+        ok = reduce_all_max<2>(d, red, lds, slot);
         if (!ok) break;
-        rr = red[0];
-        bb = red[1];
+        rr = red[0] * red[0];
+        bb = red[1] * red[1];
         if (!isfinite(rr)) {
           status = NRT_ERR_NUMERIC;
           break;
@@ -2488,7 +2505,7 @@ static std::string failure_message(const Implicit* e, const nrt_options& o, cons
   } else if (e->method == 2) {
     snprintf(msg, sizeof(msg),
              "FP32 tree solve in FP64 refinement, step %lld, %s after %d sweeps: scaled "
-             "residual %.3e (relative %.3e, refine_tol %.1e, refine_max %d)",
+             "residual max norm %.3e (relative %.3e, refine_tol %.1e, refine_max %d)",
              st.fail_step, what, st.fail_cycles, sqrt(st.fail_rr), rel, o.refine_tol,
              o.refine_max);
   } else {

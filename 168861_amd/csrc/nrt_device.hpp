@@ -48,6 +48,16 @@ __device__ __forceinline__ float wave_sum(float x) {
   return (readlane_f32(x, 0) + readlane_f32(x, 16)) + (readlane_f32(x, 32) + readlane_f32(x, 48));
 }
 
+// Full-wave maximum, uniform result (the reduction pattern of wave_sum).
+__device__ __forceinline__ double wave_max(double x) {
+  x = fmax(x, dpp_f64<0xB1>(x));
+  x = fmax(x, dpp_f64<0x4E>(x));
+  x = fmax(x, dpp_f64<0x141>(x));
+  x = fmax(x, dpp_f64<0x140>(x));
+  return fmax(fmax(readlane_f64(x, 0), readlane_f64(x, 16)),
+              fmax(readlane_f64(x, 32), readlane_f64(x, 48)));
+}
+
 // Error-free transformation a + b = s + e (Knuth two-sum).
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   s = a + b;
@@ -100,6 +110,26 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   for (int k = 0; k < NV; k++) {
     double s = 0;
     for (int w = 0; w < nw; w++) s = s + lds[k * 16 + w];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+// Maximum of NV non-negative values over the block, as block_sum.
+template <int NV>
+__device__ __forceinline__ void block_max(double (&v)[NV], double* lds) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_max(v[k]);
+    if (lane == 0) lds[k * 16 + wid] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0;
+    for (int w = 0; w < nw; w++) s = fmax(s, lds[k * 16 + w]);
     v[k] = s;
   }
   __syncthreads();
@@ -162,6 +192,30 @@ __device__ __forceinline__ void grid_partials_sum(const double* part, int stride
 #pragma unroll
     for (int k = 0; k < NV; k++) {
       double s = wave_sum(acc[k]);
+      if (threadIdx.x == 0) lds[k] = s;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) out[k] = lds[k];
+  __syncthreads();
+}
+
+// Maximum of non-negative per-block partials, as grid_partials_sum.
+template <int NV>
+__device__ __forceinline__ void grid_partials_max(const double* part, int stride, int nblocks,
+                                                  double (&out)[NV], double* lds) {
+  if (threadIdx.x < kWave) {
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kWave) {
+#pragma unroll
+      for (int k = 0; k < NV; k++) acc[k] = fmax(acc[k], part[b * stride + k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = wave_max(acc[k]);
       if (threadIdx.x == 0) lds[k] = s;
     }
   }

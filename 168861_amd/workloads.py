@@ -215,6 +215,13 @@ def sweep_tree(index, n_range=(500, 50000), depth_range=(4, 12), strain=(0.0, 0.
                          dT, steps * dT)
 
 
+def served_request_target(index, median=20000.0, sigma=1.0, cap=2_000_000):
+    """The compartment count request `index` of config 5 aims at (its first draw),
+    without building the tree."""
+    rng = SplitMix64(seed_for(5, index))
+    return min(float(cap), median * float(np.exp(sigma * rng.normal())))
+
+
 def served_request(index, median=20000.0, sigma=1.0, cap=2_000_000, duration=(5e-3, 50e-3),
                    rate=200.0, dx=2e-6, dT=25e-6):
     """Config 5 request `index`: compartment count log-normal (median, sigma),

@@ -176,8 +176,11 @@ typedef struct nrt_options {
   int32_t num_blocks;   /* 0 -> autotuned grid size for persistent kernels     */
   double tol;           /* Krylov tolerance (see tol_mode; FP64 forms)          */
   double refine_tol;    /* FP32_REFINE: outer FP64 residual tolerance, relative
-                           and Jacobi-scaled: |D^-1 (b - A x)|_2 <=
-                           refine_tol |D^-1 b|_2 with D = diag(A)             */
+                           and Jacobi-scaled (D = diag(A)). Tree solve, max
+                           norms (a bound per element, whatever N):
+                           |D^-1 (b - A x)|_inf <= refine_tol |D^-1 b|_inf;
+                           GMRES, 2-norms (its residual estimate):
+                           |D^-1 (b - A x)|_2 <= refine_tol |D^-1 b|_2        */
   int32_t refine_max;   /* FP32_REFINE tree solve: maximum refinement sweeps
                            (GMRES: each restart cycle is one sweep; cycles are
                            bounded by max_iter Arnoldi steps in total)          */
@@ -203,8 +206,10 @@ typedef struct nrt_stats {
   int32_t last_iters;       /* Krylov iterations of the last step               */
   int32_t last_status;      /* status of the last step                          */
   double last_residual;     /* ||b - A x||_2 at exit of the last step
-                               (FP32_REFINE: ||D^-1 (b - A x)||_2)              */
-  double last_rhs_norm;     /* ||b||_2 of the last step (FP32_REFINE: ||D^-1 b||) */
+                               (FP32_REFINE: ||D^-1 (b - A x)||, the norm of
+                               refine_tol's test)                               */
+  double last_rhs_norm;     /* ||b||_2 of the last step (FP32_REFINE: ||D^-1 b||,
+                               as last_residual)                                */
   double kernel_ms;         /* device time of the dominant kernel, last run     */
   int64_t kernel_launches;  /* launches of the dominant kernel, last run        */
   int64_t matrix_nnz;       /* implicit: non-zeros of the assembled CSR         */
