@@ -319,8 +319,11 @@ int nrt_batch_create(const nrt_options* opt, nrt_batch** out) {
   auto* b = new nrt_batch;
   int st = guarded_b(b, [&] {
     b->opt = resolve_options(opt);
-    NRT_REQUIRE(b->opt.kind == NRT_EXPLICIT || b->opt.krylov == NRT_DIRECT, NRT_ERR_UNSUPPORTED,
-                "implicit batches use the tree solve (krylov = NRT_DIRECT)");
+    NRT_REQUIRE(b->opt.kind == NRT_EXPLICIT || b->opt.krylov == NRT_DIRECT ||
+                    (b->opt.krylov == NRT_BICGSTAB && b->opt.precond == NRT_PRECOND_TREE),
+                NRT_ERR_UNSUPPORTED,
+                "implicit batches use the tree solve (krylov = NRT_DIRECT) or BiCGSTAB "
+                "preconditioned by it (NRT_BICGSTAB with NRT_PRECOND_TREE)");
     NRT_REQUIRE(b->opt.batch_threads == 0 || b->opt.batch_threads == 256 ||
                     b->opt.batch_threads == 512 || b->opt.batch_threads == 1024,
                 NRT_ERR_INVALID, "batch_threads must be 0, 256, 512 or 1024");

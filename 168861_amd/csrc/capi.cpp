@@ -10,6 +10,7 @@
 namespace nrt {
 int setup_on_device(const nrt_geometry* g, int device, nrt_setup_out* out);
 
+
 [[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line) {
   (void)hipGetLastError();
   const char* base = strrchr(file, '/');
@@ -96,10 +97,19 @@ nrt_options resolve_options(const nrt_options* opt) {
   NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.kind == NRT_IMPLICIT, NRT_ERR_INVALID, "bad kind");
   NRT_REQUIRE(o.precision == NRT_FP64 || o.precision == NRT_FP32_REFINE, NRT_ERR_INVALID,
               "bad precision");
-  NRT_REQUIRE(o.precision == NRT_FP64 ||
-                  (o.kind == NRT_IMPLICIT && (o.krylov == NRT_DIRECT || o.krylov == NRT_GMRES)),
+  NRT_REQUIRE(o.precond == NRT_PRECOND_NONE || o.precond == NRT_PRECOND_TREE, NRT_ERR_INVALID,
+              "bad precond");
+  NRT_REQUIRE(o.precond == NRT_PRECOND_NONE ||
+                  (o.kind == NRT_IMPLICIT && o.krylov == NRT_BICGSTAB),
               NRT_ERR_UNSUPPORTED,
-              "FP32 with FP64 refinement is available for the implicit tree solve and GMRES");
+              "the tree-solve preconditioner is built for implicit BiCGSTAB (GMRES takes none)");
+  NRT_REQUIRE(o.precision == NRT_FP64 ||
+                  (o.kind == NRT_IMPLICIT &&
+                   (o.krylov == NRT_DIRECT || o.krylov == NRT_GMRES ||
+                    (o.krylov == NRT_BICGSTAB && o.precond == NRT_PRECOND_TREE))),
+              NRT_ERR_UNSUPPORTED,
+              "FP32 with FP64 refinement is available for the implicit tree solve, GMRES and "
+              "BiCGSTAB preconditioned by the tree solve");
   NRT_REQUIRE(o.kind == NRT_EXPLICIT || o.krylov == NRT_BICGSTAB || o.krylov == NRT_GMRES ||
                   o.krylov == NRT_DIRECT,
               NRT_ERR_UNSUPPORTED, "implicit methods: NRT_BICGSTAB, NRT_GMRES or NRT_DIRECT");
@@ -109,8 +119,6 @@ nrt_options resolve_options(const nrt_options* opt) {
   NRT_REQUIRE(o.x0_mode == NRT_X0_PREVIOUS || o.x0_mode == NRT_X0_RHS ||
                   o.x0_mode == NRT_X0_ZERO,
               NRT_ERR_INVALID, "bad x0_mode");
-  NRT_REQUIRE(o.precond == NRT_PRECOND_NONE, NRT_ERR_UNSUPPORTED,
-              "preconditioned Krylov is not built; use NRT_DIRECT for the tree solve");
   NRT_REQUIRE(o.chunk >= 0 && o.chunk <= kMaxChunk && o.chunk_hi >= 0 &&
                   o.chunk_hi <= kMaxChunk,
               NRT_ERR_INVALID, "tree chunk lengths must be in [0, 4096]");

@@ -1037,6 +1037,7 @@ struct TreeArgs {
   TreePlanDev P;
   TreeWork<T> w;
   int direct;
+  int krylov;   // BiCGSTAB preconditioned by this tree solve (else direct / refinement)
   double refine_tol;
   int refine_max;
   int top_lds;  // the launch provides top_lds_bytes<T>() of dynamic LDS (grid launches)
@@ -1070,7 +1071,7 @@ __device__ void tps_factor(const ImpDev& d, const TreePlanDev& P, const TreeWork
 template <class T, bool GRID>
 __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
                                         const TreeWork<T>& w, bool acc,
-                        bool all_static, unsigned char* tl) {
+                        bool all_static, unsigned char* tl, double* out) {
   const int wid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
   const int nw = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
   for (int lv = 0; lv < P.levels; lv++) {
@@ -1142,21 +1143,203 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
       tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr);
     tsync<GRID>(d);
   }
-  // level 0 lands in the FP64 potential directly (x = solution, or x += correction)
+  // level 0 lands in the FP64 vector `out` directly (the potential: x = solution,
+  // or x += correction; a preconditioned Krylov step: z = M^-1 r)
   if (P.levels == 0) {
     const int n0 = P.nslots[0];
     for (int v = wid; v < n0; v += nw) {
-      if (acc) tps_final_out<OUT_ADD>(P, w, v, d.x);
-      else tps_final_out<OUT_ASSIGN>(P, w, v, d.x);
+      if (acc) tps_final_out<OUT_ADD>(P, w, v, out);
+      else tps_final_out<OUT_ASSIGN>(P, w, v, out);
     }
   } else {
     const int c1 = P.chunk_off[1];
     for (int ch = wid; ch < c1; ch += nw) {
-      if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, ch, d.x);
-      else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, d.x);
+      if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, ch, out);
+      else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, out);
     }
   }
   tsync<GRID>(d);
+}
+
+template <int NV, bool GRID>
+__device__ __forceinline__ bool tree_reduce(const ImpDev& d, double (&v)[NV], double* lds,
+                                            int& slot) {
+  if (GRID) return reduce_all<NV>(d, v, lds, slot);
+  block_sum<NV>(v, lds);
+  return true;
+}
+
+// One step's solve by BiCGSTAB, the reference GPU solver's method
+// (solverIm-gpu.cpp:725-843: the same recurrences for alpha, omega and beta),
+// right-preconditioned by the tree solve: p^ = M^-1 p and s^ = M^-1 s are tree
+// solves of the step's matrix (the FP64 one is exact, so the half step converges
+// in the first iteration; the FP32 one, M^-1 = the FP32 tree solve, takes a few),
+// and x is updated with p^, s^. Vectors in level-0 slot order; x0 = the previous
+// potential; the test ||b - A x||_2 <= tol (relative: tol ||b||_2) at the half and
+// the full step of every iteration; a breakdown ((r0, v) = 0, rho = 0, omega = 0)
+// restarts the shadow residual as r0 = r, at most kMaxRestarts times a step.
+struct KrylovOut {
+  int iters, solves, status, restarts;
+  double rr, bb;
+};
+template <class T, bool GRID>
+__device__ __forceinline__ KrylovOut tree_bicgstab(const ImpDev& d, const TreeArgs<T>& t,
+                                                   double* lds, unsigned char* tl, int& slot) {
+  const int r_begin = GRID ? d.vb * d.rpb : 0;
+  const int r_end = GRID ? min(d.ns0, r_begin + d.rpb) : d.ns0;
+  const int bs = blockDim.x;
+  const double* __restrict__ val = d.val;
+  const int32_t* __restrict__ rowptr = d.srowptr;
+  const int32_t* __restrict__ col = d.scol;
+  double* __restrict__ x = d.x;
+  const double* __restrict__ b = d.b;
+  double* __restrict__ r = d.r;
+  double* __restrict__ r0 = d.r0;
+  double* __restrict__ p = d.p0;
+  double* __restrict__ v = d.v0;
+  double* __restrict__ sv = d.s;
+  double* __restrict__ tv = d.t;
+  double* __restrict__ ph = d.p1;  // M^-1 p
+  double* __restrict__ sh = d.v1;  // M^-1 s
+  T* __restrict__ wr = t.w.r;
+  KrylovOut o{0, 0, NRT_ERR_NOT_CONVERGED, 0, 0.0, 0.0};
+  // r = b - A x ; r0 = r ; rho = (r0, r) ; ||b||^2
+  double red2[2] = {0.0, 0.0};
+  for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+    double ax = 0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * x[col[k]];
+    const double bi = b[i];
+    const double ri = bi + (-1.0) * ax;
+    r[i] = ri;
+    r0[i] = ri;
+    red2[0] = red2[0] + ri * ri;
+    red2[1] = red2[1] + bi * bi;
+  }
+  bool ok = tree_reduce<2, GRID>(d, red2, lds, slot);
+  double rho = red2[0], rr = red2[0];
+  o.bb = red2[1];
+  const double thr = d.tol_mode == NRT_TOL_RELATIVE ? d.tol * sqrt(o.bb) : d.tol;
+  const double limit = thr * thr;
+  if (rr <= limit) o.status = NRT_OK;
+  double alpha = 0, omega = 0, beta = 0;
+  bool first = true, restart = false;
+  while (ok && o.status == NRT_ERR_NOT_CONVERGED && o.iters < d.max_iter) {
+    // p = r + beta (p - omega v), and the tree system's right-hand side
+    const double nomega = -omega;
+    for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+      if (restart) r0[i] = r[i];
+      const double pi = first ? r[i] : r[i] + beta * (p[i] + nomega * v[i]);
+      p[i] = pi;
+      wr[i] = (T)pi;
+    }
+    restart = false;
+    tsync<GRID>(d);
+    // the matrix changed (HH rows) only before the step's first solve
+    tps_run<T, GRID>(d, t.P, t.w, false, o.solves > 0, tl, ph);
+    o.solves++;
+    // v = A p^ ; (r0, v)
+    double redA[1] = {0.0};
+    for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+      double acc = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) acc = acc + val[k] * ph[col[k]];
+      v[i] = acc;
+      redA[0] = redA[0] + acc * r0[i];
+    }
+    ok = tree_reduce<1, GRID>(d, redA, lds, slot);
+    if (!ok) break;
+    o.iters++;
+    if (!isfinite(redA[0])) {
+      o.status = NRT_ERR_NUMERIC;
+      break;
+    }
+    if (redA[0] == 0.0 || !isfinite(rho / redA[0])) {
+      if (++o.restarts > kMaxRestarts) break;
+      restart = true;
+      first = true;
+      rho = rr;
+      continue;
+    }
+    alpha = rho / redA[0];
+    const double nalpha = -(rho / redA[0]);
+    // s = r - alpha v ; ||s||^2 ; the tree system's right-hand side
+    double red1[1] = {0.0};
+    for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+      const double si = r[i] + nalpha * v[i];
+      sv[i] = si;
+      wr[i] = (T)si;
+      red1[0] = red1[0] + si * si;
+    }
+    ok = tree_reduce<1, GRID>(d, red1, lds, slot);
+    if (!ok) break;
+    if (!isfinite(red1[0])) {
+      o.status = NRT_ERR_NUMERIC;
+      break;
+    }
+    if (red1[0] <= limit) {
+      // converged at the half step: x += alpha p^
+      for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+        x[i] = x[i] + alpha * ph[i];
+        r[i] = sv[i];
+      }
+      rr = red1[0];
+      o.status = NRT_OK;
+      break;
+    }
+    tps_run<T, GRID>(d, t.P, t.w, false, true, tl, sh);
+    o.solves++;
+    // t = A s^ ; (t, s), (t, t)
+    double redB[2] = {0.0, 0.0};
+    for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+      double acc = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) acc = acc + val[k] * sh[col[k]];
+      tv[i] = acc;
+      redB[0] = redB[0] + acc * sv[i];
+      redB[1] = redB[1] + acc * acc;
+    }
+    ok = tree_reduce<2, GRID>(d, redB, lds, slot);
+    if (!ok) break;
+    if (!isfinite(redB[0]) || !isfinite(redB[1]) || redB[1] == 0.0) {
+      o.status = NRT_ERR_NUMERIC;
+      break;
+    }
+    omega = redB[0] / redB[1];
+    const double nw = -(redB[0] / redB[1]);
+    // x += alpha p^ + omega s^ ; r = s - omega t ; (r0, r), (r, r)
+    double redC[2] = {0.0, 0.0};
+    for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+      double xi = x[i] + alpha * ph[i];
+      xi = xi + omega * sh[i];
+      x[i] = xi;
+      const double ri = sv[i] + nw * tv[i];
+      r[i] = ri;
+      redC[0] = redC[0] + ri * r0[i];
+      redC[1] = redC[1] + ri * ri;
+    }
+    ok = tree_reduce<2, GRID>(d, redC, lds, slot);
+    if (!ok) break;
+    beta = (alpha / omega) * (redC[0] / rho);
+    rho = redC[0];
+    rr = redC[1];
+    first = false;
+    if (!isfinite(rr)) {
+      o.status = NRT_ERR_NUMERIC;
+      break;
+    }
+    if (rr <= limit) {
+      o.status = NRT_OK;
+      break;
+    }
+    if (!isfinite(beta) || rho == 0.0 || omega == 0.0) {
+      if (++o.restarts > kMaxRestarts) break;
+      restart = true;
+      first = true;
+      rho = rr;
+    }
+  }
+  if (!ok) o.status = NRT_ERR_HIP;
+  o.rr = rr;
+  tsync<GRID>(d);  // x complete before the next step's update and residual
+  return o;
 }
 
 // Update phase of a tree-solved step, over slots. HH elements are spread one per
@@ -1255,9 +1438,19 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     int sweeps = 0, status = NRT_OK;
     double rr = 0, bb = 0;
     bool ok = true;
+    int iters = -1;
     if (t.direct) {
-      tps_run<T, GRID>(d, t.P, t.w, false, false, tl);
+      tps_run<T, GRID>(d, t.P, t.w, false, false, tl, d.x);
       sweeps = 1;
+    } else if (t.krylov) {
+      const KrylovOut ko = tree_bicgstab<T, GRID>(d, t, lds, tl, slot);
+      iters = ko.iters;
+      sweeps = ko.solves;
+      status = ko.status;
+      rr = ko.rr;
+      bb = ko.bb;
+      ok = status != NRT_ERR_HIP;
+      if (d.vb == 0 && threadIdx.x == 0) d.st->restarts += ko.restarts;
     } else {
       while (true) {
         // FP64 residual r = b - A x into the level-0 right-hand side; the test is on
@@ -1290,7 +1483,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
           status = NRT_ERR_NOT_CONVERGED;
           break;
         }
-        tps_run<T, GRID>(d, t.P, t.w, true, sweeps > 0, tl);
+        tps_run<T, GRID>(d, t.P, t.w, true, sweeps > 0, tl, d.x);
         sweeps++;
       }
     }
@@ -1298,16 +1491,18 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
       ok = false;
     if (!ok) status = NRT_ERR_HIP;
     if (d.vb == 0 && threadIdx.x == 0) {
-      d.st->iters_total += (unsigned long long)sweeps;
+      // Krylov form: iterations, and the tree solves as sweeps
+      const int its = iters >= 0 ? iters : sweeps;
+      d.st->iters_total += (unsigned long long)its;
       if (!t.direct) d.st->sweeps_total += (unsigned long long)sweeps;
       d.st->steps += 1;
-      d.st->last_iters = sweeps;
+      d.st->last_iters = its;
       d.st->last_rr = rr;
       d.st->last_bb = bb;
       if (d.st->status == NRT_OK && status != NRT_OK) {
         d.st->status = status;
         d.st->fail_step = step;
-        d.st->fail_iter = sweeps;
+        d.st->fail_iter = its;
         d.st->fail_rr = rr;
         d.st->fail_bb = bb;
         d.st->fail_cycles = sweeps;
@@ -2020,7 +2215,8 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
     e->b.alloc(N);
     NRT_HIP(hipMemsetAsync(e->b.p, 0, sizeof(double) * N, s));
     const bool mixed = o.precision == NRT_FP32_REFINE;
-    if (o.krylov == NRT_BICGSTAB || o.krylov == NRT_GMRES) {
+    const bool precond = o.precond == NRT_PRECOND_TREE;
+    if (!precond && (o.krylov == NRT_BICGSTAB || o.krylov == NRT_GMRES)) {
       // GMRES: the residual (r0), and in FP64 its Arnoldi vectors (s, t)
       std::vector<DBuf<double>*> vecs{&e->r0};
       if (o.krylov == NRT_BICGSTAB) vecs = {&e->r0, &e->r, &e->p[0], &e->p[1], &e->v[0],
@@ -2031,7 +2227,12 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
         NRT_HIP(hipMemsetAsync(vbuf->p, 0, sizeof(double) * N, s));
       }
     }
-    e->method = o.krylov == NRT_DIRECT ? (o.precision == NRT_FP64 ? 1 : 2) : 0;
+    // 0 Krylov, 1 FP64 tree solve, 2 FP32 tree solve in FP64 refinement, 3 BiCGSTAB
+    // preconditioned by the FP64 tree solve, 4 ... by the FP32 tree solve
+    e->method = o.krylov == NRT_DIRECT ? (o.precision == NRT_FP64 ? 1 : 2)
+                : precond              ? (o.precision == NRT_FP64 ? 3 : 4)
+                                       : 0;
+    const bool tree_f64 = e->method == 1 || e->method == 3;
     if (e->method != 0) {
       // plan of the tree partition solve from the assembled pattern
       std::vector<int32_t> hrp(N + 1), hcol(nnz);
@@ -2068,8 +2269,9 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       // groups 33.7 us, 7 groups 31.7, 8 groups 32.0, 12 groups 33.9; depth-14
       // tree of 902,037 rows 389 us in grid mode, 127 with 64 groups, 132 with 80,
       // 146 with 128; FP32 + refinement: 263 -> 138 us and 799 -> 327 us).
-      int G = batch_threads > 0 ? 0 : o.tree_groups;
-      if (G == 0 && batch_threads == 0 && e->method != 0 && o.num_blocks == 0 && N >= (1 << 14))
+      // (the preconditioned Krylov form runs in block or grid launches)
+      int G = batch_threads > 0 || precond ? 0 : o.tree_groups;
+      if (G == 0 && batch_threads == 0 && !precond && o.num_blocks == 0 && N >= (1 << 14))
         G = std::min(64, e->method == 1 ? (N + 1450) / 2900 : (N + 1700) / 3400);
       if (G > 1) {
         for (int hh = 1; hh < H; hh++)
@@ -2094,8 +2296,8 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
         e->nblocks = G;
         e->threads = kLocalBlock;
       } else if (e->grid_mode) {
-        const void* k = e->method == 1 ? (const void*)k_implicit_tree<double, true>
-                                       : (const void*)k_implicit_tree<float, true>;
+        const void* k = tree_f64 ? (const void*)k_implicit_tree<double, true>
+                                 : (const void*)k_implicit_tree<float, true>;
         e->nblocks = o.num_blocks > 0 || nb <= 0 ? choose_blocks(hnd, N, k) : nb;
         e->threads = kImpBlock;
       } else {
@@ -2255,7 +2457,15 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
       e->rowptr.release();
       e->col.release();
       const size_t T = (size_t)P.total_nodes();
-      if (e->method == 1) {
+      if (precond) {
+        // the Krylov vectors, in slot order: r, r0, p, v, s, t, M^-1 p, M^-1 s
+        for (DBuf<double>* vbuf : {&e->r, &e->r0, &e->p[0], &e->v[0], &e->s, &e->t, &e->p[1],
+                                   &e->v[1]}) {
+          vbuf->alloc(ns0);
+          NRT_HIP(hipMemsetAsync(vbuf->p, 0, sizeof(double) * ns0, s));
+        }
+      }
+      if (tree_f64) {
         e->tw64.alloc(kTreeArrays * T);
         NRT_HIP(hipMemsetAsync(e->tw64.p, 0, sizeof(double) * kTreeArrays * T, s));
         if (e->local_res) {
@@ -2395,6 +2605,7 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b, T* alt = n
   return TreeArgs<T>{e->plan,
                      w,
                      e->method == 1 ? 1 : 0,
+                     e->method >= 3 ? 1 : 0,
                      o.refine_tol,
                      o.refine_max,
                      (e->grid_mode && e->plan_h.top_n > 0) ||
@@ -2463,7 +2674,7 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
     else
       NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));
   };
-  if (e->method == 1)
+  if (e->method == 1 || e->method == 3)
     launch(tree_args<double>(e, o, e->tw64.p, e->tw64alt.p), (const void*)k_implicit_tree<double, true>,
            (const void*)k_implicit_tree<double, false>,
            (const void*)(e->local_res ? (const void*)k_implicit_tree_local<double, true>
@@ -2502,6 +2713,13 @@ static std::string failure_message(const Implicit* e, const nrt_options& o, cons
              "(relative %.3e; %s)",
              e->gm, st.fail_step, what, st.fail_iter, st.fail_cycles, sqrt(st.fail_rr), rel,
              o.precision == NRT_FP32_REFINE ? "Jacobi-scaled, refine_tol" : "tol");
+  } else if (e->method >= 3) {
+    snprintf(msg, sizeof(msg),
+             "BiCGSTAB preconditioned by the %s tree solve, step %lld, %s after %d iterations "
+             "(%d tree solves): |r| = %.3e (relative %.3e, tol %.1e %s)",
+             e->method == 3 ? "FP64" : "FP32", st.fail_step, what, st.fail_iter,
+             st.fail_cycles, sqrt(st.fail_rr), rel, o.tol,
+             o.tol_mode == NRT_TOL_RELATIVE ? "relative" : "absolute");
   } else if (e->method == 2) {
     snprintf(msg, sizeof(msg),
              "FP32 tree solve in FP64 refinement, step %lld, %s after %d sweeps: scaled "
@@ -2523,7 +2741,8 @@ static int collect_stats(nrt_handle* hnd) {
   NRT_HIP(hipStreamSynchronize(hnd->stream));
   hnd->stats.steps = (int64_t)st.steps;
   hnd->stats.krylov_iters = (int64_t)st.iters_total;
-  hnd->stats.refine_sweeps = hnd->opt.precision == NRT_FP32_REFINE ? (int64_t)st.sweeps_total : 0;
+  hnd->stats.refine_sweeps =
+      hnd->opt.precision == NRT_FP32_REFINE || e->method >= 3 ? (int64_t)st.sweeps_total : 0;
   hnd->stats.last_iters = st.last_iters;
   hnd->stats.last_residual = sqrt(st.last_rr);
   hnd->stats.last_rhs_norm = sqrt(st.last_bb);
@@ -2633,7 +2852,7 @@ void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_ele
   d.a1 = a1;
   d.interval = interval;
   d.trace_elem = probe_index(e, trace_elem, interval);
-  if (e->method == 1) {
+  if (e->method == 1 || e->method == 3) {
     BatchSlot<double> S{d, tree_args<double>(e, h->opt, e->tw64.p), trace};
     std::memcpy(out, &S, sizeof(S));
   } else {
@@ -2720,7 +2939,8 @@ int64_t implicit_device_bytes(const Implicit* e) {
   const int64_t N = e->N, T = e->plan_h.total_nodes(), S = e->ns0;
   return N * (8 * 5 + 4) + S * (8 * 5 + 4) + (int64_t)e->nnz * 13 + (S + 1) * 4 +
          (int64_t)e->H * 8 * 13 + (int64_t)e->plan_buf.n * 4 +
-         T * kTreeArrays * (e->method == 1 ? 8 : 4);
+         T * kTreeArrays * (e->method == 1 || e->method == 3 ? 8 : 4) +
+         (e->method >= 3 ? S * 8 * 8 : 0);
 }
 
 }  // namespace nrt

@@ -45,8 +45,9 @@ extern "C" {
 
 #define NRT_FP64          0  /* plain form: every step in FP64                      */
 #define NRT_FP32_REFINE   1  /* FP32 solves inside FP64 iterative refinement: the
-                                tree solve (NRT_DIRECT) or GMRES's Arnoldi work
-                                (SpMV, basis, compensated dots) in FP32; the
+                                tree solve (NRT_DIRECT), GMRES's Arnoldi work
+                                (SpMV, basis, compensated dots) or BiCGSTAB's
+                                preconditioner (NRT_PRECOND_TREE) in FP32; the
                                 residual b - A x, the update of V and the HH
                                 gates stay FP64                                   */
 
@@ -55,7 +56,10 @@ extern "C" {
 #define NRT_DIRECT   2   /* exact tree partition solve (no Krylov)           */
 
 #define NRT_PRECOND_NONE 0  /* reference: unpreconditioned                    */
-#define NRT_PRECOND_TREE 1  /* exact tree solve as the preconditioner         */
+#define NRT_PRECOND_TREE 1  /* BiCGSTAB only: the tree solve of the step's
+                               matrix as a right preconditioner (FP64: exact,
+                               one iteration; FP32_REFINE: the FP32 tree solve,
+                               a few); batches and the server accept it       */
 
 #define NRT_TOL_ABSOLUTE 0  /* ||b - A x||_2 <= tol      (reference: 1e-14)  */
 #define NRT_TOL_RELATIVE 1  /* ||b - A x||_2 <= tol*||b||_2                  */

@@ -138,3 +138,23 @@ def test_no_cooperative_launches():
         hip = sorted(s for s in _undefined_symbols(exe) if s.startswith("hip"))
         assert not hip, hip
         assert any(s.startswith("nrt_") for s in _undefined_symbols(exe))
+
+
+def test_precond_option_validation(nrt):
+    """The tree-solve preconditioner goes with BiCGSTAB (FP64 or FP32 inside it);
+    GMRES takes none, and BiCGSTAB without it has no FP32 form. Rejections come
+    before any device work; accepted options fail on the NULL problem instead."""
+    import ctypes as C
+    L = nrt.lib()
+    h = C.c_void_p()
+    cases = ((dict(krylov=nrt.GMRES, precond=nrt.PRECOND_TREE), "unsupported"),
+             (dict(krylov=nrt.BICGSTAB, precond=7), "invalid"),
+             (dict(krylov=nrt.BICGSTAB, precision=nrt.FP32_REFINE), "unsupported"),
+             (dict(krylov=nrt.BICGSTAB, precond=nrt.PRECOND_TREE), "invalid"),
+             (dict(krylov=nrt.BICGSTAB, precond=nrt.PRECOND_TREE, precision=nrt.FP32_REFINE),
+              "invalid"))
+    for kw, code in cases:
+        o = nrt.default_options(kind=nrt.IMPLICIT, **kw)
+        assert L.nrt_create(None, C.byref(o), C.byref(h)) == nrt.STATUS_CODES[code], kw
+    o = nrt.default_options(kind=nrt.EXPLICIT, precond=nrt.PRECOND_TREE)
+    assert L.nrt_create(None, C.byref(o), C.byref(h)) == nrt.STATUS_CODES["unsupported"]
