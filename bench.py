@@ -1,25 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark of the implicit cable-equation step on one MI355X.
+"""Benchmark of the Neurite cable-equation solvers on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): a myelinated
-axon of 200 HH nodes of Ranvier (1 um) alternating with 1 mm passive internodes
-(100 x 10 um, Cm = 0.005 uF/cm^2, gL = 0.001 mS/cm^2), d = 10 um, N = 20,202
-elements, backward Euler at dt = 10 us, stimulus 10 nA for 0.5 ms at the first
-node. A "step" is one time step: HH gating, matrix refresh, right-hand side and
-the solve, all inside one persistent kernel launch per run; inputs are resident
-on the device.
+Headline (`value`): config 4, the batched parameter sweep: 4,096 seeded trees with
+compartment counts log-uniform in [500, 50,000] (~42.7 M compartments), per-tree
+strain and stimulus, backward Euler at dt = 25 us, all trees advanced together by
+the batched tree-solve kernel. A "step" is one time step of every tree (HH gating,
+matrix refresh, right-hand side and the exact solve). Inputs are on the device
+before the timed region; `value` = compartments x timed steps / wall time of the
+timed steps. The form reported is the fastest of FP64 and FP32 + FP64 refinement
+whose max |dV| against the oracle's exact FP64 solve (16 trees across the size
+range, same steps) is <= 1e-4 mV.
 
-Forms measured on the same workload:
-  fp64_tree_direct     exact tree partition solve in FP64
-  mixed_tree_ir        FP32 tree solves inside FP64 iterative refinement
-                       (residual b - A x and x += d in FP64, Jacobi-scaled
-                       relative residual 1e-10)
-  fp64_plain_bicgstab  the reference method: unpreconditioned BiCGSTAB,
-                       relative residual 1e-10 (few steps: it is slow)
-`value` is the fastest form whose max |dV| against the FP64 exact solve after
-the same steps is <= 1e-4 mV. Prints ONE JSON line.
+The same JSON line carries the metric's latency part, config 5 (a served stream:
+Poisson arrivals at 200 requests/s, compartments log-normal around 20,000,
+5..50 ms each at dt = 25 us) with p50/p99 request latency and its own error
+against the oracle on a sample of requests, and legs for configs 1-3. Every GPU
+form's error is taken against the oracle (test infrastructure, imported here only
+for those checks and for the CPU baselines, which time the oracle's restatement
+of the reference CPU method on this host).
 """
 import argparse
+import concurrent.futures as cf
 import json
 import os
 import sys
@@ -35,6 +36,29 @@ HBM_PEAK_GBS = 8000.0
 TOL_MV = 1e-4
 
 
+# ------------------------------------------------------------ algorithmic bytes
+def tree_bytes(N, H, nnz, steps, solves, vbytes):
+    """Algorithmic bytes of tree-solved steps (k_implicit_tree*, k_batch_tree), per
+    element, counting every chunk as one that reuses its factors:
+    update phase per step: 144 B per HH element (V, gates r/w, parameters, the
+    refreshed diagonal) and per element V 8 + code 4 + c_m dx 8 + K dx 8 + input 8
+    + the right-hand side written 8 (44 B);
+    per tree solve: the forward sweep reads r and the multiplier and writes the
+    partial right-hand side, the backward sweep reads it with the pivot reciprocal
+    and writes alpha, the back-substitution reads alpha, beta, gamma and writes V
+    (FP64: 10 values of 8 B + V 8 = 88 B; FP32: 10 values of 4 B + V read and
+    written 16 = 56 B);
+    refinement only, per residual evaluation (solves + steps of them): the matrix
+    (12 nnz + 4 (N+1)), the gathered V (8 nnz), b (8 N), the residual written into
+    the tree system (vbytes N)."""
+    per_step = 144 * H + 44 * N
+    if vbytes == 8:
+        return steps * per_step + solves * 88 * N
+    per_solve = (10 * vbytes + 16) * N
+    per_resid = 12 * nnz + 4 * (N + 1) + 8 * nnz + 8 * N + vbytes * N
+    return steps * per_step + solves * per_solve + (solves + steps) * per_resid
+
+
 def bicgstab_bytes(N, H, nnz, steps, iters):
     """Per step 144 B per HH element + 40 B per element (update phase); per
     BiCGSTAB iteration 2 SpMV (12 nnz + 4 (N+1) + 16 N) + 21 vector passes of 8 N."""
@@ -42,73 +66,12 @@ def bicgstab_bytes(N, H, nnz, steps, iters):
     return steps * (144 * H + 40 * N) + iters * (2 * spmv + 21 * 8 * N)
 
 
-def tree_bytes(N, H, nnz, steps, solves, vbytes):
-    """Algorithmic bytes of the tree-solved steps (k_implicit_tree, k_batch_tree),
-    counting every chunk as one that reuses its factors (chunks holding an HH row
-    are refactored each step and move more; they are a small share):
-    update phase per step: 144 B per HH element (V, gates r/w, parameters, the
-      refreshed diagonal) and per element V 8 + code 4 + c_m dx 8 + K dx 8 +
-      input 8 + the right-hand side written (FP64 b for the refinement: 8; into the
-      tree system: vbytes);
-    per tree solve: the forward sweep reads r and the multiplier and writes the
-      partial right-hand side, the backward sweep reads it with the pivot
-      reciprocal and the coupling and writes alpha, the back-substitution reads
-      alpha, beta, gamma (10 values) and writes V (8; the refinement also reads V);
-    refinement only, per residual evaluation (solves + steps of them): the matrix
-      (12 nnz + 4 (N+1)), the gathered V (8 nnz), b (8 N), the residual written
-      into the tree system (vbytes N)."""
-    if vbytes == 8:
-        per_step = 144 * H + (8 + 4 + 8 + 8 + 8 + 8) * N
-        per_solve = (10 * 8 + 8) * N
-        return steps * per_step + solves * per_solve
-    per_step = 144 * H + (8 + 4 + 8 + 8 + 8 + 8) * N
-    per_solve = (10 * vbytes + 16) * N
-    per_resid = 12 * nnz + 4 * (N + 1) + 8 * nnz + 8 * N + vbytes * N
-    return steps * per_step + solves * per_solve + (solves + steps) * per_resid
-
-
-def tree_kernel(nh, name, f):
-    """Name of the kernel a form's launches ran (as rocprofv3 lists it)."""
-    fixed = {"fp64_plain_bicgstab": "k_implicit_bicgstab_fp64",
-             "fp64_gmres30": "k_implicit_gmres<double>", "mixed_gmres30_ir": "k_implicit_gmres<float>"}
-    if name in fixed:
-        return fixed[name]
-    return nh.TREE_KERNELS[f["tree_form"]].format(T="double" if name.startswith("fp64") else "float")
-
-
-def run_form(nh, prob, opts, first, steps, a0, a1):
-    s = nh.Solver(prob, kind=nh.IMPLICIT, device=0, **opts)
-    if first > 1:
-        s.run(1, first - 1, a0, a1)
-    s.sync()
-    st0 = s.stats()
-    t0 = time.perf_counter()
-    s.run(first, steps, a0, a1)
-    s.sync()
-    t1 = time.perf_counter()
-    st1 = s.stats()
-    v = s.snapshot()
-    s.close()
-    return {"wall": t1 - t0, "kernel_ms": st1["kernel_ms"],
-            "inner": st1["krylov_iters"] - st0["krylov_iters"], "nnz": st1["matrix_nnz"],
-            "blocks": st1["num_blocks"], "v": v, "tree_form": st1["tree_form"],
-            # GMRES restart cycles (FP32 forms: the refinement sweeps; FP64: the
-            # restarts plus one per step)
-            "cycles": (st1["refine_sweeps"] - st0["refine_sweeps"]) if st1["refine_sweeps"]
-            else (st1["restarts"] - st0["restarts"]) + steps}
-
-
 def gmres_bytes(N, H, nnz, steps, iters, cycles, vbytes):
-    """Algorithmic bytes of GMRES(m) steps (k_implicit_gmres<double|float>), with
-    the cycles of a run taken as equally long (iters / cycles Arnoldi steps):
-    update phase per step: 144 B per HH element + 40 B per element;
-    per cycle start: the FP64 residual (matrix 12 nnz + 4 (N+1), V, b, r0 written:
-      24 N) and at its end x += V y (kk basis vectors of vbytes N, x read and
-      written 16 N);
-    per Arnoldi step k: the SpMV (values vbytes nnz + columns 4 nnz + rowptr
-      4 (N+1), the previous vector read, v_k and w written: 3 vbytes N) and CGS2
-      (two projections and two corrections each read v_0..v_k and w, the
-      corrections write w: (4 (k+1) + 6) vbytes N)."""
+    """GMRES(m) steps (k_implicit_gmres<double|float>), cycles of a run taken as
+    equally long (iters / cycles Arnoldi steps): update phase per step 144 B per HH
+    element + 40 B per element; per cycle the FP64 residual (12 nnz + 4 (N+1) +
+    24 N) and the update x += V y (kk vbytes N + 16 N); per Arnoldi step the SpMV
+    (vbytes nnz + 4 nnz + 4 (N+1) + 3 vbytes N) and CGS2 ((4 (k+1) + 6) vbytes N)."""
     if cycles <= 0:
         return steps * (144 * H + 40 * N)
     kk = iters / cycles
@@ -119,81 +82,224 @@ def gmres_bytes(N, H, nnz, steps, iters, cycles, vbytes):
 
 
 def explicit_bytes(N, H):
-    """48 B per passive element (V r/w, a, r_a dx, W dx, K dx) + 4 B code; HH
-    elements 152 B (V r/w, a, r_a dx, m/h/n r/w, 9 parameters)."""
+    """52 B per passive element (V r/w, a, r_a dx, W dx, K dx, code) and 156 B per
+    HH element (V r/w, a, r_a dx, m/h/n r/w, 9 parameters, code)."""
     return 52 * (N - H) + 156 * H
 
 
-def batched_sweep(nh, wl, count, steps):
-    """Config 4: `count` seeded trees (sizes log-uniform 500..50,000), one batch per
-    form. Returns the per-form throughput and the accuracy of the mixed form."""
-    t0 = time.perf_counter()
-    geoms = [wl.sweep_tree(i) for i in range(count)]
-    imp = [nh.setup(g, dT=r.dT)[0] for g, r in geoms]
-    exp_ = [nh.setup(g, dT=None, dt_factor=0.8)[0] for g, _ in geoms]
-    gen_s = time.perf_counter() - t0
-    Ntot = sum(p.N for p in imp)
-    H = sum(p.H for p in imp)
-    out = {"workload": f"{count} seeded trees, compartments log-uniform 500..50000, depth "
-                       "U{4..12} (capped at >= 2 compartments per segment), per-tree strain "
-                       "U[0,0.3] and stimulus U[1,20] nA at the soma for 0.5 ms; implicit "
-                       "dt = 25 us, explicit dt = 0.8 x each tree's critical step",
-           "trees": count, "compartments": Ntot, "hh_elements": H, "setup_s": gen_s}
-    sample = list(range(0, count, max(1, count // 16)))
-    snaps = {}
-    for name, kw, probs in [
-            ("implicit_fp64", dict(kind=nh.IMPLICIT, krylov=nh.DIRECT), imp),
-            ("implicit_mixed", dict(kind=nh.IMPLICIT, krylov=nh.DIRECT,
-                                    precision=nh.FP32_REFINE, refine_tol=1e-10), imp),
-            ("explicit", dict(kind=nh.EXPLICIT), exp_)]:
-        b = nh.Batch(**kw)
-        t0 = time.perf_counter()
-        ids = [b.add(p, steps + 8, 1, 20) for p in probs]
-        add_s = time.perf_counter() - t0
-        b.step(4)  # warm-up (and the workgroup-size autotuner's first launches)
-        b.step(4)
-        st0 = b.stats()
-        t0 = time.perf_counter()
-        for _ in range(steps // 10):
-            b.step(10)
-        wall = time.perf_counter() - t0
-        st1 = b.stats()
-        cs = st1["compartment_steps"] - st0["compartment_steps"]
-        kms = st1["kernel_ms"] - st0["kernel_ms"]
-        rec = {"compartment_steps_per_s": cs / wall, "ms_per_step": wall / (steps // 10 * 10) * 1e3,
-               "kernel_ms": kms, "add_s": add_s, "device_bytes": st1["device_bytes"],
-               "workgroup_threads": st1["threads"] if name != "explicit" else 256}
-        timed = steps // 10 * 10
-        if name == "explicit":
-            nb = explicit_bytes(Ntot, H) * timed
-            kern = "k_explicit_batch"
-        else:
-            snaps[name] = [b.snapshot(ids[i]) for i in sample]
-            # solves in the timed steps: scaled from each slot's total (the warm-up
-            # steps take as many solves per step as the timed ones, within a sweep)
-            nb = 0
-            vb = 8 if name == "implicit_fp64" else 4
-            for sid, p in zip(ids, probs):
-                inf = b.info(sid)
-                solves = inf["inner_iters"] * timed / max(1, inf["steps_done"])
-                nb += tree_bytes(p.N, p.H, inf["nnz"], timed, solves, vb)
-            kern = f"k_batch_tree<{'double' if vb == 8 else 'float'},{st1['threads']}>"
-        ach = nb / (kms / 1e3) / 1e9
-        rec["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "kernel": kern,
-                           "algorithmic_bytes": nb}
-        out[name] = rec
-        b.close()
-    err = 0.0
-    for i, (vm, vd) in enumerate(zip(snaps["implicit_mixed"], snaps["implicit_fp64"])):
-        m = np.ones(len(vd), bool)
-        m[[0, len(vd) - 1]] = False
-        err = max(err, float(np.max(np.abs(vm - vd)[m])) * 1e3)
-    out["implicit_mixed"]["max_abs_dV_mV_vs_fp64"] = err
-    out["implicit_mixed"]["compared_trees"] = len(sample)
+def roofline(nbytes, kernel_ms, kernel, launches, traffic=None):
+    ach = nbytes / max(launches, 1) / (kernel_ms / max(launches, 1) / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+            "launches": launches, "kernel_ms_per_launch": kernel_ms / max(launches, 1),
+            "algorithmic_bytes_per_launch": nbytes / max(launches, 1)}
+
+
+def measured_traffic(kernel, workload, steps_per_launch):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE
+    runs of this kernel on this workload (per step, copy-calibrated), scaled to the
+    launch's steps; None when no run covers it."""
+    tfile = os.path.join(REPO, "profiles", "traffic.json")
+    if not os.path.exists(tfile):
+        return None
+    t = json.load(open(tfile)).get(f"{kernel} | {workload}")
+    return None if not t else t["hbm_bytes_per_step"] * steps_per_launch
+
+
+# -------------------------------------------------------------------- helpers
+def host_info():
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable,
+            "threads_used": min(16, usable)}
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle  # the checker: errors of the GPU forms and the CPU baselines only
+    oracle.lib()
+    return oracle
+
+
+def oracle_exact(orc, p, steps, a0, a1):
+    ref = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    ref.run(1, steps, a0, a1, 0, -1)
+    return ref.snapshot()
+
+
+def mask_of(p):
+    m = np.ones(p.N, bool)
+    m[[0, p.N - 1]] = False
+    m[np.asarray(p.terminals)] = False
+    return m
+
+
+def err_mv(v, ref, p):
+    return float(np.max(np.abs(np.asarray(v) - ref)[mask_of(p)])) * 1e3
+
+
+def split_launches(total, per):
+    out = [per] * (total // per)
+    if total % per:
+        out.append(total % per)
     return out
 
 
+def cpu_pool(orc, jobs, threads, method):
+    """Runs jobs (problem, steps, a0, a1) on `threads` host threads, one problem per
+    job, with the oracle (its calls release the interpreter lock, so the threads
+    run in parallel); returns (compartment-steps, wall seconds, failures)."""
+    def one(job):
+        p, n, a0, a1 = job
+        if method == "gmres":
+            s = orc.Implicit(p, variant=orc.VAR_CPU, method=orc.M_GMRES, restart=150,
+                             tol=1e-14, tol_mode=0)
+        else:
+            s = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+        bad = 0
+        for i in range(1, n + 1):
+            s.update(a0 <= i <= a1)
+            bad += s.calculate() != 0
+        s.close()
+        return p.N * n, bad
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, jobs))
+    wall = time.perf_counter() - t0
+    return sum(r[0] for r in res), wall, sum(r[1] for r in res)
+
+
+# ------------------------------------------------------- config 4: the sweep
+def sweep_problems(nh, wl, count):
+    t0 = time.perf_counter()
+    geoms = [wl.sweep_tree(i) for i in range(count)]
+    imp = [nh.setup(g, dT=r.dT)[0] for g, r in geoms]
+    gen_s = time.perf_counter() - t0
+    return geoms, imp, gen_s
+
+
+def run_sweep_form(nh, probs, kw, warmup, steps, per_launch):
+    """Adds every problem as a slot for warmup + steps steps; untimed warm-up
+    launches (the first three also pick the workgroup size), then the timed steps."""
+    warmup = max(warmup, 3)  # the workgroup-size autotuner times the first three launches
+    b = nh.Batch(**kw)
+    t0 = time.perf_counter()
+    ids = [b.add(p, warmup + steps, 1, 20) for p in probs]
+    add_s = time.perf_counter() - t0
+    for k in split_launches(warmup, max(1, warmup // 5)):
+        b.step(k)
+    implicit = kw["kind"] == nh.IMPLICIT
+    inner0 = [b.info(i)["inner_iters"] for i in ids] if implicit else None
+    st0 = b.stats()
+    t0 = time.perf_counter()
+    for k in split_launches(steps, per_launch):
+        b.step(k)
+    wall = time.perf_counter() - t0
+    st1 = b.stats()
+    out = {"wall_s": wall, "kernel_ms": st1["kernel_ms"] - st0["kernel_ms"],
+           "launches": len(split_launches(steps, per_launch)),
+           "compartment_steps": st1["compartment_steps"] - st0["compartment_steps"],
+           "add_s": add_s, "device_bytes": st1["device_bytes"],
+           "threads": st1["threads"] if implicit else 256}
+    if implicit:
+        infos = [b.info(i) for i in ids]
+        out["solves"] = [inf["inner_iters"] - s0 for inf, s0 in zip(infos, inner0)]
+        out["nnz"] = [inf["nnz"] for inf in infos]
+        out["status"] = max(inf["status"] for inf in infos)
+    return b, ids, out
+
+
+def sweep_leg(nh, wl, orc, a, info):
+    geoms, imp, gen_s = sweep_problems(nh, wl, a.sweep_trees)
+    Ntot = sum(p.N for p in imp)
+    H = sum(p.H for p in imp)
+    order = np.argsort([p.N for p in imp])
+    sample = [int(order[k]) for k in np.linspace(0, len(imp) - 1, 16).astype(int)]
+    per_launch = min(a.steps, 10)
+    wname = f"sweep {a.sweep_trees} trees"
+    leg = {"workload": f"{a.sweep_trees} seeded trees, compartments log-uniform 500..50,000, depth "
+                       "U{4..12} (capped at >= 2 compartments per segment), per-tree strain "
+                       "U[0,0.3], stimulus U[1,20] nA at the soma for 0.5 ms (steps 1-20); "
+                       "implicit dt = 25 us; explicit dt = 0.8 x each tree's critical step",
+           "trees": a.sweep_trees, "compartments": Ntot, "hh_elements": H, "setup_s": gen_s,
+           "timed_steps": a.steps, "warmup_steps": a.warmup, "steps_per_launch": per_launch,
+           "oracle_sample": f"{len(sample)} trees across the size range, "
+                            f"N {imp[sample[0]].N}..{imp[sample[-1]].N}"}
+    forms = {"implicit_fp64": (dict(kind=nh.IMPLICIT, krylov=nh.DIRECT), 8),
+             "implicit_mixed": (dict(kind=nh.IMPLICIT, krylov=nh.DIRECT,
+                                     precision=nh.FP32_REFINE, refine_tol=1e-10), 4)}
+    for name, (kw, vb) in forms.items():
+        b, ids, r = run_sweep_form(nh, imp, kw, a.warmup, a.steps, per_launch)
+        snaps = {i: b.snapshot(ids[i]) for i in sample}
+        done = b.info(ids[sample[0]])["steps_done"]
+        b.close()
+        nb = sum(tree_bytes(p.N, p.H, nz, a.steps, sv, vb)
+                 for p, nz, sv in zip(imp, r["nnz"], r["solves"]))
+        kern = f"k_batch_tree<{'double' if vb == 8 else 'float'},{r['threads']}>"
+        rec = {"compartment_steps_per_s": r["compartment_steps"] / r["wall_s"],
+               "ms_per_step": r["wall_s"] / a.steps * 1e3, "status": r["status"],
+               "solves_per_step": sum(r["solves"]) / len(imp) / a.steps,
+               "workgroup_threads": r["threads"], "add_s": r["add_s"],
+               "device_bytes": r["device_bytes"],
+               "roofline": roofline(nb, r["kernel_ms"], kern, r["launches"],
+                                    measured_traffic(kern, wname, per_launch))}
+        if orc is not None:
+            rec["max_abs_dV_mV_vs_oracle"] = max(
+                err_mv(snaps[i], oracle_exact(orc, imp[i], done, 1, 20), imp[i]) for i in sample)
+        leg[name] = rec
+    # explicit: every tree at 0.8 x its own critical step
+    exp_ = [nh.setup(g, dT=None, dt_factor=0.8)[0] for g, _ in geoms]
+    b, ids, r = run_sweep_form(nh, exp_, dict(kind=nh.EXPLICIT), a.warmup, a.steps, per_launch)
+    ex_snap = {i: b.snapshot(ids[i]) for i in sample[::4]}
+    ex_done = b.info(ids[sample[0]])["steps_done"]
+    b.close()
+    nb = explicit_bytes(sum(p.N for p in exp_), sum(p.H for p in exp_)) * a.steps
+    rec = {"compartment_steps_per_s": r["compartment_steps"] / r["wall_s"],
+           "ms_per_step": r["wall_s"] / a.steps * 1e3,
+           "roofline": roofline(nb, r["kernel_ms"], "k_explicit_batch", a.steps,
+                                measured_traffic("k_explicit_batch", wname, 1))}
+    if orc is not None:
+        errs = []
+        for i, v in ex_snap.items():
+            ex = orc.Explicit(exp_[i])
+            ex.run(1, ex_done, 1, 20)
+            errs.append(err_mv(v, ex.snapshot(), exp_[i]))
+        rec["max_abs_dV_mV_vs_oracle"] = max(errs)
+        rec["oracle_sample"] = "4 trees of the sample, smallest to largest"
+    leg["explicit"] = rec
+    if orc is not None and not a.no_cpu_baseline:
+        th = info["threads_used"]
+        jobs = [(imp[i], 1, 1, 20) for i in range(min(64, len(imp)))]
+        cs, wall, bad = cpu_pool(orc, jobs, th, "gmres")
+        leg["cpu_baseline"] = {
+            "value": cs / wall, "unit": "compartment-steps/s", "cores": th, "kind": "port",
+            "sample": f"the first step of each of the first {len(jobs)} trees of the sweep "
+                      f"({sum(j[0].N for j in jobs)} compartments), one tree per thread on "
+                      f"{th} threads of {info['cpu_model']}, with the reference CPU method: "
+                      "restarted GMRES(150), x0 = 0, ||r|| <= 1e-14 A (solverIm-cpu.cpp "
+                      "semantics), oracle C++ build" + (f"; {bad} steps not converged" if bad else ""),
+            "wall_s": wall}
+        cs, wall, _ = cpu_pool(orc, [(imp[i], 20, 1, 20) for i in range(min(64, len(imp)))], th,
+                               "direct")
+        leg["cpu_exact_elimination"] = {
+            "value": cs / wall, "cores": th,
+            "sample": "20 steps of each of the same trees with the exact (Hines) elimination, "
+                      "one tree per thread"}
+    return leg
+
+
+# ------------------------------------------------------ config 5: the stream
 def served_requests(nh, wl, count):
     reqs = []
     for i in range(count):
@@ -203,241 +309,305 @@ def served_requests(nh, wl, count):
     return reqs
 
 
-def served_stream(nh, reqs, precision):
-    """Config 5: requests with Poisson arrivals (200/s), sizes log-normal (median
-    20,000, sigma 1, cap 2M), 5..50 ms simulated at dt = 25 us, submitted in real
-    time to the request server."""
-    count = len(reqs)
+def served_stream(nh, reqs, precision, fetch_ids):
     kw = dict(kind=nh.IMPLICIT, krylov=nh.DIRECT, precision=precision, refine_tol=1e-10)
     srv = nh.Server(steps_per_launch=16, **kw)
     t0 = time.perf_counter()
     due = 0.0
+    rids = []
     for p, n, gap in reqs:
         due += gap
         wait = t0 + due - time.perf_counter()
         if wait > 0:
             time.sleep(wait)
-        srv.submit(p, n, 1, 20)
+        rids.append(srv.submit(p, n, 1, 20))
     srv.wait_idle(600.0)
     wall = time.perf_counter() - t0
     st = srv.stats()
-    res = srv.poll()
+    fetched = {i: srv.fetch(rids[i])[1] for i in fetch_ids}
     srv.close()
-    return {"requests": count, "completed": st["completed"], "failed": st["failed"],
-            "arrival_window_s": due, "wall_s": wall,
-            "compartment_steps_per_s": st["compartment_steps"] / wall,
-            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-            "largest_request": max(p.N for p, _, _ in reqs),
-            "launches": st["launches"], "results": len(res),
-            "loop_ms": {k: st[k] for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms",
-                                           "batch_kernel_ms")}}
+    return fetched, {
+        "requests": len(reqs), "completed": st["completed"], "failed": st["failed"],
+        "arrival_window_s": due, "wall_s": wall,
+        "compartment_steps_per_s": st["compartment_steps"] / wall,
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+        "launches": st["launches"],
+        "loop_ms": {k: st[k] for k in ("launch_ms", "admit_ms", "wait_ms", "retire_ms",
+                                       "batch_kernel_ms")}}
 
 
-def dendritic_tree(nh, wl, tree_steps, krylov_steps, max_iter):
-    """Config 3: the seeded depth-14 dendritic tree (16,383 segments, ~1.8M
-    compartments, dx = 2 um, per-segment strain U[0, 0.3], HH soma, dt = 25 us).
-    GMRES(30) in FP64 (relative 1e-10) and in FP32 inside FP64 refinement (scaled
-    relative 1e-12) for `krylov_steps` steps (each step takes hundreds of Arnoldi
-    steps), the tree solves for `tree_steps`; every form's error is taken against
-    the FP64 tree solve after the same steps."""
-    t0 = time.perf_counter()
-    geom, run = wl.random_tree(wl.seed_for(3), depth=14)
-    prob, _ = nh.setup(geom, dT=run.dT)
-    setup_s = time.perf_counter() - t0
-    N, H = prob.N, prob.H
-    a0, a1 = nh.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
-    gm = dict(krylov=nh.GMRES, restart=30, max_iter=max_iter)
-    specs = {
-        "fp64_tree_direct": (tree_steps, dict(krylov=nh.DIRECT, precision=nh.FP64)),
-        "mixed_tree_ir": (tree_steps, dict(krylov=nh.DIRECT, precision=nh.FP32_REFINE,
-                                           refine_tol=1e-10)),
-        "fp64_gmres30": (krylov_steps, dict(gm, precision=nh.FP64, tol=1e-10,
-                                            tol_mode=nh.TOL_RELATIVE)),
-        # the scaled 2-norm over ~900k rows: 1e-12 keeps every element within 1e-4 mV
-        "mixed_gmres30_ir": (krylov_steps, dict(gm, precision=nh.FP32_REFINE, refine_tol=1e-12)),
-    }
+def served_leg(nh, wl, orc, a, info):
+    reqs = served_requests(nh, wl, a.serve_requests)
+    # oracle sample: every 33rd request (its full run)
+    sample = list(range(0, len(reqs), max(1, len(reqs) // 12)))[:12]
+    leg = {"workload": "Poisson arrivals 200/s, compartments log-normal (median 20000, sigma 1, "
+                       "cap 2M), 5..50 ms at dt 25 us, stimulus 10 nA for 0.5 ms",
+           "largest_request": max(p.N for p, _, _ in reqs),
+           "offered_compartment_steps": sum(p.N * n for p, n, _ in reqs),
+           "oracle_sample": f"{len(sample)} requests (every {max(1, len(reqs) // 12)}th)"}
+    for name, prec in (("fp64", nh.FP64), ("mixed", nh.FP32_REFINE)):
+        fetched, rec = served_stream(nh, reqs, prec, sample)
+        if orc is not None:
+            rec["max_abs_dV_mV_vs_oracle"] = max(
+                err_mv(fetched[i], oracle_exact(orc, reqs[i][0], reqs[i][1], 1, 20), reqs[i][0])
+                for i in sample)
+        leg[name] = rec
+    # the plain FP64 GPU form (the reference's method: unpreconditioned BiCGSTAB,
+    # relative 1e-10) on a sample of the same requests, one handle each
+    plain, fails = [], 0
+    for p, n, _ in reqs[:4]:
+        with nh.Solver(p, kind=nh.IMPLICIT, krylov=nh.BICGSTAB, tol=1e-10,
+                       tol_mode=nh.TOL_RELATIVE) as s:
+            t0 = time.perf_counter()
+            try:
+                s.run(1, 2, 1, 20)
+            except nh.NrtError:
+                fails += 1
+            s.sync()
+            plain.append((p.N * 2, time.perf_counter() - t0, s.stats()["krylov_iters"] / 2))
+    leg["fp64_plain_bicgstab"] = {
+        "compartment_steps_per_s": sum(x[0] for x in plain) / sum(x[1] for x in plain),
+        "iterations_per_step": [x[2] for x in plain], "not_converged_runs": fails,
+        "sample": "the first 4 requests of the stream, 2 steps each, alone"}
+    if orc is not None and not a.no_cpu_baseline:
+        th = info["threads_used"]
+        jobs = [(p, 1, 1, 20) for p, _, _ in reqs[:48] if p.N <= 60000][:32]
+        cs, wall, bad = cpu_pool(orc, jobs, th, "gmres")
+        leg["cpu_baseline"] = {
+            "value": cs / wall, "unit": "compartment-steps/s", "cores": th, "kind": "port",
+            "sample": f"the first step of each of the first {len(jobs)} requests of at most "
+                      f"60,000 compartments, one request per thread on {th} threads, reference "
+                      "CPU method (GMRES(150), x0 = 0, 1e-14 A)" +
+                      (f"; {bad} steps not converged" if bad else ""),
+            "wall_s": wall}
+    return leg
+
+
+# ------------------------------------------------------------ configs 1..3
+def run_solver(nh, prob, kind, opts, steps, a0, a1):
+    with nh.Solver(prob, kind=kind, device=0, **opts) as s:
+        s.sync()
+        st0 = s.stats()
+        t0 = time.perf_counter()
+        s.run(1, steps, a0, a1)
+        s.sync()
+        wall = time.perf_counter() - t0
+        st1 = s.stats()
+        v = s.snapshot()
+    return v, wall, st1, st0
+
+
+def config1_leg(nh, wl, orc, a):
+    """Config 1: the 1,000-compartment squid axon, explicit FP64, 10 ms."""
+    geom, run = wl.squid_axon()
+    _, crit = nh.setup(geom, dT=run.dT)
+    leg = {"workload": "unbranched HH cable 1 cm, d 1 um, dx 10 um (1,000 compartments + 2 "
+                       "ghosts), 10 nA for 0.5 ms at compartment 1, 10 ms",
+           "critical_dT_s": crit}
+    for label, dT in (("dt_1us_literal", run.dT), ("dt_0.8_critical", 0.8 * crit)):
+        prob, _ = nh.setup(geom, dT=dT)
+        steps = run.steps(dT)
+        a0, a1 = nh.active_range(dT, steps, run.init_time, run.end_input)
+        v, wall, st, _ = run_solver(nh, prob, nh.EXPLICIT, {}, steps, a0, a1)
+        rec = {"dT_s": dT, "steps": steps, "compartment_steps_per_s": prob.N * steps / wall,
+               "ms_per_step": wall / steps * 1e3, "finite": bool(np.all(np.isfinite(v))),
+               "max_abs_V_mV": float(np.max(np.abs(v))) * 1e3}
+        if st["kernel_ms"] > 0:
+            nb = explicit_bytes(prob.N, prob.H) * steps
+            rec["roofline"] = roofline(nb, st["kernel_ms"], "k_explicit_fp64", steps)
+        if orc is not None:
+            ex = orc.Explicit(prob)
+            t0 = time.perf_counter()
+            ex.run(1, steps, a0, a1)
+            cpu = time.perf_counter() - t0
+            rv = ex.snapshot()
+            rec["max_abs_dV_mV_vs_oracle"] = (err_mv(v, rv, prob) if np.all(np.isfinite(rv))
+                                              else None)
+            rec["identical_to_oracle"] = bool(np.array_equal(v, rv, equal_nan=True))
+            if not a.no_cpu_baseline:
+                rec["cpu_baseline"] = {"value": prob.N * steps / cpu, "cores": 1, "kind": "port",
+                                       "sample": "the full run, oracle explicit step (solverEx-"
+                                                 "cpu.cpp semantics), 1 thread"}
+        leg[label] = rec
+    return leg
+
+
+def implicit_forms(nh, prob, forms, a0, a1, orc):
+    out = {}
     refs = {}
-    for st in {tree_steps, krylov_steps} - {0}:
-        refs[st] = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP64), 1, st, a0,
-                            a1)["v"]
-    mask = np.ones(N, bool)
-    mask[[0, N - 1]] = False
-    kernels = {"fp64_gmres30": "k_implicit_gmres<double>",
-               "mixed_gmres30_ir": "k_implicit_gmres<float>"}
-    out = {"workload": "seeded binary dendritic tree, depth 14 (16,383 segments of "
-                       "U[20,200] um), Rall 3/2 diameters from 4 um, dx 2 um, strain U[0,0.3] "
-                       "per segment, HH soma, dt 25 us, stimulus 1 nA x 0.5 ms at the soma",
-           "N": N, "hh_elements": H, "setup_s": setup_s}
-    for name, (steps, opts) in specs.items():
-        if steps <= 0:
-            continue
+    for name, (steps, opts) in forms.items():
         try:
-            f = run_form(nh, prob, opts, 1, steps, a0, a1)
+            v, wall, st, st0 = run_solver(nh, prob, nh.IMPLICIT, opts, steps, a0, a1)
         except nh.NrtError as e:
             out[name] = {"steps": steps, "error": str(e)}
             continue
-        vb = 8 if name.startswith("fp64") else 4
-        if "gmres" in name:
-            nbytes = gmres_bytes(N, H, f["nnz"], steps, f["inner"], f["cycles"], vb)
+        inner = st["krylov_iters"] - st0["krylov_iters"]
+        rec = {"compartment_steps_per_s": prob.N * steps / wall, "ms_per_step": wall / steps * 1e3,
+               "steps": steps, "inner_per_step": inner / steps, "workgroups": st["num_blocks"]}
+        kr = opts.get("krylov")
+        vb = 8 if opts.get("precision", nh.FP64) == nh.FP64 else 4
+        if kr == nh.DIRECT:
+            nb = tree_bytes(prob.N, prob.H, st["matrix_nnz"], steps, inner, vb)
+            kern = nh.TREE_KERNELS[st["tree_form"]].format(T="double" if vb == 8 else "float")
+        elif kr == nh.GMRES:
+            cycles = st["refine_sweeps"] if vb == 4 else st["restarts"] + steps
+            nb = gmres_bytes(prob.N, prob.H, st["matrix_nnz"], steps, inner, cycles, vb)
+            kern = f"k_implicit_gmres<{'double' if vb == 8 else 'float'}>"
+            rec["cycles_per_step"] = cycles / steps
         else:
-            nbytes = tree_bytes(N, H, f["nnz"], steps, f["inner"], vb)
-        achieved = nbytes / (f["kernel_ms"] / 1e3) / 1e9
-        out[name] = {"compartment_steps_per_s": N * steps / f["wall"],
-                     "ms_per_step": f["wall"] / steps * 1e3, "steps": steps,
-                     "max_abs_dV_mV_vs_fp64": float(np.max(np.abs(f["v"] - refs[steps])[mask]))
-                     * 1e3,
-                     "inner_per_step": f["inner"] / steps,
-                     "cycles_per_step": f["cycles"] / steps, "workgroups": f["blocks"],
-                     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                                  "kernel": tree_kernel(nh, name, f), "kernel_ms": f["kernel_ms"],
-                                  "algorithmic_bytes": nbytes}}
-        if "gmres" not in name:
-            del out[name]["cycles_per_step"]
+            nb = bicgstab_bytes(prob.N, prob.H, st["matrix_nnz"], steps, inner)
+            kern = "k_implicit_bicgstab_fp64"
+        rec["roofline"] = roofline(nb, st["kernel_ms"], kern, 1)
+        if orc is not None:
+            if steps not in refs:
+                refs[steps] = oracle_exact(orc, prob, steps, a0, a1)
+            rec["max_abs_dV_mV_vs_oracle"] = err_mv(v, refs[steps], prob)
+        out[name] = rec
     return out
 
 
-def cpu_baseline(prob, budget_s=15.0, max_steps=3):
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import oracle  # checker library: timed here as the CPU baseline only
-    im = oracle.Implicit(prob, variant=oracle.VAR_CPU, method=oracle.M_GMRES, restart=150,
-                         tol=1e-14, tol_mode=0)
-    t0 = time.perf_counter()
-    n = 0
-    while n < max_steps and time.perf_counter() - t0 < budget_s:
-        n += 1
+def config2_leg(nh, wl, orc, a):
+    geom, run = wl.myelinated_axon()
+    prob, _ = nh.setup(geom, dT=run.dT)
+    a0, a1 = nh.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
+    n = a.config2_steps
+    forms = {"fp64_tree_direct": (n, dict(krylov=nh.DIRECT)),
+             "mixed_tree_ir": (n, dict(krylov=nh.DIRECT, precision=nh.FP32_REFINE,
+                                       refine_tol=1e-10)),
+             "fp64_plain_bicgstab": (a.plain_steps, dict(krylov=nh.BICGSTAB, tol=1e-10,
+                                                         tol_mode=nh.TOL_RELATIVE))}
+    leg = {"workload": "myelinated axon, 200 HH nodes (1 um) + 200 x 1 mm passive internodes "
+                       "(100 x 10 um), d 10 um, N 20,202, dt 10 us, stimulus 10 nA x 0.5 ms",
+           "N": prob.N, "hh_elements": prob.H}
+    leg.update(implicit_forms(nh, prob, forms, a0, a1, orc))
+    if orc is not None and not a.no_cpu_baseline:
+        im = orc.Implicit(prob, variant=orc.VAR_CPU, method=orc.M_GMRES, restart=150, tol=1e-14,
+                          tol_mode=0)
+        t0 = time.perf_counter()
+        k = 0
+        while k < 3 and time.perf_counter() - t0 < 15.0:
+            k += 1
+            im.update(a0 <= k <= a1)
+            im.calculate()
+        leg["cpu_baseline"] = {"value": prob.N * k / (time.perf_counter() - t0), "cores": 1,
+                               "kind": "port", "sample": f"the first {k} steps, reference CPU "
+                               "method (GMRES(150), x0 = 0, 1e-14 A), 1 thread"}
+        cs, wall, _ = cpu_pool(orc, [(prob, 200, a0, a1)], 1, "direct")
+        leg["cpu_exact_elimination"] = {"value": cs / wall, "cores": 1,
+                                        "sample": "200 steps, exact elimination, 1 thread"}
+    return leg
+
+
+def config3_leg(nh, wl, orc, a):
+    geom, run = wl.random_tree(wl.seed_for(3), depth=14)
+    prob, _ = nh.setup(geom, dT=run.dT)
+    a0, a1 = nh.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
+    n, kn = a.config3_steps, a.krylov3_steps
+    gm = dict(krylov=nh.GMRES, restart=30, max_iter=a.krylov3_max_iter)
+    forms = {"fp64_tree_direct": (n, dict(krylov=nh.DIRECT)),
+             "mixed_tree_ir": (n, dict(krylov=nh.DIRECT, precision=nh.FP32_REFINE,
+                                       refine_tol=1e-10)),
+             "fp64_gmres30": (kn, dict(gm, tol=1e-10, tol_mode=nh.TOL_RELATIVE)),
+             "mixed_gmres30_ir": (kn, dict(gm, precision=nh.FP32_REFINE, refine_tol=1e-12))}
+    leg = {"workload": "seeded binary dendritic tree, depth 14 (16,383 segments of U[20,200] um), "
+                       "Rall 3/2 diameters from 4 um, dx 2 um, strain U[0,0.3] per segment, HH "
+                       "soma, dt 25 us, stimulus 1 nA x 0.5 ms at the soma",
+           "N": prob.N, "hh_elements": prob.H}
+    leg.update(implicit_forms(nh, prob, forms, a0, a1, orc))
+    if orc is not None and not a.no_cpu_baseline:
+        # the reference CPU method cannot finish one step of this tree in the
+        # budget: time a bounded number of its Arnoldi iterations, which bounds its
+        # rate from above (a step needs more than that many)
+        it = a.cpu3_iters
+        im = orc.Implicit(prob, variant=orc.VAR_CPU, method=orc.M_GMRES, restart=150, tol=1e-14,
+                          tol_mode=0, max_iter=it)
+        t0 = time.perf_counter()
         im.update(True)
-        im.calculate()
-    dt = time.perf_counter() - t0
-    out = {"value": prob.N * n / dt, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
-           "sample": f"first {n} steps of the same workload (N={prob.N}) with the reference "
-                     "CPU method: restarted GMRES(150), x0=0, ||r||<=1e-14 A "
-                     "(solverIm-cpu.cpp semantics), oracle C++ build, 1 thread"}
-    ex = oracle.Implicit(prob, variant=oracle.VAR_GPU, method=oracle.M_DIRECT)
-    t0 = time.perf_counter()
-    n = 0
-    while n < 2000 and time.perf_counter() - t0 < 5.0:
-        n += 1
-        ex.update(n <= 50)
-        ex.calculate()
-    out["exact_tree_elimination_1core"] = prob.N * n / (time.perf_counter() - t0)
-    return out
+        st = im.calculate()
+        t = time.perf_counter() - t0
+        leg["cpu_baseline"] = {"value_upper_bound": prob.N / t, "cores": 1, "kind": "port",
+                               "sample": f"{it} Arnoldi iterations of step 1 with the reference "
+                                         f"CPU method (GMRES(150), x0 = 0, 1e-14 A), {t:.1f} s, "
+                                         f"step {'converged' if st == 0 else 'not converged'}: "
+                                         "the rate is below N / t", "converged": st == 0}
+        cs, wall, _ = cpu_pool(orc, [(prob, 5, a0, a1)], 1, "direct")
+        leg["cpu_exact_elimination"] = {"value": cs / wall, "cores": 1,
+                                        "sample": "5 steps, exact elimination, 1 thread"}
+    return leg
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=10)
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--steps", type=int, default=100, help="timed steps of the headline sweep")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--legs", default="sweep,served,config1,config2,config3")
     ap.add_argument("--sweep-trees", type=int, default=4096)
-    ap.add_argument("--sweep-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--tree3-steps", type=int, default=200,
-                    help="depth-14 dendritic tree: tree-solve steps (0 skips the leg)")
-    ap.add_argument("--krylov3-steps", type=int, default=4,
-                    help="depth-14 dendritic tree: GMRES(30) steps")
-    ap.add_argument("--krylov3-max-iter", type=int, default=4000,
-                    help="depth-14 dendritic tree: Arnoldi steps allowed per time step")
+    ap.add_argument("--config2-steps", type=int, default=1000)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--config3-steps", type=int, default=200)
+    ap.add_argument("--krylov3-steps", type=int, default=3)
+    ap.add_argument("--krylov3-max-iter", type=int, default=4000)
+    ap.add_argument("--cpu3-iters", type=int, default=60)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true", help="skip every oracle check")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the solvers have "
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the solvers have "
               "no multi-GPU path (each neurite or batch runs on one device)", file=sys.stderr)
         sys.exit(2)
-
     import neurite_hip as nh
     import workloads as wl
 
-    geom, run = wl.myelinated_axon()
-    prob, _ = nh.setup(geom, dT=run.dT)
-    N, H = prob.N, prob.H
-    a0, a1 = nh.active_range(run.dT, run.steps(run.dT), run.init_time, run.end_input)
-    first = a.warmup + 1
-    forms = {}
-    ref = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP64), first, a.steps, a0, a1)
-    forms["fp64_tree_direct"] = ref
-    forms["mixed_tree_ir"] = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP32_REFINE,
-                                                     refine_tol=1e-10), first, a.steps, a0, a1)
-    if not a.no_plain:
-        plain = run_form(nh, prob, dict(krylov=nh.BICGSTAB, precision=nh.FP64, tol=1e-10,
-                                        tol_mode=nh.TOL_RELATIVE), first, a.plain_steps, a0, a1)
-        pref = run_form(nh, prob, dict(krylov=nh.DIRECT, precision=nh.FP64), first,
-                        a.plain_steps, a0, a1)
-        plain["ref_v"] = pref["v"]
-        forms["fp64_plain_bicgstab"] = plain
-    mask = np.ones(N, bool)
-    mask[[0, N - 1]] = False
-    summary = {}
-    for name, f in forms.items():
-        steps = a.plain_steps if name == "fp64_plain_bicgstab" else a.steps
-        rv = f.get("ref_v", ref["v"])
-        err_mv = float(np.max(np.abs(f["v"] - rv)[mask])) * 1e3
-        if name == "fp64_plain_bicgstab":
-            nbytes = bicgstab_bytes(N, H, f["nnz"], steps, f["inner"])
-        else:
-            nbytes = tree_bytes(N, H, f["nnz"], steps, f["inner"], 8 if name.startswith("fp64")
-                                else 4)
-        summary[name] = {"compartment_steps_per_s": N * steps / f["wall"],
-                         "ms_per_step": f["wall"] / steps * 1e3, "steps": steps,
-                         "max_abs_dV_mV_vs_fp64": err_mv, "inner_per_step": f["inner"] / steps,
-                         "kernel_ms": f["kernel_ms"], "algorithmic_bytes": nbytes,
-                         "workgroups": f["blocks"]}
-    ok = {k: v for k, v in summary.items() if v["max_abs_dV_mV_vs_fp64"] <= TOL_MV}
-    best = max(ok, key=lambda k: ok[k]["compartment_steps_per_s"])
-    b = summary[best]
-    kernel = tree_kernel(nh, best, forms[best])
-    achieved = b["algorithmic_bytes"] / (b["kernel_ms"] / 1e3) / 1e9
-    # HBM bytes measured with rocprofv3 counters for this kernel and workload (per step,
-    # calibrated with a copy), scaled to this launch's steps
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", "traffic.json")
-    tkey = f"{kernel} myelinated axon N={N}"
-    if os.path.exists(tfile):
-        t = json.load(open(tfile)).get(tkey)
-        if t:
-            traffic = t["hbm_bytes_per_step"] * b["steps"]
-    out = {
-        "metric": METRIC,
-        "value": b["compartment_steps_per_s"],
-        "unit": "compartment-steps/s",
-        "n_gpus": 1,
-        "steps": b["steps"],
-        "warmup": a.warmup,
-        "ms_per_step": b["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if best.startswith("fp64") else "f32+f64-refine",
-        "data": "synthetic",
-        "config": {"workload": "myelinated axon, 200 HH nodes (1 um) + 200 x 1 mm passive "
-                               "internodes (100 x 10 um), d=10 um, N=20202, backward Euler "
-                               "dt=10 us, stimulus 10 nA x 0.5 ms",
-                   "N": N, "hh_elements": H, "nnz": forms[best]["nnz"], "form": best,
-                   "tolerance": "relative 1e-10 (refinement: Jacobi-scaled; BiCGSTAB: plain)"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-                     "launches": 1, "kernel_ms": b["kernel_ms"],
-                     "algorithmic_bytes": b["algorithmic_bytes"]},
-        "forms": summary,
-    }
-    if "fp64_plain_bicgstab" in summary:
-        out["speedup_vs_fp64_plain"] = (b["compartment_steps_per_s"] /
-                                        summary["fp64_plain_bicgstab"]["compartment_steps_per_s"])
-    if a.tree3_steps > 0:
-        out["dendritic_tree"] = dendritic_tree(nh, wl, a.tree3_steps, a.krylov3_steps,
-                                               a.krylov3_max_iter)
-    if a.sweep_trees > 0:
-        out["batched_sweep"] = batched_sweep(nh, wl, a.sweep_trees, a.sweep_steps)
-    if a.serve_requests > 0:
-        reqs = served_requests(nh, wl, a.serve_requests)
-        out["served"] = {"workload": "Poisson arrivals 200/s, compartments log-normal "
-                                     "(median 20000, sigma 1, cap 2M), 5..50 ms at dt 25 us",
-                         "fp64": served_stream(nh, reqs, nh.FP64),
-                         "mixed": served_stream(nh, reqs, nh.FP32_REFINE)}
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(prob)
+    legs = set(a.legs.split(","))
+    info = host_info()
+    orc = None if a.no_oracle else _oracle()
+    out = {"metric": METRIC, "value": None, "unit": "compartment-steps/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": None, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": None,
+           "data": "synthetic (seeded trees; no datasets)"}
+    res = {"host": info}
+    if "sweep" in legs:
+        res["sweep"] = sweep_leg(nh, wl, orc, a, info)
+    if "served" in legs:
+        res["served"] = served_leg(nh, wl, orc, a, info)
+    if "config1" in legs:
+        res["config1"] = config1_leg(nh, wl, orc, a)
+    if "config2" in legs:
+        res["config2"] = config2_leg(nh, wl, orc, a)
+    if "config3" in legs:
+        res["config3"] = config3_leg(nh, wl, orc, a)
+    if "sweep" in legs:
+        sw = res["sweep"]
+        cand = {k: sw[k] for k in ("implicit_fp64", "implicit_mixed")
+                if sw[k].get("status", 0) == 0 and
+                (orc is None or sw[k].get("max_abs_dV_mV_vs_oracle", 1e9) <= TOL_MV)}
+        best = max(cand, key=lambda k: cand[k]["compartment_steps_per_s"])
+        b = sw[best]
+        out.update({
+            "value": b["compartment_steps_per_s"], "ms_per_step": b["ms_per_step"],
+            "dtype": "f64" if best == "implicit_fp64" else "f32+f64-refine",
+            "config": {"workload": f"config 4 batched sweep: {sw['workload']}",
+                       "trees": sw["trees"], "compartments": sw["compartments"], "form": best,
+                       "max_abs_dV_mV_vs_oracle": b.get("max_abs_dV_mV_vs_oracle"),
+                       "oracle_sample": sw["oracle_sample"]},
+            "roofline": b["roofline"],
+            "cpu_baseline": sw.get("cpu_baseline")})
+    if "served" in legs:
+        sv = res["served"]
+        okf = [k for k in ("fp64", "mixed") if sv[k]["failed"] == 0 and
+               (orc is None or sv[k].get("max_abs_dV_mV_vs_oracle", 1e9) <= TOL_MV)]
+        if okf:
+            bf = max(okf, key=lambda k: sv[k]["compartment_steps_per_s"])
+            out["p50_ms"] = sv[bf]["p50_ms"]
+            out["p99_ms"] = sv[bf]["p99_ms"]
+            out["served_form"] = bf
+            out["served_compartment_steps_per_s"] = sv[bf]["compartment_steps_per_s"]
+            out["speedup_served_mixed_vs_plain_fp64_bicgstab"] = (
+                sv["mixed"]["compartment_steps_per_s"] /
+                sv["fp64_plain_bicgstab"]["compartment_steps_per_s"])
+    out.update(res)
     print(json.dumps(out), flush=True)
 
 
