@@ -55,6 +55,7 @@ struct Explicit {
   DBuf<double> trace;
   int cur = 0;  // buffer holding the latest potential (the reference's prevT)
   int64_t trace_cap = 0;
+  bool res_attr = false;  // the resident kernel's LDS limit is raised
   HHDev hd() {
     return HHDev{m.p, h.p, n.p, E_Na.p, E_K.p, E_L.p, g_Na.p, g_K.p, G_L.p, ls_Na.p, ls_K.p,
                  rest.p, hdx.p};
@@ -123,13 +124,14 @@ struct ExOps {
   double v, vm, vp, rj, rp, aj, w0, k0;
 };
 
-__device__ __forceinline__ ExOps ex_load(int j, int N, const double* __restrict__ Vo_,
-                                         const double* __restrict__ coefA_,
-                                         const double* __restrict__ rdx_,
-                                         const double* __restrict__ wdx_,
-                                         const double* __restrict__ kdx_,
-                                         const uint32_t* __restrict__ code_) {
-  const NRT_GLB double* __restrict__ Vo = glb(Vo_);
+// VO: where the potentials are read (global memory, or the LDS of the resident
+// kernel); the constants always come from global memory.
+template <class VO>
+__device__ __forceinline__ ExOps ex_load_on(int j, int N, VO Vo, const double* __restrict__ coefA_,
+                                            const double* __restrict__ rdx_,
+                                            const double* __restrict__ wdx_,
+                                            const double* __restrict__ kdx_,
+                                            const uint32_t* __restrict__ code_) {
   const NRT_GLB double* __restrict__ rdx = glb(rdx_);
   const int jm = j > 0 ? j - 1 : j, jp = j + 1 < N ? j + 1 : j;
   ExOps o;
@@ -145,6 +147,16 @@ __device__ __forceinline__ ExOps ex_load(int j, int N, const double* __restrict_
   return o;
 }
 
+__device__ __forceinline__ ExOps ex_load(int j, int N, const double* __restrict__ Vo_,
+                                         const double* __restrict__ coefA_,
+                                         const double* __restrict__ rdx_,
+                                         const double* __restrict__ wdx_,
+                                         const double* __restrict__ kdx_,
+                                         const uint32_t* __restrict__ code_) {
+  const NRT_GLB double* __restrict__ Vo = glb(Vo_);
+  return ex_load_on(j, N, Vo, coefA_, rdx_, wdx_, kdx_, code_);
+}
+
 // The loads of ex_load complete here (not later, under the branches on the code).
 __device__ __forceinline__ void ex_ready(const ExOps& o) {
   ready_v(o.c);
@@ -158,13 +170,12 @@ __device__ __forceinline__ void ex_ready(const ExOps& o) {
   ready_v(o.k0);
 }
 
-__device__ __forceinline__ void ex_finish(int j, const ExOps& o, const double* __restrict__ Vo_,
-                                          double* __restrict__ Vn_,
-                                          const double* __restrict__ rdx_,
-                                          const ExRec* __restrict__ recs_, const HHDev& hd,
-                                          double dt_ms, bool active) {
-  const NRT_GLB double* __restrict__ Vo = glb(Vo_);
-  NRT_GLB double* __restrict__ Vn = glb(Vn_);
+// NT: the new potential of a plain element is stored nontemporally (global).
+template <bool NT, class VO, class VN>
+__device__ __forceinline__ void ex_finish_on(int j, const ExOps& o, VO Vo, VN Vn,
+                                             const double* __restrict__ rdx_,
+                                             const ExRec* __restrict__ recs_, const HHDev& hd,
+                                             double dt_ms, bool active) {
   const NRT_GLB double* __restrict__ rdx = glb(rdx_);
   const uint32_t c = o.c;
   const double v = o.v;
@@ -185,7 +196,8 @@ __device__ __forceinline__ void ex_finish(int j, const ExOps& o, const double* _
     s = s + (o.vm - v) / o.rj;
     s = s + wd * v;
     s = s + kd;
-    __builtin_nontemporal_store(v + o.aj * s, Vn + j);
+    if constexpr (NT) __builtin_nontemporal_store(v + o.aj * s, &Vn[j]);
+    else Vn[j] = v + o.aj * s;
     return;
   }
   const NRT_GLB ExRec& rec = glb(recs_)[c & kCodeIdx];
@@ -205,6 +217,16 @@ __device__ __forceinline__ void ex_finish(int j, const ExOps& o, const double* _
 #pragma unroll
   for (int t = 0; t < 3; t++)
     if (rec.tgt[t] >= 0) Vn[rec.tgt[t]] = vn;
+}
+
+__device__ __forceinline__ void ex_finish(int j, const ExOps& o, const double* __restrict__ Vo_,
+                                          double* __restrict__ Vn_,
+                                          const double* __restrict__ rdx_,
+                                          const ExRec* __restrict__ recs_, const HHDev& hd,
+                                          double dt_ms, bool active) {
+  const NRT_GLB double* __restrict__ Vo = glb(Vo_);
+  NRT_GLB double* __restrict__ Vn = glb(Vn_);
+  ex_finish_on<true>(j, o, Vo, Vn, rdx_, recs_, hd, dt_ms, active);
 }
 
 __device__ __forceinline__ void explicit_element(
@@ -228,6 +250,51 @@ __global__ __launch_bounds__(256) void k_explicit_fp64(
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   explicit_element(j, N, Vo, Vn, coefA, rdx, wdx, kdx, code, recs, hd, dt_ms, active != 0);
+}
+
+// Whole runs of a small neurite in one workgroup: both potentials live in LDS
+// for the launch and the steps are separated by workgroup barriers instead of
+// kernel launches (a launch per step costs more than a thousand elements' work).
+// The element update is explicit_element's, operand for operand.
+constexpr int kResBlock = 1024;
+constexpr int kResMaxN = 8192;  // two potentials of 8 B: 128 KiB of LDS
+
+__global__ __launch_bounds__(kResBlock) void k_explicit_resident(
+    int N, const double* __restrict__ Vin, const double* __restrict__ Vother,
+    double* __restrict__ Vout, const double* __restrict__ coefA, const double* __restrict__ rdx,
+    const double* __restrict__ wdx, const double* __restrict__ kdx,
+    const uint32_t* __restrict__ code, const ExRec* __restrict__ recs, HHDev hd, double dt_ms,
+    long long first, long long num, long long a0, long long a1, int trace_elem,
+    long long interval, double* __restrict__ trace) {
+  extern __shared__ double sv[];
+  typedef __attribute__((address_space(3))) double LdsD;
+  LdsD* A = (LdsD*)sv;
+  LdsD* B = (LdsD*)sv + N;
+  const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int j = tid; j < N; j += nt) {
+    A[j] = Vin[j];
+    B[j] = Vother[j];
+  }
+  __syncthreads();
+  long long gl = 0, k = 0;
+  for (long long i = first; i < first + num; i++) {
+    const bool active = i >= a0 && i <= a1;
+    for (int j = tid; j < N; j += nt) {
+      const ExOps o = ex_load_on(j, N, (const LdsD*)A, coefA, rdx, wdx, kdx, code);
+      ex_ready(o);
+      ex_finish_on<false>(j, o, (const LdsD*)A, B, rdx, recs, hd, dt_ms, active);
+    }
+    __syncthreads();
+    LdsD* t = A;
+    A = B;
+    B = t;
+    if (interval >= 0 && ++gl > interval) {
+      gl = 0;
+      if (trace && tid == 0) trace[k] = A[trace_elem];
+      k++;
+    }
+  }
+  for (int j = tid; j < N; j += nt) Vout[j] = A[j];
 }
 
 Explicit* explicit_create(nrt_handle* hnd, const HostProblem& p) {
@@ -373,16 +440,38 @@ void explicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64
   NRT_HIP(hipEventCreate(&t0));
   NRT_HIP(hipEventCreate(&t1));
   NRT_HIP(hipEventRecord(t0, hnd->stream));
-  int64_t gl = 0, k = 0;
-  for (int64_t i = first; i < first + num; i++) {
-    const bool active = i >= a0 && i <= a1;
-    double* dst = nullptr;
-    if (tracing && ++gl > interval) {
-      gl = 0;
-      if (trace_out) dst = e->trace.p + k;
-      k++;
+  int64_t k = 0;
+  if (e->N <= kResMaxN) {
+    // one launch for the whole run; the result lands in the other buffer, which
+    // becomes the current one (the flips of num steps, then one more)
+    const size_t lds = sizeof(double) * 2 * (size_t)e->N;
+    if (!e->res_attr) {
+      NRT_HIP(hipFuncSetAttribute((const void*)k_explicit_resident,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(sizeof(double) * 2 * kResMaxN)));
+      e->res_attr = true;
     }
-    explicit_step(hnd, active, dst, trace_elem);
+    hipLaunchKernelGGL(k_explicit_resident, dim3(1), dim3(kResBlock), lds, hnd->stream, e->N,
+                       e->V[e->cur].p, e->V[1 - e->cur].p, e->V[1 - e->cur].p, e->coefA.p,
+                       e->rdx.p, e->wdx.p, e->kdx.p, e->code.p, e->recs.p, e->hd(), e->dt_ms,
+                       (long long)first, (long long)num, (long long)a0, (long long)a1,
+                       tracing ? trace_elem : 0, (long long)interval,
+                       tracing && trace_out ? e->trace.p : nullptr);
+    NRT_HIP(hipGetLastError());
+    e->cur = 1 - e->cur;
+    k = max_samples;
+  } else {
+    int64_t gl = 0;
+    for (int64_t i = first; i < first + num; i++) {
+      const bool active = i >= a0 && i <= a1;
+      double* dst = nullptr;
+      if (tracing && ++gl > interval) {
+        gl = 0;
+        if (trace_out) dst = e->trace.p + k;
+        k++;
+      }
+      explicit_step(hnd, active, dst, trace_elem);
+    }
   }
   NRT_HIP(hipEventRecord(t1, hnd->stream));
   if (trace_out && k > 0)
@@ -394,7 +483,7 @@ void explicit_run(nrt_handle* hnd, int64_t first, int64_t num, int64_t a0, int64
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   hnd->stats.kernel_ms = ms;
-  hnd->stats.kernel_launches = num;
+  hnd->stats.kernel_launches = e->N <= kResMaxN ? 1 : num;
   hnd->stats.steps += num;
   if (ns) *ns = k;
 }

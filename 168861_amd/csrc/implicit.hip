@@ -341,6 +341,41 @@ constexpr int kImpBlock = 256;
 // breakdown restarts allowed per time step before the step reports NOT_CONVERGED
 constexpr int kMaxRestarts = 64;
 
+// LDS-staged row tiles of the Krylov kernels' SpMV y_i = sum_k val[k] f(col[k]) over
+// this workgroup's rows [r_begin, r_end): the operand f (a stored vector, or one
+// recomputed from its recurrence) is evaluated once per row of a tile and of its
+// one-row halo and kept in LDS, and the rows read their neighbours from there; a
+// column outside the tile (a branch point's far daughter, a daughter's far
+// mother) is evaluated again. kSpmvTile is a multiple of the workgroup size, so
+// every row keeps its thread and every thread its order of rows (the partial
+// sums of the reductions fused into `row` do not change), and the products are
+// summed in the row's column order as before. row(i, y_i, f(i)) consumes own rows.
+constexpr int kSpmvTile = 2048;
+static_assert(kSpmvTile % kImpBlock == 0, "tile rows keep their thread");
+template <class V, class VT, class F, class ROW>
+__device__ __forceinline__ void spmv_tiles(int N, int r_begin, int r_end,
+                                           const int32_t* __restrict__ rowptr,
+                                           const int32_t* __restrict__ col, const VT* val, V* sx,
+                                           F f, ROW row) {
+  const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
+  for (int t0 = r_begin; t0 < r_end; t0 += kSpmvTile) {
+    const int t1 = min(r_end, t0 + kSpmvTile);
+    const int lo = max(0, t0 - 1), hi = min(N, t1 + 1);
+    for (int i = lo + tid; i < hi; i += bs) sx[i - lo] = f(i);
+    __syncthreads();
+    for (int i = t0 + tid; i < t1; i += bs) {
+      V acc = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const int c = col[k];
+        const V xc = (c >= lo && c < hi) ? sx[c - lo] : f(c);
+        acc = acc + val[k] * xc;
+      }
+      row(i, acc, sx[i - lo]);
+    }
+    __syncthreads();
+  }
+}
+
 // HH gating (solverIm-gpu.cpp:61-98): the per-entry parameters, and one step of
 // the gates at potential v (updates m, h, n; returns W and K).
 struct HHGate {
@@ -350,18 +385,38 @@ __device__ __forceinline__ HHGate hh_gate_load(const ImpHH& hd, int i) {
   return HHGate{hd.rest[i], hd.ls_Na[i], hd.ls_K[i], hd.g_Na[i], hd.g_K[i],
                 hd.G_L[i],  hd.E_Na[i],  hd.E_K[i],  hd.E_L[i]};
 }
-__device__ __forceinline__ void hh_gate_step(const HHGate& q, double v, double dt_ms, double& m,
-                                             double& h, double& n, double& W, double& K) {
+// The rate functions (coupling.cpp:25-48) in precision R: FP64 for the FP64 forms;
+// FP32 (expf, single-precision divisions) for the FP32 forms, whose gate state,
+// conductances and currents stay FP64.
+template <class R>
+struct HHRates {
+  double am, bm, ah, bh, an, bn;
+};
+template <class R>
+__device__ __forceinline__ HHRates<R> hh_rates(const HHGate& q, double v) {
   double vna = v + q.ls_Na;
   double vk = v + q.ls_K;
   vna = (vna - q.rest) / 0.001;
   vk = (vk - q.rest) / 0.001;
-  const double am = (2.5 - 0.1 * vna) / (exp(2.5 - 0.1 * vna) - 1);
-  const double bm = 4 * exp(-vna / 18);
-  const double ah = 0.07 * exp(-vna / 20);
-  const double bh = 1 / (exp(3 - 0.1 * vna) + 1);
-  const double an = (0.1 - 0.01 * vk) / (exp(1 - 0.1 * vk) - 1);
-  const double bn = 0.125 * exp(-vk / 80);
+  if constexpr (sizeof(R) == 4) {
+    const float un = (float)vna, uk = (float)vk;
+    return HHRates<R>{(2.5f - 0.1f * un) / (expf(2.5f - 0.1f * un) - 1.0f),
+                      4.0f * expf(-un / 18.0f),
+                      0.07f * expf(-un / 20.0f),
+                      1.0f / (expf(3.0f - 0.1f * un) + 1.0f),
+                      (0.1f - 0.01f * uk) / (expf(1.0f - 0.1f * uk) - 1.0f),
+                      0.125f * expf(-uk / 80.0f)};
+  } else {
+    return HHRates<R>{(2.5 - 0.1 * vna) / (exp(2.5 - 0.1 * vna) - 1), 4 * exp(-vna / 18),
+                      0.07 * exp(-vna / 20), 1 / (exp(3 - 0.1 * vna) + 1),
+                      (0.1 - 0.01 * vk) / (exp(1 - 0.1 * vk) - 1), 0.125 * exp(-vk / 80)};
+  }
+}
+template <class R = double>
+__device__ __forceinline__ void hh_gate_step(const HHGate& q, double v, double dt_ms, double& m,
+                                             double& h, double& n, double& W, double& K) {
+  const HHRates<R> rt = hh_rates<R>(q, v);
+  const double am = rt.am, bm = rt.bm, ah = rt.ah, bh = rt.bh, an = rt.an, bn = rt.bn;
   const double m0 = m, h0 = h, n0 = n;
   m = m0 + dt_ms * (am * (1 - m0) - bm * m0);
   h = h0 + dt_ms * (ah * (1 - h0) - bh * h0);
@@ -372,10 +427,11 @@ __device__ __forceinline__ void hh_gate_step(const HHGate& q, double v, double d
   W = -(gna + gk + gl);
   K = gna * q.E_Na + gk * q.E_K + gl * q.E_L;
 }
+template <class R = double>
 __device__ __forceinline__ void imp_gate(const ImpHH& hd, int i, double v, double dt_ms,
                                          double& W, double& K) {
   double m = hd.m[i], h = hd.h[i], n = hd.n[i];
-  hh_gate_step(hh_gate_load(hd, i), v, dt_ms, m, h, n, W, K);
+  hh_gate_step<R>(hh_gate_load(hd, i), v, dt_ms, m, h, n, W, K);
   hd.m[i] = m;
   hd.h[i] = h;
   hd.n[i] = n;
@@ -488,6 +544,7 @@ __device__ __forceinline__ bool reduce_all(const ImpDev& d, double (&v)[NV], dou
 
 __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) {
   __shared__ double lds[4 * 16];
+  __shared__ double sx[kSpmvTile + 2];
   d.vb = (int)blockIdx.x;
   const int r_begin = blockIdx.x * d.rpb;
   const int r_end = min(d.N, r_begin + d.rpb);
@@ -521,16 +578,16 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
     grid_barrier(d.bar, d.nblocks, d.err, d.spin_limit);
     // ---- r0 = b - A x0 ; r = r0 ; rho = r0.r0 ; bb = b.b
     double red2[2] = {0.0, 0.0};
-    for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
-      double ax = 0;
-      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) ax = ax + val[k] * d.x[col[k]];
-      const double bi = d.b[i];
-      const double ri = bi + (-1.0) * ax;
-      d.r0[i] = ri;
-      d.r[i] = ri;
-      red2[0] = red2[0] + ri * ri;
-      red2[1] = red2[1] + bi * bi;
-    }
+    spmv_tiles<double>(
+        d.N, r_begin, r_end, rowptr, col, val, sx, [&](int c) { return d.x[c]; },
+        [&](int i, double ax, double) {
+          const double bi = d.b[i];
+          const double ri = bi + (-1.0) * ax;
+          d.r0[i] = ri;
+          d.r[i] = ri;
+          red2[0] = red2[0] + ri * ri;
+          red2[1] = red2[1] + bi * bi;
+        });
     bool ok = reduce_all<2>(d, red2, lds, slot);
     double rho = red2[0], rr = red2[0];
     const double bb = red2[1];
@@ -550,19 +607,15 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
       // ---- phase A: p = r + beta*(p - omega*v) ; v = A p ; apr = v.r0
       // (after a breakdown of rho the shadow residual restarts as r0 = r)
       double redA[1] = {0.0};
-      for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
-        if (restart) d.r0[i] = d.r[i];
-        const double pi = first_it ? d.r[i] : d.r[i] + beta * (pold[i] + nomega * vold[i]);
-        pnew[i] = pi;
-        double acc = 0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-          const int c = col[k];
-          const double pc = first_it ? d.r[c] : d.r[c] + beta * (pold[c] + nomega * vold[c]);
-          acc = acc + val[k] * pc;
-        }
-        vnew[i] = acc;
-        redA[0] = redA[0] + acc * d.r0[i];
-      }
+      spmv_tiles<double>(
+          d.N, r_begin, r_end, rowptr, col, val, sx,
+          [&](int c) { return first_it ? d.r[c] : d.r[c] + beta * (pold[c] + nomega * vold[c]); },
+          [&](int i, double acc, double pi) {
+            if (restart) d.r0[i] = d.r[i];
+            pnew[i] = pi;
+            vnew[i] = acc;
+            redA[0] = redA[0] + acc * d.r0[i];
+          });
       restart = false;
       ok = reduce_all<1>(d, redA, lds, slot);
       if (!ok) break;
@@ -584,19 +637,16 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_bicgstab_fp64(ImpDev d) 
       const double nalpha = -(rho / redA[0]);
       // ---- phase B: s = r - alpha*v ; t = A s ; w1 = t.s ; w2 = t.t ; ss = s.s
       double redB[3] = {0.0, 0.0, 0.0};
-      for (int i = r_begin + threadIdx.x; i < r_end; i += kImpBlock) {
-        const double si = d.r[i] + nalpha * vnew[i];
-        d.s[i] = si;
-        double acc = 0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-          const int c = col[k];
-          acc = acc + val[k] * (d.r[c] + nalpha * vnew[c]);
-        }
-        d.t[i] = acc;
-        redB[0] = redB[0] + acc * si;
-        redB[1] = redB[1] + acc * acc;
-        redB[2] = redB[2] + si * si;
-      }
+      spmv_tiles<double>(
+          d.N, r_begin, r_end, rowptr, col, val, sx,
+          [&](int c) { return d.r[c] + nalpha * vnew[c]; },
+          [&](int i, double acc, double si) {
+            d.s[i] = si;
+            d.t[i] = acc;
+            redB[0] = redB[0] + acc * si;
+            redB[1] = redB[1] + acc * acc;
+            redB[2] = redB[2] + si * si;
+          });
       ok = reduce_all<3>(d, redB, lds, slot);
       if (!ok) break;
       f_w1 = redB[0];
@@ -796,6 +846,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev
   __shared__ double h2sh[kGmresMaxRestart + 1];
   __shared__ double ysh[kGmresMaxRestart];
   __shared__ double fsh[2];
+  __shared__ double sx[kSpmvTile + 2];
+  __shared__ T sxt[kSpmvTile + 2];
   d.vb = (int)blockIdx.x;
   const int r_begin = blockIdx.x * d.rpb;
   const int r_end = min(d.N, r_begin + d.rpb);
@@ -824,7 +876,7 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev
       if (c & kCodeHH) {
         const int hh = (int)(c & kCodeIdx);
         double W, K;
-        imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+        imp_gate<T>(d.hh, hh, v, d.dt_ms, W, K);
         double diag;
         if (imp_update_row(d, hh, i, W, diag) && F32)
           for (int k = rowptr[i]; k < rowptr[i + 1]; k++) g.vals[k] = (T)(val[k] / diag);
@@ -846,22 +898,23 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev
       // ---- cycle start: r0 = b - A x, beta = |r0| (FP64 in both forms; FP32
       // cycles work on the Jacobi-scaled system, r0 = D^-1 (b - A x))
       double red[2] = {0.0, 0.0};
-      for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
-        double ax = 0, dg = 1.0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-          ax = ax + val[k] * d.x[col[k]];
-          if (F32 && col[k] == i) dg = val[k];
-        }
-        double bi = d.b[i];
-        double ri = bi - ax;
-        if (F32) {
-          ri = ri / dg;
-          bi = bi / dg;
-        }
-        d.r0[i] = ri;
-        red[0] = red[0] + ri * ri;
-        red[1] = red[1] + bi * bi;
-      }
+      spmv_tiles<double>(
+          d.N, r_begin, r_end, rowptr, col, val, sx, [&](int c) { return d.x[c]; },
+          [&](int i, double ax, double) {
+            double dg = 1.0;
+            if (F32)
+              for (int k = rowptr[i]; k < rowptr[i + 1]; k++)
+                if (col[k] == i) dg = val[k];
+            double bi = d.b[i];
+            double ri = bi - ax;
+            if (F32) {
+              ri = ri / dg;
+              bi = bi / dg;
+            }
+            d.r0[i] = ri;
+            red[0] = red[0] + ri * ri;
+            red[1] = red[1] + bi * bi;
+          });
       ok = reduce_all<2>(d, red, lds, slot);
       if (!ok) break;
       rr = red[0];
@@ -899,12 +952,10 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev
         auto vk = [&](int c) -> T { return k == 0 ? (T)(d.r0[c] / div) : prev[c] / divT; };
         it++;
         // v_k (own rows) and w = A v_k
-        for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
-          g.V[(size_t)k * N + i] = vk(i);
-          T acc = 0;
-          for (int q = rowptr[i]; q < rowptr[i + 1]; q++) acc = acc + vals[q] * vk(col[q]);
+        spmv_tiles<T>(d.N, r_begin, r_end, rowptr, col, vals, sxt, vk, [&](int i, T acc, T vi) {
+          g.V[(size_t)k * N + i] = vi;
           wk[i] = acc;
-        }
+        });
         // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2; h += h2
         gmres_dots<T>(d, g, k + 1, wk, r_begin, r_end, hsh, lds, gslot);
         for (int i = r_begin + threadIdx.x; i < r_end; i += bs) {
@@ -1366,7 +1417,7 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
     const int si = d.hh_slot[hh];
     const double v = xv[si];
     double W, K, diag;
-    imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+    imp_gate<T>(d.hh, hh, v, d.dt_ms, W, K);
     if (imp_update_row(d, hh, i, W, diag)) t.w.d[si] = (T)diag;
     double bi = d.cmdx[i] * v / dT + K * d.dxv[i];
     if (active) bi = bi + d.input[i];
@@ -1871,7 +1922,7 @@ __device__ __forceinline__ void tree_update_local(const ImpDev& d, const TreeArg
     const int si = d.hh_slot[hh];
     const double v = xv[si];
     double W, K, diag;
-    imp_gate(d.hh, hh, v, d.dt_ms, W, K);
+    imp_gate<T>(d.hh, hh, v, d.dt_ms, W, K);
     if (imp_update_row(d, hh, i, W, diag)) t.w.d[si] = (T)diag;
     double bi = d.cmdx[i] * v / dT + K * d.dxv[i];
     if (active) bi = bi + d.input[i];

@@ -107,6 +107,10 @@ def measured_traffic(kernel, workload, steps_per_launch):
 
 
 # -------------------------------------------------------------------- helpers
+def say(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def host_info():
     model = "unknown"
     try:
@@ -221,6 +225,7 @@ def run_sweep_form(nh, probs, kw, warmup, steps, per_launch):
 
 
 def sweep_leg(nh, wl, orc, a, info):
+    say(f"config 4: building {a.sweep_trees} trees")
     geoms, imp, gen_s = sweep_problems(nh, wl, a.sweep_trees)
     Ntot = sum(p.N for p in imp)
     H = sum(p.H for p in imp)
@@ -240,6 +245,9 @@ def sweep_leg(nh, wl, orc, a, info):
              "implicit_mixed": (dict(kind=nh.IMPLICIT, krylov=nh.DIRECT,
                                      precision=nh.FP32_REFINE, refine_tol=1e-10), 4)}
     for name, (kw, vb) in forms.items():
+        if name not in a.sweep_forms:
+            continue
+        say(f"config 4: {name}")
         b, ids, r = run_sweep_form(nh, imp, kw, a.warmup, a.steps, per_launch)
         snaps = {i: b.snapshot(ids[i]) for i in sample}
         done = b.info(ids[sample[0]])["steps_done"]
@@ -258,7 +266,10 @@ def sweep_leg(nh, wl, orc, a, info):
             rec["max_abs_dV_mV_vs_oracle"] = max(
                 err_mv(snaps[i], oracle_exact(orc, imp[i], done, 1, 20), imp[i]) for i in sample)
         leg[name] = rec
+    if "explicit" not in a.sweep_forms:
+        return leg
     # explicit: every tree at 0.8 x its own critical step
+    say("config 4: explicit")
     exp_ = [nh.setup(g, dT=None, dt_factor=0.8)[0] for g, _ in geoms]
     b, ids, r = run_sweep_form(nh, exp_, dict(kind=nh.EXPLICIT), a.warmup, a.steps, per_launch)
     ex_snap = {i: b.snapshot(ids[i]) for i in sample[::4]}
@@ -279,6 +290,7 @@ def sweep_leg(nh, wl, orc, a, info):
         rec["oracle_sample"] = "4 trees of the sample, smallest to largest"
     leg["explicit"] = rec
     if orc is not None and not a.no_cpu_baseline:
+        say("config 4: CPU baseline")
         th = info["threads_used"]
         jobs = [(imp[i], 1, 1, 20) for i in range(min(64, len(imp)))]
         cs, wall, bad = cpu_pool(orc, jobs, th, "gmres")
@@ -346,6 +358,7 @@ def served_leg(nh, wl, orc, a, info):
            "offered_compartment_steps": sum(p.N * n for p, n, _ in reqs),
            "oracle_sample": f"{len(sample)} requests (every {max(1, len(reqs) // 12)}th)"}
     for name, prec in (("fp64", nh.FP64), ("mixed", nh.FP32_REFINE)):
+        say(f"config 5: served {name}")
         fetched, rec = served_stream(nh, reqs, prec, sample)
         if orc is not None:
             rec["max_abs_dV_mV_vs_oracle"] = max(
@@ -404,6 +417,7 @@ def config1_leg(nh, wl, orc, a):
     leg = {"workload": "unbranched HH cable 1 cm, d 1 um, dx 10 um (1,000 compartments + 2 "
                        "ghosts), 10 nA for 0.5 ms at compartment 1, 10 ms",
            "critical_dT_s": crit}
+    say("config 1")
     for label, dT in (("dt_1us_literal", run.dT), ("dt_0.8_critical", 0.8 * crit)):
         prob, _ = nh.setup(geom, dT=dT)
         steps = run.steps(dT)
@@ -414,7 +428,9 @@ def config1_leg(nh, wl, orc, a):
                "max_abs_V_mV": float(np.max(np.abs(v))) * 1e3}
         if st["kernel_ms"] > 0:
             nb = explicit_bytes(prob.N, prob.H) * steps
-            rec["roofline"] = roofline(nb, st["kernel_ms"], "k_explicit_fp64", steps)
+            launches = st["kernel_launches"]
+            kern = "k_explicit_resident" if launches == 1 else "k_explicit_fp64"
+            rec["roofline"] = roofline(nb, st["kernel_ms"], kern, launches)
         if orc is not None:
             ex = orc.Explicit(prob)
             t0 = time.perf_counter()
@@ -436,6 +452,7 @@ def implicit_forms(nh, prob, forms, a0, a1, orc):
     out = {}
     refs = {}
     for name, (steps, opts) in forms.items():
+        say(f"N={prob.N}: {name}")
         try:
             v, wall, st, st0 = run_solver(nh, prob, nh.IMPLICIT, opts, steps, a0, a1)
         except nh.NrtError as e:
@@ -543,6 +560,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--legs", default="sweep,served,config1,config2,config3")
     ap.add_argument("--sweep-trees", type=int, default=4096)
+    ap.add_argument("--sweep-forms", default="implicit_fp64,implicit_mixed,explicit")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--config2-steps", type=int, default=1000)
     ap.add_argument("--plain-steps", type=int, default=10)
@@ -581,7 +599,7 @@ def main():
     if "sweep" in legs:
         sw = res["sweep"]
         cand = {k: sw[k] for k in ("implicit_fp64", "implicit_mixed")
-                if sw[k].get("status", 0) == 0 and
+                if k in sw and sw[k].get("status", 0) == 0 and
                 (orc is None or sw[k].get("max_abs_dV_mV_vs_oracle", 1e9) <= TOL_MV)}
         best = max(cand, key=lambda k: cand[k]["compartment_steps_per_s"])
         b = sw[best]

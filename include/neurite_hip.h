@@ -47,9 +47,10 @@ extern "C" {
 #define NRT_FP32_REFINE   1  /* FP32 solves inside FP64 iterative refinement: the
                                 tree solve (NRT_DIRECT), GMRES's Arnoldi work
                                 (SpMV, basis, compensated dots) or BiCGSTAB's
-                                preconditioner (NRT_PRECOND_TREE) in FP32; the
-                                residual b - A x, the update of V and the HH
-                                gates stay FP64                                   */
+                                preconditioner (NRT_PRECOND_TREE) in FP32, and
+                                the HH rate functions in FP32; the residual
+                                b - A x, the update of V and the gate state stay
+                                FP64                                              */
 
 #define NRT_BICGSTAB 0   /* solverIm-gpu.cpp:725-843 */
 #define NRT_GMRES    1   /* solverIm-cpu.cpp:192-272 (restarted GMRES)       */

@@ -132,3 +132,52 @@ def test_tree_matches_oracle(nrt, orc):
     ref = orc.Explicit(prob)
     ref.run(1, n, 1, 1000)
     assert max_abs_diff(v, ref.snapshot()) <= TOL_V
+
+
+@pytest.mark.parametrize("name", ["ex3_hh_y", "ex6_ct_tree", "ex8_neuron"])
+def test_resident_run_bit_identical_to_per_step(name, nrt):
+    """nrt_run on up to 8,192 elements is one launch of the LDS-resident kernel;
+    update/calculate launch the per-element kernel once per step: the two give
+    the same bits, the trace included, and a run continues where the steps left
+    off (and the reverse)."""
+    g = load_golden(name)
+    p = problem_from_golden(g)
+    a0, a1 = g["active_first"], g["active_last"]
+    n, ev, te = 700, 6, g["dist_mes_points"]
+    _say(f"explicit resident vs per-step {name} N={p.N}")
+    with nrt.Solver(p, kind=nh.EXPLICIT) as s:
+        tr = s.run(1, n, a0, a1, te, ev)
+        assert s.stats()["kernel_launches"] == 1
+        v_run = s.snapshot()
+        for i in range(n + 1, n + 51):
+            s.update(a0 <= i <= a1)
+            s.calculate()
+        v_mix = s.snapshot()
+    with nrt.Solver(p, kind=nh.EXPLICIT) as s:
+        trace = []
+        for i in range(1, n + 1):
+            s.update(a0 <= i <= a1)
+            s.calculate()
+            if i % (ev + 1) == 0:
+                trace.append(s.snapshot()[te])
+        assert np.array_equal(s.snapshot(), v_run)
+        s.run(n + 1, 50, a0, a1)
+        assert np.array_equal(s.snapshot(), v_mix)
+    assert np.array_equal(tr, np.array(trace))
+
+
+def test_large_cable_per_step_path(nrt, orc):
+    """Above 8,192 elements nrt_run launches a kernel a step: a 12,000-compartment
+    HH cable against the oracle."""
+    geom, run = wl.squid_axon(n_comp=12000, length=1.2e-1)
+    prob, crit = nrt.setup(geom)
+    dT = 0.8 * crit
+    prob = prob.with_dt(dT)
+    steps = 400
+    with nrt.Solver(prob, kind=nh.EXPLICIT) as s:
+        s.run(1, steps, 1, 300)
+        assert s.stats()["kernel_launches"] == steps
+        v = s.snapshot()
+    ref = orc.Explicit(prob)
+    ref.run(1, steps, 1, 300)
+    assert max_abs_diff(v, ref.snapshot()) <= TOL_V
