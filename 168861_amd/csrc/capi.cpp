@@ -9,7 +9,9 @@
 
 namespace nrt {
 int setup_on_device(const nrt_geometry* g, int device, nrt_setup_out* out);
-
+int setup_noalloc_on_device(const nrt_geometry* g, int device, void* stream,
+                            const nrt_setup_out* out, int32_t hh_capacity, double* dT_crit,
+                            int32_t* num_hh);
 
 [[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line) {
   (void)hipGetLastError();
@@ -240,6 +242,15 @@ int nrt_setup(const nrt_geometry* g, int device, nrt_setup_out* out) {
   return guarded(nullptr, [&] {
     NRT_REQUIRE(g && out, NRT_ERR_INVALID, "NULL argument");
     return setup_on_device(g, pick_device(device), out);
+  });
+}
+
+int nrt_setup_noalloc(const nrt_geometry* g, int device, void* stream,
+                      const nrt_setup_out* out, int32_t hh_capacity, double* dT_crit_out,
+                      int32_t* num_hh_out) {
+  return guarded(nullptr, [&] {
+    return setup_noalloc_on_device(g, pick_device(device), stream, out, hh_capacity, dT_crit_out,
+                                   num_hh_out);
   });
 }
 

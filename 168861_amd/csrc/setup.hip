@@ -13,7 +13,9 @@
 //   critical dT per element, min over 1..N-2          init.cpp:138-174
 // The constant pi is the reference's 3.14159265 (neurite.cpp:72).
 #include <cstring>
+#include <mutex>
 #include "nrt_internal.hpp"
+#include "nrt_device.hpp"
 
 namespace nrt {
 
@@ -52,9 +54,10 @@ struct SetupDev {
   unsigned long long* crit_min_bits;
 };
 
-__global__ void k_setup(SetupDev s) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= s.N) return;
+// Derived arrays of element k; hi is its index among the HH elements (-1 when
+// passive). HH outputs are written only below hh_cap (the capacity of the HH
+// arrays; the host path sizes them exactly).
+__device__ __forceinline__ void setup_element(const SetupDev& s, int k, int hi, int hh_cap) {
   const nrt_material mt = s.mats[s.material[k]];
   const double eps = s.epsilon[k];
   const double eps_s = sqrt(1 + eps) - 1;
@@ -68,7 +71,6 @@ __global__ void k_setup(SetupDev s) {
   double cm = (mt.per_m * kPi * d) / (mt.thickness);
   double crit;
   if (mt.kind == 1) {
-    const int hi = s.hh_index[k];
     const double rest = mt.rest_pot;
     const double ENa0 = mt.dE_Na + rest, EK0 = mt.dE_K + rest;
     double ENa, EK, lsNa, lsK;
@@ -101,18 +103,20 @@ __global__ void k_setup(SetupDev s) {
     const double m = am / (am + bm), h = ah / (ah + bh), n = an / (an + bn);
     const double vlost =
         (gNa * m * m * m * h * (rest - ENa) + gK * n * n * n * n * (rest - EK) + gL * rest) / gL;
-    s.m[hi] = m;
-    s.h[hi] = h;
-    s.n[hi] = n;
-    s.E_Na[hi] = ENa;
-    s.E_K[hi] = EK;
-    s.E_L[hi] = vlost;
-    s.g_Na[hi] = gNa;
-    s.g_K[hi] = gK;
-    s.G_L[hi] = gL;
-    s.ls_Na[hi] = lsNa;
-    s.ls_K[hi] = lsK;
-    s.hh_rest[hi] = rest;
+    if (hi >= 0 && hi < hh_cap) {
+      s.m[hi] = m;
+      s.h[hi] = h;
+      s.n[hi] = n;
+      s.E_Na[hi] = ENa;
+      s.E_K[hi] = EK;
+      s.E_L[hi] = vlost;
+      s.g_Na[hi] = gNa;
+      s.g_K[hi] = gK;
+      s.G_L[hi] = gL;
+      s.ls_Na[hi] = lsNa;
+      s.ls_K[hi] = lsK;
+      s.hh_rest[hi] = rest;
+    }
     s.W[k] = 0;
     s.K[k] = 0;
     crit = (2 * ra * cm * dx * dx) / 4;
@@ -141,6 +145,12 @@ __global__ void k_setup(SetupDev s) {
     unsigned long long bits = (unsigned long long)__double_as_longlong(crit);
     if (crit > 0) atomicMin(s.crit_min_bits, bits);
   }
+}
+
+__global__ void k_setup(SetupDev s) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= s.N) return;
+  setup_element(s, k, s.hh_index[k], 0x7fffffff);
 }
 
 int setup_on_device(const nrt_geometry* g, int device, nrt_setup_out* out) {
@@ -218,6 +228,179 @@ int setup_on_device(const nrt_geometry* g, int device, nrt_setup_out* out) {
   std::memcpy(&cmin_v, &bits, sizeof(double));
   out->dT_crit = (bits == ~0ull) ? 0.0 : cmin_v;
   out->num_hh = H;
+  return NRT_OK;
+}
+
+// ------------------------------------------------ setup without allocation
+// nrt_setup_noalloc: the same per-element setup over arrays the caller already
+// holds on the device, enqueued on the caller's stream, allocating nothing. What
+// nrt_setup computes on the host (each element's index among the HH elements) is
+// a device scan here: per-block counts of HH elements, an exclusive scan of those
+// counts, then each element's rank within its block. The block counts, the
+// critical-step minimum and a bad-material flag live in `scratch`, one fixed
+// array per device: calls that overlap on one device race on it.
+constexpr int kSetupBlock = 256;
+constexpr int kSetupMaxBlocks = 1 << 18;  // 67,108,864 elements at most
+constexpr size_t kSetupScratchHead = 256;   // min bits (8 B), bad-material flag (4 B)
+constexpr size_t kSetupScratchBytes = kSetupScratchHead + sizeof(int32_t) * kSetupMaxBlocks;
+static __device__ __align__(256) unsigned char scratch[kSetupScratchBytes];
+
+struct SetupScratch {
+  unsigned long long* cmin;
+  unsigned* bad;
+  int32_t* sums;
+};
+
+// Clear what a call reads before writing it (a kernel rather than a memset, so the
+// sequence can be captured in a graph).
+__global__ void k_setup_init(SetupScratch w) {
+  if (threadIdx.x == 0) {
+    *w.cmin = ~0ull;
+    *w.bad = 0u;
+  }
+}
+
+__device__ __forceinline__ int setup_is_hh(const SetupDev& s, int nmat, int k, unsigned* bad) {
+  const int mi = s.material[k];
+  if (mi < 0 || mi >= nmat) {
+    atomicOr(bad, 1u);
+    return -1;
+  }
+  return s.mats[mi].kind == 1 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kSetupBlock) void k_setup_count(SetupDev s, int nmat,
+                                                             SetupScratch w) {
+  __shared__ int wsum[kSetupBlock / kWave];
+  const int k = blockIdx.x * kSetupBlock + threadIdx.x;
+  const int hh = k < s.N ? setup_is_hh(s, nmat, k, w.bad) : 0;
+  const unsigned long long b = __ballot(hh == 1);
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int i = 0; i < kSetupBlock / kWave; i++) t += wsum[i];
+    w.sums[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of the nb block counts in place (one workgroup), the total to
+// *num_hh (-1 when an element named a material out of range).
+constexpr int kSetupScan = 1024;
+__global__ __launch_bounds__(kSetupScan) void k_setup_scan(int nb, SetupScratch w,
+                                                           int32_t* num_hh) {
+  __shared__ int sh[kSetupScan / kWave + 1];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  int carry = 0;
+  for (int base = 0; base < nb; base += kSetupScan) {
+    const int i = base + threadIdx.x;
+    const int v = i < nb ? w.sums[i] : 0;
+    int x = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) sh[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int q = 0; q < kSetupScan / kWave; q++) {
+        const int t = sh[q];
+        sh[q] = acc;
+        acc += t;
+      }
+      sh[kSetupScan / kWave] = acc;
+    }
+    __syncthreads();
+    if (i < nb) w.sums[i] = carry + sh[wid] + x - v;
+    carry += sh[kSetupScan / kWave];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *num_hh = *w.bad ? -1 : carry;
+}
+
+__global__ __launch_bounds__(kSetupBlock) void k_setup_scanned(SetupDev s, int nmat,
+                                                               SetupScratch w, int hh_cap,
+                                                               int32_t* hh_elem) {
+  __shared__ int woff[kSetupBlock / kWave];
+  const int k = blockIdx.x * kSetupBlock + threadIdx.x;
+  const int hh = k < s.N ? setup_is_hh(s, nmat, k, w.bad) : 0;
+  const unsigned long long b = __ballot(hh == 1);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) woff[wid] = __popcll(b);
+  __syncthreads();
+  int off = w.sums[blockIdx.x];
+  for (int i = 0; i < wid; i++) off += woff[i];
+  if (k >= s.N || hh < 0) return;
+  const int hi = hh == 1 ? off + __popcll(b & ((1ull << lane) - 1ull)) : -1;
+  setup_element(s, k, hi, hh_cap);
+  if (hi >= 0 && hi < hh_cap) hh_elem[hi] = k;
+}
+
+__global__ void k_setup_finish(SetupScratch w, double* dT_crit) {
+  if (threadIdx.x == 0) {
+    const unsigned long long bits = *w.cmin;
+    *dT_crit = bits == ~0ull ? 0.0 : __longlong_as_double((long long)bits);
+  }
+}
+
+// The device address of `scratch` on each device, taken once per device with that
+// device current (the array's name is only a host-side stand-in).
+static std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+static SetupScratch scratch_on(int device) {
+  NRT_REQUIRE(device >= 0 && device < kMaxDevices, NRT_ERR_INVALID, "device index too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    NRT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NRT_REQUIRE(p != nullptr, NRT_ERR_HIP, "no device address for the setup scratch");
+    g_scratch_ptr[device] = (unsigned char*)p;
+  }
+  unsigned char* b = g_scratch_ptr[device];
+  return SetupScratch{(unsigned long long*)b, (unsigned*)(b + 8),
+                      (int32_t*)(b + kSetupScratchHead)};
+}
+
+int setup_noalloc_on_device(const nrt_geometry* g, int device, void* stream,
+                            const nrt_setup_out* out, int32_t hh_capacity, double* dT_crit,
+                            int32_t* num_hh) {
+  NRT_REQUIRE(g && out && dT_crit && num_hh, NRT_ERR_INVALID, "NULL argument");
+  const int64_t N = g->num_elem;
+  NRT_REQUIRE(N >= 3, NRT_ERR_INVALID, "geometry needs at least 3 elements");
+  NRT_REQUIRE(N <= (int64_t)kSetupMaxBlocks * kSetupBlock, NRT_ERR_INVALID,
+              "nrt_setup_noalloc takes at most 67,108,864 elements (its fixed scratch holds "
+              "one count per 256 elements); use nrt_setup");
+  NRT_REQUIRE(g->diameter && g->dx0 && g->epsilon && g->material && g->materials &&
+                  g->num_materials >= 1,
+              NRT_ERR_INVALID, "geometry arrays missing");
+  NRT_REQUIRE(out->dx && out->c_m && out->r_a && out->r_m && out->W && out->K && out->crit_dT &&
+                  out->rest,
+              NRT_ERR_INVALID, "per-element output arrays missing");
+  NRT_REQUIRE(hh_capacity >= 0, NRT_ERR_INVALID, "hh_capacity < 0");
+  NRT_REQUIRE(hh_capacity == 0 ||
+                  (out->hh_elem && out->hh_m && out->hh_h && out->hh_n && out->E_Na &&
+                   out->E_K && out->E_L && out->g_Na && out->g_K && out->G_L && out->ls_Na &&
+                   out->ls_K && out->hh_rest),
+              NRT_ERR_INVALID, "HH output arrays missing");
+  NRT_HIP(hipSetDevice(device));
+  const SetupScratch w = scratch_on(device);
+  hipStream_t st = (hipStream_t)stream;
+  SetupDev s{(int)N,      g->diameter, g->dx0,      g->epsilon, g->material, g->materials,
+             nullptr,     g->myelinate, g->E_MAX,   out->dx,    out->c_m,    out->r_a,
+             out->r_m,    out->W,       out->K,     out->crit_dT, out->rest, out->hh_m,
+             out->hh_h,   out->hh_n,    out->E_Na,  out->E_K,   out->E_L,    out->g_Na,
+             out->g_K,    out->G_L,     out->ls_Na, out->ls_K,  out->hh_rest, w.cmin};
+  const int nb = grid_for(N, kSetupBlock);
+  hipLaunchKernelGGL(k_setup_init, dim3(1), dim3(64), 0, st, w);
+  hipLaunchKernelGGL(k_setup_count, dim3(nb), dim3(kSetupBlock), 0, st, s, g->num_materials, w);
+  hipLaunchKernelGGL(k_setup_scan, dim3(1), dim3(kSetupScan), 0, st, nb, w, num_hh);
+  hipLaunchKernelGGL(k_setup_scanned, dim3(nb), dim3(kSetupBlock), 0, st, s, g->num_materials, w,
+                     (int)hh_capacity, out->hh_elem);
+  hipLaunchKernelGGL(k_setup_finish, dim3(1), dim3(64), 0, st, w, dT_crit);
+  NRT_HIP(hipGetLastError());
   return NRT_OK;
 }
 

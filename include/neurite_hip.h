@@ -243,6 +243,27 @@ void nrt_default_options(nrt_options* opt);
 /* Setup chain on the device: geometry -> derived arrays (init.cpp:138-298). */
 int nrt_setup(const nrt_geometry* g, int device, nrt_setup_out* out);
 
+/*
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nrt_setup over arrays the caller already holds on the
+ * device, enqueued on `stream` (a hipStream_t; NULL: the default stream) without
+ * waiting, allocating nothing. Every array pointer of `g` (diameter, dx0,
+ * epsilon, material, materials: num_materials nrt_material records) and of
+ * `out` is a device pointer; the topology fields of `g` are not read. The
+ * per-element outputs take num_elem values; the HH outputs and hh_elem take the
+ * first hh_capacity HH elements. On completion *num_hh_out (device) holds the
+ * number of HH elements (-1 if an element names a material out of range; then
+ * the outputs are incomplete) and *dT_crit_out (device) the critical step, as
+ * nrt_setup reports them; with *num_hh_out > hh_capacity the HH outputs hold
+ * only the first hh_capacity. At most 67,108,864 elements: the kernels keep their
+ * per-block counts in a fixed per-device scratch array of 1 MiB + 256 B, held
+ * until the process ends. Calls that overlap on one device race on that array.
+ */
+int nrt_setup_noalloc(const nrt_geometry* g, int device, void* stream,
+                      const nrt_setup_out* out, int32_t hh_capacity, double* dT_crit_out,
+                      int32_t* num_hh_out);
+
 /* Solver lifecycle: the Solver interface of inc/solver.h. */
 int nrt_create(const nrt_problem* p, const nrt_options* opt, nrt_handle** out);
 int nrt_update(nrt_handle* h, int input_active);      /* Solver::update    */
