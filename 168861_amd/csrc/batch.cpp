@@ -4,6 +4,7 @@
 // and advances all slots with one launch per call (implicit, persistent over the
 // k steps) or one graph of k steps (explicit).
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -36,7 +37,14 @@ int64_t explicit_device_bytes(const Explicit* e);
 int explicit_batch_block_elems();
 
 namespace {
-constexpr int kTuneThreads[] = {256, 512, 1024};
+// Workgroup sizes the autotuner tries, in this order, for each size class of slots.
+constexpr int kTuneThreads[] = {256, 512, 1024, 128};
+constexpr int kTuneCount = (int)(sizeof(kTuneThreads) / sizeof(int));
+// Size classes (rows): each class runs as a launch of its own with its own
+// workgroup size, so small trees and large ones each get the size that serves them
+// best (rows per worker differ by an order of magnitude across a sweep).
+constexpr int kClasses = 3;
+inline int size_class(int N) { return N < 4096 ? 0 : (N < 16384 ? 1 : 2); }
 // workers assumed by a slot's chunk choice: the size the workgroup autotuner
 // picks for the FP64 sweep (4,096 trees: planned for 512, 2.16 ms per step; for
 // 256, 2.06; with level-0 chunks of at most 16 rows, 2.00)
@@ -102,12 +110,35 @@ struct nrt_batch {
   bool pending = false;  // a launch is in flight (finish() accounts for it)
   int64_t pend_cs = 0, pend_ss = 0, pend_launches = 0;
   int pend_threads = 0;
-  int threads = 0;                        // implicit workgroup size (0: tuning)
-  std::vector<std::pair<int, double>> tune;  // (threads, ms per compartment-step)
+  // implicit: per size class, the workgroup size (0: still tuning), the tuning
+  // record (threads, ms per compartment-step), the launch in flight and its events
+  int cthreads[kClasses] = {0, 0, 0};
+  std::vector<std::pair<int, double>> ctune[kClasses];
+  // implicit, set by nrt_batch_tune: chunk lengths new slots of a class are planned
+  // with (0: the options' value / auto rule)
+  int cchunk[kClasses] = {0, 0, 0}, cchunk_hi[kClasses] = {0, 0, 0};
+  struct ClassLaunch {
+    bool on;
+    int th;
+    int64_t cs;
+  } pend_cls[kClasses] = {};
+  hipEvent_t cev[kClasses][2] = {};
+  hipStream_t side[kClasses] = {};  // streams of the class launches besides `stream`
   nrt_batch_stats stats{};
   std::string last_error;
 
   bool is_implicit() const { return opt.kind == NRT_IMPLICIT; }
+
+  // Options a new slot of N rows is planned with.
+  nrt_options slot_options(int N) const {
+    nrt_options o = opt;
+    const int c = size_class(N);
+    if (is_implicit() && cchunk[c] > 0) {
+      o.chunk = cchunk[c];
+      o.chunk_hi = cchunk_hi[c];
+    }
+    return o;
+  }
 
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -164,10 +195,10 @@ struct nrt_batch {
     dirty = false;
   }
 
-  int pick_threads() {
+  int pick_threads(int c) {
     if (opt.batch_threads > 0) return opt.batch_threads;
-    if (threads > 0) return threads;
-    return kTuneThreads[tune.size()];
+    if (cthreads[c] > 0) return cthreads[c];
+    return kTuneThreads[ctune[c].size()];
   }
 
   // Enqueues steps of every active slot and returns: k each (lockstep), or, with
@@ -182,12 +213,13 @@ struct nrt_batch {
     if (dirty) rebuild();
     g += k;
     if (order.empty() || k == 0) return;
-    int th = 0;
+    int th[kClasses] = {0, 0, 0};
+    std::vector<int2> seg;  // wgmap range of each class
     if (is_implicit()) {
-      th = pick_threads();
-      plan_workgroups(th);
+      for (int c = 0; c < kClasses; c++) th[c] = pick_threads(c);
+      seg = plan_workgroups(th);
     }
-    int64_t cs = 0, ss = 0;
+    int64_t cs = 0, ss = 0, ccs[kClasses] = {0, 0, 0};
     const size_t wb = implicit_slot_work_bytes();
     for (size_t i = 0; i < order.size(); i++) {
       Slot& s = *slots[order[i]];
@@ -201,17 +233,42 @@ struct nrt_batch {
       s.done += adv;
       ss += adv;
       cs += adv * s.N;
+      ccs[size_class(s.N)] += adv * s.N;
     }
     if (is_implicit()) {
       NRT_HIP(hipMemcpyAsync(work.p, work_h.data(), work_h.size(), hipMemcpyHostToDevice,
                              stream));
       wgmap.upload(wgmap_h, stream);
+      // One launch per size class present, running concurrently: the largest class
+      // on the batch stream, the others on side streams forked from it and joined
+      // back, so that no launch waits for another's tail. A slot spanning several
+      // workgroups still finds them all resident: its launch alone fits the
+      // co-resident limit, and the other launches' workgroups wait on nothing.
+      pend_launches = 0;
+      int64_t best = -1;
       NRT_HIP(hipEventRecord(ev0, stream));
-      implicit_batch_launch(opt.precision, th, table.p, work.p, wgmap.p, (int)wgmap_h.size(),
-                            stream);
+      hipStream_t on = nullptr;
+      for (int c = kClasses - 1; c >= 0; c--) {
+        pend_cls[c] = ClassLaunch{false, th[c], ccs[c]};
+        const int n = seg[c].y - seg[c].x;
+        if (n <= 0) continue;
+        const hipStream_t cs_ = on == nullptr ? stream : side[c];
+        if (cs_ != stream) NRT_HIP(hipStreamWaitEvent(cs_, ev0, 0));
+        on = cs_;
+        NRT_HIP(hipEventRecord(cev[c][0], cs_));
+        implicit_batch_launch(opt.precision, th[c], table.p, work.p, wgmap.p + seg[c].x, n,
+                              cs_);
+        NRT_HIP(hipEventRecord(cev[c][1], cs_));
+        pend_cls[c].on = true;
+        pend_launches++;
+        if (ccs[c] > best) {
+          best = ccs[c];
+          pend_threads = th[c];
+        }
+      }
+      for (int c = 0; c < kClasses; c++)
+        if (pend_cls[c].on) NRT_HIP(hipStreamWaitEvent(stream, cev[c][1], 0));
       NRT_HIP(hipEventRecord(ev1, stream));
-      pend_threads = th;
-      pend_launches = 1;
     } else {
       if (!gexec || graph_k != k) {
         drop_graph();
@@ -236,30 +293,47 @@ struct nrt_batch {
   // at most opt.num_blocks when set), and only while every workgroup of the launch
   // can be resident at once -- otherwise the largest grids are halved until they
   // fit, down to one workgroup per slot, which needs no co-residency.
-  void plan_workgroups(int th) {
-    auto it = capacity.find(th);
-    if (it == capacity.end()) it = capacity.emplace(th, implicit_batch_capacity(opt.precision, th)).first;
-    const int cap = it->second;
+  // Each size class is its own launch, so the co-residency limit applies per class
+  // (with that class's workgroup size). Returns the wgmap range [x, y) of each class;
+  // the table is in decreasing size, so a class is a contiguous run of entries, and
+  // wgmap_h holds the classes largest first.
+  std::vector<int2> plan_workgroups(const int (&th)[kClasses]) {
     const int vmax = opt.num_blocks > 0 ? std::min(opt.num_blocks, kMaxSlotBlocks) : kMaxSlotBlocks;
-    long long total = 0;
+    std::vector<int> first(kClasses, (int)order.size()), last(kClasses, 0);
     for (size_t i = 0; i < order.size(); i++) {
       const int N = slots[order[i]]->N;
+      const int c = size_class(N);
+      first[c] = std::min(first[c], (int)i);
+      last[c] = std::max(last[c], (int)i + 1);
       vn[i] = std::max(1, std::min(vmax, (N + kRowsPerWorkgroup - 1) / kRowsPerWorkgroup));
-      total += vn[i];
     }
-    while (total > cap) {
-      size_t big = 0;
-      for (size_t i = 1; i < vn.size(); i++)
-        if (vn[i] > vn[big]) big = i;
-      if (vn[big] == 1) break;  // one workgroup per slot: no co-residency needed
-      const int half = vn[big] / 2;
-      total -= vn[big] - half;
-      vn[big] = half;
-    }
-    if (total > cap) std::fill(vn.begin(), vn.end(), 1);
+    std::vector<int2> seg(kClasses, make_int2(0, 0));
     wgmap_h.clear();
-    for (size_t i = 0; i < order.size(); i++)
-      for (int b = 0; b < vn[i]; b++) wgmap_h.push_back(make_int2((int)i, b));
+    for (int c = kClasses - 1; c >= 0; c--) {
+      seg[c] = make_int2((int)wgmap_h.size(), (int)wgmap_h.size());
+      if (first[c] >= last[c]) continue;
+      auto it = capacity.find(th[c]);
+      if (it == capacity.end())
+        it = capacity.emplace(th[c], implicit_batch_capacity(opt.precision, th[c])).first;
+      const int cap = it->second;
+      long long total = 0;
+      for (int i = first[c]; i < last[c]; i++) total += vn[i];
+      while (total > cap) {
+        int big = first[c];
+        for (int i = first[c] + 1; i < last[c]; i++)
+          if (vn[i] > vn[big]) big = i;
+        if (vn[big] == 1) break;  // one workgroup per slot: no co-residency needed
+        const int half = vn[big] / 2;
+        total -= vn[big] - half;
+        vn[big] = half;
+      }
+      if (total > cap)
+        for (int i = first[c]; i < last[c]; i++) vn[i] = 1;
+      for (int i = first[c]; i < last[c]; i++)
+        for (int b = 0; b < vn[i]; b++) wgmap_h.push_back(make_int2(i, b));
+      seg[c].y = (int)wgmap_h.size();
+    }
+    return seg;
   }
 
   // Waits for the launch in flight and accounts for it.
@@ -267,17 +341,27 @@ struct nrt_batch {
     if (!pending) return;
     pending = false;
     float ms = 0;
-    NRT_HIP(hipEventSynchronize(ev1));
-    NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     if (is_implicit()) {
-      if (opt.batch_threads <= 0 && threads == 0 && pend_cs > 0) {
-        tune.emplace_back(pend_threads, ms / (double)pend_cs);
-        if (tune.size() == sizeof(kTuneThreads) / sizeof(int))
-          threads = std::min_element(tune.begin(), tune.end(),
-                                     [](auto& a, auto& b) { return a.second < b.second; })
-                        ->first;
+      NRT_HIP(hipEventSynchronize(ev1));
+      NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));  // the launches together
+      for (int c = 0; c < kClasses; c++) {
+        if (!pend_cls[c].on) continue;
+        float mc = 0;  // this class's launch (overlapping the others)
+        NRT_HIP(hipEventElapsedTime(&mc, cev[c][0], cev[c][1]));
+        pend_cls[c].on = false;
+        if (opt.batch_threads <= 0 && cthreads[c] == 0 && pend_cls[c].cs > 0) {
+          ctune[c].emplace_back(pend_cls[c].th, mc / (double)pend_cls[c].cs);
+          if ((int)ctune[c].size() == kTuneCount)
+            cthreads[c] = std::min_element(ctune[c].begin(), ctune[c].end(), [](auto& a, auto& b) {
+                            return a.second < b.second;
+                          })->first;
+        }
+        stats.class_threads[c] = pend_cls[c].th;
       }
       stats.threads = pend_threads;
+    } else {
+      NRT_HIP(hipEventSynchronize(ev1));
+      NRT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     }
     stats.launches += pend_launches;
     stats.kernel_ms += ms;
@@ -324,9 +408,10 @@ int nrt_batch_create(const nrt_options* opt, nrt_batch** out) {
                 NRT_ERR_UNSUPPORTED,
                 "implicit batches use the tree solve (krylov = NRT_DIRECT) or BiCGSTAB "
                 "preconditioned by it (NRT_BICGSTAB with NRT_PRECOND_TREE)");
-    NRT_REQUIRE(b->opt.batch_threads == 0 || b->opt.batch_threads == 256 ||
-                    b->opt.batch_threads == 512 || b->opt.batch_threads == 1024,
-                NRT_ERR_INVALID, "batch_threads must be 0, 256, 512 or 1024");
+    NRT_REQUIRE(b->opt.batch_threads == 0 || b->opt.batch_threads == 128 ||
+                    b->opt.batch_threads == 256 || b->opt.batch_threads == 512 ||
+                    b->opt.batch_threads == 1024,
+                NRT_ERR_INVALID, "batch_threads must be 0, 128, 256, 512 or 1024");
     b->device = pick_device(b->opt.device);
     b->opt.device = b->device;
     NRT_HIP(hipSetDevice(b->device));
@@ -334,6 +419,11 @@ int nrt_batch_create(const nrt_options* opt, nrt_batch** out) {
     NRT_HIP(hipStreamCreateWithFlags(&b->adm, hipStreamNonBlocking));
     NRT_HIP(hipEventCreate(&b->ev0));
     NRT_HIP(hipEventCreate(&b->ev1));
+    for (int c = 0; c < kClasses; c++) {
+      for (int e = 0; e < 2; e++) NRT_HIP(hipEventCreate(&b->cev[c][e]));
+      if (b->opt.kind == NRT_IMPLICIT && c < kClasses - 1)
+        NRT_HIP(hipStreamCreateWithFlags(&b->side[c], hipStreamNonBlocking));
+    }
     return NRT_OK;
   });
   if (st != NRT_OK) {
@@ -356,7 +446,7 @@ int nrt_batch_add(nrt_batch* b, const nrt_problem* p, int64_t num_steps, int64_t
     NRT_HIP(hipSetDevice(b->device));
     batch_admission_check(b, p->num_elem, num_steps, trace_every, 0);
     // built on the admission stream (overlapping a launch in flight)
-    nrt_handle* h = handle_create(p, b->opt, b->adm, batch_plan_threads(b));
+    nrt_handle* h = handle_create(p, b->slot_options(p->num_elem), b->adm, batch_plan_threads(b));
     *slot = batch_add_handle(b, h, num_steps, a0, a1, trace_elem, trace_every);
     return NRT_OK;
   });
@@ -463,7 +553,125 @@ int nrt_batch_get_stats(nrt_batch* b, nrt_batch_stats* out) {
       if (s->left() > 0) out->active_slots++;
       out->device_bytes += s->bytes;
     }
-    if (b->is_implicit()) out->threads = b->pick_threads();
+    if (b->is_implicit()) {
+      int best = -1;
+      int64_t most = -1, per[kClasses] = {0, 0, 0};
+      for (auto& s : b->slots)
+        if (s) per[size_class(s->N)] += s->N;
+      for (int c = 0; c < kClasses; c++) {
+        out->class_threads[c] = b->pick_threads(c);
+        if (per[c] > most) {
+          most = per[c];
+          best = c;
+        }
+      }
+      out->threads = b->pick_threads(best);
+    }
+    return NRT_OK;
+  });
+}
+
+// Launch-shape tuning: trial batches of one size class each, built from the
+// caller's sample, over every (plan, workgroup size) pair; a plan under which any
+// trial slot fails is not chosen.
+int nrt_batch_tune(nrt_batch* b, const nrt_problem* const* sample, int32_t n, int64_t steps,
+                   int64_t rows_per_class, nrt_batch_tune_result* out) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    NRT_REQUIRE(out && (sample || n == 0) && n >= 0, NRT_ERR_INVALID, "NULL argument");
+    NRT_REQUIRE(steps >= 1 && rows_per_class >= 1, NRT_ERR_INVALID,
+                "steps and rows_per_class must be >= 1");
+    NRT_REQUIRE(b->is_implicit(), NRT_ERR_UNSUPPORTED,
+                "launch shapes are tuned for implicit batches only");
+    NRT_HIP(hipSetDevice(b->device));
+    b->finish();
+    *out = nrt_batch_tune_result{};
+    const auto t0 = std::chrono::steady_clock::now();
+    // each class: its problems in increasing size, every k-th one so that the
+    // sample spans the class within the row budget
+    std::vector<const nrt_problem*> cls[kClasses];
+    for (int i = 0; i < n; i++) {
+      NRT_REQUIRE(sample[i] && sample[i]->num_elem >= 4, NRT_ERR_INVALID,
+                  "sample problem " + std::to_string(i) + " is NULL or has < 4 elements");
+      cls[size_class(sample[i]->num_elem)].push_back(sample[i]);
+    }
+    struct Plan {
+      int chunk, chunk_hi;
+    };
+    std::vector<Plan> plans;
+    if (b->opt.chunk > 0) plans.push_back({b->opt.chunk, b->opt.chunk_hi});
+    else plans = {{0, 0}, {8, 8}, {12, 8}, {16, 8}, {24, 8}, {16, 5}, {16, 12}};
+    std::vector<int> ths;
+    if (b->opt.batch_threads > 0) ths.push_back(b->opt.batch_threads);
+    else ths.assign(std::begin(kTuneThreads), std::end(kTuneThreads));
+    const int64_t run_steps = 2 * steps * (int64_t)ths.size();
+    for (int c = 0; c < kClasses; c++) {
+      auto& v = cls[c];
+      if (v.empty()) continue;
+      std::stable_sort(v.begin(), v.end(), [](const nrt_problem* x, const nrt_problem* y) {
+        return x->num_elem < y->num_elem;
+      });
+      int64_t total = 0;
+      for (auto* p : v) total += p->num_elem;
+      const size_t stride = (size_t)std::max<int64_t>(1, (total + rows_per_class - 1) / rows_per_class);
+      std::vector<const nrt_problem*> pick;
+      for (size_t i = std::min(stride / 2, v.size() - 1); i < v.size(); i += stride)
+        pick.push_back(v[i]);
+      for (auto* p : pick) out->rows[c] += p->num_elem;
+      out->slots[c] = (int32_t)pick.size();
+      double best = 0;
+      for (const Plan& pl : plans) {
+        nrt_options o = b->opt;
+        o.chunk = pl.chunk;
+        o.chunk_hi = pl.chunk_hi;
+        o.batch_threads = 0;
+        nrt_batch* t = nullptr;
+        int st = nrt_batch_create(&o, &t);
+        NRT_REQUIRE(st == NRT_OK, st, "trial batch: create failed");
+        std::unique_ptr<nrt_batch, int (*)(nrt_batch*)> own(t, nrt_batch_destroy);
+        for (auto* p : pick) {
+          int32_t id = 0;
+          st = nrt_batch_add(t, p, run_steps, 1, 20, 0, -1, &id);
+          NRT_REQUIRE(st == NRT_OK, st, std::string("trial batch: ") + t->last_error);
+        }
+        std::vector<std::pair<int, double>> timed;
+        for (int th : ths) {
+          t->opt.batch_threads = th;  // the launch shape of this trial
+          t->step(steps);             // untimed
+          const double k0 = t->stats.kernel_ms;
+          const int64_t cs0 = t->stats.compartment_steps;
+          t->step(steps);
+          const int64_t cs = t->stats.compartment_steps - cs0;
+          timed.emplace_back(th, cs > 0 ? (t->stats.kernel_ms - k0) * 1e6 / (double)cs : 0.0);
+          out->trials++;
+        }
+        bool ok = true;
+        for (auto& sl : t->slots) {
+          int sst = NRT_OK;
+          int64_t inner = 0;
+          implicit_slot_stats(sl->h, &sst, &inner);
+          ok = ok && sst == NRT_OK;
+        }
+        if (!ok) continue;
+        for (auto& [th, ns] : timed) {
+          if (pl.chunk == 0 && th == 256) out->auto_ns_per_row_step[c] = ns;
+          if (best == 0 || ns < best) {
+            best = ns;
+            out->threads[c] = th;
+            out->chunk[c] = pl.chunk;
+            out->chunk_hi[c] = pl.chunk_hi;
+          }
+        }
+      }
+      NRT_REQUIRE(best > 0, NRT_ERR_NUMERIC,
+                  "every trial plan failed for size class " + std::to_string(c));
+      out->ns_per_row_step[c] = best;
+      b->cchunk[c] = out->chunk[c];
+      b->cchunk_hi[c] = out->chunk_hi[c];
+      b->cthreads[c] = out->threads[c];
+      b->ctune[c].clear();
+    }
+    out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return NRT_OK;
   });
 }
@@ -482,6 +690,11 @@ int nrt_batch_destroy(nrt_batch* b) {
   b->table.release();
   b->blk.release();
   b->gdev.release();
+  for (int c = 0; c < kClasses; c++) {
+    for (int e = 0; e < 2; e++)
+      if (b->cev[c][e]) (void)hipEventDestroy(b->cev[c][e]);
+    if (b->side[c]) (void)hipStreamDestroy(b->side[c]);
+  }
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -514,7 +727,7 @@ int batch_plan_threads(const nrt_batch* b) {
   return b->opt.batch_threads > 0 ? b->opt.batch_threads : kPlanThreads;
 }
 
-const nrt_options& batch_options(const nrt_batch* b) { return b->opt; }
+nrt_options batch_slot_options(const nrt_batch* b, int N) { return b->slot_options(N); }
 
 int batch_add_handle(nrt_batch* b, nrt_handle* h, int64_t num_steps, int64_t a0, int64_t a1,
                      int32_t trace_elem, int64_t trace_every) {

@@ -15,7 +15,7 @@ int64_t batch_footprint(const nrt_batch* b, int N, int64_t num_steps, int64_t tr
 void batch_admission_check(const nrt_batch* b, int N, int64_t num_steps, int64_t trace_every,
                            int64_t reserved);
 int batch_plan_threads(const nrt_batch* b);  // batch_threads for handle_create
-const nrt_options& batch_options(const nrt_batch* b);
+nrt_options batch_slot_options(const nrt_batch* b, int N);  // options a slot is planned with
 // Registers a handle built elsewhere (handle_create with the batch's options and
 // batch_plan_threads) as a new slot; the batch owns it from then on.
 int batch_add_handle(nrt_batch* b, nrt_handle* h, int64_t num_steps, int64_t a0, int64_t a1,

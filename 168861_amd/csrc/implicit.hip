@@ -1148,7 +1148,23 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
     }
     tsync<GRID>(d);
   }
-  if (GRID && tl != nullptr && P.top_n > 0) {
+  if (!GRID && tl != nullptr && P.top_n > 0 && P.fin_cut == 0) {
+    // block mode: the whole final forest staged in LDS, a height per pass
+    const TopLds<T> L = TopLds<T>::carve(tl, kBlockTopMax);
+    const int ns = top_stage_count(P), nh = P.heights;
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) top_stage(P, w, L, j);
+    __syncthreads();
+    for (int h = 0; h < nh; h++) {
+      for (int j = L.hoff[h] + (int)threadIdx.x; j < L.hoff[h + 1]; j += blockDim.x) top_up(L, j);
+      __syncthreads();
+    }
+    for (int h = nh - 1; h >= 0; h--) {
+      for (int j = L.hoff[h] + (int)threadIdx.x; j < L.hoff[h + 1]; j += blockDim.x) top_down(L, j);
+      __syncthreads();
+    }
+    for (int j = threadIdx.x; j < P.top_n; j += blockDim.x) top_store(P, w, L, j);
+    __syncthreads();
+  } else if (GRID && tl != nullptr && P.top_n > 0) {
     if (d.vb == 0) {
       const TopLds<T> L = TopLds<T>::carve(tl);
       const int ns = top_stage_count(P), nh = P.heights - cut;
@@ -2102,7 +2118,7 @@ __global__ __launch_bounds__(GRID ? kImpBlock : kTreeBlock) void k_implicit_tree
   __shared__ double lds[4 * 16];
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   d.vb = GRID ? (int)blockIdx.x : 0;
-  tree_steps<T, GRID>(d, t, lds, GRID && t.top_lds ? dyn_lds : nullptr);
+  tree_steps<T, GRID>(d, t, lds, t.top_lds ? dyn_lds : nullptr);
 }
 
 // Batched form: independent trees ("slots"), one workgroup each, no grid-wide
@@ -2124,37 +2140,41 @@ struct SlotWork {
   int vn, pad;
 };
 
+// The slot record (descriptor pointers and constants, uniform over the workgroup)
+// is staged once in LDS and the solve reads its fields from there: a per-lane copy
+// would live in scratch (the record is over a kilobyte), and with a dozen waves per
+// CU that scratch outgrows the caches and its reloads go to HBM.
 template <class T, int BS>
 __global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __restrict__ slots,
                                                   const SlotWork* __restrict__ work,
                                                   const int2* __restrict__ wgmap) {
   __shared__ double lds[4 * 16];
+  __shared__ __align__(16) BatchSlot<T> rec;
+  extern __shared__ __align__(16) unsigned char dyn_lds[];  // a staged final forest
   const int2 m = wgmap[blockIdx.x];  // (slot, workgroup index in its grid)
   const SlotWork wk = work[m.x];
   if (wk.steps <= 0) return;
-  // The slot records are read through the constant address space (the kernel never
-  // writes them): pointers loaded from there are known to be global, so the solve's
-  // loads and stores compile to global, not generic (flat), instructions.
-  const __attribute__((address_space(4))) BatchSlot<T>* cs =
-      (const __attribute__((address_space(4))) BatchSlot<T>*)slots;
-  ImpDev d;
-  __builtin_memcpy(&d, (const void*)&cs[m.x].d, sizeof(ImpDev));
-  const TreeArgs<T>& t = *(const TreeArgs<T>*)&cs[m.x].t;
-  double* const trace_base = cs[m.x].trace_base;
-  d.first = wk.done + 1;
-  d.num = wk.steps;
-  if (d.interval >= 0) {
-    d.gl0 = wk.done % (d.interval + 1);
-    d.trace = trace_base ? trace_base + wk.done / (d.interval + 1) : nullptr;
+  static_assert(sizeof(BatchSlot<T>) % 4 == 0, "slot record in words");
+  constexpr int kWords = (int)(sizeof(BatchSlot<T>) / 4);
+  const uint32_t* __restrict__ src = (const uint32_t*)&slots[m.x];
+  uint32_t* dst = (uint32_t*)&rec;
+  for (int i = (int)threadIdx.x; i < kWords; i += BS) dst[i] = src[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ImpDev& d = rec.d;
+    d.first = wk.done + 1;
+    d.num = wk.steps;
+    if (d.interval >= 0) {
+      d.gl0 = wk.done % (d.interval + 1);
+      d.trace = rec.trace_base ? rec.trace_base + wk.done / (d.interval + 1) : nullptr;
+    }
+    d.vb = m.y;
+    d.nblocks = wk.vn;
+    if (wk.vn > 1) d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
   }
-  d.vb = m.y;
-  d.nblocks = wk.vn;
-  if (wk.vn > 1) {
-    d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
-    tree_steps<T, true>(d, t, lds, nullptr);
-  } else {
-    tree_steps<T, false>(d, t, lds, nullptr);
-  }
+  __syncthreads();
+  if (wk.vn > 1) tree_steps<T, true>(rec.d, rec.t, lds, nullptr);
+  else tree_steps<T, false>(rec.d, rec.t, lds, rec.t.top_lds ? dyn_lds : nullptr);
 }
 
 // ---------------------------------------------------------------- host side
@@ -2454,6 +2474,8 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
                                std::min(kMaxSlotBlocks, (N + kRowsPerWorkgroup - 1) /
                                                             kRowsPerWorkgroup),
                                batch_threads);
+      // one workgroup: the final forest in LDS when it is small enough
+      if (!e->grid_mode && !e->local_mode) stage_block_final_forest(e->plan_h, kBlockTopMax);
       const TreePlanHost& P = e->plan_h;
       // one device buffer holding every plan array
       std::vector<int32_t> flat;
@@ -2660,7 +2682,9 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b, T* alt = n
                      o.refine_tol,
                      o.refine_max,
                      (e->grid_mode && e->plan_h.top_n > 0) ||
-                             (e->local_mode && e->plan_h.top_lds_ok)
+                             (e->local_mode && e->plan_h.top_lds_ok) ||
+                             (!e->grid_mode && !e->local_mode && e->plan_h.top_n > 0 &&
+                              e->plan_h.fin_cut == 0)
                          ? 1
                          : 0,
                      e->local_mode ? e->hh_goff.p : nullptr,
@@ -2723,7 +2747,8 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
       NRT_HIP(hipLaunchKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, shmem,
                                          hnd->stream));
     else
-      NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));
+      NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args,
+                              targs.top_lds ? top_lds_bytes<TT>(kBlockTopMax) : 0, hnd->stream));
   };
   if (e->method == 1 || e->method == 3)
     launch(tree_args<double>(e, o, e->tw64.p, e->tw64alt.p), (const void*)k_implicit_tree<double, true>,
@@ -2925,7 +2950,7 @@ static int batch_capacity_of() {
   NRT_HIP(hipGetDevice(&dev));
   NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_batch_tree<T, BS>, BS,
-                                                       0));
+                                                       top_lds_bytes<T>(kBlockTopMax)));
   return cus * std::max(per, 0);
 }
 
@@ -2933,10 +2958,12 @@ static int batch_capacity_of() {
 // idle device): virtual grids of several workgroups are only formed within it.
 int implicit_batch_capacity(int precision, int threads) {
   if (precision == NRT_FP64) {
+    if (threads == 128) return batch_capacity_of<double, 128>();
     if (threads == 256) return batch_capacity_of<double, 256>();
     if (threads == 512) return batch_capacity_of<double, 512>();
     return batch_capacity_of<double, 1024>();
   }
+  if (threads == 128) return batch_capacity_of<float, 128>();
   if (threads == 256) return batch_capacity_of<float, 256>();
   if (threads == 512) return batch_capacity_of<float, 512>();
   return batch_capacity_of<float, 1024>();
@@ -2947,16 +2974,21 @@ void implicit_batch_launch(int precision, int threads, const void* table, const 
   if (nwg <= 0) return;
   const dim3 grid(nwg);
   const auto* w = (const SlotWork*)work;
+  // every workgroup gets room for a staged final forest (slots whose forest fits)
   if (precision == NRT_FP64) {
     auto* t = (const BatchSlot<double>*)table;
-    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<double, 256>), grid, dim3(256), 0, s, t, w, wgmap);
-    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<double, 512>), grid, dim3(512), 0, s, t, w, wgmap);
-    else hipLaunchKernelGGL((k_batch_tree<double, 1024>), grid, dim3(1024), 0, s, t, w, wgmap);
+    const size_t sh = top_lds_bytes<double>(kBlockTopMax);
+    if (threads == 128) hipLaunchKernelGGL((k_batch_tree<double, 128>), grid, dim3(128), sh, s, t, w, wgmap);
+    else if (threads == 256) hipLaunchKernelGGL((k_batch_tree<double, 256>), grid, dim3(256), sh, s, t, w, wgmap);
+    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<double, 512>), grid, dim3(512), sh, s, t, w, wgmap);
+    else hipLaunchKernelGGL((k_batch_tree<double, 1024>), grid, dim3(1024), sh, s, t, w, wgmap);
   } else {
     auto* t = (const BatchSlot<float>*)table;
-    if (threads == 256) hipLaunchKernelGGL((k_batch_tree<float, 256>), grid, dim3(256), 0, s, t, w, wgmap);
-    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<float, 512>), grid, dim3(512), 0, s, t, w, wgmap);
-    else hipLaunchKernelGGL((k_batch_tree<float, 1024>), grid, dim3(1024), 0, s, t, w, wgmap);
+    const size_t sh = top_lds_bytes<float>(kBlockTopMax);
+    if (threads == 128) hipLaunchKernelGGL((k_batch_tree<float, 128>), grid, dim3(128), sh, s, t, w, wgmap);
+    else if (threads == 256) hipLaunchKernelGGL((k_batch_tree<float, 256>), grid, dim3(256), sh, s, t, w, wgmap);
+    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<float, 512>), grid, dim3(512), sh, s, t, w, wgmap);
+    else hipLaunchKernelGGL((k_batch_tree<float, 1024>), grid, dim3(1024), sh, s, t, w, wgmap);
   }
   NRT_HIP(hipGetLastError());
 }

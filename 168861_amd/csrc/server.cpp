@@ -244,7 +244,7 @@ struct nrt_server {
           b.h = handle_create(&r.prob->view, o, stream, 0);
         } else {
           batch_admission_check(batch, r.N(), r.num_steps, r.every, 0);
-          b.h = handle_create(&r.prob->view, batch_options(batch), stream,
+          b.h = handle_create(&r.prob->view, batch_slot_options(batch, r.N()), stream,
                               batch_plan_threads(batch));
         }
       } catch (const Error& e) {

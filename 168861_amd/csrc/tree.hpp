@@ -138,6 +138,12 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chu
 // top fits in kTopLdsNodes LDS entries cost a tenth (the lists for that are
 // built). No split when groups <= 1.
 void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage_top = false);
+// A launch of one workgroup (block mode, a batch slot) eliminates the whole final
+// forest in LDS when it has at most max_entries nodes: top lists over every height
+// (fin_cut 0); plans already split for a grid keep theirs.
+void stage_block_final_forest(TreePlanHost& P, int max_entries);
+// LDS entries a block-mode launch provides for its staged final forest.
+constexpr int kBlockTopMax = 768;
 
 // Local mode row groups: G contiguous ranges of rows of (nearly) equal size.
 std::vector<int32_t> row_groups(int N, int G);

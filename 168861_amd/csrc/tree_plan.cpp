@@ -498,6 +498,23 @@ static void stage_final_top(TreePlanHost& P, int cut) {
   }
 }
 
+void stage_block_final_forest(TreePlanHost& P, int max_entries) {
+  const int n = (int)P.fin_par.size();
+  if (P.fin_cut != 0 || P.top_n != 0 || n == 0 || n > max_entries) return;
+  std::vector<uint8_t> all(n, 1);
+  stage_top_set(P, all, max_entries);
+  if (!P.top_lds_ok || P.fin_cut != 0) {
+    P.fin_cut = 0;
+    P.top_n = P.top_ext = 0;
+    P.top_lds_ok = 0;
+    P.top_node.clear();
+    P.top_hoff.clear();
+    P.top_cptr.clear();
+    P.top_child.clear();
+    P.top_par.clear();
+  }
+}
+
 void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage_top) {
   P.fin_cut = 0;
   P.sub_groups = 0;
@@ -735,6 +752,7 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
                                      dyn_rows ? &dyn : nullptr, local ? &rgrp : nullptr);
     if (local) check_local_plan(H);
     else partition_final_forest(H, groups, 256, groups > 0);
+    if (groups == 0) stage_block_final_forest(H, kBlockTopMax);  // as a block-mode launch
     if (!local) {
       // the split's promise to the device: a subtree node's children sit in its own
       // group at lower heights, and every node is placed exactly once

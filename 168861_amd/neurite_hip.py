@@ -108,11 +108,19 @@ class SlotInfo(C.Structure):
                 ("trace_samples", C.c_int64), ("nnz", C.c_int64)]
 
 
+class BatchTuneResult(C.Structure):
+    _fields_ = [("threads", C.c_int32 * 3), ("chunk", C.c_int32 * 3),
+                ("chunk_hi", C.c_int32 * 3), ("slots", C.c_int32 * 3), ("rows", C.c_int64 * 3),
+                ("ns_per_row_step", C.c_double * 3), ("auto_ns_per_row_step", C.c_double * 3),
+                ("trials", C.c_int32), ("reserved", C.c_int32), ("seconds", C.c_double)]
+
+
 class BatchStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("slot_steps", C.c_int64),
                 ("compartment_steps", C.c_int64), ("kernel_ms", C.c_double),
                 ("active_slots", C.c_int32), ("threads", C.c_int32),
-                ("device_bytes", C.c_int64)]
+                ("device_bytes", C.c_int64), ("class_threads", C.c_int32 * 3),
+                ("reserved", C.c_int32)]
 
 
 class ServerConfig(C.Structure):
@@ -178,6 +186,8 @@ def lib():
         L.nrt_batch_slot_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SlotInfo)]
         L.nrt_batch_remove.argtypes = [C.c_void_p, C.c_int32]
         L.nrt_batch_get_stats.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
+        L.nrt_batch_tune.argtypes = [C.c_void_p, C.POINTER(C.POINTER(ProblemStruct)), C.c_int32,
+                                     C.c_int64, C.c_int64, C.POINTER(BatchTuneResult)]
         L.nrt_batch_last_error.argtypes = [C.c_void_p]
         L.nrt_batch_last_error.restype = C.c_char_p
         L.nrt_batch_destroy.argtypes = [C.c_void_p]
@@ -387,6 +397,28 @@ class Batch:
     def step(self, k):
         self._c(lib().nrt_batch_step(self.b, k))
 
+    def tune(self, problems, steps=10, rows_per_class=4_000_000):
+        """Launch shapes per size class (nrt_batch_tune) from a sample of problems:
+        returns, per class (rows < 4096, < 16384, the rest), the chosen workgroup
+        size and chunk lengths with their kernel time per compartment-step."""
+        structs = [p.struct() for p in problems]
+        arr = (C.POINTER(ProblemStruct) * max(1, len(structs)))(
+            *[C.pointer(x) for x in structs])
+        r = BatchTuneResult()
+        self._c(lib().nrt_batch_tune(self.b, arr, len(structs), steps, rows_per_class,
+                                     C.byref(r)))
+        names = ("N<4096", "4096<=N<16384", "N>=16384")
+        out = {"trials": r.trials, "seconds": r.seconds, "classes": {}}
+        for c, name in enumerate(names):
+            if r.slots[c] == 0:
+                continue
+            out["classes"][name] = {
+                "threads": r.threads[c], "chunk": r.chunk[c] or "auto",
+                "chunk_hi": r.chunk_hi[c] or "auto", "sample_trees": r.slots[c],
+                "sample_rows": r.rows[c], "ns_per_compartment_step": r.ns_per_row_step[c],
+                "auto_256_ns_per_compartment_step": r.auto_ns_per_row_step[c]}
+        return out
+
     def snapshot(self, slot):
         out = np.empty(self._keep[slot])
         self._c(lib().nrt_batch_snapshot(self.b, slot, out.ctypes.data_as(_dp)))
@@ -412,7 +444,9 @@ class Batch:
     def stats(self):
         s = BatchStats()
         self._c(lib().nrt_batch_get_stats(self.b, C.byref(s)))
-        return {f: getattr(s, f) for f, _ in BatchStats._fields_}
+        out = {f: getattr(s, f) for f, _ in BatchStats._fields_ if f != "reserved"}
+        out["class_threads"] = list(s.class_threads)
+        return out
 
     def close(self):
         if getattr(self, "b", None):

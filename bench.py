@@ -193,11 +193,15 @@ def sweep_problems(nh, wl, count):
     return geoms, imp, gen_s
 
 
-def run_sweep_form(nh, probs, kw, warmup, steps, per_launch):
+def run_sweep_form(nh, probs, kw, warmup, steps, per_launch, tune_rows=0):
     """Adds every problem as a slot for warmup + steps steps; untimed warm-up
-    launches (the first three also pick the workgroup size), then the timed steps."""
-    warmup = max(warmup, 3)  # the workgroup-size autotuner times the first three launches
+    launches, then the timed steps. Implicit: the launch shapes (chunk lengths and
+    workgroup size per size class) come from nrt_batch_tune on a sample of the same
+    problems (tune_rows compartments per class) before the slots are added; without
+    it, the first four launches pick each class's workgroup size."""
+    warmup = max(warmup, 4)
     b = nh.Batch(**kw)
+    tuned = b.tune(probs, steps=per_launch, rows_per_class=tune_rows) if tune_rows > 0 else None
     t0 = time.perf_counter()
     ids = [b.add(p, warmup + steps, 1, 20) for p in probs]
     add_s = time.perf_counter() - t0
@@ -215,7 +219,9 @@ def run_sweep_form(nh, probs, kw, warmup, steps, per_launch):
            "launches": len(split_launches(steps, per_launch)),
            "compartment_steps": st1["compartment_steps"] - st0["compartment_steps"],
            "add_s": add_s, "device_bytes": st1["device_bytes"],
-           "threads": st1["threads"] if implicit else 256}
+           "threads": st1["threads"] if implicit else 256,
+           "class_threads": list(st1["class_threads"]) if implicit else None,
+           "tune": tuned}
     if implicit:
         infos = [b.info(i) for i in ids]
         out["solves"] = [inf["inner_iters"] - s0 for inf, s0 in zip(infos, inner0)]
@@ -248,17 +254,23 @@ def sweep_leg(nh, wl, orc, a, info):
         if name not in a.sweep_forms:
             continue
         say(f"config 4: {name}")
-        b, ids, r = run_sweep_form(nh, imp, kw, a.warmup, a.steps, per_launch)
+        b, ids, r = run_sweep_form(nh, imp, kw, a.warmup, a.steps, per_launch, a.tune_rows)
         snaps = {i: b.snapshot(ids[i]) for i in sample}
         done = b.info(ids[sample[0]])["steps_done"]
         b.close()
         nb = sum(tree_bytes(p.N, p.H, nz, a.steps, sv, vb)
                  for p, nz, sv in zip(imp, r["nnz"], r["solves"]))
-        kern = f"k_batch_tree<{'double' if vb == 8 else 'float'},{r['threads']}>"
+        # one k_batch_tree launch per size class, concurrent: the roofline is the
+        # group's (all classes' bytes over the time from the first start to the last end)
+        kern = f"k_batch_tree<{'double' if vb == 8 else 'float'},*>"
         rec = {"compartment_steps_per_s": r["compartment_steps"] / r["wall_s"],
                "ms_per_step": r["wall_s"] / a.steps * 1e3, "status": r["status"],
                "solves_per_step": sum(r["solves"]) / len(imp) / a.steps,
-               "workgroup_threads": r["threads"], "add_s": r["add_s"],
+               "workgroup_threads": r["threads"],
+               "workgroup_threads_by_size_class": dict(zip(
+                   ["N<4096", "4096<=N<16384", "N>=16384"], r["class_threads"])),
+               "launch_shape_tuning": r["tune"],
+               "add_s": r["add_s"],
                "device_bytes": r["device_bytes"],
                "roofline": roofline(nb, r["kernel_ms"], kern, r["launches"],
                                     measured_traffic(kern, wname, per_launch))}
@@ -561,6 +573,9 @@ def main():
     ap.add_argument("--legs", default="sweep,served,config1,config2,config3")
     ap.add_argument("--sweep-trees", type=int, default=4096)
     ap.add_argument("--sweep-forms", default="implicit_fp64,implicit_mixed,explicit")
+    ap.add_argument("--tune-rows", type=int, default=4_000_000,
+                    help="compartments per size class the sweep's launch-shape tuner times "
+                         "(0: no tuning)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--config2-steps", type=int, default=1000)
     ap.add_argument("--plain-steps", type=int, default=10)

@@ -193,8 +193,9 @@ typedef struct nrt_options {
   int32_t chunk;        /* tree solve: chain chunk length at level 0 (0 -> auto) */
   int32_t chunk_hi;     /* tree solve: chunk length of the reduced levels
                            (0 -> chunk if set, else auto)                        */
-  int32_t batch_threads;/* batch (implicit): workgroup size per tree, 256, 512
-                           or 1024 (0 -> autotuned on the first launches)        */
+  int32_t batch_threads;/* batch (implicit): workgroup size per tree, 128, 256,
+                           512 or 1024 (0 -> autotuned on the first launches,
+                           for each size class of trees separately)              */
   int32_t tree_groups;  /* tree solve, local mode: this many workgroups of 512
                            threads, each owning a contiguous share of the rows
                            (its chunks and final nodes), one or two grid
@@ -291,9 +292,10 @@ int nrt_sync(nrt_handle* h);
  * No counterpart in the reference, which advances one neurite per process
  * (neurite.cpp:185-255). A batch advances many independent neurites ("slots")
  * together, each slot being exactly the problem a handle would solve alone:
- *   implicit: one launch per nrt_batch_step moves every active slot by up to k
- *     steps, one workgroup per tree, each with its own tree solve and its own
- *     refinement scalars (NRT_DIRECT, FP64 or FP32_REFINE);
+ *   implicit: each nrt_batch_step moves every active slot by up to k steps in
+ *     one launch per size class of slots (< 4,096 rows, < 16,384, the rest), a
+ *     workgroup per tree (two or more for the largest), each with its own tree
+ *     solve and its own refinement scalars (NRT_DIRECT, FP64 or FP32_REFINE);
  *   explicit: one thread per element over all slots, one launch per step, the k
  *     steps of a call captured once as a HIP graph and replayed.
  * Slots keep their whole state on the device and join or leave between calls
@@ -319,8 +321,13 @@ typedef struct nrt_batch_stats {
   int64_t compartment_steps; /* sum over slots of N x steps advanced          */
   double kernel_ms;          /* device time of the step launches              */
   int32_t active_slots;      /* slots with steps left                         */
-  int32_t threads;           /* implicit: workgroup size in use               */
+  int32_t threads;           /* implicit: workgroup size of the size class
+                                holding most compartments                     */
   int64_t device_bytes;      /* device memory held by the slots               */
+  int32_t class_threads[3];  /* implicit: workgroup size per size class (slots
+                                of < 4,096 rows, < 16,384, the rest), each its
+                                own launch, tuned by timing its first launches */
+  int32_t reserved;
 } nrt_batch_stats;
 
 int nrt_batch_create(const nrt_options* opt, nrt_batch** out);
@@ -335,6 +342,35 @@ int nrt_batch_trace(nrt_batch* b, int32_t slot, double* host_out, int64_t capaci
 int nrt_batch_slot_info(nrt_batch* b, int32_t slot, nrt_slot_info* info);
 int nrt_batch_remove(nrt_batch* b, int32_t slot);
 int nrt_batch_get_stats(nrt_batch* b, nrt_batch_stats* out);
+
+/* Launch shapes of an implicit batch, per size class. nrt_batch_tune times trial
+ * batches built from a caller's sample of problems (each class: its problems,
+ * largest first, up to rows_per_class compartments) over every pair of
+ *   plan:   (level-0 chunk, reduced-level chunk) in {auto, 8/8, 12/8, 16/8, 24/8,
+ *           16/5, 16/12} (auto: the rule nrt_options.chunk = 0 applies), and
+ *   threads in {128, 256, 512, 1024} (workgroup size),
+ * one untimed launch and one timed launch of `steps` steps each, and keeps the
+ * pair with the least kernel time per compartment-step. Slots added afterwards are
+ * planned with their class's chunk lengths and launched with its workgroup size
+ * (slots already in the batch keep their plans). Classes with no problem in the
+ * sample keep the defaults (chunks by the auto rule, workgroup size tuned on the
+ * first launches). An explicit chunk or batch_threads in the batch's options
+ * restricts the candidates to that value. */
+typedef struct nrt_batch_tune_result {
+  int32_t threads[3];            /* chosen workgroup size per size class (0: no
+                                    sample in that class)                        */
+  int32_t chunk[3];              /* chosen level-0 chunk length (0: auto rule)    */
+  int32_t chunk_hi[3];           /* chosen reduced-level chunk length (0: auto)   */
+  int32_t slots[3];              /* sample problems timed per class               */
+  int64_t rows[3];               /* their compartments                            */
+  double ns_per_row_step[3];     /* kernel time per compartment-step, chosen pair */
+  double auto_ns_per_row_step[3];/* the same with auto chunks and 256 threads     */
+  int32_t trials;                /* (plan, threads) pairs timed                   */
+  int32_t reserved;
+  double seconds;                /* wall time of the tuning                       */
+} nrt_batch_tune_result;
+int nrt_batch_tune(nrt_batch* b, const nrt_problem* const* sample, int32_t n, int64_t steps,
+                   int64_t rows_per_class, nrt_batch_tune_result* out);
 const char* nrt_batch_last_error(const nrt_batch* b);
 int nrt_batch_destroy(nrt_batch* b);
 

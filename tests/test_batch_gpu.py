@@ -49,20 +49,26 @@ def _oracle(orc, p, steps, a0, a1):
     return ref.snapshot()
 
 
-@pytest.mark.parametrize("threads", [256, 1024, 0])
+TUNED = (128, 256, 512, 1024)
+
+
+@pytest.mark.parametrize("threads", [128, 256, 1024, 0])
 def test_implicit_batch_matches_alone(threads, nrt, orc):
     probs = _problems(nrt)
     steps, a0, a1 = 60, 1, 25
     _say(f"implicit batch fp64 threads={threads} sizes={[p.N for p in probs]}")
     with nh.Batch(kind=nh.IMPLICIT, krylov=nh.DIRECT, batch_threads=threads) as b:
         ids = [b.add(p, steps, a0, a1) for p in probs]
-        for _ in range(4):  # 4 calls of 15 steps (the autotuner takes the first 3)
-            b.step(15)
+        for _ in range(5):  # 5 calls of 12 steps (the autotuner takes the first 4)
+            b.step(12)
         st = b.stats()
         assert st["active_slots"] == 0
         assert st["compartment_steps"] == steps * sum(p.N for p in probs)
         if threads == 0:
-            assert st["threads"] in (256, 512, 1024)
+            assert st["threads"] in TUNED
+            assert all(t in TUNED for t in st["class_threads"])
+        else:
+            assert st["class_threads"] == [threads] * 3
         for sid, p in zip(ids, probs):
             v = b.snapshot(sid)
             info = b.info(sid)
@@ -72,6 +78,80 @@ def test_implicit_batch_matches_alone(threads, nrt, orc):
             assert max_abs_diff(v, ref, ghost_mask(p)) <= TOL_SAME
             exact = _oracle(orc, p, steps, a0, a1)
             assert max_abs_diff(v, exact, ghost_mask(p)) <= TOL_V
+
+
+def _class_sample(nrt, per_class):
+    """Sweep trees, `per_class` of each size class (< 4,096 rows, < 16,384, the rest)."""
+    picked = {}
+    for i in range(400):
+        geom, run = wl.sweep_tree(i)
+        p = nrt.setup(geom, dT=run.dT)[0]
+        c = 0 if p.N < 4096 else (1 if p.N < 16384 else 2)
+        if len(picked.get(c, [])) < per_class:
+            picked.setdefault(c, []).append(p)
+        if all(len(picked.get(c, [])) == per_class for c in range(3)):
+            break
+    return [p for c in range(3) for p in picked[c]]
+
+
+def test_implicit_batch_size_classes(nrt, orc):
+    """Slots of every size class run as launches of their own (concurrently), each
+    class with the workgroup size its autotuner picks; every slot gives what it gives
+    alone."""
+    probs = _class_sample(nrt, 2)
+    steps, a0, a1 = 30, 1, 20
+    _say(f"implicit batch size classes sizes={[p.N for p in probs]}")
+    with nh.Batch(kind=nh.IMPLICIT, krylov=nh.DIRECT) as b:
+        ids = [b.add(p, steps, a0, a1) for p in probs]
+        for _ in range(6):
+            b.step(5)
+        st = b.stats()
+        assert st["active_slots"] == 0
+        assert st["launches"] == 6 * 3  # one launch per class per call
+        assert all(t in TUNED for t in st["class_threads"])
+        for sid, p in zip(ids, probs):
+            info = b.info(sid)
+            assert info["status"] == 0 and info["steps_done"] == steps
+            v = b.snapshot(sid)
+            assert max_abs_diff(v, _alone(p, steps, a0, a1), ghost_mask(p)) <= TOL_SAME
+            assert max_abs_diff(v, _oracle(orc, p, steps, a0, a1), ghost_mask(p)) <= TOL_V
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_batch_tune_per_class(form, nrt, orc):
+    """nrt_batch_tune times every (chunk plan, workgroup size) pair per size class on
+    a sample; slots added afterwards use the class's choice and still match the
+    oracle (FP64: and the single-problem solve)."""
+    probs = _class_sample(nrt, 6)
+    kw = dict(kind=nh.IMPLICIT, krylov=nh.DIRECT)
+    if form == "mixed":
+        kw.update(precision=nh.FP32_REFINE, refine_tol=1e-10)
+    steps, a0, a1 = 16, 1, 20
+    _say(f"batch tune {form} sizes={[p.N for p in probs]}")
+    with nh.Batch(**kw) as b:
+        r = b.tune(probs, steps=2, rows_per_class=60_000)
+        _say(f"  tune: {r}")
+        assert r["trials"] == 3 * 7 * 4
+        assert set(r["classes"]) == {"N<4096", "4096<=N<16384", "N>=16384"}
+        for c in r["classes"].values():
+            assert c["threads"] in TUNED
+            assert (c["chunk"], c["chunk_hi"]) in {("auto", "auto"), (8, 8), (12, 8), (16, 8),
+                                                   (24, 8), (16, 5), (16, 12)}
+            assert 0 < c["ns_per_compartment_step"] <= c["auto_256_ns_per_compartment_step"]
+            assert 1 <= c["sample_trees"] <= 6
+        st = b.stats()
+        assert st["class_threads"] == [r["classes"][k]["threads"] for k in
+                                       ("N<4096", "4096<=N<16384", "N>=16384")]
+        ids = [b.add(p, steps, a0, a1) for p in probs]
+        b.step(steps)
+        for sid, p in zip(ids, probs):
+            info = b.info(sid)
+            assert info["status"] == 0 and info["steps_done"] == steps
+            v = b.snapshot(sid)
+            if form == "fp64":
+                assert max_abs_diff(v, _alone(p, steps, a0, a1), ghost_mask(p)) <= TOL_SAME
+            assert max_abs_diff(v, _oracle(orc, p, steps, a0, a1), ghost_mask(p)) <= (
+                TOL_V if form == "fp64" else TOL_MIXED)
 
 
 def test_implicit_batch_mixed_precision(nrt, orc):
