@@ -15,11 +15,11 @@ namespace nrt {
 size_t implicit_slot_bytes(int precision);
 void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_elem,
                         long long interval, double* trace, void* out);
-void implicit_batch_launch(int precision, int threads, const void* table, const void* work,
-                           const int2* wgmap, int nwg, hipStream_t s);
+void implicit_batch_launch(int precision, int krylov, int threads, const void* table,
+                           const void* work, const int2* wgmap, int nwg, hipStream_t s);
 size_t implicit_slot_work_bytes();
 void implicit_slot_work(void* out, long long done, long long steps, int vn);
-int implicit_batch_capacity(int precision, int threads);
+int implicit_batch_capacity(int precision, int krylov, int threads);
 void implicit_slot_stats(nrt_handle* h, int* status, int64_t* inner);
 void implicit_slot_stats_on(nrt_handle* h, hipStream_t s, int* status, int64_t* inner);
 void implicit_snapshot_on(nrt_handle* hnd, double* host_out, hipStream_t s);
@@ -256,8 +256,8 @@ struct nrt_batch {
         if (cs_ != stream) NRT_HIP(hipStreamWaitEvent(cs_, ev0, 0));
         on = cs_;
         NRT_HIP(hipEventRecord(cev[c][0], cs_));
-        implicit_batch_launch(opt.precision, th[c], table.p, work.p, wgmap.p + seg[c].x, n,
-                              cs_);
+        implicit_batch_launch(opt.precision, opt.krylov, th[c], table.p, work.p,
+                              wgmap.p + seg[c].x, n, cs_);
         NRT_HIP(hipEventRecord(cev[c][1], cs_));
         pend_cls[c].on = true;
         pend_launches++;
@@ -314,7 +314,8 @@ struct nrt_batch {
       if (first[c] >= last[c]) continue;
       auto it = capacity.find(th[c]);
       if (it == capacity.end())
-        it = capacity.emplace(th[c], implicit_batch_capacity(opt.precision, th[c])).first;
+        it = capacity.emplace(th[c], implicit_batch_capacity(opt.precision, opt.krylov, th[c]))
+                 .first;
       const int cap = it->second;
       long long total = 0;
       for (int i = first[c]; i < last[c]; i++) total += vn[i];

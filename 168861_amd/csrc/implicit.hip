@@ -1148,23 +1148,7 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
     }
     tsync<GRID>(d);
   }
-  if (!GRID && tl != nullptr && P.top_n > 0 && P.fin_cut == 0) {
-    // block mode: the whole final forest staged in LDS, a height per pass
-    const TopLds<T> L = TopLds<T>::carve(tl, kBlockTopMax);
-    const int ns = top_stage_count(P), nh = P.heights;
-    for (int j = threadIdx.x; j < ns; j += blockDim.x) top_stage(P, w, L, j);
-    __syncthreads();
-    for (int h = 0; h < nh; h++) {
-      for (int j = L.hoff[h] + (int)threadIdx.x; j < L.hoff[h + 1]; j += blockDim.x) top_up(L, j);
-      __syncthreads();
-    }
-    for (int h = nh - 1; h >= 0; h--) {
-      for (int j = L.hoff[h] + (int)threadIdx.x; j < L.hoff[h + 1]; j += blockDim.x) top_down(L, j);
-      __syncthreads();
-    }
-    for (int j = threadIdx.x; j < P.top_n; j += blockDim.x) top_store(P, w, L, j);
-    __syncthreads();
-  } else if (GRID && tl != nullptr && P.top_n > 0) {
+  if (GRID && tl != nullptr && P.top_n > 0) {
     if (d.vb == 0) {
       const TopLds<T> L = TopLds<T>::carve(tl);
       const int ns = top_stage_count(P), nh = P.heights - cut;
@@ -1180,6 +1164,18 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
         __syncthreads();
       }
       for (int j = threadIdx.x; j < P.top_n; j += blockDim.x) top_store(P, w, L, j);
+    }
+  } else if ((!GRID || d.vb == 0) && cut == 0 && P.ftask_hn > 0) {
+    // unsplit: chain tasks, a workgroup barrier per task height
+    for (int h = 0; h < P.ftask_hn; h++) {
+      for (int t = P.ftask_hoff[h] + (int)threadIdx.x; t < P.ftask_hoff[h + 1]; t += blockDim.x)
+        tps_task_up(P, w, t);
+      __syncthreads();
+    }
+    for (int h = P.ftask_hn - 1; h >= 0; h--) {
+      for (int t = P.ftask_hoff[h] + (int)threadIdx.x; t < P.ftask_hoff[h + 1]; t += blockDim.x)
+        tps_task_down(P, w, t);
+      __syncthreads();
     }
   } else if (!GRID || d.vb == 0) {
     for (int h = cut; h < P.heights; h++) {
@@ -1469,11 +1465,19 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
   }
 }
 
+// The form of each step's solve: taken from TreeArgs at run time (kSolveAny), or
+// fixed at compile time, so that a kernel holds only the code of one form (the
+// batch kernel: the registers of the Krylov and refinement loops would otherwise
+// spill in the direct one).
+constexpr int kSolveAny = -1, kSolveDirect = 0, kSolveRefine = 1, kSolveKrylov = 2;
+
 // Steps d.first .. d.first+d.num-1 of one tree (GRID: the whole launch works on
 // it; else one workgroup does).
-template <class T, bool GRID>
+template <class T, bool GRID, int MODE = kSolveAny>
 __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t, double* lds,
                                            unsigned char* tl) {
+  const bool direct = MODE == kSolveAny ? t.direct != 0 : MODE == kSolveDirect;
+  const bool krylov = MODE == kSolveAny ? t.krylov != 0 : MODE == kSolveKrylov;
   // slot ranges (the tree solve's vectors are in level-0 slot order)
   const int r_begin = GRID ? d.vb * d.rpb : 0;
   const int r_end = GRID ? min(d.ns0, r_begin + d.rpb) : d.ns0;
@@ -1506,10 +1510,10 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     double rr = 0, bb = 0;
     bool ok = true;
     int iters = -1;
-    if (t.direct) {
+    if (direct) {
       tps_run<T, GRID>(d, t.P, t.w, false, false, tl, d.x);
       sweeps = 1;
-    } else if (t.krylov) {
+    } else if (krylov) {
       const KrylovOut ko = tree_bicgstab<T, GRID>(d, t, lds, tl, slot);
       iters = ko.iters;
       sweeps = ko.solves;
@@ -1561,7 +1565,7 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
       // Krylov form: iterations, and the tree solves as sweeps
       const int its = iters >= 0 ? iters : sweeps;
       d.st->iters_total += (unsigned long long)its;
-      if (!t.direct) d.st->sweeps_total += (unsigned long long)sweeps;
+      if (!direct) d.st->sweeps_total += (unsigned long long)sweeps;
       d.st->steps += 1;
       d.st->last_iters = its;
       d.st->last_rr = rr;
@@ -2144,13 +2148,12 @@ struct SlotWork {
 // is staged once in LDS and the solve reads its fields from there: a per-lane copy
 // would live in scratch (the record is over a kilobyte), and with a dozen waves per
 // CU that scratch outgrows the caches and its reloads go to HBM.
-template <class T, int BS>
+template <class T, int BS, int MODE>
 __global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __restrict__ slots,
                                                   const SlotWork* __restrict__ work,
                                                   const int2* __restrict__ wgmap) {
   __shared__ double lds[4 * 16];
   __shared__ __align__(16) BatchSlot<T> rec;
-  extern __shared__ __align__(16) unsigned char dyn_lds[];  // a staged final forest
   const int2 m = wgmap[blockIdx.x];  // (slot, workgroup index in its grid)
   const SlotWork wk = work[m.x];
   if (wk.steps <= 0) return;
@@ -2173,8 +2176,8 @@ __global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __rest
     if (wk.vn > 1) d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
   }
   __syncthreads();
-  if (wk.vn > 1) tree_steps<T, true>(rec.d, rec.t, lds, nullptr);
-  else tree_steps<T, false>(rec.d, rec.t, lds, rec.t.top_lds ? dyn_lds : nullptr);
+  if (wk.vn > 1) tree_steps<T, true, MODE>(rec.d, rec.t, lds, nullptr);
+  else tree_steps<T, false, MODE>(rec.d, rec.t, lds, nullptr);
 }
 
 // ---------------------------------------------------------------- host side
@@ -2474,8 +2477,6 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
                                std::min(kMaxSlotBlocks, (N + kRowsPerWorkgroup - 1) /
                                                             kRowsPerWorkgroup),
                                batch_threads);
-      // one workgroup: the final forest in LDS when it is small enough
-      if (!e->grid_mode && !e->local_mode) stage_block_final_forest(e->plan_h, kBlockTopMax);
       const TreePlanHost& P = e->plan_h;
       // one device buffer holding every plan array
       std::vector<int32_t> flat;
@@ -2682,9 +2683,7 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b, T* alt = n
                      o.refine_tol,
                      o.refine_max,
                      (e->grid_mode && e->plan_h.top_n > 0) ||
-                             (e->local_mode && e->plan_h.top_lds_ok) ||
-                             (!e->grid_mode && !e->local_mode && e->plan_h.top_n > 0 &&
-                              e->plan_h.fin_cut == 0)
+                             (e->local_mode && e->plan_h.top_lds_ok)
                          ? 1
                          : 0,
                      e->local_mode ? e->hh_goff.p : nullptr,
@@ -2747,8 +2746,7 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
       NRT_HIP(hipLaunchKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, shmem,
                                          hnd->stream));
     else
-      NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args,
-                              targs.top_lds ? top_lds_bytes<TT>(kBlockTopMax) : 0, hnd->stream));
+      NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));
   };
   if (e->method == 1 || e->method == 3)
     launch(tree_args<double>(e, o, e->tw64.p, e->tw64alt.p), (const void*)k_implicit_tree<double, true>,
@@ -2944,53 +2942,47 @@ void implicit_slot_work(void* out, long long done, long long steps, int vn) {
   std::memcpy(out, &w, sizeof(w));
 }
 
-template <class T, int BS>
-static int batch_capacity_of() {
-  int dev = 0, cus = 0, per = 0;
-  NRT_HIP(hipGetDevice(&dev));
-  NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_batch_tree<T, BS>, BS,
-                                                       top_lds_bytes<T>(kBlockTopMax)));
-  return cus * std::max(per, 0);
+// Solve form of a batch's slots (all alike): the kernel instantiation to launch.
+static int batch_mode(int precision, int krylov) {
+  if (krylov == NRT_BICGSTAB) return kSolveKrylov;
+  return precision == NRT_FP64 ? kSolveDirect : kSolveRefine;
+}
+
+// The k_batch_tree instantiation for (value type, solve form, workgroup size).
+template <class T, int MODE>
+static const void* batch_kernel_of(int threads) {
+  if (threads == 128) return (const void*)k_batch_tree<T, 128, MODE>;
+  if (threads == 256) return (const void*)k_batch_tree<T, 256, MODE>;
+  if (threads == 512) return (const void*)k_batch_tree<T, 512, MODE>;
+  return (const void*)k_batch_tree<T, 1024, MODE>;
+}
+static const void* batch_kernel(int precision, int krylov, int threads) {
+  const int mode = batch_mode(precision, krylov);
+  if (precision == NRT_FP64)
+    return mode == kSolveKrylov ? batch_kernel_of<double, kSolveKrylov>(threads)
+                                : batch_kernel_of<double, kSolveDirect>(threads);
+  return mode == kSolveKrylov ? batch_kernel_of<float, kSolveKrylov>(threads)
+                              : batch_kernel_of<float, kSolveRefine>(threads);
 }
 
 // Workgroups of the batched kernel that can be resident at once (an otherwise
 // idle device): virtual grids of several workgroups are only formed within it.
-int implicit_batch_capacity(int precision, int threads) {
-  if (precision == NRT_FP64) {
-    if (threads == 128) return batch_capacity_of<double, 128>();
-    if (threads == 256) return batch_capacity_of<double, 256>();
-    if (threads == 512) return batch_capacity_of<double, 512>();
-    return batch_capacity_of<double, 1024>();
-  }
-  if (threads == 128) return batch_capacity_of<float, 128>();
-  if (threads == 256) return batch_capacity_of<float, 256>();
-  if (threads == 512) return batch_capacity_of<float, 512>();
-  return batch_capacity_of<float, 1024>();
+int implicit_batch_capacity(int precision, int krylov, int threads) {
+  int dev = 0, cus = 0, per = 0;
+  NRT_HIP(hipGetDevice(&dev));
+  NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per, batch_kernel(precision, krylov, threads), threads, 0));
+  return cus * std::max(per, 0);
 }
 
-void implicit_batch_launch(int precision, int threads, const void* table, const void* work,
-                           const int2* wgmap, int nwg, hipStream_t s) {
+void implicit_batch_launch(int precision, int krylov, int threads, const void* table,
+                           const void* work, const int2* wgmap, int nwg, hipStream_t s) {
   if (nwg <= 0) return;
-  const dim3 grid(nwg);
-  const auto* w = (const SlotWork*)work;
-  // every workgroup gets room for a staged final forest (slots whose forest fits)
-  if (precision == NRT_FP64) {
-    auto* t = (const BatchSlot<double>*)table;
-    const size_t sh = top_lds_bytes<double>(kBlockTopMax);
-    if (threads == 128) hipLaunchKernelGGL((k_batch_tree<double, 128>), grid, dim3(128), sh, s, t, w, wgmap);
-    else if (threads == 256) hipLaunchKernelGGL((k_batch_tree<double, 256>), grid, dim3(256), sh, s, t, w, wgmap);
-    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<double, 512>), grid, dim3(512), sh, s, t, w, wgmap);
-    else hipLaunchKernelGGL((k_batch_tree<double, 1024>), grid, dim3(1024), sh, s, t, w, wgmap);
-  } else {
-    auto* t = (const BatchSlot<float>*)table;
-    const size_t sh = top_lds_bytes<float>(kBlockTopMax);
-    if (threads == 128) hipLaunchKernelGGL((k_batch_tree<float, 128>), grid, dim3(128), sh, s, t, w, wgmap);
-    else if (threads == 256) hipLaunchKernelGGL((k_batch_tree<float, 256>), grid, dim3(256), sh, s, t, w, wgmap);
-    else if (threads == 512) hipLaunchKernelGGL((k_batch_tree<float, 512>), grid, dim3(512), sh, s, t, w, wgmap);
-    else hipLaunchKernelGGL((k_batch_tree<float, 1024>), grid, dim3(1024), sh, s, t, w, wgmap);
-  }
-  NRT_HIP(hipGetLastError());
+  const SlotWork* w = (const SlotWork*)work;
+  void* args[] = {(void*)&table, (void*)&w, (void*)&wgmap};
+  NRT_HIP(hipLaunchKernel(batch_kernel(precision, krylov, threads), dim3(nwg), dim3(threads), args,
+                          0, s));
 }
 
 // Steps first..first+num-1 enqueued on the handle's stream without waiting; the

@@ -54,6 +54,15 @@ struct TreePlanHost {
   std::vector<int32_t> fin_hoff;    // height boundaries into fin_order, H+1 entries
   std::vector<int32_t> fin_cptr, fin_child;  // children of final-forest nodes (CSR)
   int heights = 0;                  // H
+  // Chain tasks of the final forest: a node with exactly one child is eliminated
+  // right after that child by the same thread (no barrier between), so a task is a
+  // maximal single-child chain, bottom to top, and tasks are ordered by task height
+  // (1 + the largest task height among the bottom node's children). A forest of
+  // segments (start and end node each) takes half as many passes as by heights.
+  int ftask_hn = 0;                 // task heights
+  std::vector<int32_t> ftask_hoff;  // task-height boundaries into tasks, ftask_hn+1
+  std::vector<int32_t> ftask_off;   // task t: nodes [ftask_off[t], ftask_off[t+1])
+  std::vector<int32_t> ftask_node;  // ... of ftask_node, bottom to top
   // Final forest split for grid launches (partition_final_forest): the nodes below
   // height fin_cut form whole subtrees, dealt to sub_groups groups that workgroups
   // eliminate side by side; only heights >= fin_cut are left to one workgroup.
@@ -138,12 +147,6 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par, int chunk, int chu
 // top fits in kTopLdsNodes LDS entries cost a tenth (the lists for that are
 // built). No split when groups <= 1.
 void partition_final_forest(TreePlanHost& P, int groups, int threads, bool stage_top = false);
-// A launch of one workgroup (block mode, a batch slot) eliminates the whole final
-// forest in LDS when it has at most max_entries nodes: top lists over every height
-// (fin_cut 0); plans already split for a grid keep theirs.
-void stage_block_final_forest(TreePlanHost& P, int max_entries);
-// LDS entries a block-mode launch provides for its staged final forest.
-constexpr int kBlockTopMax = 768;
 
 // Local mode row groups: G contiguous ranges of rows of (nearly) equal size.
 std::vector<int32_t> row_groups(int N, int G);
@@ -164,7 +167,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
 // exactly its length: an access past the end of any one array leaves its
 // allocation (host emulation under a bounds-checking build).
 TreePlanDev plan_view_parts(const TreePlanHost& P, std::vector<std::vector<int32_t>>& store);
-constexpr int kPlanParts = 31;
+constexpr int kPlanParts = 34;
 
 // The matrix with its rows in level-0 slot order (padding slots are empty rows):
 // columns as slots, each entry's role (0 diagonal, 1 coupling to the parent, 2 to
@@ -181,7 +184,7 @@ SlotCsr slot_order_csr(const TreePlanHost& P, int N, const int32_t* rowptr, cons
 
 // Device view of a plan (pointers into one device allocation).
 struct TreePlanDev {
-  int levels, heights;
+  int levels, heights, ftask_hn;
   int fin_cut, sub_groups;
   int top_n, top_ext;
   int groups, top_lds_ok, lds_nodes, top_wide, top_narrow, top_kids;
@@ -200,6 +203,9 @@ struct TreePlanDev {
   const int32_t* fin_hoff;
   const int32_t* fin_cptr;
   const int32_t* fin_child;
+  const int32_t* ftask_hoff;
+  const int32_t* ftask_off;
+  const int32_t* ftask_node;
   const int32_t* sub_order;
   const int32_t* sub_hoff;
   const int32_t* top_node;
@@ -545,6 +551,89 @@ NRT_HD void tps_final_down(const TreePlanDev& P, const TreeWork<T>& w, int v) {
   T acc = w.r[base + v];
   if (p >= 0) acc = acc - w.up[base + v] * w.x[base + p];
   w.x[base + v] = acc * tps_rcp(w.ca[base + v]);
+}
+
+// Final forest by chain tasks (TreePlanHost::ftask_*), task t: its bottom node as
+// tps_final_up (children in other tasks), then each node above it from the one
+// below, whose pivot and right-hand side are still in registers. The operands that
+// do not depend on the chain (the nodes, their d, r, up and dn) are loaded before
+// the chain's arithmetic; the arithmetic is tps_final_up's, operand for operand.
+constexpr int kTaskBatch = 4;
+template <class T>
+NRT_HD void tps_task_up(const TreePlanDev& P, const TreeWork<T>& w, int t) {
+  const int base = P.node_off[P.levels];
+  const int k0 = P.ftask_off[t], k1 = P.ftask_off[t + 1];
+  int v = P.ftask_node[k0];
+  T dv = w.d[base + v], rv = w.r[base + v];
+  for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) {
+    const int c = base + P.fin_child[k];
+    const T f = w.dn[c] * tps_rcp(w.ca[c]);
+    dv = dv - f * w.up[c];
+    rv = rv - f * w.r[c];
+  }
+  w.ca[base + v] = dv;
+  w.r[base + v] = rv;
+  for (int k = k0 + 1; k < k1; k += kTaskBatch) {
+    int u[kTaskBatch];
+    T bd[kTaskBatch], br[kTaskBatch], bup[kTaskBatch], bdn[kTaskBatch];
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k + j < k1) {
+        const int c = base + P.ftask_node[k + j - 1];  // the node below: its only child
+        u[j] = base + P.ftask_node[k + j];
+        bd[j] = w.d[u[j]];
+        br[j] = w.r[u[j]];
+        bup[j] = w.up[c];
+        bdn[j] = w.dn[c];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k + j < k1) {
+        const T f = bdn[j] * tps_rcp(dv);
+        dv = bd[j] - f * bup[j];
+        rv = br[j] - f * rv;
+        w.ca[u[j]] = dv;
+        w.r[u[j]] = rv;
+      }
+    }
+  }
+}
+
+// Back-substitution of task t, top to bottom (tps_final_down's arithmetic): the
+// top node from its parent's solution (a task of a larger height), each node below
+// from the one above.
+template <class T>
+NRT_HD void tps_task_down(const TreePlanDev& P, const TreeWork<T>& w, int t) {
+  const int base = P.node_off[P.levels];
+  const int k0 = P.ftask_off[t], k1 = P.ftask_off[t + 1];
+  const int v = P.ftask_node[k1 - 1];
+  const int p = P.fin_par[v];
+  T acc = w.r[base + v];
+  if (p >= 0) acc = acc - w.up[base + v] * w.x[base + p];
+  T xv = acc * tps_rcp(w.ca[base + v]);
+  w.x[base + v] = xv;
+  for (int k = k1 - 2; k >= k0; k -= kTaskBatch) {
+    int u[kTaskBatch];
+    T br[kTaskBatch], bup[kTaskBatch], bca[kTaskBatch];
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k - j >= k0) {
+        u[j] = base + P.ftask_node[k - j];
+        br[j] = w.r[u[j]];
+        bup[j] = w.up[u[j]];
+        bca[j] = w.ca[u[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k - j >= k0) {
+        const T a = br[j] - bup[j] * xv;
+        xv = a * tps_rcp(bca[j]);
+        w.x[u[j]] = xv;
+      }
+    }
+  }
 }
 
 // The top of the final forest staged in LDS (TreePlanHost::top_*): one workgroup

@@ -95,6 +95,47 @@ struct LocalLevel {
 };
 }  // namespace
 
+// Chain tasks of the final forest (TreePlanHost::ftask_*). Children have larger
+// indices than their parents, so one pass from the last node settles every task
+// before its parent is seen.
+static void build_final_tasks(TreePlanHost& P) {
+  const int n = (int)P.fin_par.size();
+  std::vector<int32_t> task(n, -1), th;  // task of each node; height of each task
+  std::vector<std::vector<int32_t>> nodes;
+  for (int u = n - 1; u >= 0; u--) {
+    const int nc = P.fin_cptr[u + 1] - P.fin_cptr[u];
+    if (nc == 1) {  // continues its only child's task (whose top that child is)
+      const int c = P.fin_child[P.fin_cptr[u]];
+      task[u] = task[c];
+      nodes[task[u]].push_back(u);
+    } else {
+      int h = 0;
+      for (int k = P.fin_cptr[u]; k < P.fin_cptr[u + 1]; k++)
+        h = std::max(h, th[task[P.fin_child[k]]] + 1);
+      task[u] = (int)nodes.size();
+      nodes.push_back({u});
+      th.push_back(h);
+    }
+  }
+  const int nt = (int)nodes.size();
+  int hn = 0;
+  for (int t = 0; t < nt; t++) hn = std::max(hn, th[t] + 1);
+  P.ftask_hn = n > 0 ? hn : 0;
+  P.ftask_hoff.assign(P.ftask_hn + 1, 0);
+  for (int t = 0; t < nt; t++) P.ftask_hoff[th[t] + 1]++;
+  for (int h = 0; h < P.ftask_hn; h++) P.ftask_hoff[h + 1] += P.ftask_hoff[h];
+  std::vector<int32_t> fill(P.ftask_hoff.begin(), P.ftask_hoff.end() - 1), order(nt);
+  for (int t = 0; t < nt; t++) order[fill[th[t]]++] = t;
+  P.ftask_off.assign(nt + 1, 0);
+  P.ftask_node.clear();
+  P.ftask_node.reserve(n);
+  for (int i = 0; i < nt; i++) {
+    const auto& v = nodes[order[i]];
+    P.ftask_node.insert(P.ftask_node.end(), v.begin(), v.end());
+    P.ftask_off[i + 1] = (int32_t)P.ftask_node.size();
+  }
+}
+
 TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int chunk_hi,
                              const std::vector<uint8_t>* dyn_rows,
                              const std::vector<int32_t>* groups, bool wide) {
@@ -287,6 +328,7 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
   std::vector<int32_t> pos(P.fin_cptr.begin(), P.fin_cptr.end() - 1);
   for (int u = 0; u < n; u++)
     if (fp[u] >= 0) P.fin_child[pos[fp[u]]++] = u;
+  build_final_tasks(P);
   P.nchunk.push_back(0);  // keeps nchunk[levels] addressable
   if (G > 0) {
     // local final nodes: the whole subtree in the node's group (children have
@@ -398,7 +440,8 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
   const std::vector<int32_t>* p[kPlanParts] = {
       &P.nslots,    &P.nchunk,    &P.node_off,  &P.next,      &P.chunk_off, &P.chunk_len,
       &P.chunk_dyn, &P.slot0,     &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
-      &P.fin_cptr,  &P.fin_child, &P.sub_order, &P.sub_hoff,  &P.top_node,  &P.top_hoff,
+      &P.fin_cptr,  &P.fin_child, &P.ftask_hoff, &P.ftask_off, &P.ftask_node,
+      &P.sub_order, &P.sub_hoff,  &P.top_node,  &P.top_hoff,
       &P.top_cptr,  &P.top_child, &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
       &P.lbase,     &P.lnext,     &P.lf_cptr,   &P.lf_child,  &P.lf_par,    &P.wide_off,
       &P.wide_col};
@@ -406,16 +449,17 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
 }
 
 static TreePlanDev plan_from(const TreePlanHost& P, const int32_t* const* a) {
-  return TreePlanDev{P.levels,     P.heights,    P.fin_cut,   P.sub_groups, P.top_n,
-                     P.top_ext,    P.groups,     P.top_lds_ok, P.lds_nodes, P.top_wide,
-                     P.top_narrow, P.top_kids,   P.gl_cols,   P.gl_fin,     P.gl_edges,
-                     P.gl_wide,    a[0],         a[1],        a[2],         a[3],
-                     a[4],         a[5],         a[6],        a[7],         a[8],
-                     a[9],         a[10],        a[11],       a[12],        a[13],
-                     a[14],        a[15],        a[16],       a[17],        a[18],
-                     a[19],        a[20],        a[21],       a[22],        a[23],
-                     a[24],        a[25],        a[26],       a[27],        a[28],
-                     a[29],        a[30]};
+  return TreePlanDev{P.levels,     P.heights,    P.ftask_hn,  P.fin_cut,    P.sub_groups,
+                     P.top_n,      P.top_ext,    P.groups,    P.top_lds_ok, P.lds_nodes,
+                     P.top_wide,   P.top_narrow, P.top_kids,  P.gl_cols,    P.gl_fin,
+                     P.gl_edges,   P.gl_wide,    a[0],        a[1],         a[2],
+                     a[3],         a[4],         a[5],        a[6],         a[7],
+                     a[8],         a[9],         a[10],       a[11],        a[12],
+                     a[13],        a[14],        a[15],       a[16],        a[17],
+                     a[18],        a[19],        a[20],       a[21],        a[22],
+                     a[23],        a[24],        a[25],       a[26],        a[27],
+                     a[28],        a[29],        a[30],       a[31],        a[32],
+                     a[33]};
 }
 
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
@@ -490,23 +534,6 @@ static void stage_final_top(TreePlanHost& P, int cut) {
   P.fin_cut = cut;
   if (!P.top_lds_ok) {
     P.top_n = P.top_ext = 0;
-    P.top_node.clear();
-    P.top_hoff.clear();
-    P.top_cptr.clear();
-    P.top_child.clear();
-    P.top_par.clear();
-  }
-}
-
-void stage_block_final_forest(TreePlanHost& P, int max_entries) {
-  const int n = (int)P.fin_par.size();
-  if (P.fin_cut != 0 || P.top_n != 0 || n == 0 || n > max_entries) return;
-  std::vector<uint8_t> all(n, 1);
-  stage_top_set(P, all, max_entries);
-  if (!P.top_lds_ok || P.fin_cut != 0) {
-    P.fin_cut = 0;
-    P.top_n = P.top_ext = 0;
-    P.top_lds_ok = 0;
     P.top_node.clear();
     P.top_hoff.clear();
     P.top_cptr.clear();
@@ -752,7 +779,6 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
                                      dyn_rows ? &dyn : nullptr, local ? &rgrp : nullptr);
     if (local) check_local_plan(H);
     else partition_final_forest(H, groups, 256, groups > 0);
-    if (groups == 0) stage_block_final_forest(H, kBlockTopMax);  // as a block-mode launch
     if (!local) {
       // the split's promise to the device: a subtree node's children sit in its own
       // group at lower heights, and every node is placed exactly once
@@ -776,6 +802,31 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
             throw std::runtime_error("final forest split: a child outside its parent's group");
         }
       }
+    }
+    {
+      // the chain tasks' promise: every node once; above a task's bottom node, each
+      // node's only child is the node below it; the bottom node's children sit in
+      // tasks of lower task heights
+      const int n = (int)H.fin_par.size();
+      std::vector<int32_t> th(n, -1);
+      for (int h = 0; h < H.ftask_hn; h++)
+        for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++)
+          for (int k = H.ftask_off[t]; k < H.ftask_off[t + 1]; k++) {
+            const int u = H.ftask_node[k];
+            if (th[u] >= 0) throw std::runtime_error("chain tasks: node placed twice");
+            th[u] = h;
+            const int nc = H.fin_cptr[u + 1] - H.fin_cptr[u];
+            if (k > H.ftask_off[t]) {
+              if (nc != 1 || H.fin_child[H.fin_cptr[u]] != H.ftask_node[k - 1])
+                throw std::runtime_error("chain tasks: a chain node without its one child below");
+            } else {
+              for (int q = H.fin_cptr[u]; q < H.fin_cptr[u + 1]; q++)
+                if (th[H.fin_child[q]] < 0 || th[H.fin_child[q]] >= h)
+                  throw std::runtime_error("chain tasks: a child not in a lower task height");
+            }
+          }
+      for (int u = 0; u < n; u++)
+        if (th[u] < 0) throw std::runtime_error("chain tasks: node never placed");
     }
     // every plan array and every work array in an allocation of its own, of exactly
     // its length, so that a bounds-checking build of this hook catches an access
@@ -968,6 +1019,12 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
         for (int h = nh - 1; h >= 0; h--)
           for (int j = L.hoff[h]; j < L.hoff[h + 1]; j++) top_down(L, j);
         for (int j = 0; j < H.top_n; j++) top_store(P, w, L, j);
+      } else if (cut == 0 && H.ftask_hn > 0) {
+        // unsplit: by chain tasks, task heights up, then down
+        for (int h = 0; h < H.ftask_hn; h++)
+          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++) tps_task_up(P, w, t);
+        for (int h = H.ftask_hn - 1; h >= 0; h--)
+          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++) tps_task_down(P, w, t);
       } else {
         for (int h = cut; h < H.heights; h++)
           for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
