@@ -15,8 +15,11 @@ namespace nrt {
 size_t implicit_slot_bytes(int precision);
 void implicit_slot_fill(nrt_handle* h, long long a0, long long a1, int trace_elem,
                         long long interval, double* trace, void* out);
-void implicit_batch_launch(int precision, int krylov, int threads, const void* table,
-                           const void* work, const int2* wgmap, int nwg, hipStream_t s);
+size_t implicit_wgrec_bytes(int precision, int nwg);
+void implicit_batch_expand(int precision, const void* table, const void* work,
+                           const int2* wgmap, int nwg, void* wgrec, hipStream_t s);
+void implicit_batch_launch(int precision, int krylov, int threads, const void* wgrec, int nwg,
+                           hipStream_t s);
 size_t implicit_slot_work_bytes();
 void implicit_slot_work(void* out, long long done, long long steps, int vn);
 int implicit_batch_capacity(int precision, int krylov, int threads);
@@ -99,6 +102,7 @@ struct nrt_batch {
   DBuf<uint8_t> work;  // implicit: steps and workgroups per table entry
   std::vector<uint8_t> work_h;
   DBuf<int2> wgmap;    // implicit: (table entry, index in its virtual grid) per workgroup
+  DBuf<uint8_t> wgrec;  // implicit: each workgroup's slot record for the launch
   std::vector<int2> wgmap_h;
   std::vector<int> vn;  // workgroups per table entry in the launch being built
   std::map<int, int> capacity;  // resident workgroups per workgroup size
@@ -239,6 +243,11 @@ struct nrt_batch {
       NRT_HIP(hipMemcpyAsync(work.p, work_h.data(), work_h.size(), hipMemcpyHostToDevice,
                              stream));
       wgmap.upload(wgmap_h, stream);
+      const int nwg = (int)wgmap_h.size();
+      const size_t rb = implicit_wgrec_bytes(opt.precision, nwg);
+      if (wgrec.n < rb) wgrec.alloc(rb);
+      implicit_batch_expand(opt.precision, table.p, work.p, wgmap.p, nwg, wgrec.p, stream);
+      const size_t rec1 = implicit_wgrec_bytes(opt.precision, 1);
       // One launch per size class present, running concurrently: the largest class
       // on the batch stream, the others on side streams forked from it and joined
       // back, so that no launch waits for another's tail. A slot spanning several
@@ -256,8 +265,8 @@ struct nrt_batch {
         if (cs_ != stream) NRT_HIP(hipStreamWaitEvent(cs_, ev0, 0));
         on = cs_;
         NRT_HIP(hipEventRecord(cev[c][0], cs_));
-        implicit_batch_launch(opt.precision, opt.krylov, th[c], table.p, work.p,
-                              wgmap.p + seg[c].x, n, cs_);
+        implicit_batch_launch(opt.precision, opt.krylov, th[c], wgrec.p + rec1 * seg[c].x, n,
+                              cs_);
         NRT_HIP(hipEventRecord(cev[c][1], cs_));
         pend_cls[c].on = true;
         pend_launches++;
@@ -673,6 +682,26 @@ int nrt_batch_tune(nrt_batch* b, const nrt_problem* const* sample, int32_t n, in
       b->ctune[c].clear();
     }
     out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return NRT_OK;
+  });
+}
+
+int nrt_batch_set_shape(nrt_batch* b, int32_t size_class, int32_t threads, int32_t chunk,
+                        int32_t chunk_hi) {
+  if (!b) return NRT_ERR_INVALID;
+  return guarded_b(b, [&] {
+    NRT_REQUIRE(b->is_implicit(), NRT_ERR_UNSUPPORTED, "launch shapes are for implicit batches");
+    NRT_REQUIRE(size_class >= 0 && size_class < kClasses, NRT_ERR_INVALID,
+                "size_class must be 0, 1 or 2");
+    NRT_REQUIRE(threads == 0 || threads == 128 || threads == 256 || threads == 512 ||
+                    threads == 1024,
+                NRT_ERR_INVALID, "threads must be 0, 128, 256, 512 or 1024");
+    NRT_REQUIRE(chunk >= 0 && chunk <= kMaxChunk && chunk_hi >= 0 && chunk_hi <= kMaxChunk,
+                NRT_ERR_INVALID, "chunk lengths must be in [0, 4096]");
+    b->cthreads[size_class] = threads;
+    b->ctune[size_class].clear();
+    b->cchunk[size_class] = chunk;
+    b->cchunk_hi[size_class] = chunk > 0 ? chunk_hi : 0;
     return NRT_OK;
   });
 }

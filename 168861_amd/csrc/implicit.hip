@@ -1119,16 +1119,21 @@ __device__ void tps_factor(const ImpDev& d, const TreePlanDev& P, const TreeWork
 // eliminated in full, the others sweep the right-hand side with their kept
 // factors; all_static: the matrix did not change since the last solve (a later
 // refinement sweep of the same step), every chunk reuses its factors.
+// rs (direct solves): the right-hand side of the rows of static level-0 chunks is
+// computed from the state as they are swept (the update phase wrote only the rows
+// of P.dyn0).
 template <class T, bool GRID>
 __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
                                         const TreeWork<T>& w, bool acc,
-                        bool all_static, unsigned char* tl, double* out) {
+                        bool all_static, unsigned char* tl, double* out,
+                        const TpsRhsUpdate<T>* rs = nullptr) {
   const int wid = GRID ? d.vb * (int)blockDim.x + (int)threadIdx.x : (int)threadIdx.x;
   const int nw = GRID ? d.nblocks * (int)blockDim.x : (int)blockDim.x;
   for (int lv = 0; lv < P.levels; lv++) {
     const int c1 = P.chunk_off[lv + 1];
     for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw) {
       if (!all_static && P.chunk_dyn[ch]) tps_reduce_chunk(P, w, lv, ch);
+      else if (lv == 0 && rs != nullptr) tps_reduce_chunk_static(P, w, 0, ch, *rs);
       else tps_reduce_chunk_static(P, w, lv, ch);
     }
     tsync<GRID>(d);
@@ -1436,6 +1441,19 @@ __device__ __forceinline__ void tree_update(const ImpDev& d, const TreeArgs<T>& 
     if (t.direct) rt[si] = (T)bi;
     else b[si] = bi;
   }
+  if (t.direct) {
+    // direct solve: only the rows of dynamic chunks (the static ones are computed
+    // by the sweep); the HH rows among them were written above
+    const int32_t* __restrict__ dl = t.P.dyn0;
+    for (int k = tid; k < t.P.ndyn0; k += nt) {
+      const int i = dl[k];
+      if (code[i] & (kCodeHH | kCodeSkip)) continue;
+      double bi = cmdx[i] * xv[i] / dT + kdx[i];
+      if (active) bi = bi + input[i];
+      rt[i] = (T)bi;
+    }
+    return;
+  }
   // the other slots, loads of kTpsBatch slots per thread issued together
   const int st = (int)blockDim.x;
   for (int i0 = r0 + (int)threadIdx.x; i0 < r1; i0 += kTpsBatch * st) {
@@ -1511,7 +1529,8 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
     bool ok = true;
     int iters = -1;
     if (direct) {
-      tps_run<T, GRID>(d, t.P, t.w, false, false, tl, d.x);
+      const TpsRhsUpdate<T> rs{d.x, d.scmdx, d.skdx, d.sinput, d.dT, active};
+      tps_run<T, GRID>(d, t.P, t.w, false, false, tl, d.x, &rs);
       sweeps = 1;
     } else if (krylov) {
       const KrylovOut ko = tree_bicgstab<T, GRID>(d, t, lds, tl, slot);
@@ -2148,35 +2167,46 @@ struct SlotWork {
 // is staged once in LDS and the solve reads its fields from there: a per-lane copy
 // would live in scratch (the record is over a kilobyte), and with a dozen waves per
 // CU that scratch outgrows the caches and its reloads go to HBM.
-template <class T, int BS, int MODE>
-__global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __restrict__ slots,
-                                                  const SlotWork* __restrict__ work,
-                                                  const int2* __restrict__ wgmap) {
-  __shared__ double lds[4 * 16];
-  __shared__ __align__(16) BatchSlot<T> rec;
-  const int2 m = wgmap[blockIdx.x];  // (slot, workgroup index in its grid)
-  const SlotWork wk = work[m.x];
-  if (wk.steps <= 0) return;
+// Per-workgroup slot records of a launch: workgroup w of the step kernel runs
+// wgmap[w] = (table entry, index in its virtual grid), and its record is that
+// entry's with this launch's steps, trace position and grid filled in (num = 0:
+// nothing to do). Written by a kernel of its own so that the step kernel reads its
+// record through a read-only pointer at a workgroup-uniform address: scalar loads
+// into SGPRs, cached, instead of per-lane copies in registers or LDS.
+template <class T>
+__global__ __launch_bounds__(64) void k_batch_expand(const BatchSlot<T>* __restrict__ slots,
+                                                     const SlotWork* __restrict__ work,
+                                                     const int2* __restrict__ wgmap,
+                                                     BatchSlot<T>* __restrict__ wgrec) {
+  const int2 m = wgmap[blockIdx.x];
   static_assert(sizeof(BatchSlot<T>) % 4 == 0, "slot record in words");
   constexpr int kWords = (int)(sizeof(BatchSlot<T>) / 4);
   const uint32_t* __restrict__ src = (const uint32_t*)&slots[m.x];
-  uint32_t* dst = (uint32_t*)&rec;
-  for (int i = (int)threadIdx.x; i < kWords; i += BS) dst[i] = src[i];
+  uint32_t* __restrict__ dst = (uint32_t*)&wgrec[blockIdx.x];
+  for (int i = (int)threadIdx.x; i < kWords; i += 64) dst[i] = src[i];
   __syncthreads();
   if (threadIdx.x == 0) {
-    ImpDev& d = rec.d;
+    const SlotWork wk = work[m.x];
+    ImpDev& d = wgrec[blockIdx.x].d;
     d.first = wk.done + 1;
-    d.num = wk.steps;
+    d.num = wk.steps > 0 ? wk.steps : 0;
     if (d.interval >= 0) {
       d.gl0 = wk.done % (d.interval + 1);
-      d.trace = rec.trace_base ? rec.trace_base + wk.done / (d.interval + 1) : nullptr;
+      const BatchSlot<T>& S = slots[m.x];
+      d.trace = S.trace_base ? S.trace_base + wk.done / (d.interval + 1) : nullptr;
     }
     d.vb = m.y;
     d.nblocks = wk.vn;
     if (wk.vn > 1) d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
   }
-  __syncthreads();
-  if (wk.vn > 1) tree_steps<T, true, MODE>(rec.d, rec.t, lds, nullptr);
+}
+
+template <class T, int BS, int MODE>
+__global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __restrict__ wgrec) {
+  __shared__ double lds[4 * 16];
+  const BatchSlot<T>& rec = wgrec[blockIdx.x];
+  if (rec.d.num <= 0) return;
+  if (rec.d.nblocks > 1) tree_steps<T, true, MODE>(rec.d, rec.t, lds, nullptr);
   else tree_steps<T, false, MODE>(rec.d, rec.t, lds, nullptr);
 }
 
@@ -2976,11 +3006,27 @@ int implicit_batch_capacity(int precision, int krylov, int threads) {
   return cus * std::max(per, 0);
 }
 
-void implicit_batch_launch(int precision, int krylov, int threads, const void* table,
-                           const void* work, const int2* wgmap, int nwg, hipStream_t s) {
+size_t implicit_wgrec_bytes(int precision, int nwg) {
+  return (size_t)std::max(nwg, 0) * implicit_slot_bytes(precision);
+}
+
+void implicit_batch_expand(int precision, const void* table, const void* work,
+                           const int2* wgmap, int nwg, void* wgrec, hipStream_t s) {
   if (nwg <= 0) return;
-  const SlotWork* w = (const SlotWork*)work;
-  void* args[] = {(void*)&table, (void*)&w, (void*)&wgmap};
+  const auto* w = (const SlotWork*)work;
+  if (precision == NRT_FP64)
+    hipLaunchKernelGGL(k_batch_expand<double>, dim3(nwg), dim3(64), 0, s,
+                       (const BatchSlot<double>*)table, w, wgmap, (BatchSlot<double>*)wgrec);
+  else
+    hipLaunchKernelGGL(k_batch_expand<float>, dim3(nwg), dim3(64), 0, s,
+                       (const BatchSlot<float>*)table, w, wgmap, (BatchSlot<float>*)wgrec);
+  NRT_HIP(hipGetLastError());
+}
+
+void implicit_batch_launch(int precision, int krylov, int threads, const void* wgrec, int nwg,
+                           hipStream_t s) {
+  if (nwg <= 0) return;
+  void* args[] = {(void*)&wgrec};
   NRT_HIP(hipLaunchKernel(batch_kernel(precision, krylov, threads), dim3(nwg), dim3(threads), args,
                           0, s));
 }

@@ -63,6 +63,10 @@ struct TreePlanHost {
   std::vector<int32_t> ftask_hoff;  // task-height boundaries into tasks, ftask_hn+1
   std::vector<int32_t> ftask_off;   // task t: nodes [ftask_off[t], ftask_off[t+1])
   std::vector<int32_t> ftask_node;  // ... of ftask_node, bottom to top
+  // Level-0 slots of the rows in dynamic chunks (all rows when levels == 0): the
+  // rows whose right-hand side a step's update phase writes for a direct solve;
+  // the static chunks compute theirs as they are swept (TpsRhsUpdate).
+  std::vector<int32_t> dyn0;
   // Final forest split for grid launches (partition_final_forest): the nodes below
   // height fin_cut form whole subtrees, dealt to sub_groups groups that workgroups
   // eliminate side by side; only heights >= fin_cut are left to one workgroup.
@@ -167,7 +171,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
 // exactly its length: an access past the end of any one array leaves its
 // allocation (host emulation under a bounds-checking build).
 TreePlanDev plan_view_parts(const TreePlanHost& P, std::vector<std::vector<int32_t>>& store);
-constexpr int kPlanParts = 34;
+constexpr int kPlanParts = 35;
 
 // The matrix with its rows in level-0 slot order (padding slots are empty rows):
 // columns as slots, each entry's role (0 diagonal, 1 coupling to the parent, 2 to
@@ -184,7 +188,7 @@ SlotCsr slot_order_csr(const TreePlanHost& P, int N, const int32_t* rowptr, cons
 
 // Device view of a plan (pointers into one device allocation).
 struct TreePlanDev {
-  int levels, heights, ftask_hn;
+  int levels, heights, ftask_hn, ndyn0;
   int fin_cut, sub_groups;
   int top_n, top_ext;
   int groups, top_lds_ok, lds_nodes, top_wide, top_narrow, top_kids;
@@ -206,6 +210,7 @@ struct TreePlanDev {
   const int32_t* ftask_hoff;
   const int32_t* ftask_off;
   const int32_t* ftask_node;
+  const int32_t* dyn0;
   const int32_t* sub_order;
   const int32_t* sub_hoff;
   const int32_t* top_node;
@@ -301,6 +306,9 @@ NRT_HD T tps_rcp(T x) {
 // their operands do not depend on them, so a batch of rows is fetched at once and
 // the latency is paid once per batch rather than once per row.
 constexpr int kTpsBatch = 4;
+// Rows per batch of the static sweep and of the back-substitution (their loads do
+// not depend on the chain's arithmetic, so a whole batch is one memory round trip).
+constexpr int kTpsBatchS = 8;
 
 // Where a chunk's two surviving ends go: the slots of the next level in the same
 // arrays (level-synchronous layout). End codes are slot indices.
@@ -450,38 +458,60 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
 // reduced matrix values of its ends) are those of its last full elimination: the
 // same operations on r as tps_reduce_chunk, in the same order, so the results are
 // identical; the sweeps are multiply-adds with no division on the dependent path.
-template <class W, class E>
+// Right-hand side of a row for the static sweep: stored (RhsArray), or computed
+// from the state as a direct solve's update phase would store it (TpsRhsUpdate:
+// b = c_m dx V / dT + K dx (+ input while the stimulus is on), the same operations).
+template <class T>
+struct RhsArray {
+  const T* r;
+  NRT_HD T operator()(int g) const { return r[g]; }
+};
+template <class T>
+struct TpsRhsUpdate {
+  const double* x;
+  const double* cmdx;
+  const double* kdx;
+  const double* input;
+  double dT;
+  bool active;
+  NRT_HD T operator()(int g) const {
+    double bi = cmdx[g] * x[g] / dT + kdx[g];
+    if (active) bi = bi + input[g];
+    return (T)bi;
+  }
+};
+
+template <class W, class E, class R>
 NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int ee,
-                                const E& ends) {
+                                const E& ends, const R& r) {
   using T = typename W::value_type;
   using P = decltype(w.d);
-  const P __restrict__ r = w.r;
   const P __restrict__ ca = w.ca;
   const P __restrict__ dn = w.dn;
   const P __restrict__ cf = w.cf;
   const P __restrict__ ci = w.ci;
   if (m <= 2) {
-    for (int k = 0; k < m; k++) ends.put_r(k == 0 ? es : ee, r[base + k * NC]);
+    for (int k = 0; k < m; k++) ends.put_r(k == 0 ? es : ee, r(base + k * NC));
     return;
   }
-  const T r0 = r[base];
+  const T r0 = r(base);
   const T dn1 = dn[base + NC];
   // forward: rp_k = r_k - f_k rp_{k-1}
-  T rp = r[base + NC];
+  T rp = r(base + NC);
   ca[base + NC] = rp;
-  for (int k0 = 2; k0 <= m - 1; k0 += kTpsBatch) {
-    T br[kTpsBatch], bf[kTpsBatch];
+  for (int k0 = 2; k0 <= m - 1; k0 += kTpsBatchS) {
+    T br[kTpsBatchS], bf[kTpsBatchS];
 #pragma unroll
-    for (int j = 0; j < kTpsBatch; j++) {
+    for (int j = 0; j < kTpsBatchS; j++) {
       const int k = k0 + j;
       if (k <= m - 1) {
         const int g = base + k * NC;
-        br[j] = r[g];
+        br[j] = r(g);
         bf[j] = cf[g];
       }
     }
 #pragma unroll
-    for (int j = 0; j < kTpsBatch; j++) {
+    for (int j = 0; j < kTpsBatchS; j++) {
       const int k = k0 + j;
       if (k <= m - 1) {
         rp = br[j] - bf[j] * rp;
@@ -494,10 +524,10 @@ NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int
   const int g2 = base + (m - 2) * NC;
   T al = ca[g2] * ci[g2];
   ca[g2] = al;
-  for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatch) {
-    T bp[kTpsBatch], bi[kTpsBatch], bn[kTpsBatch];
+  for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatchS) {
+    T bp[kTpsBatchS], bi[kTpsBatchS], bn[kTpsBatchS];
 #pragma unroll
-    for (int j = 0; j < kTpsBatch; j++) {
+    for (int j = 0; j < kTpsBatchS; j++) {
       const int k = k0 - j;
       if (k >= 1) {
         const int g = base + k * NC;
@@ -507,7 +537,7 @@ NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int
       }
     }
 #pragma unroll
-    for (int j = 0; j < kTpsBatch; j++) {
+    for (int j = 0; j < kTpsBatchS; j++) {
       const int k = k0 - j;
       if (k >= 1) {
         al = (bp[j] - bn[j] * al) * bi[j];
@@ -518,13 +548,25 @@ NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int
   ends.put_r(es, r0 - dn1 * al);
 }
 
-template <class T>
-NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
+template <class W, class E>
+NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int ee,
+                                const E& ends) {
+  using T = typename W::value_type;
+  chunk_reduce_static(w, base, NC, m, es, ee, ends, RhsArray<T>{w.r});
+}
+
+template <class T, class R>
+NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch,
+                                    const R& rhs) {
   const int NC = P.nchunk[lv], m = P.chunk_len[ch];
   const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
   const int es = nbase + P.next[base];
   const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
-  chunk_reduce_static(w, base, NC, m, es, ee, EndsSlots<T>{w});
+  chunk_reduce_static(w, base, NC, m, es, ee, EndsSlots<T>{w}, rhs);
+}
+template <class T>
+NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
+  tps_reduce_chunk_static(P, w, lv, ch, RhsArray<T>{w.r});
 }
 
 // Final forest, elimination of node v (all of its children are final already).
@@ -752,10 +794,10 @@ NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, cons
   const P __restrict__ ca = w.ca;
   const P __restrict__ cb = w.cb;
   const P __restrict__ cg = w.cg;
-  for (int k0 = 1; k0 <= m - 2; k0 += kTpsBatch) {
-    T a[kTpsBatch], b[kTpsBatch], g[kTpsBatch];
+  for (int k0 = 1; k0 <= m - 2; k0 += kTpsBatchS) {
+    T a[kTpsBatchS], b[kTpsBatchS], g[kTpsBatchS];
 #pragma unroll
-    for (int j = 0; j < kTpsBatch; j++) {
+    for (int j = 0; j < kTpsBatchS; j++) {
       const int k = k0 + j;
       if (k <= m - 2) {
         const int gg = base + k * NC;
@@ -765,7 +807,7 @@ NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, cons
       }
     }
 #pragma unroll
-    for (int j = 0; j < kTpsBatch; j++) {
+    for (int j = 0; j < kTpsBatchS; j++) {
       const int k = k0 + j;
       if (k <= m - 2) tps_put<MODE>(w, base + k * NC, a[j] + b[j] * xs + g[j] * xe, out);
     }

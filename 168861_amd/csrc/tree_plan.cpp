@@ -304,6 +304,23 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
   P.slot0 = lv[0].slot;
   P.row0.assign(P.nslots[0], -1);
   for (int i = 0; i < N0; i++) P.row0[P.slot0[i]] = i;
+  // level-0 rows whose right-hand side a step's update writes: those of the
+  // dynamic chunks (a direct solve computes the others as it sweeps their chunks),
+  // or every row when nothing is reduced
+  P.dyn0.clear();
+  if (P.levels == 0) {
+    for (int v = 0; v < P.nslots[0]; v++)
+      if (P.row0[v] >= 0) P.dyn0.push_back(v);
+  } else {
+    const int nc0 = P.chunk_off[1] - P.chunk_off[0];
+    for (int c = 0; c < nc0; c++) {
+      if (!(P.chunk_dyn[P.chunk_off[0] + c] & 1)) continue;
+      for (int k = 0; k < P.chunk_len[P.chunk_off[0] + c]; k++) {
+        const int v = k * P.nchunk[0] + c;
+        if (P.row0[v] >= 0) P.dyn0.push_back(v);
+      }
+    }
+  }
   // final forest (local index == slot): heights, children, order
   const std::vector<int32_t>& fp = lv[nl - 1].par;
   const int n = (int)fp.size();
@@ -440,7 +457,7 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
   const std::vector<int32_t>* p[kPlanParts] = {
       &P.nslots,    &P.nchunk,    &P.node_off,  &P.next,      &P.chunk_off, &P.chunk_len,
       &P.chunk_dyn, &P.slot0,     &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
-      &P.fin_cptr,  &P.fin_child, &P.ftask_hoff, &P.ftask_off, &P.ftask_node,
+      &P.fin_cptr,  &P.fin_child, &P.ftask_hoff, &P.ftask_off, &P.ftask_node, &P.dyn0,
       &P.sub_order, &P.sub_hoff,  &P.top_node,  &P.top_hoff,
       &P.top_cptr,  &P.top_child, &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
       &P.lbase,     &P.lnext,     &P.lf_cptr,   &P.lf_child,  &P.lf_par,    &P.wide_off,
@@ -449,17 +466,17 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
 }
 
 static TreePlanDev plan_from(const TreePlanHost& P, const int32_t* const* a) {
-  return TreePlanDev{P.levels,     P.heights,    P.ftask_hn,  P.fin_cut,    P.sub_groups,
-                     P.top_n,      P.top_ext,    P.groups,    P.top_lds_ok, P.lds_nodes,
-                     P.top_wide,   P.top_narrow, P.top_kids,  P.gl_cols,    P.gl_fin,
-                     P.gl_edges,   P.gl_wide,    a[0],        a[1],         a[2],
-                     a[3],         a[4],         a[5],        a[6],         a[7],
-                     a[8],         a[9],         a[10],       a[11],        a[12],
-                     a[13],        a[14],        a[15],       a[16],        a[17],
-                     a[18],        a[19],        a[20],       a[21],        a[22],
-                     a[23],        a[24],        a[25],       a[26],        a[27],
-                     a[28],        a[29],        a[30],       a[31],        a[32],
-                     a[33]};
+  return TreePlanDev{P.levels,     P.heights,    P.ftask_hn,  (int)P.dyn0.size(),
+                     P.fin_cut,    P.sub_groups, P.top_n,     P.top_ext,    P.groups,
+                     P.top_lds_ok, P.lds_nodes,  P.top_wide,  P.top_narrow, P.top_kids,
+                     P.gl_cols,    P.gl_fin,     P.gl_edges,  P.gl_wide,    a[0],
+                     a[1],         a[2],         a[3],        a[4],         a[5],
+                     a[6],         a[7],         a[8],        a[9],         a[10],
+                     a[11],        a[12],        a[13],       a[14],        a[15],
+                     a[16],        a[17],        a[18],       a[19],        a[20],
+                     a[21],        a[22],        a[23],       a[24],        a[25],
+                     a[26],        a[27],        a[28],       a[29],        a[30],
+                     a[31],        a[32],        a[33],       a[34]};
 }
 
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
@@ -802,6 +819,20 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
             throw std::runtime_error("final forest split: a child outside its parent's group");
         }
       }
+    }
+    {
+      // the update's promise to a direct solve: every row is either in dyn0 (its
+      // right-hand side written by the update) or in a static level-0 chunk (computed
+      // by the sweep), never both
+      std::vector<int> seen(H.nslots[0], 0);
+      for (int v : H.dyn0) seen[v] += 1;
+      if (H.levels > 0)
+        for (int c = 0; c < H.chunk_off[1]; c++)
+          if (!(H.chunk_dyn[c] & 1))
+            for (int k = 0; k < H.chunk_len[c]; k++) seen[k * H.nchunk[0] + c] += 2;
+      for (int v = 0; v < H.nslots[0]; v++)
+        if ((H.row0[v] >= 0) != (seen[v] == 1 || seen[v] == 2))
+          throw std::runtime_error("update rows: a row written twice or never");
     }
     {
       // the chain tasks' promise: every node once; above a task's bottom node, each

@@ -108,6 +108,9 @@ class SlotInfo(C.Structure):
                 ("trace_samples", C.c_int64), ("nnz", C.c_int64)]
 
 
+SIZE_CLASSES = ("N<4096", "4096<=N<16384", "N>=16384")  # batch slot size classes (rows)
+
+
 class BatchTuneResult(C.Structure):
     _fields_ = [("threads", C.c_int32 * 3), ("chunk", C.c_int32 * 3),
                 ("chunk_hi", C.c_int32 * 3), ("slots", C.c_int32 * 3), ("rows", C.c_int64 * 3),
@@ -186,6 +189,8 @@ def lib():
         L.nrt_batch_slot_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SlotInfo)]
         L.nrt_batch_remove.argtypes = [C.c_void_p, C.c_int32]
         L.nrt_batch_get_stats.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
+        L.nrt_batch_set_shape.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32]
         L.nrt_batch_tune.argtypes = [C.c_void_p, C.POINTER(C.POINTER(ProblemStruct)), C.c_int32,
                                      C.c_int64, C.c_int64, C.POINTER(BatchTuneResult)]
         L.nrt_batch_last_error.argtypes = [C.c_void_p]
@@ -397,6 +402,16 @@ class Batch:
     def step(self, k):
         self._c(lib().nrt_batch_step(self.b, k))
 
+    def set_shapes(self, classes):
+        """Launch shapes per size class, as tune() returns them ({class name:
+        {"threads", "chunk", "chunk_hi"}}; "auto" chunks are the auto rule)."""
+        for c, name in enumerate(SIZE_CLASSES):
+            if name in classes:
+                sh = classes[name]
+                ch = 0 if sh["chunk"] == "auto" else int(sh["chunk"])
+                chh = 0 if sh["chunk_hi"] == "auto" else int(sh["chunk_hi"])
+                self._c(lib().nrt_batch_set_shape(self.b, c, int(sh["threads"]), ch, chh))
+
     def tune(self, problems, steps=10, rows_per_class=4_000_000):
         """Launch shapes per size class (nrt_batch_tune) from a sample of problems:
         returns, per class (rows < 4096, < 16384, the rest), the chosen workgroup
@@ -407,7 +422,7 @@ class Batch:
         r = BatchTuneResult()
         self._c(lib().nrt_batch_tune(self.b, arr, len(structs), steps, rows_per_class,
                                      C.byref(r)))
-        names = ("N<4096", "4096<=N<16384", "N>=16384")
+        names = SIZE_CLASSES
         out = {"trials": r.trials, "seconds": r.seconds, "classes": {}}
         for c, name in enumerate(names):
             if r.slots[c] == 0:

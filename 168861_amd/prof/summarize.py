@@ -1,23 +1,40 @@
-"""Reduces the rocprofv3 output of profile_sweep.sh to a small JSON.
+"""Reduces the rocprofv3 output of profile_sweep.sh to small JSON files.
 
-Timed region: bench.py times the last `timed_steps` steps of each sweep form in
-launches of `steps_per_launch` steps; an implicit launch is one k_batch_tree
-dispatch per size class of slots, running concurrently (a "group"). So:
-  * k_batch_tree<double|float, *>: the last timed_steps / steps_per_launch groups
-    of that precision (a group: dispatches overlapping in time); their span (first
-    start to last end) and the summed counters of their dispatches;
-  * k_explicit_batch: the last timed_steps dispatches (one per step).
-Counters: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), and the same counters on
-the copy kernels of prof/copy_calib, which move a known number of bytes: the
-factor (known bytes / counted bytes) of the copy with the same access width
-(8-byte lanes) scales the kernels' counts to HBM bytes.
+  summarize.py reduce RAW OUT [--timed-steps N] [--steps-per-launch S] [--classes C]
+      whatever RAW holds (kernel trace, FETCH_SIZE, WRITE_SIZE, copy calibration);
+  summarize.py merge OUT IN...
+      merges reduced files (each run on a box of its own) and scales the counters.
+
+Timed region: bench.py times the last N steps of each sweep form in launches of S
+steps; an implicit launch is one k_batch_tree dispatch per size class of slots (C
+of them in the sweep), running concurrently (a "group"). So:
+  * k_batch_tree<double|float, *>: the last N / S groups of that value type (a
+    group: dispatches overlapping in time), their span (first start to last end),
+    and the counters of their N / S * C dispatches (the last ones of that type);
+  * k_explicit_batch: the last N dispatches (one per step).
+Counters: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), the HBM-side request
+counters TCC_EA0_RDREQ_DRAM_32B_sum and TCC_EA0_WRREQ_WRITE_DRAM_32B_sum (32-byte
+units), and the same counters on the copy kernels of copy_calib, which move a known
+number of bytes: the factor (known bytes / counted bytes) of the copy with the
+kernels' access width (8-byte lanes, copy_f64) scales the kernels' counts to HBM
+bytes.
 """
-import argparse
 import csv
 import glob
 import json
 import os
 import re
+import sys
+
+FAMILIES = {"k_batch_tree<double,*>": lambda n: "k_batch_tree<double" in n,
+            "k_batch_tree<float,*>": lambda n: "k_batch_tree<float" in n,
+            "k_explicit_batch": lambda n: core(n) == "k_explicit_batch"}
+
+
+def core(name):
+    """Bare function name: no namespace, template arguments or parameters."""
+    n = re.sub(r"<.*>", "", re.sub(r"\(.*", "", name or ""))
+    return n.strip().split(" ")[-1].split("::")[-1]
 
 
 def rows(pattern):
@@ -32,30 +49,26 @@ def kname(r):
     return r.get("Kernel_Name") or r.get("Kernel-Name") or r.get("Name")
 
 
-FAMILIES = {"k_batch_tree<double,*>": lambda n: "k_batch_tree<double" in n,
-            "k_batch_tree<float,*>": lambda n: "k_batch_tree<float" in n,
-            "k_explicit_batch": lambda n: core(n) == "k_explicit_batch"}
-
-
-def core(name):
-    """Bare function name: no namespace, template arguments or parameters."""
-    n = re.sub(r"<.*>", "", re.sub(r"\(.*", "", name or ""))
-    return n.strip().split(" ")[-1].split("::")[-1]
+# counter: (run tag, bytes per unit): FETCH_SIZE / WRITE_SIZE count KiB, the DRAM
+# request counters 32-byte units
+UNITS = {"FETCH_SIZE": ("fetch", 1024), "WRITE_SIZE": ("write", 1024),
+         "TCC_EA0_RDREQ_DRAM_32B_sum": ("dram", 32),
+         "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum": ("dram", 32)}
 
 
 def counters(raw, tag, counter):
     """Per dispatch, in dispatch order: (kernel name, value in bytes)."""
     out = []
+    unit = UNITS[counter][1]
     for r in rows(os.path.join(raw, tag, "**", "*counter_collection.csv")):
-        if r["Counter_Name"] != counter:
-            continue
-        out.append((int(r.get("Dispatch_Id", 0)), kname(r), float(r["Counter_Value"]) * 1024.0))
+        if r["Counter_Name"] == counter:
+            out.append((int(r.get("Dispatch_Id", 0)), kname(r), float(r["Counter_Value"]) * unit))
     out.sort()
     return [(n, v) for _, n, v in out]
 
 
 def groups(dispatches):
-    """Dispatches (start, end) sorted by start, merged while they overlap."""
+    """Dispatches (start, end), merged while they overlap."""
     g = []
     for s, e in sorted(dispatches):
         if g and s <= g[-1][1]:
@@ -65,67 +78,102 @@ def groups(dispatches):
     return g
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("raw")
-    ap.add_argument("out")
-    ap.add_argument("--timed-steps", type=int, default=40)
-    ap.add_argument("--steps-per-launch", type=int, default=10)
-    a = ap.parse_args()
-    nlaunch = a.timed_steps // a.steps_per_launch
-    res = {"timed_steps": a.timed_steps, "steps_per_launch": a.steps_per_launch, "stats": {}}
-    for r in rows(os.path.join(a.raw, "trace", "**", "*kernel_stats.csv")):
-        if any(f(r["Name"]) for f in FAMILIES.values()) or "copy_f64" in r["Name"]:
-            res["stats"][r["Name"]] = {k: r[k] for k in ("Calls", "TotalDurationNs",
-                                                         "AverageNs", "MinNs", "MaxNs")}
-    trace = rows(os.path.join(a.raw, "trace", "**", "*kernel_trace.csv"))
-    # copy calibration: known bytes / counted bytes, per access width and counter
+def reduce(raw, out, timed_steps=40, per_launch=10, classes=3):
+    nlaunch = timed_steps // per_launch
+    res = {"timed_steps": timed_steps, "steps_per_launch": per_launch, "kernels": {}}
+    stats = {}
+    for r in rows(os.path.join(raw, "trace", "**", "*kernel_stats.csv")):
+        if any(f(r["Name"]) for f in FAMILIES.values()):
+            stats[r["Name"]] = {k: r[k] for k in ("Calls", "TotalDurationNs", "AverageNs",
+                                                  "MinNs", "MaxNs")}
+    if stats:
+        res["stats_all_dispatches"] = stats
+    trace = rows(os.path.join(raw, "trace", "**", "*kernel_trace.csv"))
+    for label, match in FAMILIES.items():
+        k = {}
+        explicit = label == "k_explicit_batch"
+        mine = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in trace
+                if match(kname(r))]
+        if mine:
+            if explicit:
+                spans = [e - s for s, e in sorted(mine)[-timed_steps:]]
+            else:
+                g = groups(mine)[-nlaunch:]
+                spans = [e - s for s, e, _ in g]
+                k["dispatches_per_launch_seen"] = [n for _, _, n in g]
+            k["timed_launches"] = len(spans)
+            k["steps_per_launch"] = 1 if explicit else per_launch
+            k["mean_launch_ns"] = sum(spans) / len(spans)
+        take = timed_steps if explicit else nlaunch * classes
+        for counter, (tag, _) in UNITS.items():
+            mv = [v for n, v in counters(raw, tag, counter) if match(n)][-take:]
+            if mv:
+                k[counter + "_bytes_per_launch"] = sum(mv) / (take if explicit else nlaunch)
+                k["steps_per_launch"] = 1 if explicit else per_launch
+        if k:
+            res["kernels"][label] = k
     cal = {}
-    for tag, counter in (("copy_fetch", "FETCH_SIZE"), ("copy_write", "WRITE_SIZE")):
+    for counter, (tag, _) in UNITS.items():
+        tag = "copy_" + tag
         per = {}
-        for n, v in counters(a.raw, tag, counter):
+        for n, v in counters(raw, tag, counter):
             per.setdefault(core(n), []).append(v)
         for n, vs in per.items():
             known = 8.0 * (1 << 27)
-            cal.setdefault(n, {})[counter] = {"counted_bytes_mean": sum(vs) / len(vs),
-                                              "known_bytes": known,
-                                              "factor": known / (sum(vs) / len(vs))}
-    res["copy_calibration"] = cal
-    fx = {c: cal.get("copy_f64", {}).get(c, {}).get("factor") for c in ("FETCH_SIZE", "WRITE_SIZE")}
-    fetch = counters(a.raw, "fetch", "FETCH_SIZE")
-    write = counters(a.raw, "write", "WRITE_SIZE")
-    res["kernels"] = {}
-    for label, match in FAMILIES.items():
-        mine = [r for r in trace if match(kname(r))]
-        if not mine:
-            continue
-        d = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in mine]
-        explicit = label == "k_explicit_batch"
-        if explicit:
-            last = sorted(d)[-a.timed_steps:]
-            spans = [e - s for s, e in last]
-            per_group, steps_per = 1, 1
-        else:
-            g = groups(d)[-nlaunch:]
-            spans = [e - s for s, e, _ in g]
-            per_group = max(n for _, _, n in g)
-            steps_per = a.steps_per_launch
-        k = {"timed_launches": len(spans), "dispatches_per_launch": per_group,
-             "steps_per_launch": steps_per, "mean_launch_ns": sum(spans) / len(spans)}
-        ntake = len(spans) * per_group
-        for counter, vals in (("FETCH_SIZE", fetch), ("WRITE_SIZE", write)):
-            mv = [v for n, v in vals if match(n)][-ntake:]
-            if mv:
-                k[counter + "_bytes_per_launch"] = sum(mv) / len(spans)
-        if fx["FETCH_SIZE"] and fx["WRITE_SIZE"] and "FETCH_SIZE_bytes_per_launch" in k and \
-                "WRITE_SIZE_bytes_per_launch" in k:
-            hbm = (k["FETCH_SIZE_bytes_per_launch"] * fx["FETCH_SIZE"] +
-                   k["WRITE_SIZE_bytes_per_launch"] * fx["WRITE_SIZE"])
-            k["hbm_bytes_per_launch"] = hbm
-            k["hbm_bytes_per_step"] = hbm / steps_per
-        res["kernels"][label] = k
-    json.dump(res, open(a.out, "w"), indent=1)
-    print(json.dumps(res["kernels"], indent=1))
+            mean = sum(vs) / len(vs)
+            cal.setdefault(n, {})[counter] = {"counted_bytes_mean": mean, "known_bytes": known,
+                                              "factor": known / mean}
+    if cal:
+        res["copy_calibration"] = cal
+    json.dump(res, open(out, "w"), indent=1)
+    print(json.dumps(res, indent=1))
+
+
+def merge(out, ins):
+    res = {}
+
+    def deep(a, b):
+        for k, v in b.items():
+            if isinstance(v, dict) and isinstance(a.get(k), dict):
+                deep(a[k], v)
+            else:
+                a[k] = v
+    for f in ins:
+        deep(res, json.load(open(f)))
+    cal = res.get("copy_calibration", {}).get("copy_f64", {})
+    ff = cal.get("FETCH_SIZE", {}).get("factor")
+    fw = cal.get("WRITE_SIZE", {}).get("factor")
+    res["scale"] = {"FETCH_SIZE": ff, "WRITE_SIZE": fw, "from": "copy_f64 (8-byte lanes)"}
+    dr = cal.get("TCC_EA0_RDREQ_DRAM_32B_sum", {}).get("factor")
+    dw = cal.get("TCC_EA0_WRREQ_WRITE_DRAM_32B_sum", {}).get("factor")
+    res["scale"].update({"DRAM_READ": dr, "DRAM_WRITE": dw})
+    for k in res.get("kernels", {}).values():
+        if dr and dw and "TCC_EA0_RDREQ_DRAM_32B_sum_bytes_per_launch" in k:
+            k["dram_bytes_per_step"] = (
+                k["TCC_EA0_RDREQ_DRAM_32B_sum_bytes_per_launch"] * dr +
+                k["TCC_EA0_WRREQ_WRITE_DRAM_32B_sum_bytes_per_launch"] * dw) / k["steps_per_launch"]
+        if ff and fw and "FETCH_SIZE_bytes_per_launch" in k and "WRITE_SIZE_bytes_per_launch" in k:
+            k["hbm_read_bytes_per_launch"] = k["FETCH_SIZE_bytes_per_launch"] * ff
+            k["hbm_write_bytes_per_launch"] = k["WRITE_SIZE_bytes_per_launch"] * fw
+            k["hbm_bytes_per_launch"] = k["hbm_read_bytes_per_launch"] + k["hbm_write_bytes_per_launch"]
+            k["hbm_bytes_per_step"] = k["hbm_bytes_per_launch"] / k["steps_per_launch"]
+    json.dump(res, open(out, "w"), indent=1)
+    print(json.dumps(res.get("kernels", {}), indent=1))
+
+
+def main():
+    if len(sys.argv) >= 4 and sys.argv[1] == "reduce":
+        kw = {}
+        args = sys.argv[4:]
+        for i in range(0, len(args) - 1, 2):
+            key = {"--timed-steps": "timed_steps", "--steps-per-launch": "per_launch",
+                   "--classes": "classes"}[args[i]]
+            kw[key] = int(args[i + 1])
+        reduce(sys.argv[2], sys.argv[3], **kw)
+    elif len(sys.argv) >= 4 and sys.argv[1] == "merge":
+        merge(sys.argv[2], sys.argv[3:])
+    else:
+        sys.exit(__doc__)
 
 
 if __name__ == "__main__":

@@ -193,15 +193,21 @@ def sweep_problems(nh, wl, count):
     return geoms, imp, gen_s
 
 
-def run_sweep_form(nh, probs, kw, warmup, steps, per_launch, tune_rows=0):
+def run_sweep_form(nh, probs, kw, warmup, steps, per_launch, tune_rows=0, shapes=None):
     """Adds every problem as a slot for warmup + steps steps; untimed warm-up
     launches, then the timed steps. Implicit: the launch shapes (chunk lengths and
-    workgroup size per size class) come from nrt_batch_tune on a sample of the same
-    problems (tune_rows compartments per class) before the slots are added; without
-    it, the first four launches pick each class's workgroup size."""
+    workgroup size per size class) are `shapes` when given (a tune() result kept
+    from an earlier run), else come from nrt_batch_tune on a sample of the same
+    problems (tune_rows compartments per class) before the slots are added; with
+    neither, the first four launches pick each class's workgroup size."""
     warmup = max(warmup, 4)
     b = nh.Batch(**kw)
-    tuned = b.tune(probs, steps=per_launch, rows_per_class=tune_rows) if tune_rows > 0 else None
+    tuned = None
+    if shapes is not None:
+        b.set_shapes(shapes["classes"])
+        tuned = dict(shapes, reused=True)
+    elif tune_rows > 0:
+        tuned = b.tune(probs, steps=per_launch, rows_per_class=tune_rows)
     t0 = time.perf_counter()
     ids = [b.add(p, warmup + steps, 1, 20) for p in probs]
     add_s = time.perf_counter() - t0
@@ -230,6 +236,19 @@ def run_sweep_form(nh, probs, kw, warmup, steps, per_launch, tune_rows=0):
     return b, ids, out
 
 
+def load_shapes(path):
+    """Launch shapes kept by an earlier run ({form: tune() result}), if any."""
+    if not path or not os.path.exists(path):
+        return {}
+    return json.load(open(path))
+
+
+def save_shapes(path, form, tuned):
+    kept = load_shapes(path)
+    kept[form] = tuned
+    json.dump(kept, open(path, "w"), indent=1)
+
+
 def sweep_leg(nh, wl, orc, a, info):
     say(f"config 4: building {a.sweep_trees} trees")
     geoms, imp, gen_s = sweep_problems(nh, wl, a.sweep_trees)
@@ -254,7 +273,11 @@ def sweep_leg(nh, wl, orc, a, info):
         if name not in a.sweep_forms:
             continue
         say(f"config 4: {name}")
-        b, ids, r = run_sweep_form(nh, imp, kw, a.warmup, a.steps, per_launch, a.tune_rows)
+        kept = load_shapes(a.shapes).get(name)
+        b, ids, r = run_sweep_form(nh, imp, kw, a.warmup, a.steps, per_launch, a.tune_rows,
+                                   kept)
+        if a.shapes and kept is None and r["tune"] is not None:
+            save_shapes(a.shapes, name, r["tune"])
         snaps = {i: b.snapshot(ids[i]) for i in sample}
         done = b.info(ids[sample[0]])["steps_done"]
         b.close()
@@ -573,6 +596,9 @@ def main():
     ap.add_argument("--legs", default="sweep,served,config1,config2,config3")
     ap.add_argument("--sweep-trees", type=int, default=4096)
     ap.add_argument("--sweep-forms", default="implicit_fp64,implicit_mixed,explicit")
+    ap.add_argument("--shapes", default=None,
+                    help="JSON file of launch shapes per sweep form: used when it holds "
+                         "the form, else the tuned shapes are written to it")
     ap.add_argument("--tune-rows", type=int, default=4_000_000,
                     help="compartments per size class the sweep's launch-shape tuner times "
                          "(0: no tuning)")

@@ -371,6 +371,12 @@ typedef struct nrt_batch_tune_result {
 } nrt_batch_tune_result;
 int nrt_batch_tune(nrt_batch* b, const nrt_problem* const* sample, int32_t n, int64_t steps,
                    int64_t rows_per_class, nrt_batch_tune_result* out);
+/* Sets the launch shape of one size class (0: < 4,096 rows, 1: < 16,384, 2: the
+ * rest) as nrt_batch_tune would: workgroup size 128, 256, 512 or 1024 (0: tuned on
+ * the first launches), chunk lengths (0: the auto rule). A shape tuned once can so
+ * be kept and reused by later batches of the same kind of trees. */
+int nrt_batch_set_shape(nrt_batch* b, int32_t size_class, int32_t threads, int32_t chunk,
+                        int32_t chunk_hi);
 const char* nrt_batch_last_error(const nrt_batch* b);
 int nrt_batch_destroy(nrt_batch* b);
 

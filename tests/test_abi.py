@@ -76,6 +76,7 @@ def test_batch_entry_points_without_device(nrt):
     assert not b.value
     assert L.nrt_batch_step(None, 1) == nrt.STATUS_CODES["invalid"]
     assert L.nrt_batch_tune(None, None, 0, 1, 1, None) == nrt.STATUS_CODES["invalid"]
+    assert L.nrt_batch_set_shape(None, 0, 256, 16, 8) == nrt.STATUS_CODES["invalid"]
     assert L.nrt_batch_destroy(None) == 0
 
 

@@ -140,8 +140,10 @@ def test_batch_tune_per_class(form, nrt, orc):
             assert 0 < c["ns_per_compartment_step"] <= c["auto_256_ns_per_compartment_step"]
             assert 1 <= c["sample_trees"] <= 6
         st = b.stats()
-        assert st["class_threads"] == [r["classes"][k]["threads"] for k in
-                                       ("N<4096", "4096<=N<16384", "N>=16384")]
+        assert st["class_threads"] == [r["classes"][k]["threads"] for k in nh.SIZE_CLASSES]
+        with nh.Batch(**kw) as b2:  # the same shapes kept and set on another batch
+            b2.set_shapes(r["classes"])
+            assert b2.stats()["class_threads"] == st["class_threads"]
         ids = [b.add(p, steps, a0, a1) for p in probs]
         b.step(steps)
         for sid, p in zip(ids, probs):
