@@ -642,7 +642,8 @@ def main():
         cand = {k: sw[k] for k in ("implicit_fp64", "implicit_mixed")
                 if k in sw and sw[k].get("status", 0) == 0 and
                 (orc is None or sw[k].get("max_abs_dV_mV_vs_oracle", 1e9) <= TOL_MV)}
-        best = max(cand, key=lambda k: cand[k]["compartment_steps_per_s"])
+        best = max(cand, key=lambda k: cand[k]["compartment_steps_per_s"]) if cand else None
+    if "sweep" in legs and best is not None:
         b = sw[best]
         out.update({
             "value": b["compartment_steps_per_s"], "ms_per_step": b["ms_per_step"],
