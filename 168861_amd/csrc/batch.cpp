@@ -219,9 +219,26 @@ struct nrt_batch {
     if (order.empty() || k == 0) return;
     int th[kClasses] = {0, 0, 0};
     std::vector<int2> seg;  // wgmap range of each class
+    bool merged = false;    // one launch for all classes
+    int dom = 0;            // the class holding the most rows
     if (is_implicit()) {
+      int64_t rows[kClasses] = {0, 0, 0};
+      for (int id : order) rows[size_class(slots[id]->N)] += slots[id]->N;
+      for (int c = 1; c < kClasses; c++)
+        if (rows[c] > rows[dom]) dom = c;
       for (int c = 0; c < kClasses; c++) th[c] = pick_threads(c);
       seg = plan_workgroups(th);
+      // Classes run as launches of their own only while each fills at least half
+      // the device's co-resident workgroups (a large sweep); otherwise (a few
+      // requests of a server, say) one launch, with the dominant class's size.
+      for (int c = 0; c < kClasses; c++) {
+        const int n = seg[c].y - seg[c].x;
+        if (n > 0 && n < capacity.at(th[c]) / 2) merged = true;
+      }
+      if (merged) {
+        for (int c = 0; c < kClasses; c++) th[c] = th[dom];
+        seg = plan_workgroups(th);
+      }
     }
     int64_t cs = 0, ss = 0, ccs[kClasses] = {0, 0, 0};
     const size_t wb = implicit_slot_work_bytes();
@@ -257,6 +274,15 @@ struct nrt_batch {
       int64_t best = -1;
       NRT_HIP(hipEventRecord(ev0, stream));
       hipStream_t on = nullptr;
+      if (merged) {  // the whole wgmap as the dominant class's launch
+        seg.assign(kClasses, make_int2(0, 0));
+        seg[dom] = make_int2(0, nwg);
+        for (int c = 0; c < kClasses; c++)
+          if (c != dom) {
+            ccs[dom] += ccs[c];
+            ccs[c] = 0;
+          }
+      }
       for (int c = kClasses - 1; c >= 0; c--) {
         pend_cls[c] = ClassLaunch{false, th[c], ccs[c]};
         const int n = seg[c].y - seg[c].x;

@@ -14,9 +14,9 @@
 //      persistent-kernel launch per request
 //      (more elements than big_threshold, which would hold a whole batch launch
 //      back if one workgroup had to carry it), all on the batch stream,
-//   3. retires the requests completed by the previous launch while this one
-//      runs (final V and probe trace copied out over its own stream; the slot's
-//      memory back to the pool, which may let the FIFO head in), then waits.
+//   3. waits for the launch and retires the requests it completed (final V and
+//      probe trace copied out over the server's own stream; the slot's memory
+//      back to the pool, which may let the FIFO head in).
 // Latency of a request = completion time - submission time (steady clock).
 #include <algorithm>
 #include <chrono>
@@ -320,9 +320,8 @@ struct nrt_server {
     }
   }
 
-  // Completed batch slots of the previous launch, while the next one runs: their
-  // results come over the server's own stream and their memory goes back to the
-  // pool; none of that waits for the launch in flight.
+  // Completed batch slots of the launch just finished: their results come over the
+  // server's own stream and their memory goes back to the pool.
   void retire_slots(std::vector<int>& done) {
     for (int slot : done) {
       auto it = running.find(slot);
@@ -384,7 +383,7 @@ struct nrt_server {
       NRT_HIP(hipSetDevice(opt.device));
       NRT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
       std::deque<Built> got;
-      std::vector<int> done;  // batch slots completed by the last launch
+      std::vector<int> done;  // batch slots completed by the launch just finished
       while (true) {
         {
           std::unique_lock<std::mutex> g(mu);
@@ -408,29 +407,32 @@ struct nrt_server {
         const double t0 = now_s();
         join_ready(got);
         const long long k = cfg.steps_per_launch;
-        const bool stepping = running.size() > done.size() || !big.empty();
+        const bool stepping = !running.empty() || !big.empty();
         if (stepping) {
           if (opt.kind == NRT_IMPLICIT) batch_launch_budget(batch, k, k * kBudgetElems, 8 * k);
           else batch_launch(batch, k);
           launch_big(k);
         }
         const double t1 = now_s();
-        retire_slots(done);  // overlaps the launch
-        const double t2 = now_s();
         if (stepping) {
           batch_finish(batch);
           NRT_HIP(hipStreamSynchronize(batch_stream(batch)));
         }
-        const double t3 = now_s();
+        const double t2 = now_s();
+        // The slots this launch completed are retired at once, before the next
+        // launch: copies on another stream while a launch runs were found to wait
+        // for it anyway (streams share the process's few hardware queues), so an
+        // overlap saved nothing and delayed each result by a launch.
         for (auto& kv : running)
           if (batch_slot_complete(batch, kv.first)) done.push_back(kv.first);
+        retire_slots(done);
         retire_big();
-        const double t4 = now_s();
+        const double t3 = now_s();
         std::lock_guard<std::mutex> g(mu);
         if (stepping) stats.launches++;
         stats.launch_ms += (t1 - t0) * 1e3;
-        stats.retire_ms += (t2 - t1) * 1e3 + (t4 - t3) * 1e3;
-        stats.wait_ms += (t3 - t2) * 1e3;
+        stats.retire_ms += (t3 - t2) * 1e3;
+        stats.wait_ms += (t2 - t1) * 1e3;
       }
     } catch (const Error& e) {
       fail(e.status, e.what());

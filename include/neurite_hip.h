@@ -292,10 +292,12 @@ int nrt_sync(nrt_handle* h);
  * No counterpart in the reference, which advances one neurite per process
  * (neurite.cpp:185-255). A batch advances many independent neurites ("slots")
  * together, each slot being exactly the problem a handle would solve alone:
- *   implicit: each nrt_batch_step moves every active slot by up to k steps in
- *     one launch per size class of slots (< 4,096 rows, < 16,384, the rest), a
+ *   implicit: each nrt_batch_step moves every active slot by up to k steps, a
  *     workgroup per tree (two or more for the largest), each with its own tree
- *     solve and its own refinement scalars (NRT_DIRECT, FP64 or FP32_REFINE);
+ *     solve and its own refinement scalars (NRT_DIRECT, FP64 or FP32_REFINE); the
+ *     slots of each size class (< 4,096 rows, < 16,384, the rest) run as a launch
+ *     of their own, concurrently, while every class fills at least half the
+ *     device, else all in one launch;
  *   explicit: one thread per element over all slots, one launch per step, the k
  *     steps of a call captured once as a HIP graph and replayed.
  * Slots keep their whole state on the device and join or leave between calls
@@ -325,8 +327,10 @@ typedef struct nrt_batch_stats {
                                 holding most compartments                     */
   int64_t device_bytes;      /* device memory held by the slots               */
   int32_t class_threads[3];  /* implicit: workgroup size per size class (slots
-                                of < 4,096 rows, < 16,384, the rest), each its
-                                own launch, tuned by timing its first launches */
+                                of < 4,096 rows, < 16,384, the rest), tuned by
+                                timing the class's first launches (classes that
+                                share a launch run with the size of the class
+                                holding the most rows)                         */
   int32_t reserved;
 } nrt_batch_stats;
 
