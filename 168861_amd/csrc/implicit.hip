@@ -1174,12 +1174,12 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
     // unsplit: chain tasks, a workgroup barrier per task height
     for (int h = 0; h < P.ftask_hn; h++) {
       for (int t = P.ftask_hoff[h] + (int)threadIdx.x; t < P.ftask_hoff[h + 1]; t += blockDim.x)
-        tps_task_up(P, w, t);
+        tps_task_up(P, w, P.ftask_off, P.ftask_node, t);
       __syncthreads();
     }
     for (int h = P.ftask_hn - 1; h >= 0; h--) {
       for (int t = P.ftask_hoff[h] + (int)threadIdx.x; t < P.ftask_hoff[h + 1]; t += blockDim.x)
-        tps_task_down(P, w, t);
+        tps_task_down(P, w, P.ftask_off, P.ftask_node, t);
       __syncthreads();
     }
   } else if (!GRID || d.vb == 0) {
@@ -1687,7 +1687,7 @@ __device__ __forceinline__ void local_top(const ImpDev& d, const TreePlanDev& P,
 template <class T>
 __device__ __forceinline__ void tps_run_local(const ImpDev& d, const TreePlanDev& P, const TreeWork<T>& w,
                               bool acc, bool all_static, unsigned char* tl) {
-  const int g = d.vb, G = P.groups, H = P.heights, L = P.levels;
+  const int g = d.vb, G = P.groups, L = P.levels;
   const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
   for (int lv = 0; lv < L; lv++) {
     const int32_t* gc = P.gcol + lv * (G + 1);
@@ -1699,18 +1699,20 @@ __device__ __forceinline__ void tps_run_local(const ImpDev& d, const TreePlanDev
     }
     __syncthreads();
   }
-  const int32_t* lh = P.loc_hoff + g * H;
-  for (int h = 0; h < H; h++) {
-    const int k0 = lh[h], k1 = lh[h + 1];
+  // the group's local final nodes by chain tasks (a barrier per task height)
+  const int TH = P.ltask_hn;
+  const int32_t* lt = P.ltask_hoff + g * TH;
+  for (int h = 0; h < TH; h++) {
+    const int k0 = lt[h], k1 = lt[h + 1];
     if (k0 == k1) continue;  // uniform over the workgroup
-    for (int k = k0 + tid; k < k1; k += nt) tps_final_up(P, w, P.loc_order[k]);
+    for (int t = k0 + tid; t < k1; t += nt) tps_task_up(P, w, P.ltask_off, P.ltask_node, t);
     __syncthreads();
   }
   if (P.top_n > 0) local_top<T>(d, P, w, tl);
-  for (int h = H - 1; h >= 0; h--) {
-    const int k0 = lh[h], k1 = lh[h + 1];
+  for (int h = TH - 1; h >= 0; h--) {
+    const int k0 = lt[h], k1 = lt[h + 1];
     if (k0 == k1) continue;
-    for (int k = k0 + tid; k < k1; k += nt) tps_final_down(P, w, P.loc_order[k]);
+    for (int t = k0 + tid; t < k1; t += nt) tps_task_down(P, w, P.ltask_off, P.ltask_node, t);
     __syncthreads();
   }
   for (int lv = L - 1; lv >= 1; lv--) {

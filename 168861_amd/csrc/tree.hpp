@@ -63,6 +63,10 @@ struct TreePlanHost {
   std::vector<int32_t> ftask_hoff;  // task-height boundaries into tasks, ftask_hn+1
   std::vector<int32_t> ftask_off;   // task t: nodes [ftask_off[t], ftask_off[t+1])
   std::vector<int32_t> ftask_node;  // ... of ftask_node, bottom to top
+  // Local mode: the chain tasks of each group's local final nodes, group g's tasks
+  // of task height h being [ltask_hoff[g*ltask_hn+h], +1) of ltask_off / ltask_node.
+  int ltask_hn = 0;
+  std::vector<int32_t> ltask_hoff, ltask_off, ltask_node;
   // Level-0 slots of the rows in dynamic chunks (all rows when levels == 0): the
   // rows whose right-hand side a step's update phase writes for a direct solve;
   // the static chunks compute theirs as they are swept (TpsRhsUpdate).
@@ -171,7 +175,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
 // exactly its length: an access past the end of any one array leaves its
 // allocation (host emulation under a bounds-checking build).
 TreePlanDev plan_view_parts(const TreePlanHost& P, std::vector<std::vector<int32_t>>& store);
-constexpr int kPlanParts = 35;
+constexpr int kPlanParts = 38;
 
 // The matrix with its rows in level-0 slot order (padding slots are empty rows):
 // columns as slots, each entry's role (0 diagonal, 1 coupling to the parent, 2 to
@@ -188,7 +192,7 @@ SlotCsr slot_order_csr(const TreePlanHost& P, int N, const int32_t* rowptr, cons
 
 // Device view of a plan (pointers into one device allocation).
 struct TreePlanDev {
-  int levels, heights, ftask_hn, ndyn0;
+  int levels, heights, ftask_hn, ndyn0, ltask_hn;
   int fin_cut, sub_groups;
   int top_n, top_ext;
   int groups, top_lds_ok, lds_nodes, top_wide, top_narrow, top_kids;
@@ -211,6 +215,9 @@ struct TreePlanDev {
   const int32_t* ftask_off;
   const int32_t* ftask_node;
   const int32_t* dyn0;
+  const int32_t* ltask_hoff;
+  const int32_t* ltask_off;
+  const int32_t* ltask_node;
   const int32_t* sub_order;
   const int32_t* sub_hoff;
   const int32_t* top_node;
@@ -602,10 +609,11 @@ NRT_HD void tps_final_down(const TreePlanDev& P, const TreeWork<T>& w, int v) {
 // the chain's arithmetic; the arithmetic is tps_final_up's, operand for operand.
 constexpr int kTaskBatch = 4;
 template <class T>
-NRT_HD void tps_task_up(const TreePlanDev& P, const TreeWork<T>& w, int t) {
+NRT_HD void tps_task_up(const TreePlanDev& P, const TreeWork<T>& w, const int32_t* toff,
+                        const int32_t* tnode, int t) {
   const int base = P.node_off[P.levels];
-  const int k0 = P.ftask_off[t], k1 = P.ftask_off[t + 1];
-  int v = P.ftask_node[k0];
+  const int k0 = toff[t], k1 = toff[t + 1];
+  int v = tnode[k0];
   T dv = w.d[base + v], rv = w.r[base + v];
   for (int k = P.fin_cptr[v]; k < P.fin_cptr[v + 1]; k++) {
     const int c = base + P.fin_child[k];
@@ -621,8 +629,8 @@ NRT_HD void tps_task_up(const TreePlanDev& P, const TreeWork<T>& w, int t) {
 #pragma unroll
     for (int j = 0; j < kTaskBatch; j++) {
       if (k + j < k1) {
-        const int c = base + P.ftask_node[k + j - 1];  // the node below: its only child
-        u[j] = base + P.ftask_node[k + j];
+        const int c = base + tnode[k + j - 1];  // the node below: its only child
+        u[j] = base + tnode[k + j];
         bd[j] = w.d[u[j]];
         br[j] = w.r[u[j]];
         bup[j] = w.up[c];
@@ -646,10 +654,11 @@ NRT_HD void tps_task_up(const TreePlanDev& P, const TreeWork<T>& w, int t) {
 // top node from its parent's solution (a task of a larger height), each node below
 // from the one above.
 template <class T>
-NRT_HD void tps_task_down(const TreePlanDev& P, const TreeWork<T>& w, int t) {
+NRT_HD void tps_task_down(const TreePlanDev& P, const TreeWork<T>& w, const int32_t* toff,
+                          const int32_t* tnode, int t) {
   const int base = P.node_off[P.levels];
-  const int k0 = P.ftask_off[t], k1 = P.ftask_off[t + 1];
-  const int v = P.ftask_node[k1 - 1];
+  const int k0 = toff[t], k1 = toff[t + 1];
+  const int v = tnode[k1 - 1];
   const int p = P.fin_par[v];
   T acc = w.r[base + v];
   if (p >= 0) acc = acc - w.up[base + v] * w.x[base + p];
@@ -661,7 +670,7 @@ NRT_HD void tps_task_down(const TreePlanDev& P, const TreeWork<T>& w, int t) {
 #pragma unroll
     for (int j = 0; j < kTaskBatch; j++) {
       if (k - j >= k0) {
-        u[j] = base + P.ftask_node[k - j];
+        u[j] = base + tnode[k - j];
         br[j] = w.r[u[j]];
         bup[j] = w.up[u[j]];
         bca[j] = w.ca[u[j]];

@@ -136,6 +136,53 @@ static void build_final_tasks(TreePlanHost& P) {
   }
 }
 
+// Chain tasks of each group's local final nodes (TreePlanHost::ltask_*): as
+// build_final_tasks, within a group (a local node's children are local nodes of
+// its group), the groups' task lists side by side.
+static void build_local_tasks(TreePlanHost& P, int G, int H) {
+  const int n = (int)P.fin_par.size();
+  std::vector<int32_t> task(n, -1);
+  std::vector<std::vector<int32_t>> gth(G);                // task heights, per group
+  std::vector<std::vector<std::vector<int32_t>>> gnodes(G);  // task nodes, per group
+  int hn = 0;
+  for (int g = 0; g < G; g++) {
+    auto& th = gth[g];
+    auto& nodes = gnodes[g];
+    for (int k = P.loc_hoff[(size_t)g * H]; k < P.loc_hoff[(size_t)g * H + H]; k++) {
+      const int u = P.loc_order[k];  // by height: children first
+      const int nc = P.fin_cptr[u + 1] - P.fin_cptr[u];
+      if (nc == 1) {
+        const int c = P.fin_child[P.fin_cptr[u]];
+        task[u] = task[c];
+        nodes[task[u]].push_back(u);
+      } else {
+        int h = 0;
+        for (int q = P.fin_cptr[u]; q < P.fin_cptr[u + 1]; q++)
+          h = std::max(h, th[task[P.fin_child[q]]] + 1);
+        task[u] = (int)nodes.size();
+        nodes.push_back({u});
+        th.push_back(h);
+        hn = std::max(hn, h + 1);
+      }
+    }
+  }
+  P.ltask_hn = hn;
+  P.ltask_hoff.assign((size_t)G * hn + 1, 0);
+  P.ltask_off.assign(1, 0);
+  P.ltask_node.clear();
+  for (int g = 0; g < G; g++)
+    for (int h = 0; h < hn; h++) {
+      P.ltask_hoff[(size_t)g * hn + h] = (int32_t)P.ltask_off.size() - 1;
+      for (size_t t = 0; t < gth[g].size(); t++) {
+        if (gth[g][t] != h) continue;
+        const auto& v = gnodes[g][t];
+        P.ltask_node.insert(P.ltask_node.end(), v.begin(), v.end());
+        P.ltask_off.push_back((int32_t)P.ltask_node.size());
+      }
+    }
+  P.ltask_hoff[(size_t)G * hn] = (int32_t)P.ltask_off.size() - 1;
+}
+
 TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int chunk_hi,
                              const std::vector<uint8_t>* dyn_rows,
                              const std::vector<int32_t>* groups, bool wide) {
@@ -365,6 +412,7 @@ TreePlanHost build_tree_plan(const std::vector<int32_t>& par0, int chunk, int ch
         const int u = P.fin_order[q];
         if (!mixed[u]) P.loc_order[lf[(size_t)fg[u] * H + h[u]]++] = u;
       }
+    build_local_tasks(P, G, H);
     stage_top_set(P, mixed, kTopLdsLocal);
     // group-resident indices
     const int L = P.levels;
@@ -458,6 +506,7 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
       &P.nslots,    &P.nchunk,    &P.node_off,  &P.next,      &P.chunk_off, &P.chunk_len,
       &P.chunk_dyn, &P.slot0,     &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
       &P.fin_cptr,  &P.fin_child, &P.ftask_hoff, &P.ftask_off, &P.ftask_node, &P.dyn0,
+      &P.ltask_hoff, &P.ltask_off, &P.ltask_node,
       &P.sub_order, &P.sub_hoff,  &P.top_node,  &P.top_hoff,
       &P.top_cptr,  &P.top_child, &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
       &P.lbase,     &P.lnext,     &P.lf_cptr,   &P.lf_child,  &P.lf_par,    &P.wide_off,
@@ -467,16 +516,17 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
 
 static TreePlanDev plan_from(const TreePlanHost& P, const int32_t* const* a) {
   return TreePlanDev{P.levels,     P.heights,    P.ftask_hn,  (int)P.dyn0.size(),
-                     P.fin_cut,    P.sub_groups, P.top_n,     P.top_ext,    P.groups,
-                     P.top_lds_ok, P.lds_nodes,  P.top_wide,  P.top_narrow, P.top_kids,
-                     P.gl_cols,    P.gl_fin,     P.gl_edges,  P.gl_wide,    a[0],
-                     a[1],         a[2],         a[3],        a[4],         a[5],
-                     a[6],         a[7],         a[8],        a[9],         a[10],
-                     a[11],        a[12],        a[13],       a[14],        a[15],
-                     a[16],        a[17],        a[18],       a[19],        a[20],
-                     a[21],        a[22],        a[23],       a[24],        a[25],
-                     a[26],        a[27],        a[28],       a[29],        a[30],
-                     a[31],        a[32],        a[33],       a[34]};
+                     P.ltask_hn,   P.fin_cut,    P.sub_groups, P.top_n,     P.top_ext,
+                     P.groups,     P.top_lds_ok, P.lds_nodes, P.top_wide,   P.top_narrow,
+                     P.top_kids,   P.gl_cols,    P.gl_fin,    P.gl_edges,   P.gl_wide,
+                     a[0],         a[1],         a[2],        a[3],         a[4],
+                     a[5],         a[6],         a[7],        a[8],         a[9],
+                     a[10],        a[11],        a[12],       a[13],        a[14],
+                     a[15],        a[16],        a[17],       a[18],        a[19],
+                     a[20],        a[21],        a[22],       a[23],        a[24],
+                     a[25],        a[26],        a[27],       a[28],        a[29],
+                     a[30],        a[31],        a[32],       a[33],        a[34],
+                     a[35],        a[36],        a[37]};
 }
 
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
@@ -859,6 +909,39 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
       for (int u = 0; u < n; u++)
         if (th[u] < 0) throw std::runtime_error("chain tasks: node never placed");
     }
+    if (local) {
+      // the same for each group's local chain tasks: exactly its local final nodes,
+      // a bottom node's children in its group at lower task heights
+      const int n = (int)H.fin_par.size(), G = H.groups, TH = H.ltask_hn, Hh = H.heights;
+      std::vector<int32_t> th(n, -1), grp(n, -1);
+      for (int g = 0; g < G; g++)
+        for (int k = H.loc_hoff[(size_t)g * Hh]; k < H.loc_hoff[(size_t)g * Hh + Hh]; k++)
+          grp[H.loc_order[k]] = g;
+      int placed = 0;
+      for (int g = 0; g < G; g++)
+        for (int h = 0; h < TH; h++)
+          for (int t = H.ltask_hoff[g * TH + h]; t < H.ltask_hoff[g * TH + h + 1]; t++)
+            for (int k = H.ltask_off[t]; k < H.ltask_off[t + 1]; k++) {
+              const int u = H.ltask_node[k];
+              if (grp[u] != g || th[u] >= 0)
+                throw std::runtime_error("local chain tasks: a node placed twice or in another group");
+              th[u] = h;
+              placed++;
+              const int nc = H.fin_cptr[u + 1] - H.fin_cptr[u];
+              if (k > H.ltask_off[t]) {
+                if (nc != 1 || H.fin_child[H.fin_cptr[u]] != H.ltask_node[k - 1])
+                  throw std::runtime_error("local chain tasks: a chain node without its one child");
+              } else {
+                for (int q = H.fin_cptr[u]; q < H.fin_cptr[u + 1]; q++) {
+                  const int c = H.fin_child[q];
+                  if (grp[c] != g || th[c] < 0 || th[c] >= h)
+                    throw std::runtime_error("local chain tasks: a child not below in its group");
+                }
+              }
+            }
+      if (placed != (int)H.loc_order.size())
+        throw std::runtime_error("local chain tasks: a local node never placed");
+    }
     // every plan array and every work array in an allocation of its own, of exactly
     // its length, so that a bounds-checking build of this hook catches an access
     // past any one of them (the device keeps them in one buffer each)
@@ -1004,7 +1087,7 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
         // local mode in the order of its launch: every group alone up to its local
         // final nodes (here last group first: no group may need another's work),
         // the top, then every group back down
-        const int G = H.groups, Hh = H.heights;
+        const int G = H.groups;
         for (int g = G - 1; g >= 0; g--) {
           for (int lv = 0; lv < H.levels; lv++) {
             const int* gc = H.gcol.data() + (size_t)lv * (G + 1);
@@ -1013,15 +1096,17 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
               else tps_reduce_chunk(P, w, lv, ch);
             }
           }
-          for (int h = 0; h < Hh; h++)
-            for (int k = H.loc_hoff[g * Hh + h]; k < H.loc_hoff[g * Hh + h + 1]; k++)
-              tps_final_up(P, w, H.loc_order[k]);
+          const int TH = H.ltask_hn;
+          for (int h = 0; h < TH; h++)
+            for (int t = H.ltask_hoff[g * TH + h]; t < H.ltask_hoff[g * TH + h + 1]; t++)
+              tps_task_up(P, w, P.ltask_off, P.ltask_node, t);
         }
         solve_top();
         for (int g = 0; g < G; g++) {
-          for (int h = Hh - 1; h >= 0; h--)
-            for (int k = H.loc_hoff[g * Hh + h]; k < H.loc_hoff[g * Hh + h + 1]; k++)
-              tps_final_down(P, w, H.loc_order[k]);
+          const int TH = H.ltask_hn;
+          for (int h = TH - 1; h >= 0; h--)
+            for (int t = H.ltask_hoff[g * TH + h]; t < H.ltask_hoff[g * TH + h + 1]; t++)
+              tps_task_down(P, w, P.ltask_off, P.ltask_node, t);
           for (int lv = H.levels - 1; lv >= 0; lv--) {
             const int* gc = H.gcol.data() + (size_t)lv * (G + 1);
             for (int ch = H.chunk_off[lv] + gc[g]; ch < H.chunk_off[lv] + gc[g + 1]; ch++) {
@@ -1053,9 +1138,11 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
       } else if (cut == 0 && H.ftask_hn > 0) {
         // unsplit: by chain tasks, task heights up, then down
         for (int h = 0; h < H.ftask_hn; h++)
-          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++) tps_task_up(P, w, t);
+          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++)
+            tps_task_up(P, w, P.ftask_off, P.ftask_node, t);
         for (int h = H.ftask_hn - 1; h >= 0; h--)
-          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++) tps_task_down(P, w, t);
+          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++)
+            tps_task_down(P, w, P.ftask_off, P.ftask_node, t);
       } else {
         for (int h = cut; h < H.heights; h++)
           for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)
