@@ -111,6 +111,15 @@ __device__ __forceinline__ void ready_s(X x) {
   asm volatile("" ::"s"(x));
 }
 
+// Element kinds of the explicit step (code >> 29; the low 29 bits are an index):
+// the flag bits kCodeHH, kCodeRec, kCodeSkip alone or HH with Skip keep their
+// meaning (chain element, HH chain element with its HH index, exception record,
+// terminal/ghost); the three unused combinations carry an element with one
+// exception and no record: a daughter (index = its mother), a branch point (index =
+// its left daughter), a source of the sealed end right after it (no index).
+constexpr uint32_t kExPlain = 0, kExSkip = 1, kExRecord = 2, kExBranch = 3, kExHH = 4,
+                   kExSkipHH = 5, kExDaughter = 6, kExCopyNext = 7;
+
 // Explicit step of element j of N: reads Vo, writes Vn[j] and its sealed-end copies.
 // A plain chain element (no record) couples to j - 1 and j + 1; its operands are
 // fetched with its code (ex_load), before the code is known, so a thread waits for
@@ -177,27 +186,35 @@ __device__ __forceinline__ void ex_finish_on(int j, const ExOps& o, VO Vo, VN Vn
                                              const ExRec* __restrict__ recs_, const HHDev& hd,
                                              double dt_ms, bool active) {
   const NRT_GLB double* __restrict__ rdx = glb(rdx_);
-  const uint32_t c = o.c;
+  const uint32_t c = o.c, kind = c >> 29, idx = c & kCodeIdx;
   const double v = o.v;
-  if (c & kCodeSkip) {
+  if (kind == kExSkip || kind == kExSkipHH) {
     // Terminal / ghost: its potential is a sealed-end copy written by its source,
     // but an HH terminal still advances its gates (solverEx-cpu.cpp:62).
-    if (c & kCodeHH) {
+    if (kind == kExSkipHH) {
       double wd, kd;
-      hh_gate(hd, (int)(c & kCodeIdx), v, dt_ms, wd, kd);
+      hh_gate(hd, (int)idx, v, dt_ms, wd, kd);
     }
     return;
   }
-  if (!(c & kCodeRec)) {
+  if (kind != kExRecord) {
+    // A chain element, or one whose single exception sits in its code: a daughter
+    // (its mother elsewhere), a branch point (its left daughter), the source of
+    // the sealed end right after it. One load waits on the code at most, none on
+    // a record; the operations are the record path's, in its order.
     double wd = o.w0, kd = o.k0;
-    if (c & kCodeHH) hh_gate(hd, (int)(c & kCodeIdx), v, dt_ms, wd, kd);
-    // DR term + mother term + W*dX*V + K*dX
+    if (kind == kExHH) hh_gate(hd, (int)idx, v, dt_ms, wd, kd);
+    const double vm = kind == kExDaughter ? Vo[idx] : o.vm;
+    // (DL term +) DR term + mother term + W*dX*V + K*dX
     double s = (o.vp - v) / o.rp;
-    s = s + (o.vm - v) / o.rj;
+    if (kind == kExBranch) s = (Vo[idx] - v) / rdx[idx] + s;
+    s = s + (vm - v) / o.rj;
     s = s + wd * v;
     s = s + kd;
-    if constexpr (NT) __builtin_nontemporal_store(v + o.aj * s, &Vn[j]);
-    else Vn[j] = v + o.aj * s;
+    const double vn = v + o.aj * s;
+    if constexpr (NT) __builtin_nontemporal_store(vn, &Vn[j]);
+    else Vn[j] = vn;
+    if (kind == kExCopyNext) Vn[j + 1] = vn;
     return;
   }
   const NRT_GLB ExRec& rec = glb(recs_)[c & kCodeIdx];
@@ -327,6 +344,8 @@ Explicit* explicit_create(nrt_handle* hnd, const HostProblem& p) {
       NRT_REQUIRE(tg.size() <= 3, NRT_ERR_TOPOLOGY,
                   "more than three sealed ends copy element " + std::to_string(src));
     }
+    NRT_REQUIRE((uint32_t)N <= kCodeIdx, NRT_ERR_INVALID,
+                "more elements than an element code can index");
     std::vector<uint32_t> code(N, 0);
     std::vector<ExRec> recs;
     std::vector<double> coefA(N, 0.0), rdx(N), wdx(N, 0.0), kdx(N, 0.0);
@@ -351,8 +370,24 @@ Explicit* explicit_create(nrt_handle* hnd, const HostProblem& p) {
       const bool plain = mo == j - 1 && dr == j + 1 && !p.branching[j] && p.input[j] == 0.0 &&
                          targets[j].empty();
       if (plain) {
-        code[j] = hh >= 0 ? (kCodeHH | (uint32_t)hh) : 0u;
+        code[j] = hh >= 0 ? (kCodeHH | (uint32_t)hh) : kExPlain << 29;
         continue;
+      }
+      // one exception, passive, no stimulus: in the code, no record
+      if (hh < 0 && p.input[j] == 0.0 && dr == j + 1) {
+        const bool br = p.branching[j] != 0, own = mo == j - 1;
+        if (!br && !own && targets[j].empty()) {
+          code[j] = (kExDaughter << 29) | (uint32_t)mo;
+          continue;
+        }
+        if (br && own && targets[j].empty()) {
+          code[j] = (kExBranch << 29) | (uint32_t)dl;
+          continue;
+        }
+        if (!br && own && targets[j].size() == 1 && targets[j][0] == j + 1) {
+          code[j] = kExCopyNext << 29;
+          continue;
+        }
       }
       ExRec r{};
       r.mother = mo;
