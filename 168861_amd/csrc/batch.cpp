@@ -33,8 +33,8 @@ size_t explicit_slot_bytes();
 void explicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long long a0,
                         long long a1, int trace_elem, long long interval, double* trace,
                         void* out);
-void explicit_batch_step(const void* slots, const int2* blk, int nblk, int nslots,
-                         long long* gstep, hipStream_t s);
+void explicit_batch_steps(const void* slots, const int2* blk, int nblk, int nslots,
+                          const long long* gbase, int k, hipStream_t s);
 const double* explicit_slot_v(nrt_handle* h, long long done);
 int64_t explicit_device_bytes(const Explicit* e);
 int explicit_batch_block_elems();
@@ -106,7 +106,8 @@ struct nrt_batch {
   std::vector<int2> wgmap_h;
   std::vector<int> vn;  // workgroups per table entry in the launch being built
   std::map<int, int> capacity;  // resident workgroups per workgroup size
-  DBuf<long long> gdev;
+  DBuf<long long> gdev;  // explicit: the first batch step of the call being replayed
+  long long gbase_h = 0;  // its host source (kept until the launch is finished)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   long long graph_k = 0;
@@ -193,7 +194,6 @@ struct nrt_batch {
       blk.upload(bl, stream);
       nblk = (int)bl.size();
       if (!gdev.p) gdev.alloc(1);
-      NRT_HIP(hipMemcpyAsync(gdev.p, &g, sizeof(long long), hipMemcpyHostToDevice, stream));
     }
     NRT_HIP(hipStreamSynchronize(stream));
     dirty = false;
@@ -305,15 +305,19 @@ struct nrt_batch {
         if (pend_cls[c].on) NRT_HIP(hipStreamWaitEvent(stream, cev[c][1], 0));
       NRT_HIP(hipEventRecord(ev1, stream));
     } else {
+      NRT_REQUIRE(k <= (1ll << 30), NRT_ERR_INVALID, "k too large for one call");
       if (!gexec || graph_k != k) {
         drop_graph();
         NRT_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (long long i = 0; i < k; i++)
-          explicit_batch_step(table.p, blk.p, nblk, (int)order.size(), gdev.p, stream);
+        explicit_batch_steps(table.p, blk.p, nblk, (int)order.size(), gdev.p, (int)k, stream);
         NRT_HIP(hipStreamEndCapture(stream, &graph));
         NRT_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         graph_k = k;
       }
+      // the first batch step of this call (g already counts its k steps); the graph's
+      // launches take it from device memory, set here before each replay
+      gbase_h = g - k;
+      NRT_HIP(hipMemcpyAsync(gdev.p, &gbase_h, sizeof(long long), hipMemcpyHostToDevice, stream));
       NRT_HIP(hipEventRecord(ev0, stream));
       NRT_HIP(hipGraphLaunch(gexec, stream));
       NRT_HIP(hipEventRecord(ev1, stream));

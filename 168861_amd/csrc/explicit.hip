@@ -546,14 +546,33 @@ struct ExSlot {
   double* trace;
 };
 
+// Probe samples of the slots whose trace is due after `done` of their steps
+// (done = batch steps completed - s0), from the buffer holding that step's V.
+__device__ __forceinline__ void ex_batch_probes(const ExSlot* __restrict__ slots, int nslots,
+                                                long long steps_completed, int t0, int nt) {
+  for (int si = t0; si < nslots; si += nt) {
+    const ExSlot& S = slots[si];
+    const long long done = steps_completed - S.s0;
+    if (S.interval < 0 || !S.trace || done < 1 || done > S.num_steps) continue;
+    if (done % (S.interval + 1) != 0) continue;
+    const int cur = (S.cur0 + (int)(done & 1)) & 1;
+    S.trace[done / (S.interval + 1) - 1] = S.V[cur][S.trace_elem];
+  }
+}
+
+// Batch step gbase[0] + i. Block 0 also records the probes due after the step
+// before (whose V this step only reads), so a step is one launch; the probes of a
+// call's last step are recorded by k_explicit_batch_trace after it.
 __global__ __launch_bounds__(256) void k_explicit_batch(const ExSlot* __restrict__ slots,
                                                         const int2* __restrict__ blk,
-                                                        const long long* __restrict__ gstep) {
+                                                        const long long* __restrict__ gbase,
+                                                        int i, int nslots) {
   const int2 b = blk[blockIdx.x];
   const ExSlot& S = slots[b.x];
   // the slot's fields are read before the one exit test (not short-circuit), so the
   // scalar loads are in flight together
-  const long long s0 = S.s0, nst = S.num_steps, g = *gstep, a0 = S.a0, a1 = S.a1;
+  const long long s0 = S.s0, nst = S.num_steps, g = *gbase + i, a0 = S.a0, a1 = S.a1;
+  if (blockIdx.x == 0 && i > 0) ex_batch_probes(slots, nslots, g, (int)threadIdx.x, 256);
   const int N = S.N, cur0 = S.cur0;
   const double* V0 = S.V[0];
   const double* V1 = S.V[1];
@@ -597,20 +616,12 @@ __global__ __launch_bounds__(256) void k_explicit_batch(const ExSlot* __restrict
 
 int explicit_batch_block_elems() { return 256 * kExEPT; }
 
-// After a step: the probe samples due at it, then the step counter advances.
+// After a call's last step (batch step gbase[0] + k - 1): its probe samples.
 __global__ void k_explicit_batch_trace(const ExSlot* __restrict__ slots, int nslots,
-                                       const long long* __restrict__ gstep) {
-  const int si = blockIdx.x * blockDim.x + threadIdx.x;
-  if (si >= nslots) return;
-  const ExSlot& S = slots[si];
-  const long long done = *gstep - S.s0 + 1;  // steps done including this one
-  if (S.interval < 0 || !S.trace || done < 1 || done > S.num_steps) return;
-  if (done % (S.interval + 1) != 0) return;
-  const int cur = (S.cur0 + (int)(done & 1)) & 1;
-  S.trace[done / (S.interval + 1) - 1] = S.V[cur][S.trace_elem];
+                                       const long long* __restrict__ gbase, int k) {
+  ex_batch_probes(slots, nslots, *gbase + k, blockIdx.x * blockDim.x + threadIdx.x,
+                  gridDim.x * blockDim.x);
 }
-
-__global__ void k_batch_advance(long long* gstep) { *gstep = *gstep + 1; }
 
 size_t explicit_slot_bytes() { return sizeof(ExSlot); }
 
@@ -641,15 +652,17 @@ void explicit_slot_fill(nrt_handle* h, long long s0, long long num_steps, long l
   std::memcpy(out, &S, sizeof(S));
 }
 
-// One batch step: the element kernel over `nblk` blocks, the probes, the counter.
-void explicit_batch_step(const void* slots, const int2* blk, int nblk, int nslots,
-                         long long* gstep, hipStream_t s) {
+// k batch steps from batch step gbase[0]: one launch of the element kernel per step
+// over `nblk` blocks, then the last step's probes.
+void explicit_batch_steps(const void* slots, const int2* blk, int nblk, int nslots,
+                          const long long* gbase, int k, hipStream_t s) {
   const auto* S = (const ExSlot*)slots;
-  if (nblk > 0) hipLaunchKernelGGL(k_explicit_batch, dim3(nblk), dim3(256), 0, s, S, blk, gstep);
-  if (nslots > 0)
+  for (int i = 0; i < k; i++)
+    if (nblk > 0)
+      hipLaunchKernelGGL(k_explicit_batch, dim3(nblk), dim3(256), 0, s, S, blk, gbase, i, nslots);
+  if (nslots > 0 && k > 0)
     hipLaunchKernelGGL(k_explicit_batch_trace, dim3(grid_for(nslots, 256)), dim3(256), 0, s, S,
-                       nslots, gstep);
-  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, s, gstep);
+                       nslots, gbase, k);
   NRT_HIP(hipGetLastError());
 }
 
