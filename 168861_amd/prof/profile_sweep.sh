@@ -7,6 +7,8 @@
 #   write    WRITE_SIZE;
 #   dram     TCC_EA0_RDREQ_DRAM_32B_sum and TCC_EA0_WRREQ_WRITE_DRAM_32B_sum (HBM-side
 #            read and write requests in 32-byte units: a cross-check of the two above);
+#   sq       wave-state counters (SQ_WAVE_CYCLES, SQ_WAIT_ANY, SQ_WAIT_INST_ANY,
+#            SQ_ACTIVE_INST_ANY, SQ_ACTIVE_INST_VALU, ...): where the waves' cycles go;
 #   calib    the three counter runs on copy kernels of known size (8- and 16-byte lanes);
 #   summary  summarize.py reduce: what this box's runs left, reduced to OUT.json
 #            (runs on separate boxes are combined by summarize.py merge).
@@ -27,6 +29,7 @@ mkdir -p "$RAW"
 SHAPES=${SHAPES:-$RAW/shapes.json}
 LOG=${PROF_LOG:-/dev/null}
 DRAM="TCC_EA0_RDREQ_DRAM_32B_sum TCC_EA0_WRREQ_WRITE_DRAM_32B_sum"
+SQ="SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_VALU SQ_INSTS_VALU"
 B="python3 bench.py --legs sweep --steps $STEPS --warmup 5 --no-oracle --no-cpu-baseline --shapes $SHAPES"
 for ph in "$@"; do
   case $ph in
@@ -42,6 +45,10 @@ for ph in "$@"; do
     dram)
       echo "[prof] DRAM request counters: $B"
       timeout -s KILL 900 rocprofv3 --pmc $DRAM --output-format csv -d "$RAW/dram" -o sweep -- $B > /dev/null 2>> "$LOG" || exit $?
+      ;;
+    sq)
+      echo "[prof] wave-state counters: $B"
+      timeout -s KILL 900 rocprofv3 --pmc $SQ --output-format csv -d "$RAW/sq" -o sweep -- $B > /dev/null 2>> "$LOG" || exit $?
       ;;
     calib)
       echo "[prof] copy calibration"

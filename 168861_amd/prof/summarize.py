@@ -12,6 +12,9 @@ of them in the sweep), running concurrently (a "group"). So:
     group: dispatches overlapping in time), their span (first start to last end),
     and the counters of their N / S * C dispatches (the last ones of that type);
   * k_explicit_batch: the last N dispatches (one per step).
+Wave states: SQ_WAVE_CYCLES and the cycles waves spent waiting (SQ_WAIT_ANY: on a
+memory, LDS or barrier dependency; SQ_WAIT_INST_ANY: for an instruction to issue) or
+issuing (SQ_ACTIVE_INST_*), summed over the same dispatches.
 Counters: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), the HBM-side request
 counters TCC_EA0_RDREQ_DRAM_32B_sum and TCC_EA0_WRREQ_WRITE_DRAM_32B_sum (32-byte
 units), and the same counters on the copy kernels of copy_calib, which move a known
@@ -54,17 +57,41 @@ def kname(r):
 UNITS = {"FETCH_SIZE": ("fetch", 1024), "WRITE_SIZE": ("write", 1024),
          "TCC_EA0_RDREQ_DRAM_32B_sum": ("dram", 32),
          "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum": ("dram", 32)}
+# wave-state counters (counts, summed over the timed dispatches)
+SQ = ["SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY",
+      "SQ_ACTIVE_INST_ANY", "SQ_ACTIVE_INST_VALU", "SQ_INSTS_VALU"]
+
+
+_loaded = {}
+
+
+def _run(raw, tag):
+    """Counter name -> [(dispatch id, kernel name, summed value)], one pass per run."""
+    key = (raw, tag)
+    if key not in _loaded:
+        acc = {}
+        for f in sorted(glob.glob(os.path.join(raw, tag, "**", "*counter_collection.csv"),
+                                  recursive=True)):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    k = (r["Counter_Name"], int(r.get("Dispatch_Id", 0)))
+                    if k in acc:
+                        acc[k][1] += float(r["Counter_Value"])
+                    else:
+                        acc[k] = [kname(r), float(r["Counter_Value"])]
+        per = {}
+        for (c, d), (n, v) in acc.items():
+            per.setdefault(c, []).append((d, n, v))
+        for v in per.values():
+            v.sort()
+        _loaded[key] = per
+    return _loaded[key]
 
 
 def counters(raw, tag, counter):
-    """Per dispatch, in dispatch order: (kernel name, value in bytes)."""
-    out = []
-    unit = UNITS[counter][1]
-    for r in rows(os.path.join(raw, tag, "**", "*counter_collection.csv")):
-        if r["Counter_Name"] == counter:
-            out.append((int(r.get("Dispatch_Id", 0)), kname(r), float(r["Counter_Value"]) * unit))
-    out.sort()
-    return [(n, v) for _, n, v in out]
+    """Per dispatch, in dispatch order: (kernel name, value in bytes, or a count)."""
+    unit = UNITS[counter][1] if counter in UNITS else 1
+    return [(n, v * unit) for _, n, v in _run(raw, tag).get(counter, [])]
 
 
 def groups(dispatches):
@@ -110,6 +137,17 @@ def reduce(raw, out, timed_steps=40, per_launch=10, classes=3):
             if mv:
                 k[counter + "_bytes_per_launch"] = sum(mv) / (take if explicit else nlaunch)
                 k["steps_per_launch"] = 1 if explicit else per_launch
+        sq = {}
+        for counter in SQ:
+            mv = [v for n, v in counters(raw, "sq", counter) if match(n)][-take:]
+            if mv:
+                sq[counter] = sum(mv)
+        if sq.get("SQ_WAVE_CYCLES"):
+            wc = sq["SQ_WAVE_CYCLES"]
+            sq["share_of_wave_cycles"] = {c: sq[c] / wc for c in (
+                "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_ANY", "SQ_ACTIVE_INST_VALU")
+                if c in sq}
+            k["wave_states"] = sq
         if k:
             res["kernels"][label] = k
     cal = {}
