@@ -134,8 +134,9 @@ def reduce(raw, out, timed_steps=40, per_launch=10, classes=3):
         take = timed_steps if explicit else nlaunch * classes
         for counter, (tag, _) in UNITS.items():
             mv = [v for n, v in counters(raw, tag, counter) if match(n)][-take:]
-            if mv:
-                k[counter + "_bytes_per_launch"] = sum(mv) / (take if explicit else nlaunch)
+            if mv:  # per launch: one dispatch (explicit) or a group of `classes`
+                k[counter + "_bytes_per_launch"] = sum(mv) / (len(mv) if explicit else
+                                                              len(mv) / classes)
                 k["steps_per_launch"] = 1 if explicit else per_launch
         sq = {}
         for counter in SQ:
