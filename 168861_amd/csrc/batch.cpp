@@ -119,6 +119,9 @@ struct nrt_batch {
   // record (threads, ms per compartment-step), the launch in flight and its events
   int cthreads[kClasses] = {0, 0, 0};
   std::vector<std::pair<int, double>> ctune[kClasses];
+  // a class's first launch also fills and factors its slots (later launches reuse
+  // the factors), so it is not a timing sample
+  bool cwarm[kClasses] = {false, false, false};
   // implicit, set by nrt_batch_tune: chunk lengths new slots of a class are planned
   // with (0: the options' value / auto rule)
   int cchunk[kClasses] = {0, 0, 0}, cchunk_hi[kClasses] = {0, 0, 0};
@@ -389,7 +392,9 @@ struct nrt_batch {
         float mc = 0;  // this class's launch (overlapping the others)
         NRT_HIP(hipEventElapsedTime(&mc, cev[c][0], cev[c][1]));
         pend_cls[c].on = false;
-        if (opt.batch_threads <= 0 && cthreads[c] == 0 && pend_cls[c].cs > 0) {
+        if (opt.batch_threads <= 0 && cthreads[c] == 0 && pend_cls[c].cs > 0 && !cwarm[c]) {
+          cwarm[c] = true;
+        } else if (opt.batch_threads <= 0 && cthreads[c] == 0 && pend_cls[c].cs > 0) {
           ctune[c].emplace_back(pend_cls[c].th, mc / (double)pend_cls[c].cs);
           if ((int)ctune[c].size() == kTuneCount)
             cthreads[c] = std::min_element(ctune[c].begin(), ctune[c].end(), [](auto& a, auto& b) {

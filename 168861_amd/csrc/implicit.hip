@@ -25,7 +25,10 @@
 // input, code (uint32); HH SoA as in the explicit step plus row type/position/DL.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 #include <type_traits>
 #include "nrt_internal.hpp"
 #include "nrt_device.hpp"
@@ -126,6 +129,11 @@ struct ImpDev {
   const int32_t* scol;      // column slots
   const uint8_t* srole;     // 0 diagonal, 1 coupling to the parent, 2 to a child
   const int32_t* hh_slot;   // slot of each HH entry's row
+  // the tree solve's work arrays already hold this plan's matrix and factors (a
+  // batch slot after its first launch): the launch skips the fill and the full
+  // factorization (static chunks keep theirs between launches, dynamic ones are
+  // eliminated again by every step)
+  int keep_factors;
 };
 
 struct Implicit {
@@ -1081,6 +1089,8 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev
 // grid mode (large trees): every workgroup takes a share of the rows and chunks,
 // grid barriers separate the levels, and workgroup 0 solves the compressed forest.
 constexpr int kTreeBlock = 1024;
+// Largest staged final forest (dynamic LDS of one workgroup, block launches).
+constexpr size_t kForestLdsMax = 64 * 1024;
 constexpr int kTreeArrays = 10;  // d, up, dn, r, x, ca, cb, cg, cf, ci
 
 template <class T>
@@ -1091,7 +1101,8 @@ struct TreeArgs {
   int krylov;   // BiCGSTAB preconditioned by this tree solve (else direct / refinement)
   double refine_tol;
   int refine_max;
-  int top_lds;  // the launch provides top_lds_bytes<T>() of dynamic LDS (grid launches)
+  int top_lds;  // the launch provides dynamic LDS: top_lds_bytes<T>() (grid launches) or
+                // forest_lds_bytes<T>(P) (block launches)
   const int32_t* hh_goff;  // local mode: HH entries of group g are [hh_goff[g], +1)
   TreeWork<T> walt;        // group-resident form: the second top-input buffer
 };
@@ -1170,6 +1181,25 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
       }
       for (int j = threadIdx.x; j < P.top_n; j += blockDim.x) top_store(P, w, L, j);
     }
+  } else if (!GRID && tl != nullptr && P.ftask_hn > 0) {
+    // one workgroup with room for the whole forest (tl holds forest_lds_bytes):
+    // staged in LDS at task positions, a workgroup barrier per task height
+    const ForestLds<T> L = ForestLds<T>::carve(tl, P);
+    const int ns = forest_stage_count(P);
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) forest_stage(P, w, L, j);
+    __syncthreads();
+    for (int h = 0; h < P.ftask_hn; h++) {
+      for (int t = L.thoff[h] + (int)threadIdx.x; t < L.thoff[h + 1]; t += blockDim.x)
+        forest_up(L, t);
+      __syncthreads();
+    }
+    for (int h = P.ftask_hn - 1; h >= 0; h--) {
+      for (int t = L.thoff[h] + (int)threadIdx.x; t < L.thoff[h + 1]; t += blockDim.x)
+        forest_down(L, t);
+      __syncthreads();
+    }
+    for (int j = threadIdx.x; j < P.fin_n; j += blockDim.x) forest_store(P, w, L, j);
+    __syncthreads();
   } else if ((!GRID || d.vb == 0) && cut == 0 && P.ftask_hn > 0) {
     // unsplit: chain tasks, a workgroup barrier per task height
     for (int h = 0; h < P.ftask_hn; h++) {
@@ -1509,17 +1539,19 @@ __device__ __forceinline__ void tree_steps(const ImpDev& d, const TreeArgs<T>& t
   // Level-0 tree system from the matrix. Off-diagonals never change; the diagonal
   // of HH rows is refreshed by the update phase of every step, by the thread that
   // owns the HH entry (not the slot), hence the barrier before the first update.
-  for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
-    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-      const T v = (T)val[k];
-      const int role = d.srole[k];
-      if (role == 0) t.w.d[i] = v;
-      else if (role == 1) t.w.up[i] = v;
-      else t.w.dn[col[k]] = v;
+  if (!d.keep_factors) {
+    for (int i = r_begin + (int)threadIdx.x; i < r_end; i += bs) {
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const T v = (T)val[k];
+        const int role = d.srole[k];
+        if (role == 0) t.w.d[i] = v;
+        else if (role == 1) t.w.up[i] = v;
+        else t.w.dn[col[k]] = v;
+      }
     }
+    tsync<GRID>(d);
+    tps_factor<T, GRID>(d, t.P, t.w);
   }
-  tsync<GRID>(d);
-  tps_factor<T, GRID>(d, t.P, t.w);
   for (long long step = d.first; step < d.first + d.num; step++) {
     const bool active = step >= d.a0 && step <= d.a1;
     tree_update<T, GRID>(d, t, r_begin, r_end, active);
@@ -2165,10 +2197,6 @@ struct SlotWork {
   int vn, pad;
 };
 
-// The slot record (descriptor pointers and constants, uniform over the workgroup)
-// is staged once in LDS and the solve reads its fields from there: a per-lane copy
-// would live in scratch (the record is over a kilobyte), and with a dozen waves per
-// CU that scratch outgrows the caches and its reloads go to HBM.
 // Per-workgroup slot records of a launch: workgroup w of the step kernel runs
 // wgmap[w] = (table entry, index in its virtual grid), and its record is that
 // entry's with this launch's steps, trace position and grid filled in (num = 0:
@@ -2199,17 +2227,27 @@ __global__ __launch_bounds__(64) void k_batch_expand(const BatchSlot<T>* __restr
     }
     d.vb = m.y;
     d.nblocks = wk.vn;
+    d.keep_factors = wk.done > 0;  // an earlier launch filled and factored the arrays
     if (wk.vn > 1) d.rpb = (d.ns0 + wk.vn - 1) / wk.vn;
   }
 }
 
+// forest_lds: bytes of dynamic LDS the launch gives each workgroup; a one-workgroup
+// slot whose final forest fits there solves it in LDS.
 template <class T, int BS, int MODE>
-__global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __restrict__ wgrec) {
+__global__ __launch_bounds__(BS, 3) void k_batch_tree(const BatchSlot<T>* __restrict__ wgrec,
+                                                      int forest_lds) {
   __shared__ double lds[4 * 16];
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
   const BatchSlot<T>& rec = wgrec[blockIdx.x];
   if (rec.d.num <= 0) return;
-  if (rec.d.nblocks > 1) tree_steps<T, true, MODE>(rec.d, rec.t, lds, nullptr);
-  else tree_steps<T, false, MODE>(rec.d, rec.t, lds, nullptr);
+  if (rec.d.nblocks > 1) {
+    tree_steps<T, true, MODE>(rec.d, rec.t, lds, nullptr);
+  } else {
+    const size_t fb = forest_lds_bytes<T>(rec.t.P);
+    tree_steps<T, false, MODE>(rec.d, rec.t, lds,
+                               fb > 0 && fb <= (size_t)forest_lds ? dyn_lds : nullptr);
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -2715,7 +2753,10 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b, T* alt = n
                      o.refine_tol,
                      o.refine_max,
                      (e->grid_mode && e->plan_h.top_n > 0) ||
-                             (e->local_mode && e->plan_h.top_lds_ok)
+                             (e->local_mode && e->plan_h.top_lds_ok) ||
+                             (!e->grid_mode && !e->local_mode &&
+                              forest_lds_bytes<T>(e->plan) > 0 &&
+                              forest_lds_bytes<T>(e->plan) <= kForestLdsMax)
                          ? 1
                          : 0,
                      e->local_mode ? e->hh_goff.p : nullptr,
@@ -2778,7 +2819,8 @@ static int launch_persistent(nrt_handle* hnd, int64_t first, int64_t num, int64_
       NRT_HIP(hipLaunchKernel(kgrid, dim3(e->nblocks), dim3(kImpBlock), args, shmem,
                                          hnd->stream));
     else
-      NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args, 0, hnd->stream));
+      NRT_HIP(hipLaunchKernel(kblock, dim3(1), dim3(kTreeBlock), args,
+                              targs.top_lds ? forest_lds_bytes<TT>(targs.P) : 0, hnd->stream));
   };
   if (e->method == 1 || e->method == 3)
     launch(tree_args<double>(e, o, e->tw64.p, e->tw64alt.p), (const void*)k_implicit_tree<double, true>,
@@ -2997,6 +3039,44 @@ static const void* batch_kernel(int precision, int krylov, int threads) {
                               : batch_kernel_of<float, kSolveRefine>(threads);
 }
 
+// Dynamic LDS a batch launch gives each workgroup for a staged final forest: as much
+// as leaves the kernel's occupancy (set by its registers) unchanged, at most
+// kForestLdsMax. Per device and kernel, computed once.
+static size_t batch_forest_lds(int precision, int krylov, int threads) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, size_t> known;
+  int dev = 0;
+  NRT_HIP(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, precision, batch_mode(precision, krylov), threads);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = known.find(key);
+    if (it != known.end()) return it->second;
+  }
+  const void* k = batch_kernel(precision, krylov, threads);
+  int per0 = 0, per_cu_lds = 0;
+  NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per0, k, threads, 0));
+  NRT_HIP(hipDeviceGetAttribute(&per_cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                dev));
+  hipFuncAttributes fa{};
+  NRT_HIP(hipFuncGetAttributes(&fa, k));
+  size_t cap = 0;
+  if (per0 > 0 && per_cu_lds > 0) {
+    const size_t share = (size_t)per_cu_lds / (size_t)per0;
+    cap = share > fa.sharedSizeBytes ? share - fa.sharedSizeBytes : 0;
+    cap = std::min(cap, kForestLdsMax) & ~(size_t)1023;
+    int per = 0;
+    while (cap > 0) {
+      NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, cap));
+      if (per >= per0) break;
+      cap -= 1024;
+    }
+  }
+  std::lock_guard<std::mutex> g(mu);
+  known[key] = cap;
+  return cap;
+}
+
 // Workgroups of the batched kernel that can be resident at once (an otherwise
 // idle device): virtual grids of several workgroups are only formed within it.
 int implicit_batch_capacity(int precision, int krylov, int threads) {
@@ -3004,7 +3084,8 @@ int implicit_batch_capacity(int precision, int krylov, int threads) {
   NRT_HIP(hipGetDevice(&dev));
   NRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per, batch_kernel(precision, krylov, threads), threads, 0));
+      &per, batch_kernel(precision, krylov, threads), threads,
+      batch_forest_lds(precision, krylov, threads)));
   return cus * std::max(per, 0);
 }
 
@@ -3028,9 +3109,10 @@ void implicit_batch_expand(int precision, const void* table, const void* work,
 void implicit_batch_launch(int precision, int krylov, int threads, const void* wgrec, int nwg,
                            hipStream_t s) {
   if (nwg <= 0) return;
-  void* args[] = {(void*)&wgrec};
+  int lds = (int)batch_forest_lds(precision, krylov, threads);
+  void* args[] = {(void*)&wgrec, (void*)&lds};
   NRT_HIP(hipLaunchKernel(batch_kernel(precision, krylov, threads), dim3(nwg), dim3(threads), args,
-                          0, s));
+                          (size_t)lds, s));
 }
 
 // Steps first..first+num-1 enqueued on the handle's stream without waiting; the

@@ -63,6 +63,11 @@ struct TreePlanHost {
   std::vector<int32_t> ftask_hoff;  // task-height boundaries into tasks, ftask_hn+1
   std::vector<int32_t> ftask_off;   // task t: nodes [ftask_off[t], ftask_off[t+1])
   std::vector<int32_t> ftask_node;  // ... of ftask_node, bottom to top
+  // The same tasks in task positions (index k into ftask_node) for a workgroup that
+  // stages the whole forest in LDS at those positions: the position of the parent
+  // of each task's top node (-1: a root), and the positions of the children of its
+  // bottom node, [ftask_cptr[t], ftask_cptr[t+1]) of ftask_child, in fin_child order.
+  std::vector<int32_t> ftask_par, ftask_cptr, ftask_child;
   // Local mode: the chain tasks of each group's local final nodes, group g's tasks
   // of task height h being [ltask_hoff[g*ltask_hn+h], +1) of ltask_off / ltask_node.
   int ltask_hn = 0;
@@ -175,7 +180,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
 // exactly its length: an access past the end of any one array leaves its
 // allocation (host emulation under a bounds-checking build).
 TreePlanDev plan_view_parts(const TreePlanHost& P, std::vector<std::vector<int32_t>>& store);
-constexpr int kPlanParts = 38;
+constexpr int kPlanParts = 41;
 
 // The matrix with its rows in level-0 slot order (padding slots are empty rows):
 // columns as slots, each entry's role (0 diagonal, 1 coupling to the parent, 2 to
@@ -193,6 +198,7 @@ SlotCsr slot_order_csr(const TreePlanHost& P, int N, const int32_t* rowptr, cons
 // Device view of a plan (pointers into one device allocation).
 struct TreePlanDev {
   int levels, heights, ftask_hn, ndyn0, ltask_hn;
+  int fin_n, ftask_n, ftask_nc;  // final-forest nodes, chain tasks, bottom-node children
   int fin_cut, sub_groups;
   int top_n, top_ext;
   int groups, top_lds_ok, lds_nodes, top_wide, top_narrow, top_kids;
@@ -214,6 +220,9 @@ struct TreePlanDev {
   const int32_t* ftask_hoff;
   const int32_t* ftask_off;
   const int32_t* ftask_node;
+  const int32_t* ftask_par;
+  const int32_t* ftask_cptr;
+  const int32_t* ftask_child;
   const int32_t* dyn0;
   const int32_t* ltask_hoff;
   const int32_t* ltask_off;
@@ -685,6 +694,119 @@ NRT_HD void tps_task_down(const TreePlanDev& P, const TreeWork<T>& w, const int3
       }
     }
   }
+}
+
+// The whole final forest staged in LDS by the one workgroup that solves it (block
+// launches), at task positions: one load of each node's d, r, up, dn and of the
+// task lists, every task height up and down in LDS with a workgroup barrier each,
+// one store of x. The arithmetic is tps_task_up/down's, operand for operand, so
+// the solution is the same to the bit. The eliminated pivot overwrites d and the
+// solution overwrites r (neither is needed again), and the global d, r and ca of
+// the final nodes are left as they were.
+template <class T>
+struct ForestLds {
+  T* d;
+  T* r;
+  T* up;
+  T* dn;
+  int32_t* node;   // final-local node of each position
+  int32_t* toff;   // task t: positions [toff[t], toff[t+1])
+  int32_t* cptr;   // children of task t's bottom node: [cptr[t], cptr[t+1]) of child
+  int32_t* child;
+  int32_t* par;    // position of the parent of task t's top node, or -1
+  int32_t* thoff;  // task height h: tasks [thoff[h], thoff[h+1])
+  NRT_HD static ForestLds carve(void* p, const TreePlanDev& P) {
+    const int n = P.fin_n, nt = P.ftask_n;
+    ForestLds L;
+    L.d = (T*)p;
+    L.r = L.d + n;
+    L.up = L.r + n;
+    L.dn = L.up + n;
+    L.node = (int32_t*)(L.dn + n);
+    L.toff = L.node + n;
+    L.cptr = L.toff + nt + 1;
+    L.child = L.cptr + nt + 1;
+    L.par = L.child + P.ftask_nc;
+    L.thoff = L.par + nt;
+    return L;
+  }
+};
+// LDS bytes of a staged final forest (0: the plan has no chain tasks).
+template <class T>
+NRT_HD size_t forest_lds_bytes(const TreePlanDev& P) {
+  if (P.ftask_hn <= 0) return 0;
+  const size_t n = (size_t)P.fin_n, nt = (size_t)P.ftask_n;
+  const size_t b = 4 * n * sizeof(T) +
+                   (n + 2 * (nt + 1) + (size_t)P.ftask_nc + nt + (size_t)P.ftask_hn + 1) *
+                       sizeof(int32_t);
+  return (b + 15) & ~(size_t)15;
+}
+// Staging, lane j < forest_stage_count: node values and task lists.
+template <class T>
+NRT_HD void forest_stage(const TreePlanDev& P, const TreeWork<T>& w, const ForestLds<T>& L,
+                         int j) {
+  if (j < P.fin_n) {
+    const int v = P.ftask_node[j];
+    const int g = P.node_off[P.levels] + v;
+    L.node[j] = v;
+    L.d[j] = w.d[g];
+    L.r[j] = w.r[g];
+    L.up[j] = w.up[g];
+    L.dn[j] = w.dn[g];
+  }
+  if (j <= P.ftask_n) {
+    L.toff[j] = P.ftask_off[j];
+    L.cptr[j] = P.ftask_cptr[j];
+    if (j < P.ftask_n) L.par[j] = P.ftask_par[j];
+  }
+  if (j < P.ftask_nc) L.child[j] = P.ftask_child[j];
+  if (j <= P.ftask_hn) L.thoff[j] = P.ftask_hoff[j];
+}
+NRT_HD int forest_stage_count(const TreePlanDev& P) {
+  const int a = P.fin_n > P.ftask_n + 1 ? P.fin_n : P.ftask_n + 1;
+  const int b = P.ftask_nc > P.ftask_hn + 1 ? P.ftask_nc : P.ftask_hn + 1;
+  return a > b ? a : b;
+}
+// Task t, bottom to top (tps_task_up).
+template <class T>
+NRT_HD void forest_up(const ForestLds<T>& L, int t) {
+  const int k0 = L.toff[t], k1 = L.toff[t + 1];
+  T dv = L.d[k0], rv = L.r[k0];
+  for (int q = L.cptr[t]; q < L.cptr[t + 1]; q++) {
+    const int c = L.child[q];
+    const T f = L.dn[c] * tps_rcp(L.d[c]);
+    dv = dv - f * L.up[c];
+    rv = rv - f * L.r[c];
+  }
+  L.d[k0] = dv;
+  L.r[k0] = rv;
+  for (int k = k0 + 1; k < k1; k++) {
+    const T f = L.dn[k - 1] * tps_rcp(dv);
+    dv = L.d[k] - f * L.up[k - 1];
+    rv = L.r[k] - f * rv;
+    L.d[k] = dv;
+    L.r[k] = rv;
+  }
+}
+// Task t, top to bottom (tps_task_down): the solution replaces r.
+template <class T>
+NRT_HD void forest_down(const ForestLds<T>& L, int t) {
+  const int k0 = L.toff[t], k1 = L.toff[t + 1];
+  const int p = L.par[t];
+  T acc = L.r[k1 - 1];
+  if (p >= 0) acc = acc - L.up[k1 - 1] * L.r[p];
+  T xv = acc * tps_rcp(L.d[k1 - 1]);
+  L.r[k1 - 1] = xv;
+  for (int k = k1 - 2; k >= k0; k--) {
+    const T a = L.r[k] - L.up[k] * xv;
+    xv = a * tps_rcp(L.d[k]);
+    L.r[k] = xv;
+  }
+}
+template <class T>
+NRT_HD void forest_store(const TreePlanDev& P, const TreeWork<T>& w, const ForestLds<T>& L,
+                         int j) {
+  w.x[P.node_off[P.levels] + L.node[j]] = L.r[j];
 }
 
 // The top of the final forest staged in LDS (TreePlanHost::top_*): one workgroup

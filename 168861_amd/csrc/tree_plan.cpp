@@ -1,5 +1,6 @@
 // Host-side plan of the tree partition solve (see tree.hpp).
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include "tree.hpp"
@@ -133,6 +134,19 @@ static void build_final_tasks(TreePlanHost& P) {
     const auto& v = nodes[order[i]];
     P.ftask_node.insert(P.ftask_node.end(), v.begin(), v.end());
     P.ftask_off[i + 1] = (int32_t)P.ftask_node.size();
+  }
+  // the same tasks in positions (the staged forest's layout)
+  std::vector<int32_t> pos(n);
+  for (int k = 0; k < n; k++) pos[P.ftask_node[k]] = k;
+  P.ftask_par.assign(nt, -1);
+  P.ftask_cptr.assign(nt + 1, 0);
+  P.ftask_child.clear();
+  for (int i = 0; i < nt; i++) {
+    const int top = P.ftask_node[P.ftask_off[i + 1] - 1], bot = P.ftask_node[P.ftask_off[i]];
+    if (P.fin_par[top] >= 0) P.ftask_par[i] = pos[P.fin_par[top]];
+    for (int k = P.fin_cptr[bot]; k < P.fin_cptr[bot + 1]; k++)
+      P.ftask_child.push_back(pos[P.fin_child[k]]);
+    P.ftask_cptr[i + 1] = (int32_t)P.ftask_child.size();
   }
 }
 
@@ -505,7 +519,8 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
   const std::vector<int32_t>* p[kPlanParts] = {
       &P.nslots,    &P.nchunk,    &P.node_off,  &P.next,      &P.chunk_off, &P.chunk_len,
       &P.chunk_dyn, &P.slot0,     &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
-      &P.fin_cptr,  &P.fin_child, &P.ftask_hoff, &P.ftask_off, &P.ftask_node, &P.dyn0,
+      &P.fin_cptr,  &P.fin_child, &P.ftask_hoff, &P.ftask_off, &P.ftask_node,
+      &P.ftask_par, &P.ftask_cptr, &P.ftask_child, &P.dyn0,
       &P.ltask_hoff, &P.ltask_off, &P.ltask_node,
       &P.sub_order, &P.sub_hoff,  &P.top_node,  &P.top_hoff,
       &P.top_cptr,  &P.top_child, &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
@@ -515,8 +530,10 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
 }
 
 static TreePlanDev plan_from(const TreePlanHost& P, const int32_t* const* a) {
+  const int ntask = P.ftask_off.empty() ? 0 : (int)P.ftask_off.size() - 1;
   return TreePlanDev{P.levels,     P.heights,    P.ftask_hn,  (int)P.dyn0.size(),
-                     P.ltask_hn,   P.fin_cut,    P.sub_groups, P.top_n,     P.top_ext,
+                     P.ltask_hn,   (int)P.fin_par.size(), ntask, (int)P.ftask_child.size(),
+                     P.fin_cut,    P.sub_groups, P.top_n,     P.top_ext,
                      P.groups,     P.top_lds_ok, P.lds_nodes, P.top_wide,   P.top_narrow,
                      P.top_kids,   P.gl_cols,    P.gl_fin,    P.gl_edges,   P.gl_wide,
                      a[0],         a[1],         a[2],        a[3],         a[4],
@@ -526,7 +543,8 @@ static TreePlanDev plan_from(const TreePlanHost& P, const int32_t* const* a) {
                      a[20],        a[21],        a[22],       a[23],        a[24],
                      a[25],        a[26],        a[27],       a[28],        a[29],
                      a[30],        a[31],        a[32],       a[33],        a[34],
-                     a[35],        a[36],        a[37]};
+                     a[35],        a[36],        a[37],       a[38],        a[39],
+                     a[40]};
 }
 
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
@@ -1136,13 +1154,27 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           for (int j = L.hoff[h]; j < L.hoff[h + 1]; j++) top_down(L, j);
         for (int j = 0; j < H.top_n; j++) top_store(P, w, L, j);
       } else if (cut == 0 && H.ftask_hn > 0) {
-        // unsplit: by chain tasks, task heights up, then down
+        // unsplit: the forest staged at task positions (host memory of exactly its
+        // size standing in for LDS), then by chain tasks from the work arrays; the
+        // two must give the same solution to the bit
+        std::vector<unsigned char> lds(forest_lds_bytes<double>(P));
+        const ForestLds<double> L = ForestLds<double>::carve(lds.data(), P);
+        for (int j = 0; j < forest_stage_count(P); j++) forest_stage(P, w, L, j);
+        for (int h = 0; h < H.ftask_hn; h++)
+          for (int t = L.thoff[h]; t < L.thoff[h + 1]; t++) forest_up(L, t);
+        for (int h = H.ftask_hn - 1; h >= 0; h--)
+          for (int t = L.thoff[h]; t < L.thoff[h + 1]; t++) forest_down(L, t);
+        for (int j = 0; j < P.fin_n; j++) forest_store(P, w, L, j);
+        const int fb = H.node_off[H.levels];
+        const std::vector<double> staged(w.x + fb, w.x + fb + P.fin_n);
         for (int h = 0; h < H.ftask_hn; h++)
           for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++)
             tps_task_up(P, w, P.ftask_off, P.ftask_node, t);
         for (int h = H.ftask_hn - 1; h >= 0; h--)
           for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++)
             tps_task_down(P, w, P.ftask_off, P.ftask_node, t);
+        if (std::memcmp(staged.data(), w.x + fb, sizeof(double) * P.fin_n) != 0)
+          throw std::runtime_error("staged final forest: a solution differs from the chain tasks'");
       } else {
         for (int h = cut; h < H.heights; h++)
           for (int k = H.fin_hoff[h]; k < H.fin_hoff[h + 1]; k++)

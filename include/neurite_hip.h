@@ -328,9 +328,10 @@ typedef struct nrt_batch_stats {
   int64_t device_bytes;      /* device memory held by the slots               */
   int32_t class_threads[3];  /* implicit: workgroup size per size class (slots
                                 of < 4,096 rows, < 16,384, the rest), tuned by
-                                timing the class's first launches (classes that
-                                share a launch run with the size of the class
-                                holding the most rows)                         */
+                                timing the class's launches 2 to 5 (its first
+                                also factors its slots; classes that share a
+                                launch run with the size of the class holding
+                                the most rows)                                 */
   int32_t reserved;
 } nrt_batch_stats;
 

@@ -59,7 +59,7 @@ def test_implicit_batch_matches_alone(threads, nrt, orc):
     _say(f"implicit batch fp64 threads={threads} sizes={[p.N for p in probs]}")
     with nh.Batch(kind=nh.IMPLICIT, krylov=nh.DIRECT, batch_threads=threads) as b:
         ids = [b.add(p, steps, a0, a1) for p in probs]
-        for _ in range(5):  # 5 calls of 12 steps (the autotuner takes the first 4)
+        for _ in range(5):  # 5 calls of 12 steps (the autotuner times calls 2-5)
             b.step(12)
         st = b.stats()
         assert st["active_slots"] == 0
