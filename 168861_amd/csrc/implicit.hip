@@ -164,6 +164,7 @@ struct Implicit {
   bool local_mode = false;  // tree solve in local mode (plan_h.groups workgroups)
   bool local_res = false;   // ... in its group-resident form
   size_t local_lds = 0;     // dynamic LDS of a local-mode launch
+  size_t forest_lds_max = 0;  // largest staged final forest of a block launch (bytes)
   DBuf<int32_t> hh_goff;
   int threads = 256;
   TreePlanHost plan_h;
@@ -1089,8 +1090,9 @@ __global__ __launch_bounds__(kImpBlock) void k_implicit_gmres(ImpDev d, GmresDev
 // grid mode (large trees): every workgroup takes a share of the rows and chunks,
 // grid barriers separate the levels, and workgroup 0 solves the compressed forest.
 constexpr int kTreeBlock = 1024;
-// Largest staged final forest (dynamic LDS of one workgroup, block launches).
-constexpr size_t kForestLdsMax = 64 * 1024;
+// Largest staged final forest (dynamic LDS of one workgroup, block launches), within
+// the device's limit per workgroup.
+constexpr size_t kForestLdsMax = 160 * 1024;
 constexpr int kTreeArrays = 10;  // d, up, dn, r, x, ca, cb, cg, cf, ci
 
 template <class T>
@@ -2287,6 +2289,12 @@ Implicit* implicit_create(nrt_handle* hnd, const HostProblem& p, int batch_threa
     e->N = N;
     e->H = H;
     e->dT = p.dT;
+    {
+      int per_block = 0;
+      NRT_HIP(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                    hnd->device));
+      e->forest_lds_max = std::min(kForestLdsMax, (size_t)std::max(per_block, 0));
+    }
     std::vector<int32_t> hh_of(N, -1);
     for (int i = 0; i < H; i++) {
       NRT_REQUIRE(hh_of[p.hh_elem[i]] < 0, NRT_ERR_INVALID, "duplicate HH element index");
@@ -2756,7 +2764,7 @@ static TreeArgs<T> tree_args(Implicit* e, const nrt_options& o, T* b, T* alt = n
                              (e->local_mode && e->plan_h.top_lds_ok) ||
                              (!e->grid_mode && !e->local_mode &&
                               forest_lds_bytes<T>(e->plan) > 0 &&
-                              forest_lds_bytes<T>(e->plan) <= kForestLdsMax)
+                              forest_lds_bytes<T>(e->plan) <= e->forest_lds_max)
                          ? 1
                          : 0,
                      e->local_mode ? e->hh_goff.p : nullptr,
@@ -3054,16 +3062,19 @@ static size_t batch_forest_lds(int precision, int krylov, int threads) {
     if (it != known.end()) return it->second;
   }
   const void* k = batch_kernel(precision, krylov, threads);
-  int per0 = 0, per_cu_lds = 0;
+  int per0 = 0, per_cu_lds = 0, per_block = 0;
   NRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per0, k, threads, 0));
   NRT_HIP(hipDeviceGetAttribute(&per_cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                                 dev));
+  NRT_HIP(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   hipFuncAttributes fa{};
   NRT_HIP(hipFuncGetAttributes(&fa, k));
   size_t cap = 0;
   if (per0 > 0 && per_cu_lds > 0) {
     const size_t share = (size_t)per_cu_lds / (size_t)per0;
     cap = share > fa.sharedSizeBytes ? share - fa.sharedSizeBytes : 0;
+    const size_t blk = (size_t)std::max(per_block, 0);
+    cap = std::min(cap, blk > fa.sharedSizeBytes ? blk - fa.sharedSizeBytes : 0);
     cap = std::min(cap, kForestLdsMax) & ~(size_t)1023;
     int per = 0;
     while (cap > 0) {
