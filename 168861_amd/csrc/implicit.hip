@@ -1093,7 +1093,7 @@ constexpr int kTreeBlock = 1024;
 // Largest staged final forest (dynamic LDS of one workgroup, block launches), within
 // the device's limit per workgroup.
 constexpr size_t kForestLdsMax = 160 * 1024;
-constexpr int kTreeArrays = 10;  // d, up, dn, r, x, ca, cb, cg, cf, ci
+constexpr int kTreeArrays = 11;  // d, up, dn, r, x, ca, cb, cg, cf, ci, cw
 
 template <class T>
 struct TreeArgs {
@@ -1240,7 +1240,7 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
   for (int lv = P.levels - 1; lv >= 1; lv--) {
     const int c1 = P.chunk_off[lv + 1];
     for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw)
-      tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr);
+      tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr, all_static || !P.chunk_dyn[ch]);
     tsync<GRID>(d);
   }
   // level 0 lands in the FP64 vector `out` directly (the potential: x = solution,
@@ -1254,8 +1254,9 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
   } else {
     const int c1 = P.chunk_off[1];
     for (int ch = wid; ch < c1; ch += nw) {
-      if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, ch, out);
-      else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, out);
+      const bool rp = all_static || !P.chunk_dyn[ch];
+      if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, ch, out, rp);
+      else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, out, rp);
     }
   }
   tsync<GRID>(d);
@@ -1752,14 +1753,17 @@ __device__ __forceinline__ void tps_run_local(const ImpDev& d, const TreePlanDev
   for (int lv = L - 1; lv >= 1; lv--) {
     const int32_t* gc = P.gcol + lv * (G + 1);
     const int c1 = gc[g + 1];
-    for (int c = gc[g] + tid; c < c1; c += nt)
-      tps_back_chunk<OUT_LEVEL>(P, w, lv, P.chunk_off[lv] + c, nullptr);
+    for (int c = gc[g] + tid; c < c1; c += nt) {
+      const int ch = P.chunk_off[lv] + c;
+      tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr, all_static || !P.chunk_dyn[ch]);
+    }
     __syncthreads();
   }
   const int c1 = P.gcol[g + 1];
   for (int c = P.gcol[g] + tid; c < c1; c += nt) {
-    if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, c, d.x);
-    else tps_back_chunk<OUT_ASSIGN>(P, w, 0, c, d.x);
+    const bool rp = all_static || !P.chunk_dyn[c];
+    if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, c, d.x, rp);
+    else tps_back_chunk<OUT_ASSIGN>(P, w, 0, c, d.x, rp);
   }
   __syncthreads();
 }
@@ -2746,8 +2750,8 @@ static int32_t probe_index(const Implicit* e, int32_t elem, int64_t interval) {
 
 template <class T>
 static TreeWork<T> tree_work(T* b, size_t n) {
-  return TreeWork<T>{b,         b + n,     b + 2 * n, b + 3 * n, b + 4 * n,
-                     b + 5 * n, b + 6 * n, b + 7 * n, b + 8 * n, b + 9 * n};
+  return TreeWork<T>{b,         b + n,     b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n,
+                     b + 6 * n, b + 7 * n, b + 8 * n, b + 9 * n, b + 10 * n};
 }
 
 template <class T>

@@ -264,6 +264,7 @@ struct TreeWork {
   T* cg;
   T* cf;
   T* ci;
+  T* cw;  // interior rows: weight of the row's r' in the start row's alpha (WSUM)
 };
 
 #if defined(__HIPCC__)
@@ -325,6 +326,9 @@ constexpr int kTpsBatch = 4;
 // Rows per batch of the static sweep and of the back-substitution (their loads do
 // not depend on the chain's arithmetic, so a whole batch is one memory round trip).
 constexpr int kTpsBatchS = 8;
+// Rows per batch of the forward-only static sweep and of the back-substitution from
+// r' (more operands per row: the weights, or the alphas' recurrence).
+constexpr int kTpsBatchW = 4;
 
 // Where a chunk's two surviving ends go: the slots of the next level in the same
 // arrays (level-synchronous layout). End codes are slot indices.
@@ -343,7 +347,13 @@ struct EndsSlots {
 
 // Eliminate the interior of the chunk of m rows at base, base + stride, ... in the
 // arrays w; its ends (codes es for row 0, ee for row m-1) go through `ends`.
-template <class W, class E>
+// WSUM: the start row's reduced right-hand side takes alpha_1 as the sum
+// sum_k cw_k r'_k over the interior rows (k = 1, 2, ...), with the weights
+// cw_k = (1/pivot_k) prod_{j<k} (-dn_{j+1} / pivot_j) kept in w.cw, so that a
+// chunk that reuses its factors needs only the forward sweep (chunk_reduce_static
+// <true>) and gets the same value to the bit; the interior alphas are then made
+// by the back-substitution (chunk_back<MODE, true>).
+template <bool WSUM = false, class W, class E>
 NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, const E& ends) {
   using T = typename W::value_type;
   using P = decltype(w.d);
@@ -379,6 +389,7 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   T pgk = gk, prp = rp;  // values of the row before the last one
   T inv = T(0);          // reciprocal of that row's pivot
   T dnl = dn1;           // dn of the last row
+  T wp = T(1), acc = T(0);  // WSUM: running product of the weights, and the sum
   for (int k0 = 2; k0 <= m - 1; k0 += kTpsBatch) {
     T bu[kTpsBatch], bd[kTpsBatch], bn[kTpsBatch], br[kTpsBatch];
 #pragma unroll
@@ -401,6 +412,12 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
         const int g = base + k * NC;
         inv = tps_rcp(dp);
         ci[g - NC] = inv;
+        if constexpr (WSUM) {  // row k-1 (interior): its weight, and its share of alpha_1
+          const T cwk = inv * wp;
+          w.cw[g - NC] = cwk;
+          acc = acc + cwk * rp;
+          wp = wp * (-bn[j] * inv);
+        }
         const T f = bu[j] * inv;
         dp = bd[j] - f * bn[j];
         gk = -f * gk;
@@ -456,7 +473,8 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   }
   // start row s: up_s x_P + (d_s + dn_1 beta_1) x_s + dn_1 gamma_1 x_e = r_s - dn_1 alpha_1;
   // end row e: its dn is the s -> e coupling
-  ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * al);
+  if constexpr (WSUM) ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * acc);
+  else ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * al);
   ends.put(ee, end_d, end_up, dn1 * ga, end_r);
 }
 
@@ -467,7 +485,7 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
   const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
   const int es = nbase + P.next[base];
   const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
-  chunk_reduce(w, base, NC, m, es, ee, EndsSlots<T>{w});
+  chunk_reduce<true>(w, base, NC, m, es, ee, EndsSlots<T>{w});
 }
 
 // Right-hand side only, for a chunk whose factors (f, 1/pivot, beta, gamma, the
@@ -497,7 +515,7 @@ struct TpsRhsUpdate {
   }
 };
 
-template <class W, class E, class R>
+template <bool WSUM = false, class W, class E, class R>
 NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int ee,
                                 const E& ends, const R& r) {
   using T = typename W::value_type;
@@ -512,6 +530,42 @@ NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int
   }
   const T r0 = r(base);
   const T dn1 = dn[base + NC];
+  if constexpr (WSUM) {
+    // forward only: r'_k into ca (the back-substitution makes the alphas), alpha_1
+    // as chunk_reduce<true> sums it
+    const P __restrict__ cw = w.cw;
+    T rp = r(base + NC);
+    ca[base + NC] = rp;
+    T acc = T(0);
+    acc = acc + cw[base + NC] * rp;
+    for (int k0 = 2; k0 <= m - 1; k0 += kTpsBatchW) {
+      T br[kTpsBatchW], bf[kTpsBatchW], bw[kTpsBatchW];
+#pragma unroll
+      for (int j = 0; j < kTpsBatchW; j++) {
+        const int k = k0 + j;
+        if (k <= m - 1) {
+          const int g = base + k * NC;
+          br[j] = r(g);
+          bf[j] = cf[g];
+          bw[j] = k <= m - 2 ? cw[g] : T(0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kTpsBatchW; j++) {
+        const int k = k0 + j;
+        if (k <= m - 1) {
+          rp = br[j] - bf[j] * rp;
+          if (k <= m - 2) {
+            ca[base + k * NC] = rp;
+            acc = acc + bw[j] * rp;
+          }
+        }
+      }
+    }
+    ends.put_r(ee, rp);
+    ends.put_r(es, r0 - dn1 * acc);
+    return;
+  }
   // forward: rp_k = r_k - f_k rp_{k-1}
   T rp = r(base + NC);
   ca[base + NC] = rp;
@@ -578,7 +632,7 @@ NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, 
   const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
   const int es = nbase + P.next[base];
   const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
-  chunk_reduce_static(w, base, NC, m, es, ee, EndsSlots<T>{w}, rhs);
+  chunk_reduce_static<true>(w, base, NC, m, es, ee, EndsSlots<T>{w}, rhs);
 }
 template <class T>
 NRT_HD void tps_reduce_chunk_static(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch) {
@@ -912,7 +966,9 @@ NRT_HD void tps_put(const W& w, int g, T v, double* out) {
 
 // Back-substitution of the chunk of m rows at base, base + NC, ... in w from the
 // solution at its ends (codes es, ee, read through `ends`).
-template <int MODE, class W, class E>
+// FROM_RP: ca holds the forward sweep's r'_k (chunk_reduce_static<true>), and the
+// alphas are made here, by chunk_reduce's backward recurrence, operand for operand.
+template <int MODE, bool FROM_RP = false, class W, class E>
 NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, const E& ends,
                        double* out) {
   using T = typename W::value_type;
@@ -925,6 +981,38 @@ NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, cons
   const P __restrict__ ca = w.ca;
   const P __restrict__ cb = w.cb;
   const P __restrict__ cg = w.cg;
+  if constexpr (FROM_RP) {
+    if (m <= 2) return;
+    const P __restrict__ ci = w.ci;
+    const P __restrict__ dn = w.dn;
+    const int g2 = base + (m - 2) * NC;
+    T al = ca[g2] * ci[g2];
+    tps_put<MODE>(w, g2, al + cb[g2] * xs + cg[g2] * xe, out);
+    for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatchW) {
+      T bp[kTpsBatchW], bi[kTpsBatchW], bn[kTpsBatchW], b[kTpsBatchW], g[kTpsBatchW];
+#pragma unroll
+      for (int j = 0; j < kTpsBatchW; j++) {
+        const int k = k0 - j;
+        if (k >= 1) {
+          const int gg = base + k * NC;
+          bp[j] = ca[gg];
+          bi[j] = ci[gg];
+          bn[j] = dn[gg + NC];
+          b[j] = cb[gg];
+          g[j] = cg[gg];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kTpsBatchW; j++) {
+        const int k = k0 - j;
+        if (k >= 1) {
+          al = (bp[j] - bn[j] * al) * bi[j];
+          tps_put<MODE>(w, base + k * NC, al + b[j] * xs + g[j] * xe, out);
+        }
+      }
+    }
+    return;
+  }
   for (int k0 = 1; k0 <= m - 2; k0 += kTpsBatchS) {
     T a[kTpsBatchS], b[kTpsBatchS], g[kTpsBatchS];
 #pragma unroll
@@ -945,15 +1033,18 @@ NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, cons
   }
 }
 
-// Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1.
+// Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1;
+// from_rp: the chunk was last swept by tps_reduce_chunk_static (its alphas are made
+// here), else eliminated in full by tps_reduce_chunk.
 template <int MODE, class T>
 NRT_HD void tps_back_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch,
-                           double* out) {
+                           double* out, bool from_rp) {
   const int NC = P.nchunk[lv], m = P.chunk_len[ch];
   const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
   const int es = nbase + P.next[base];
   const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
-  chunk_back<MODE>(w, base, NC, m, es, ee, EndsSlots<T>{w}, out);
+  if (from_rp) chunk_back<MODE, true>(w, base, NC, m, es, ee, EndsSlots<T>{w}, out);
+  else chunk_back<MODE>(w, base, NC, m, es, ee, EndsSlots<T>{w}, out);
 }
 
 // Level 0 as the final forest itself (a system with no chain longer than 2).

@@ -966,9 +966,10 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
     std::vector<std::vector<int32_t>> parts;
     TreePlanDev P = plan_view_parts(H, parts);
     const size_t T = (size_t)H.total_nodes();
-    std::vector<std::vector<double>> wa(10, std::vector<double>(T, 0.0));
-    TreeWork<double> w{wa[0].data(), wa[1].data(), wa[2].data(), wa[3].data(), wa[4].data(),
-                       wa[5].data(), wa[6].data(), wa[7].data(), wa[8].data(), wa[9].data()};
+    std::vector<std::vector<double>> wa(11, std::vector<double>(T, 0.0));
+    TreeWork<double> w{wa[0].data(), wa[1].data(), wa[2].data(), wa[3].data(),
+                       wa[4].data(), wa[5].data(), wa[6].data(), wa[7].data(),
+                       wa[8].data(), wa[9].data(), wa[10].data()};
     const int32_t* s0 = H.slot0.data();
     for (int i = 0; i < N; i++) {
       for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
@@ -1128,8 +1129,9 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           for (int lv = H.levels - 1; lv >= 0; lv--) {
             const int* gc = H.gcol.data() + (size_t)lv * (G + 1);
             for (int ch = H.chunk_off[lv] + gc[g]; ch < H.chunk_off[lv] + gc[g + 1]; ch++) {
-              if (lv > 0) tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr);
-              else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
+              const bool rp = reuse && !H.chunk_dyn[ch];
+              if (lv > 0) tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr, rp);
+              else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data(), rp);
             }
           }
         }
@@ -1189,12 +1191,12 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
             tps_final_down(P, w, H.sub_order[k]);
       for (int lv = H.levels - 1; lv >= 1; lv--)
         for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
-          tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data());
+          tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data(), reuse && !H.chunk_dyn[ch]);
       if (H.levels == 0) {
         for (int v = 0; v < H.nslots[0]; v++) tps_final_out<OUT_ASSIGN>(P, w, v, xs.data());
       } else {
         for (int ch = H.chunk_off[0]; ch < H.chunk_off[1]; ch++)
-          tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
+          tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data(), reuse && !H.chunk_dyn[ch]);
       }
     }
     for (int i = 0; i < N; i++) x[i] = xs[s0[i]];
