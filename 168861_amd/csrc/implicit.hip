@@ -1206,12 +1206,12 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
     // unsplit: chain tasks, a workgroup barrier per task height
     for (int h = 0; h < P.ftask_hn; h++) {
       for (int t = P.ftask_hoff[h] + (int)threadIdx.x; t < P.ftask_hoff[h + 1]; t += blockDim.x)
-        tps_task_up(P, w, P.ftask_off, P.ftask_node, t);
+        tps_ftask_up(P, w, t);
       __syncthreads();
     }
     for (int h = P.ftask_hn - 1; h >= 0; h--) {
       for (int t = P.ftask_hoff[h] + (int)threadIdx.x; t < P.ftask_hoff[h + 1]; t += blockDim.x)
-        tps_task_down(P, w, P.ftask_off, P.ftask_node, t);
+        tps_ftask_down(P, w, t);
       __syncthreads();
     }
   } else if (!GRID || d.vb == 0) {

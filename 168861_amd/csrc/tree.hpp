@@ -68,6 +68,9 @@ struct TreePlanHost {
   // of each task's top node (-1: a root), and the positions of the children of its
   // bottom node, [ftask_cptr[t], ftask_cptr[t+1]) of ftask_child, in fin_child order.
   std::vector<int32_t> ftask_par, ftask_cptr, ftask_child;
+  // ... and as final-local nodes (for the solve from the work arrays): the parent
+  // node of each task's top (-1: a root), the children of its bottom node.
+  std::vector<int32_t> ftask_pnode, ftask_cnode;
   // Local mode: the chain tasks of each group's local final nodes, group g's tasks
   // of task height h being [ltask_hoff[g*ltask_hn+h], +1) of ltask_off / ltask_node.
   int ltask_hn = 0;
@@ -180,7 +183,7 @@ TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<in
 // exactly its length: an access past the end of any one array leaves its
 // allocation (host emulation under a bounds-checking build).
 TreePlanDev plan_view_parts(const TreePlanHost& P, std::vector<std::vector<int32_t>>& store);
-constexpr int kPlanParts = 41;
+constexpr int kPlanParts = 43;
 
 // The matrix with its rows in level-0 slot order (padding slots are empty rows):
 // columns as slots, each entry's role (0 diagonal, 1 coupling to the parent, 2 to
@@ -223,6 +226,8 @@ struct TreePlanDev {
   const int32_t* ftask_par;
   const int32_t* ftask_cptr;
   const int32_t* ftask_child;
+  const int32_t* ftask_pnode;
+  const int32_t* ftask_cnode;
   const int32_t* dyn0;
   const int32_t* ltask_hoff;
   const int32_t* ltask_off;
@@ -734,6 +739,103 @@ NRT_HD void tps_task_down(const TreePlanDev& P, const TreeWork<T>& w, const int3
     for (int j = 0; j < kTaskBatch; j++) {
       if (k - j >= k0) {
         u[j] = base + tnode[k - j];
+        br[j] = w.r[u[j]];
+        bup[j] = w.up[u[j]];
+        bca[j] = w.ca[u[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k - j >= k0) {
+        const T a = br[j] - bup[j] * xv;
+        xv = a * tps_rcp(bca[j]);
+        w.x[u[j]] = xv;
+      }
+    }
+  }
+}
+
+// The same chain tasks from the work arrays with the task-indexed lists (ftask_cptr,
+// ftask_cnode, ftask_pnode): a task's bounds, child and parent nodes come from
+// arrays indexed by the task, so its loads wait on two earlier loads at most rather
+// than on a chain of node lookups. Operand for operand tps_task_up/down.
+constexpr int kTaskKids = 4;
+template <class T>
+NRT_HD void tps_ftask_up(const TreePlanDev& P, const TreeWork<T>& w, int t) {
+  const int base = P.node_off[P.levels];
+  const int k0 = P.ftask_off[t], k1 = P.ftask_off[t + 1];
+  const int q0 = P.ftask_cptr[t], q1 = P.ftask_cptr[t + 1];
+  const int v = base + P.ftask_node[k0];
+  T dv = w.d[v], rv = w.r[v];
+  for (int q = q0; q < q1; q += kTaskKids) {
+    int c[kTaskKids];
+    T bdn[kTaskKids], bca[kTaskKids], bup[kTaskKids], br[kTaskKids];
+#pragma unroll
+    for (int j = 0; j < kTaskKids; j++)
+      if (q + j < q1) c[j] = base + P.ftask_cnode[q + j];
+#pragma unroll
+    for (int j = 0; j < kTaskKids; j++) {
+      if (q + j < q1) {
+        bdn[j] = w.dn[c[j]];
+        bca[j] = w.ca[c[j]];
+        bup[j] = w.up[c[j]];
+        br[j] = w.r[c[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTaskKids; j++) {
+      if (q + j < q1) {
+        const T f = bdn[j] * tps_rcp(bca[j]);
+        dv = dv - f * bup[j];
+        rv = rv - f * br[j];
+      }
+    }
+  }
+  w.ca[v] = dv;
+  w.r[v] = rv;
+  for (int k = k0 + 1; k < k1; k += kTaskBatch) {
+    int u[kTaskBatch];
+    T bd[kTaskBatch], br[kTaskBatch], bup[kTaskBatch], bdn[kTaskBatch];
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k + j < k1) {
+        const int c = base + P.ftask_node[k + j - 1];  // the node below: its only child
+        u[j] = base + P.ftask_node[k + j];
+        bd[j] = w.d[u[j]];
+        br[j] = w.r[u[j]];
+        bup[j] = w.up[c];
+        bdn[j] = w.dn[c];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k + j < k1) {
+        const T f = bdn[j] * tps_rcp(dv);
+        dv = bd[j] - f * bup[j];
+        rv = br[j] - f * rv;
+        w.ca[u[j]] = dv;
+        w.r[u[j]] = rv;
+      }
+    }
+  }
+}
+template <class T>
+NRT_HD void tps_ftask_down(const TreePlanDev& P, const TreeWork<T>& w, int t) {
+  const int base = P.node_off[P.levels];
+  const int k0 = P.ftask_off[t], k1 = P.ftask_off[t + 1];
+  const int pn = P.ftask_pnode[t];
+  const int v = base + P.ftask_node[k1 - 1];
+  T acc = w.r[v];
+  if (pn >= 0) acc = acc - w.up[v] * w.x[base + pn];
+  T xv = acc * tps_rcp(w.ca[v]);
+  w.x[v] = xv;
+  for (int k = k1 - 2; k >= k0; k -= kTaskBatch) {
+    int u[kTaskBatch];
+    T br[kTaskBatch], bup[kTaskBatch], bca[kTaskBatch];
+#pragma unroll
+    for (int j = 0; j < kTaskBatch; j++) {
+      if (k - j >= k0) {
+        u[j] = base + P.ftask_node[k - j];
         br[j] = w.r[u[j]];
         bup[j] = w.up[u[j]];
         bca[j] = w.ca[u[j]];

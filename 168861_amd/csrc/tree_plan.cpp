@@ -139,13 +139,20 @@ static void build_final_tasks(TreePlanHost& P) {
   std::vector<int32_t> pos(n);
   for (int k = 0; k < n; k++) pos[P.ftask_node[k]] = k;
   P.ftask_par.assign(nt, -1);
+  P.ftask_pnode.assign(nt, -1);
   P.ftask_cptr.assign(nt + 1, 0);
   P.ftask_child.clear();
+  P.ftask_cnode.clear();
   for (int i = 0; i < nt; i++) {
     const int top = P.ftask_node[P.ftask_off[i + 1] - 1], bot = P.ftask_node[P.ftask_off[i]];
-    if (P.fin_par[top] >= 0) P.ftask_par[i] = pos[P.fin_par[top]];
-    for (int k = P.fin_cptr[bot]; k < P.fin_cptr[bot + 1]; k++)
+    if (P.fin_par[top] >= 0) {
+      P.ftask_par[i] = pos[P.fin_par[top]];
+      P.ftask_pnode[i] = P.fin_par[top];
+    }
+    for (int k = P.fin_cptr[bot]; k < P.fin_cptr[bot + 1]; k++) {
       P.ftask_child.push_back(pos[P.fin_child[k]]);
+      P.ftask_cnode.push_back(P.fin_child[k]);
+    }
     P.ftask_cptr[i + 1] = (int32_t)P.ftask_child.size();
   }
 }
@@ -520,7 +527,7 @@ static void plan_parts(const TreePlanHost& P, const std::vector<int32_t>* (&part
       &P.nslots,    &P.nchunk,    &P.node_off,  &P.next,      &P.chunk_off, &P.chunk_len,
       &P.chunk_dyn, &P.slot0,     &P.row0,      &P.fin_par,   &P.fin_order, &P.fin_hoff,
       &P.fin_cptr,  &P.fin_child, &P.ftask_hoff, &P.ftask_off, &P.ftask_node,
-      &P.ftask_par, &P.ftask_cptr, &P.ftask_child, &P.dyn0,
+      &P.ftask_par, &P.ftask_cptr, &P.ftask_child, &P.ftask_pnode, &P.ftask_cnode, &P.dyn0,
       &P.ltask_hoff, &P.ltask_off, &P.ltask_node,
       &P.sub_order, &P.sub_hoff,  &P.top_node,  &P.top_hoff,
       &P.top_cptr,  &P.top_child, &P.top_par,   &P.gcol,      &P.loc_order, &P.loc_hoff,
@@ -544,7 +551,7 @@ static TreePlanDev plan_from(const TreePlanHost& P, const int32_t* const* a) {
                      a[25],        a[26],        a[27],       a[28],        a[29],
                      a[30],        a[31],        a[32],       a[33],        a[34],
                      a[35],        a[36],        a[37],       a[38],        a[39],
-                     a[40]};
+                     a[40],        a[41],        a[42]};
 }
 
 TreePlanDev plan_view(const TreePlanHost& P, const int32_t* base, std::vector<int32_t>* flat) {
@@ -1170,11 +1177,9 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
         const int fb = H.node_off[H.levels];
         const std::vector<double> staged(w.x + fb, w.x + fb + P.fin_n);
         for (int h = 0; h < H.ftask_hn; h++)
-          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++)
-            tps_task_up(P, w, P.ftask_off, P.ftask_node, t);
+          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++) tps_ftask_up(P, w, t);
         for (int h = H.ftask_hn - 1; h >= 0; h--)
-          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++)
-            tps_task_down(P, w, P.ftask_off, P.ftask_node, t);
+          for (int t = H.ftask_hoff[h]; t < H.ftask_hoff[h + 1]; t++) tps_ftask_down(P, w, t);
         if (std::memcmp(staged.data(), w.x + fb, sizeof(double) * P.fin_n) != 0)
           throw std::runtime_error("staged final forest: a solution differs from the chain tasks'");
       } else {
