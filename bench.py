@@ -43,18 +43,17 @@ def tree_bytes(N, H, nnz, steps, solves, vbytes):
     update phase per step: 144 B per HH element (V, gates r/w, parameters, the
     refreshed diagonal) and per element V 8 + code 4 + c_m dx 8 + K dx 8 + input 8
     + the right-hand side written 8 (44 B);
-    per tree solve: the forward sweep reads r and the multiplier and writes the
-    partial right-hand side, the backward sweep reads it with the pivot reciprocal
-    and writes alpha, the back-substitution reads alpha, beta, gamma and writes V
-    (FP64: 10 values of 8 B + V 8 = 88 B; FP32: 10 values of 4 B + V read and
-    written 16 = 56 B);
+    per tree solve: the forward sweep reads r, the multiplier and the weight and
+    writes the partial right-hand side r'; the back-substitution reads r', the pivot
+    reciprocal, the coupling to the next row, beta and gamma and writes V (FP64: 9
+    values of 8 B + V 8 = 80 B; FP32: 9 values of 4 B + V read and written 16 = 52 B);
     refinement only, per residual evaluation (solves + steps of them): the matrix
     (12 nnz + 4 (N+1)), the gathered V (8 nnz), b (8 N), the residual written into
     the tree system (vbytes N)."""
     per_step = 144 * H + 44 * N
     if vbytes == 8:
-        return steps * per_step + solves * 88 * N
-    per_solve = (10 * vbytes + 16) * N
+        return steps * per_step + solves * 80 * N
+    per_solve = (9 * vbytes + 16) * N
     per_resid = 12 * nnz + 4 * (N + 1) + 8 * nnz + 8 * N + vbytes * N
     return steps * per_step + solves * per_solve + (solves + steps) * per_resid
 
