@@ -350,6 +350,19 @@ struct EndsSlots {
   NRT_HD T x(int e) const { return w.x[e]; }
 };
 
+// Right-hand side of a row read from the array r (RowsOf keeps the array's address
+// space: LDS arrays of the group-resident form stay LDS accesses).
+template <class T>
+struct RhsArray {
+  const T* r;
+  NRT_HD T operator()(int g) const { return r[g]; }
+};
+template <class Ptr>
+struct RowsOf {
+  Ptr r;
+  NRT_HD auto operator()(int g) const { return r[g]; }
+};
+
 // Eliminate the interior of the chunk of m rows at base, base + stride, ... in the
 // arrays w; its ends (codes es for row 0, ee for row m-1) go through `ends`.
 // WSUM: the start row's reduced right-hand side takes alpha_1 as the sum
@@ -358,14 +371,15 @@ struct EndsSlots {
 // chunk that reuses its factors needs only the forward sweep (chunk_reduce_static
 // <true>) and gets the same value to the bit; the interior alphas are then made
 // by the back-substitution (chunk_back<MODE, true>).
-template <bool WSUM = false, class W, class E>
-NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, const E& ends) {
+// rf(g): the right-hand side of row g (the array r, or computed as it is read).
+template <bool WSUM, class W, class E, class R>
+NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, const E& ends,
+                         const R& rf) {
   using T = typename W::value_type;
   using P = decltype(w.d);
   const P __restrict__ d = w.d;
   const P __restrict__ up = w.up;
   const P __restrict__ dn = w.dn;
-  const P __restrict__ r = w.r;
   const P __restrict__ ca = w.ca;
   const P __restrict__ cb = w.cb;
   const P __restrict__ cg = w.cg;
@@ -374,12 +388,12 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   if (m <= 2) {  // nothing to eliminate: the nodes pass through
     for (int k = 0; k < m; k++) {
       const int g = base + k * NC;
-      ends.put(k == 0 ? es : ee, d[g], up[g], dn[g], r[g]);
+      ends.put(k == 0 ? es : ee, d[g], up[g], dn[g], rf(g));
     }
     return;
   }
   // start row s = 0 and first interior row 1
-  const T d0 = d[base], u0 = up[base], n0 = dn[base], r0 = r[base];
+  const T d0 = d[base], u0 = up[base], n0 = dn[base], r0 = rf(base);
   const int g1 = base + NC;
   const T dn1 = dn[g1];
   // forward sweep over rows k = 1..m-1:
@@ -388,7 +402,7 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   // end row of the reduced system: G_e x_s + D_e x_e + (children) = R_e.
   // Each row's pivot reciprocal is computed once, here, and kept in ci: the
   // backward sweep and the steps that reuse the factors only multiply.
-  T dp = d[g1], gk = up[g1], rp = r[g1];
+  T dp = d[g1], gk = up[g1], rp = rf(g1);
   cb[g1] = gk;
   cg[g1] = rp;
   T pgk = gk, prp = rp;  // values of the row before the last one
@@ -405,7 +419,7 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
         bu[j] = up[g];
         bd[j] = d[g];
         bn[j] = dn[g];
-        br[j] = r[g];
+        br[j] = rf(g);
       }
     }
 #pragma unroll
@@ -482,6 +496,10 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   else ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * al);
   ends.put(ee, end_d, end_up, dn1 * ga, end_r);
 }
+template <bool WSUM = false, class W, class E>
+NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, const E& ends) {
+  chunk_reduce<WSUM>(w, base, NC, m, es, ee, ends, RowsOf<decltype(w.r)>{w.r});
+}
 
 // Eliminate the interior of chunk `ch` of level `lv` and write its ends to lv+1.
 template <class T>
@@ -500,11 +518,6 @@ NRT_HD void tps_reduce_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv,
 // Right-hand side of a row for the static sweep: stored (RhsArray), or computed
 // from the state as a direct solve's update phase would store it (TpsRhsUpdate:
 // b = c_m dx V / dT + K dx (+ input while the stimulus is on), the same operations).
-template <class T>
-struct RhsArray {
-  const T* r;
-  NRT_HD T operator()(int g) const { return r[g]; }
-};
 template <class T>
 struct TpsRhsUpdate {
   const double* x;
@@ -626,8 +639,7 @@ NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int
 template <class W, class E>
 NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int ee,
                                 const E& ends) {
-  using T = typename W::value_type;
-  chunk_reduce_static(w, base, NC, m, es, ee, ends, RhsArray<T>{w.r});
+  chunk_reduce_static(w, base, NC, m, es, ee, ends, RowsOf<decltype(w.r)>{w.r});
 }
 
 template <class T, class R>
