@@ -1240,7 +1240,7 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
   for (int lv = P.levels - 1; lv >= 1; lv--) {
     const int c1 = P.chunk_off[lv + 1];
     for (int ch = P.chunk_off[lv] + wid; ch < c1; ch += nw)
-      tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr, all_static || !P.chunk_dyn[ch]);
+      tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr);
     tsync<GRID>(d);
   }
   // level 0 lands in the FP64 vector `out` directly (the potential: x = solution,
@@ -1254,9 +1254,8 @@ __device__ __forceinline__ void tps_run(const ImpDev& d, const TreePlanDev& P,
   } else {
     const int c1 = P.chunk_off[1];
     for (int ch = wid; ch < c1; ch += nw) {
-      const bool rp = all_static || !P.chunk_dyn[ch];
-      if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, ch, out, rp);
-      else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, out, rp);
+      if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, ch, out);
+      else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, out);
     }
   }
   tsync<GRID>(d);
@@ -1753,17 +1752,14 @@ __device__ __forceinline__ void tps_run_local(const ImpDev& d, const TreePlanDev
   for (int lv = L - 1; lv >= 1; lv--) {
     const int32_t* gc = P.gcol + lv * (G + 1);
     const int c1 = gc[g + 1];
-    for (int c = gc[g] + tid; c < c1; c += nt) {
-      const int ch = P.chunk_off[lv] + c;
-      tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr, all_static || !P.chunk_dyn[ch]);
-    }
+    for (int c = gc[g] + tid; c < c1; c += nt)
+      tps_back_chunk<OUT_LEVEL>(P, w, lv, P.chunk_off[lv] + c, nullptr);
     __syncthreads();
   }
   const int c1 = P.gcol[g + 1];
   for (int c = P.gcol[g] + tid; c < c1; c += nt) {
-    const bool rp = all_static || !P.chunk_dyn[c];
-    if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, c, d.x, rp);
-    else tps_back_chunk<OUT_ASSIGN>(P, w, 0, c, d.x, rp);
+    if (acc) tps_back_chunk<OUT_ADD>(P, w, 0, c, d.x);
+    else tps_back_chunk<OUT_ASSIGN>(P, w, 0, c, d.x);
   }
   __syncthreads();
 }

@@ -404,7 +404,8 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   // backward sweep and the steps that reuse the factors only multiply.
   T dp = d[g1], gk = up[g1], rp = rf(g1);
   cb[g1] = gk;
-  cg[g1] = rp;
+  if constexpr (WSUM) ca[g1] = rp;  // r' in ca (the back-substitution's recurrence)
+  else cg[g1] = rp;
   T pgk = gk, prp = rp;  // values of the row before the last one
   T inv = T(0);          // reciprocal of that row's pivot
   T dnl = dn1;           // dn of the last row
@@ -445,12 +446,44 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
         cf[g] = f;
         if (k <= m - 2) {
           cb[g] = gk;
-          cg[g] = rp;
+          if constexpr (WSUM) ca[g] = rp;
+          else cg[g] = rp;
         }
       }
     }
   }
   const T end_d = dp, end_up = gk, end_r = rp;
+  if constexpr (WSUM) {
+    // backward sweep for beta_1 and gamma_1 only (the reduced start row and the
+    // s -> e coupling): the interior keeps (g, r') in (cb, ca), which the
+    // back-substitution's recurrence reads (chunk_back<MODE, true>)
+    T be = -pgk * inv, ga = -dnl * inv;
+    for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatch) {
+      T bi[kTpsBatch], bg[kTpsBatch], bn[kTpsBatch];
+#pragma unroll
+      for (int j = 0; j < kTpsBatch; j++) {
+        const int k = k0 - j;
+        if (k >= 1) {
+          const int g = base + k * NC;
+          bi[j] = ci[g];
+          bg[j] = cb[g];
+          bn[j] = dn[g + NC];  // coupling k -> k+1
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kTpsBatch; j++) {
+        const int k = k0 - j;
+        if (k >= 1) {
+          const T dcur = bn[j];
+          be = (-bg[j] - dcur * be) * bi[j];
+          ga = -dcur * ga * bi[j];
+        }
+      }
+    }
+    ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * acc);
+    ends.put(ee, end_d, end_up, dn1 * ga, end_r);
+    return;
+  }
   // backward sweep: x_k = alpha_k + beta_k x_s + gamma_k x_e, into (ca, cb, cg)
   T al, be, ga;
   {
@@ -492,8 +525,7 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   }
   // start row s: up_s x_P + (d_s + dn_1 beta_1) x_s + dn_1 gamma_1 x_e = r_s - dn_1 alpha_1;
   // end row e: its dn is the s -> e coupling
-  if constexpr (WSUM) ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * acc);
-  else ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * al);
+  ends.put(es, d0 + dn1 * be, u0, n0, r0 - dn1 * al);
   ends.put(ee, end_d, end_up, dn1 * ga, end_r);
 }
 template <bool WSUM = false, class W, class E>
@@ -1080,8 +1112,8 @@ NRT_HD void tps_put(const W& w, int g, T v, double* out) {
 
 // Back-substitution of the chunk of m rows at base, base + NC, ... in w from the
 // solution at its ends (codes es, ee, read through `ends`).
-// FROM_RP: ca holds the forward sweep's r'_k (chunk_reduce_static<true>), and the
-// alphas are made here, by chunk_reduce's backward recurrence, operand for operand.
+// FROM_RP: chunks swept by chunk_reduce<true> or chunk_reduce_static<true> (ca holds
+// r'_k, cb the fill-in g_k): the interior by the back-substitution's recurrence.
 template <int MODE, bool FROM_RP = false, class W, class E>
 NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, const E& ends,
                        double* out) {
@@ -1096,14 +1128,14 @@ NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, cons
   const P __restrict__ cb = w.cb;
   const P __restrict__ cg = w.cg;
   if constexpr (FROM_RP) {
-    if (m <= 2) return;
+    // the forward sweep's rows d'_k x_k + dn_{k+1} x_{k+1} + g_k x_s = r'_k, solved
+    // from the end down: x_k = (r'_k - dn_{k+1} x_{k+1} - g_k x_s) / d'_k, with
+    // (r', g) in (ca, cb)
     const P __restrict__ ci = w.ci;
     const P __restrict__ dn = w.dn;
-    const int g2 = base + (m - 2) * NC;
-    T al = ca[g2] * ci[g2];
-    tps_put<MODE>(w, g2, al + cb[g2] * xs + cg[g2] * xe, out);
-    for (int k0 = m - 3; k0 >= 1; k0 -= kTpsBatchW) {
-      T bp[kTpsBatchW], bi[kTpsBatchW], bn[kTpsBatchW], b[kTpsBatchW], g[kTpsBatchW];
+    T xn = xe;
+    for (int k0 = m - 2; k0 >= 1; k0 -= kTpsBatchW) {
+      T bp[kTpsBatchW], bi[kTpsBatchW], bn[kTpsBatchW], bg[kTpsBatchW];
 #pragma unroll
       for (int j = 0; j < kTpsBatchW; j++) {
         const int k = k0 - j;
@@ -1112,16 +1144,15 @@ NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, cons
           bp[j] = ca[gg];
           bi[j] = ci[gg];
           bn[j] = dn[gg + NC];
-          b[j] = cb[gg];
-          g[j] = cg[gg];
+          bg[j] = cb[gg];
         }
       }
 #pragma unroll
       for (int j = 0; j < kTpsBatchW; j++) {
         const int k = k0 - j;
         if (k >= 1) {
-          al = (bp[j] - bn[j] * al) * bi[j];
-          tps_put<MODE>(w, base + k * NC, al + b[j] * xs + g[j] * xe, out);
+          xn = (bp[j] - bn[j] * xn - bg[j] * xs) * bi[j];
+          tps_put<MODE>(w, base + k * NC, xn, out);
         }
       }
     }
@@ -1147,18 +1178,16 @@ NRT_HD void chunk_back(const W& w, int base, int NC, int m, int es, int ee, cons
   }
 }
 
-// Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1;
-// from_rp: the chunk was last swept by tps_reduce_chunk_static (its alphas are made
-// here), else eliminated in full by tps_reduce_chunk.
+// Back-substitution of chunk `ch` of level `lv` from the solution of level lv+1
+// (after tps_reduce_chunk or tps_reduce_chunk_static).
 template <int MODE, class T>
 NRT_HD void tps_back_chunk(const TreePlanDev& P, const TreeWork<T>& w, int lv, int ch,
-                           double* out, bool from_rp) {
+                           double* out) {
   const int NC = P.nchunk[lv], m = P.chunk_len[ch];
   const int base = P.node_off[lv] + ch - P.chunk_off[lv], nbase = P.node_off[lv + 1];
   const int es = nbase + P.next[base];
   const int ee = m >= 2 ? nbase + P.next[base + (m - 1) * NC] : es;
-  if (from_rp) chunk_back<MODE, true>(w, base, NC, m, es, ee, EndsSlots<T>{w}, out);
-  else chunk_back<MODE>(w, base, NC, m, es, ee, EndsSlots<T>{w}, out);
+  chunk_back<MODE, true>(w, base, NC, m, es, ee, EndsSlots<T>{w}, out);
 }
 
 // Level 0 as the final forest itself (a system with no chain longer than 2).

@@ -1136,9 +1136,8 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           for (int lv = H.levels - 1; lv >= 0; lv--) {
             const int* gc = H.gcol.data() + (size_t)lv * (G + 1);
             for (int ch = H.chunk_off[lv] + gc[g]; ch < H.chunk_off[lv] + gc[g + 1]; ch++) {
-              const bool rp = reuse && !H.chunk_dyn[ch];
-              if (lv > 0) tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr, rp);
-              else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data(), rp);
+              if (lv > 0) tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, nullptr);
+              else tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
             }
           }
         }
@@ -1196,12 +1195,12 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
             tps_final_down(P, w, H.sub_order[k]);
       for (int lv = H.levels - 1; lv >= 1; lv--)
         for (int ch = H.chunk_off[lv]; ch < H.chunk_off[lv + 1]; ch++)
-          tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data(), reuse && !H.chunk_dyn[ch]);
+          tps_back_chunk<OUT_LEVEL>(P, w, lv, ch, xs.data());
       if (H.levels == 0) {
         for (int v = 0; v < H.nslots[0]; v++) tps_final_out<OUT_ASSIGN>(P, w, v, xs.data());
       } else {
         for (int ch = H.chunk_off[0]; ch < H.chunk_off[1]; ch++)
-          tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data(), reuse && !H.chunk_dyn[ch]);
+          tps_back_chunk<OUT_ASSIGN>(P, w, 0, ch, xs.data());
       }
     }
     for (int i = 0; i < N; i++) x[i] = xs[s0[i]];
