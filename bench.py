@@ -45,15 +45,15 @@ def tree_bytes(N, H, nnz, steps, solves, vbytes):
     + the right-hand side written 8 (44 B);
     per tree solve: the forward sweep reads r, the multiplier and the weight and
     writes the partial right-hand side r'; the back-substitution reads r', the pivot
-    reciprocal, the coupling to the next row, beta and gamma and writes V (FP64: 9
-    values of 8 B + V 8 = 80 B; FP32: 9 values of 4 B + V read and written 16 = 52 B);
+    reciprocal, the coupling to the next row and the fill-in g and writes V (FP64: 8
+    values of 8 B + V 8 = 72 B; FP32: 8 values of 4 B + V read and written 16 = 48 B);
     refinement only, per residual evaluation (solves + steps of them): the matrix
     (12 nnz + 4 (N+1)), the gathered V (8 nnz), b (8 N), the residual written into
     the tree system (vbytes N)."""
     per_step = 144 * H + 44 * N
     if vbytes == 8:
-        return steps * per_step + solves * 80 * N
-    per_solve = (9 * vbytes + 16) * N
+        return steps * per_step + solves * 72 * N
+    per_solve = (8 * vbytes + 16) * N
     per_resid = 12 * nnz + 4 * (N + 1) + 8 * nnz + 8 * N + vbytes * N
     return steps * per_step + solves * per_solve + (solves + steps) * per_resid
 
