@@ -253,9 +253,11 @@ struct TreePlanDev {
 
 // Per-value work arrays of one solve, over all levels (total_nodes each): d, up
 // (row -> parent coupling), dn (parent -> row coupling), r, x; the elimination
-// coefficients (three arrays a, b, g, used as (d', g', r') by the forward sweep
-// and as (alpha, beta, gamma) afterwards); and the factors kept for the steps
-// that reuse them: the forward multipliers f and the pivots' reciprocals.
+// coefficients ca, cb, cg (interior rows: the forward sweep's r' in ca and fill-in
+// g in cb, read by the back-substitution's recurrence; the group-resident form's
+// LDS sweeps keep (alpha, beta, gamma) there instead); and the factors kept for the
+// steps that reuse them: the forward multipliers f, the pivots' reciprocals and
+// the weights of alpha_1 (cw).
 template <class T>
 struct TreeWork {
   using value_type = T;
@@ -398,8 +400,8 @@ NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, co
   const T dn1 = dn[g1];
   // forward sweep over rows k = 1..m-1:
   //   d'_k x_k + dn_{k+1} x_{k+1} + g_k x_s = r'_k,
-  // stored as (ca, cb, cg) = (d', g, r') for interior rows; row m-1 becomes the
-  // end row of the reduced system: G_e x_s + D_e x_e + (children) = R_e.
+  // stored as (cb, ca) = (g, r') for interior rows (WSUM; else (cb, cg)); row m-1
+  // becomes the end row of the reduced system: G_e x_s + D_e x_e + (children) = R_e.
   // Each row's pivot reciprocal is computed once, here, and kept in ci: the
   // backward sweep and the steps that reuse the factors only multiply.
   T dp = d[g1], gk = up[g1], rp = rf(g1);
