@@ -9,14 +9,15 @@
 //   * chains (runs of consecutive rows where each row's only child is the next
 //     row) are cut into chunks of at most C rows;
 //   * each chunk eliminates its interior rows into its two end rows (a forward
-//     and a backward sweep, one worker per chunk) giving a reduced system on the
-//     chunk ends with a direct end-to-end coupling; the reduced system is again a
-//     forest of chains, so this repeats until every chain has at most two nodes;
-//   * the final compressed forest is eliminated by height (a node pulls the
-//     contributions of its children, so siblings never write the same parent) and
-//     back-substituted by height;
-//   * eliminated rows are recovered in parallel as x = alpha + beta*x_s + gamma*x_e
-//     (one worker per chunk again, from the solution of the chunk's two ends).
+//     sweep, plus a backward one for the reduced rows' coefficients when the chunk's
+//     values changed; one worker per chunk) giving a reduced system on the chunk
+//     ends with a direct end-to-end coupling; the reduced system is again a forest
+//     of chains, so this repeats until every chain has at most two nodes;
+//   * the final compressed forest is eliminated by chain tasks ordered by height (a
+//     node pulls the contributions of its children, so siblings never write the
+//     same parent) and back-substituted in the reverse order;
+//   * eliminated rows are recovered in parallel from the solution of the chunk's two
+//     ends by the forward rows' recurrence (one worker per chunk again).
 // The plan (levels, chunks, heights) depends only on the topology and is built
 // once on the host; only values flow per solve.
 //
@@ -334,7 +335,7 @@ constexpr int kTpsBatch = 4;
 // not depend on the chain's arithmetic, so a whole batch is one memory round trip).
 constexpr int kTpsBatchS = 8;
 // Rows per batch of the forward-only static sweep and of the back-substitution from
-// r' (more operands per row: the weights, or the alphas' recurrence).
+// r' (more operands per row: the weights, or the recurrence's coefficients).
 constexpr int kTpsBatchW = 4;
 
 // Where a chunk's two surviving ends go: the slots of the next level in the same
@@ -371,8 +372,8 @@ struct RowsOf {
 // sum_k cw_k r'_k over the interior rows (k = 1, 2, ...), with the weights
 // cw_k = (1/pivot_k) prod_{j<k} (-dn_{j+1} / pivot_j) kept in w.cw, so that a
 // chunk that reuses its factors needs only the forward sweep (chunk_reduce_static
-// <true>) and gets the same value to the bit; the interior alphas are then made
-// by the back-substitution (chunk_back<MODE, true>).
+// <true>) and gets the same value to the bit; the interior rows are then solved
+// by the back-substitution's recurrence (chunk_back<MODE, true>).
 // rf(g): the right-hand side of row g (the array r, or computed as it is read).
 template <bool WSUM, class W, class E, class R>
 NRT_HD void chunk_reduce(const W& w, int base, int NC, int m, int es, int ee, const E& ends,
@@ -583,7 +584,7 @@ NRT_HD void chunk_reduce_static(const W& w, int base, int NC, int m, int es, int
   const T r0 = r(base);
   const T dn1 = dn[base + NC];
   if constexpr (WSUM) {
-    // forward only: r'_k into ca (the back-substitution makes the alphas), alpha_1
+    // forward only: r'_k into ca (the back-substitution solves the interior), alpha_1
     // as chunk_reduce<true> sums it
     const P __restrict__ cw = w.cw;
     T rp = r(base + NC);
