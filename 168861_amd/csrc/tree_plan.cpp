@@ -933,6 +933,28 @@ extern "C" int nrt_test_tree_solve_host(int N, const int32_t* rowptr, const int3
           }
       for (int u = 0; u < n; u++)
         if (th[u] < 0) throw std::runtime_error("chain tasks: node never placed");
+      // the task-indexed lists: per task, the parent of its top node and the children
+      // of its bottom node, as nodes and as positions in ftask_node
+      const int nt = H.ftask_hn > 0 ? H.ftask_hoff[H.ftask_hn] : 0;
+      if ((int)H.ftask_par.size() != nt || (int)H.ftask_pnode.size() != nt ||
+          (int)H.ftask_cptr.size() != nt + 1 || H.ftask_child.size() != H.ftask_cnode.size())
+        throw std::runtime_error("task lists: sizes");
+      std::vector<int32_t> pos(n, -1);
+      for (int k = 0; k < n; k++) pos[H.ftask_node[k]] = k;
+      for (int t = 0; t < nt; t++) {
+        const int top = H.ftask_node[H.ftask_off[t + 1] - 1], bot = H.ftask_node[H.ftask_off[t]];
+        const int p = H.fin_par[top];
+        if (H.ftask_pnode[t] != p || H.ftask_par[t] != (p >= 0 ? pos[p] : -1))
+          throw std::runtime_error("task lists: the parent of a task's top node");
+        const int q0 = H.ftask_cptr[t], q1 = H.ftask_cptr[t + 1];
+        if (q1 - q0 != H.fin_cptr[bot + 1] - H.fin_cptr[bot])
+          throw std::runtime_error("task lists: the children of a task's bottom node");
+        for (int q = q0; q < q1; q++) {
+          const int c = H.fin_child[H.fin_cptr[bot] + (q - q0)];
+          if (H.ftask_cnode[q] != c || H.ftask_child[q] != pos[c])
+            throw std::runtime_error("task lists: the children of a task's bottom node");
+        }
+      }
     }
     if (local) {
       // the same for each group's local chain tasks: exactly its local final nodes,
