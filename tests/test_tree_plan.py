@@ -121,6 +121,33 @@ def test_tps_reused_factors_identical(name, chunk, chunk_hi, orc):
         assert np.max(np.abs(reuse - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("index", [0, 5, 176, 504, 400])
+def test_tps_sweep_trees(index, orc):
+    """Trees of the 4,096-tree batched sweep (603 to 46,979 rows; 176 and 504 have
+    final forests of 5,118 and 10,238 nodes, past the LDS of a workgroup) with the
+    chunk lengths its size classes use (auto 8..16, 16/8, 24/8, 16/12), HH rows
+    dynamic and factors reused: the hook also solves each final forest both staged
+    (LDS layout) and by the task-indexed lists and requires the same result to the
+    bit, and the reused-factor solve must equal the full one to the bit and the exact
+    solve to 1e-12."""
+    geom, run = wl.sweep_tree(index)
+    s, bufs, H = _setup(orc, geom)
+    p = _problem(geom, s, bufs, H, run.dT)
+    rp, col, val, rhs = _system(orc, p)
+    n = len(rhs)
+    ref = orc.tree_solve(rp, col, val, rhs)
+    dyn = np.zeros(n, np.uint8)
+    dyn[np.asarray(bufs["hh_elem"][:H])] = 1
+    rng = np.random.default_rng(index)
+    for chunk, chunk_hi in [(min(16, max(8, -(-n // 256))), 8), (16, 8), (24, 8), (16, 12)]:
+        full, lv, _ = tps_host(rp, col, val, rhs, chunk, chunk_hi)
+        reuse, _, _ = tps_host(rp, col, val, rhs, chunk, chunk_hi, dyn=dyn,
+                               rhs_factor=rng.standard_normal(n))
+        assert lv >= 1
+        assert np.array_equal(full, reuse), (chunk, chunk_hi)
+        assert np.max(np.abs(full - ref)) <= 1e-12 * np.max(np.abs(ref)), (chunk, chunk_hi)
+
+
 @pytest.mark.parametrize("groups", [2, 7, 64, 256])
 def test_final_forest_split(groups, orc):
     """Grid launches split the final forest into subtrees per workgroup group; the
