@@ -8,7 +8,7 @@ phase indexes. The first case is the block-mode launch of the HH Y-branch at 100
 the critical step (N = 165, level-0 and reduced chunks of 8 rows, HH rows dynamic:
 the plan k_implicit_tree<double, false> ran when a GPU memory fault was recorded
 with a build of uncommitted sources); the others cover every golden system in the
-block, grid-split and local forms.
+block, grid-split and local forms, and trees of the 4,096-tree batched sweep.
 """
 import os
 import shutil
@@ -87,5 +87,28 @@ def test_golden_systems_in_bounds(name, chunk, chunk_hi, groups, checker, tmp_pa
     p, rp, col, val, rhs, rf, dyn = _system(orc, name, 100)
     x = _run(checker, str(tmp_path), rp, col, val, rhs, rf, p.hh_elem, chunk, chunk_hi, groups,
              dyn)
+    ref = orc.tree_solve(rp, col, val, rhs)
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("index", [0, 504, 400])
+@pytest.mark.parametrize("chunk,chunk_hi", [(8, 8), (24, 8)])
+def test_sweep_trees_in_bounds(index, chunk, chunk_hi, checker, tmp_path, orc):
+    """Trees of the 4,096-tree batched sweep (603, 35,146 and 46,979 rows; tree 504's
+    final forest has 10,238 nodes) at its size classes' chunk lengths, HH rows
+    dynamic: the staged forest's exact-size buffer, the task-indexed lists and the
+    one-pass sweeps stay in bounds."""
+    import workloads as wl
+    from test_oracle_math import _setup, _problem
+    geom, run = wl.sweep_tree(index)
+    s, bufs, H = _setup(orc, geom)
+    p = _problem(geom, s, bufs, H, run.dT)
+    im = orc.Implicit(p, variant=orc.VAR_GPU, method=orc.M_DIRECT)
+    im.update(True)
+    rp, col, val, rhs = im.matrix()
+    dyn = np.zeros(p.N, np.uint8)
+    dyn[np.asarray(p.hh_elem, dtype=np.int64)] = 1
+    rf = np.cos(np.arange(p.N)) * np.max(np.abs(rhs))
+    x = _run(checker, str(tmp_path), rp, col, val, rhs, rf, p.hh_elem, chunk, chunk_hi, 0, dyn)
     ref = orc.tree_solve(rp, col, val, rhs)
     assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
